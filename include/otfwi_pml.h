@@ -1,0 +1,166 @@
+/*
+ * libotfwi_pml.so -- MI355X (gfx950) split-field-PML acoustic propagator,
+ * boundary-saving adjoint gradient and L2 / Wasserstein-2 misfits.
+ *
+ * Two C ABIs are exported:
+ *
+ *  1. The reference's own six symbols, with identical signatures and
+ *     semantics, so the reference's ctypes wrapper (cuda_interface_pml.py:9-30)
+ *     can load this library in place of simulator/bin/cuda_regression_PML_single.so.
+ *
+ *  2. A handle-based API with integer error codes (0 = success, otherwise a
+ *     negative OTFWI_E_* code; otfwi_last_error() gives the message), batched
+ *     shots, and device-resident misfits.  The package's Python wrapper uses
+ *     this one.
+ *
+ * Conventions shared by both (the reference's): fields and models are
+ * row-major (Z, X), x fastest; traces are (n_rx, T) per shot; point lists are
+ * given as separate x and z arrays in the caller's order; units are normalized
+ * (c = v dt/dx, constvec = {dx, dz, dt}, usually {1, 1, 1}).  Sources,
+ * receivers and ring points must lie in the updated interior
+ * 4 <= x < X-4, 4 <= z < Z-4 and must not repeat within one list.
+ * All calls are synchronous with respect to the calling thread.
+ */
+#ifndef OTFWI_PML_H
+#define OTFWI_PML_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------- */
+/* 1. Reference ABI (regression_PML_single.cu).  Process-global state, FP32.  */
+/* ---------------------------------------------------------------------- */
+
+/* Host callback: computes the adjoint source from the simulated traces.
+ * Both buffers hold n_rx*T floats and belong to the library.
+ * Replaces the adj_func typedef, regression_PML_single.cu:19. */
+typedef void (*otfwi_adj_fn)(float *simulated, float *adjoint_source);
+
+/* regression_PML_single.cu:102-153.  Allocates the model, copies c^2, the
+ * dense (Z,X) damping arrays, sources (n_src*T) and receivers, and hands back
+ * a library-owned pinned buffer of n_rx*T floats through *rec.  `init` is
+ * accepted and ignored (the reference never allocates its target). */
+void init_memory_sim(int X, int Z, int T, float *cquad, float *const_vec, float *d_x,
+                     float *d_z, int n_src, int *src_x, int *src_z, int n_rx, int *rx_x,
+                     int *rx_z, float *src, float *init, float **rec);
+
+/* regression_PML_single.cu:229-250.  Sets the boundary ring (n_rev points)
+ * and hands back a library-owned pinned buffer of Z*X floats through *grad.
+ * `obs` is not used (the reference copies it but never reads it). */
+void init_memory_reg(float *obs, float **grad, int *rev_x, int *rev_z, int n_rev);
+
+/* regression_PML_single.cu:253-257.  New c^2 (Z*X floats). */
+void setCquad(float *cquad);
+
+/* regression_PML_single.cu:618-702.  One forward run firing source
+ * idx_source (-1: all sources at once); the traces land in the buffer given
+ * by init_memory_sim (sample 0 is 0).  en_out (video frames) is ignored. */
+void cuda_simulate(int en_out, int idx_source);
+
+/* regression_PML_single.cu:704-846.  Forward run with ring save, host
+ * callback for the adjoint source, reverse reconstruction + adjoint run +
+ * imaging; the gradient lands in the buffer given by init_memory_reg.
+ * *mse is set to 0, as in the reference. */
+void cuda_grad_ext(float *mse, otfwi_adj_fn adj, int idx_source);
+
+/* regression_PML_single.cu:39-43.  Stores a callback pointer; unused, as in
+ * the reference. */
+void def_python_adj(void *fn);
+
+/* ---------------------------------------------------------------------- */
+/* 2. Handle API.                                                          */
+/* ---------------------------------------------------------------------- */
+
+enum {
+    OTFWI_OK = 0,
+    OTFWI_E_ARG = -1,       /* bad argument (shape, range, missing setup) */
+    OTFWI_E_HIP = -2,       /* HIP runtime error */
+    OTFWI_E_MEM = -3,       /* device allocation failed */
+    OTFWI_E_STATE = -4,     /* call order violated (e.g. adjoint before forward) */
+    OTFWI_E_NUMERIC = -5    /* non-finite or unsupported numeric input */
+};
+
+enum { OTFWI_MISFIT_L2 = 0, OTFWI_MISFIT_W2 = 1 };
+
+typedef struct otfwi_model otfwi_model;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *otfwi_last_error(void);
+
+/* Library / device information: writes a short description (device name,
+ * gfx arch, CU count) into buf. */
+int otfwi_device_info(char *buf, int len);
+
+/* New model of X x Z cells and T time samples.  precision: 4 = FP32 state and
+ * arithmetic (production), 8 = FP64 (validation build).  constvec = {dx, dz,
+ * dt}.  Replaces the allocation half of init_memory_sim
+ * (regression_PML_single.cu:102-153). */
+int otfwi_create(otfwi_model **out, int X, int Z, int T, int precision, const float *constvec);
+int otfwi_destroy(otfwi_model *m);
+
+/* Dense (Z,X) damping arrays as Cuda_interface_PML._calcPML produces them
+ * (cuda_interface_pml.py:116-132).  d_x must be constant along z and d_z
+ * along x (separable profiles); the kernels read the reference's one-cell
+ * shifted values d_x[z][x+1] and d_z[z+1][x]. */
+int otfwi_set_damping(otfwi_model *m, const float *d_x, const float *d_z);
+
+/* c^2, (Z,X) FP32 host array (setCquad, regression_PML_single.cu:253-257). */
+int otfwi_set_cquad(otfwi_model *m, const float *cquad);
+/* c^2 from device memory (FP32, Z*X contiguous). */
+int otfwi_set_cquad_device(otfwi_model *m, const float *cquad_dev);
+
+/* Sources: positions and wavelets (n_src*T FP32); the kernels inject the
+ * running sum of each wavelet (somaFonteIntegral, regression_PML_single.cu:276-292). */
+int otfwi_set_sources(otfwi_model *m, int n_src, const int *src_x, const int *src_z,
+                      const float *wavelets);
+int otfwi_set_receivers(otfwi_model *m, int n_rx, const int *rx_x, const int *rx_z);
+/* Boundary-saving ring (init_regression, cuda_interface_pml.py:134-159). */
+int otfwi_set_ring(otfwi_model *m, int n_ring, const int *ring_x, const int *ring_z);
+/* Observed traces for n_shots shots, (n_shots, n_rx, T) FP32. */
+int otfwi_set_observed(otfwi_model *m, int n_shots, const float *obs);
+
+/* Tile heights of the forward / adjoint step kernels (16, 32 or 64; 0 keeps
+ * the current choice) and the largest number of shots resident at once
+ * (0 = all). */
+int otfwi_set_tuning(otfwi_model *m, int th_fwd, int th_adj, int max_batch);
+int otfwi_get_tuning(otfwi_model *m, int *th_fwd, int *th_adj, int *max_batch);
+
+/* Forward runs of n_shots shots, shot k firing source shot_src[k] (-1: all
+ * sources).  rec_out (may be NULL) receives (n_shots, n_rx, T) FP32 traces.
+ * Replaces the per-shot loop of Cuda_interface_PML.simulate
+ * (cuda_interface_pml.py:185-206) over cuda_simulate. */
+int otfwi_forward(otfwi_model *m, int n_shots, const int *shot_src, float *rec_out);
+
+/* Whole gradient with the misfit on the device: forward with ring save,
+ * misfit + adjoint source per trace, reverse/adjoint run, FP64 mean over
+ * shots.  misfit_out (may be NULL): per-shot misfit (n_shots FP64);
+ * grad_out: (Z,X) FP32 mean gradient (may be NULL); rec_out may be NULL.
+ * Replaces Cuda_interface_PML.calc_grad (cuda_interface_pml.py:208-236)
+ * with adj_fun = adjoint_functions.L2 or .wasserstein. */
+int otfwi_gradient(otfwi_model *m, int n_shots, const int *shot_src, int misfit_kind,
+                   double *misfit_out, float *grad_out, float *rec_out);
+
+/* Same, writing the FP32 (Z,X) mean gradient to device memory. */
+int otfwi_gradient_device(otfwi_model *m, int n_shots, const int *shot_src, int misfit_kind,
+                          double *misfit_out, float *grad_out_dev);
+
+/* Two-stage gradient for a host-side adjoint source (an arbitrary adj_fun,
+ * cuda_interface_pml.py:238-249): otfwi_forward_for_grad runs the forward
+ * with ring save and returns the traces; otfwi_adjoint_from_host takes the
+ * (n_shots, n_rx, T) FP32 adjoint source and returns the mean gradient.
+ * At most max_batch shots (otfwi_set_tuning) per pair of calls. */
+int otfwi_forward_for_grad(otfwi_model *m, int n_shots, const int *shot_src, float *rec_out);
+int otfwi_adjoint_from_host(otfwi_model *m, const float *adj_src, float *grad_out);
+
+/* Stand-alone misfit on host arrays (n_traces of T samples, FP64 in and
+ * out, as adj_wrapper hands them over): per-trace misfit and adjoint source.
+ * Replaces adjoint_functions.L2 / .wasserstein (adjoint_functions.py:8-11,
+ * 25-67). */
+int otfwi_misfit(int misfit_kind, int n_traces, int T, const double *sim, const double *obs,
+                 double *misfit_per_trace, double *adj_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OTFWI_PML_H */

@@ -1,0 +1,109 @@
+"""ctypes binding of libotfwi_pml.so (include/otfwi_pml.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (``make -C
+otfwi_amd/csrc``).  There is no fallback: importing the package without the
+built library, or calling it without a GPU, raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libotfwi_pml.so")
+
+OK, E_ARG, E_HIP, E_MEM, E_STATE, E_NUMERIC = 0, -1, -2, -3, -4, -5
+MISFIT_L2, MISFIT_W2 = 0, 1
+
+# (name, restype, argtypes) of every exported symbol declared in include/otfwi_pml.h
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_SIGS = [
+    ("init_memory_sim", None, [_i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp, _i, _vp, _vp, _vp,
+                               _vp, _vp]),
+    ("init_memory_reg", None, [_vp, _vp, _vp, _vp, _i]),
+    ("setCquad", None, [_vp]),
+    ("cuda_simulate", None, [_i, _i]),
+    ("cuda_grad_ext", None, [_vp, _vp, _i]),
+    ("def_python_adj", None, [_vp]),
+    ("otfwi_last_error", ctypes.c_char_p, []),
+    ("otfwi_device_info", _i, [ctypes.c_char_p, _i]),
+    ("otfwi_create", _i, [_vp, _i, _i, _i, _i, _vp]),
+    ("otfwi_destroy", _i, [_vp]),
+    ("otfwi_set_damping", _i, [_vp, _vp, _vp]),
+    ("otfwi_set_cquad", _i, [_vp, _vp]),
+    ("otfwi_set_cquad_device", _i, [_vp, _vp]),
+    ("otfwi_set_sources", _i, [_vp, _i, _vp, _vp, _vp]),
+    ("otfwi_set_receivers", _i, [_vp, _i, _vp, _vp]),
+    ("otfwi_set_ring", _i, [_vp, _i, _vp, _vp]),
+    ("otfwi_set_observed", _i, [_vp, _i, _vp]),
+    ("otfwi_set_tuning", _i, [_vp, _i, _i, _i]),
+    ("otfwi_get_tuning", _i, [_vp, _vp, _vp, _vp]),
+    ("otfwi_forward", _i, [_vp, _i, _vp, _vp]),
+    ("otfwi_gradient", _i, [_vp, _i, _vp, _i, _vp, _vp, _vp]),
+    ("otfwi_gradient_device", _i, [_vp, _i, _vp, _i, _vp, _vp]),
+    ("otfwi_forward_for_grad", _i, [_vp, _i, _vp, _vp]),
+    ("otfwi_adjoint_from_host", _i, [_vp, _vp, _vp]),
+    ("otfwi_misfit", _i, [_i, _i, _i, _vp, _vp, _vp, _vp]),
+]
+
+EXPORTED = [s[0] for s in _SIGS]
+
+_LIB = None
+
+
+class OtfwiError(RuntimeError):
+    """Error code + message from libotfwi_pml."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def load():
+    """Load the library and declare every signature.  Raises ImportError when
+    it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                "(make -C otfwi_amd/csrc); there is no CPU fallback")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, res, args in _SIGS:
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = lib
+    return _LIB
+
+
+def check(rc):
+    if rc != OK:
+        msg = load().otfwi_last_error()
+        raise OtfwiError(rc, msg.decode() if msg else "")
+    return rc
+
+
+def ptr(a):
+    """Data pointer of a C-contiguous numpy array (None stays None)."""
+    if a is None:
+        return None
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def device_info():
+    buf = ctypes.create_string_buffer(256)
+    check(load().otfwi_device_info(buf, 256))
+    return buf.value.decode()

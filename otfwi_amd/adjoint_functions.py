@@ -1,0 +1,58 @@
+"""Misfits with adjoint sources, computed on the GPU.
+
+Same names, signatures and return values as the reference's
+adjoint_functions.py: ``f(sim, obs) -> (misfit, adjoint_source (n_rx, T)
+float64)``.
+
+* :func:`L2` -- adjoint_functions.py:8-11: (sum (sim-obs)^2, sim-obs).
+* :func:`wasserstein` -- adjoint_functions.py:25-67: per-trace 1D W2 of the
+  affinely shifted, normalized traces, with the unit-spacing transport-map
+  integral as adjoint source; one wavefront per trace with FP64 CDF scans and
+  inverse-CDF searches.
+
+:class:`otfwi_amd.cuda_interface_pml.Cuda_interface_PML` recognizes these two
+functions and keeps the whole gradient on the device; any other callable is
+called on the host once per shot, as the reference's callback does.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, ptr
+
+__all__ = ["L2", "wasserstein"]
+
+
+def _run(kind, sim, obs):
+    s = np.ascontiguousarray(sim, dtype=np.float64)
+    o = np.ascontiguousarray(obs, dtype=np.float64)
+    squeeze = s.ndim == 1
+    if squeeze:
+        s, o = s[None], o[None]
+    if s.shape != o.shape or s.ndim != 2:
+        raise ValueError(f"sim {s.shape} and obs {o.shape} must be equal (n_rx, T) arrays")
+    n, T = s.shape
+    mis = np.zeros(n, np.float64)
+    adj = np.zeros((n, T), np.float64)
+    check(_lib.load().otfwi_misfit(kind, n, T, ptr(s), ptr(o), ptr(mis), ptr(adj)))
+    # traces are accumulated in trace order (adjoint_functions.py:33, 64)
+    total = 0.0
+    for v in mis:
+        total += v
+    return total, (adj[0] if squeeze else adj)
+
+
+def L2(sim, obs):
+    """(sum((sim-obs)^2), sim-obs) -- adjoint_functions.py:8-11."""
+    return _run(_lib.MISFIT_L2, sim, obs)
+
+
+def wasserstein(sim, obs):
+    """1D Wasserstein-2 misfit and adjoint source -- adjoint_functions.py:25-67."""
+    return _run(_lib.MISFIT_W2, sim, obs)
+
+
+# markers the gradient driver uses to keep these misfits on the device
+L2._otfwi_misfit = "l2"
+wasserstein._otfwi_misfit = "w2"

@@ -1,0 +1,110 @@
+"""Synthetic workloads: normalized velocity models, Ricker sources and
+transducer arrays for the benchmark configurations.
+
+Units follow the reference's convention (cuda_interface_pml.py:58, 99;
+setups/setup1_OCNDT.py:22-28): the propagator runs with dx = dz = dt = 1 and
+``c`` is the Courant number v*dt/dx.  At CFL 0.5 with v_max = 5900 m/s,
+dt = 0.5*dx/5900, so c(5900) = 0.5.
+
+A source or receiver "on the top row" is placed on row ``pml`` (the first
+undamped row): rows z < 4 are never updated by the scheme and the PML rows
+attenuate, so a transducer there would be dead or damped.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+V_BG = 5900.0          # background (steel) velocity, m/s
+V_MAX = 5900.0
+CFL = 0.5
+
+
+def courant(v, v_max=V_MAX, cfl=CFL):
+    """Normalized velocity c = v*dt/dx at dt = cfl*dx/v_max."""
+    return np.asarray(v, dtype=np.float64) * cfl / v_max
+
+
+def ricker(f0, dt, T, t0=None):
+    """Ricker wavelet (1 - 2 (pi f (t-t0))^2) exp(-(pi f (t-t0))^2) sampled at
+    t = k*dt, k = 0..T-1; t0 defaults to 1.2/f0."""
+    if t0 is None:
+        t0 = 1.2 / f0
+    t = np.arange(T) * dt - t0
+    a = (math.pi * f0 * t) ** 2
+    return (1.0 - 2.0 * a) * np.exp(-a)
+
+
+def linear_array(N, n_elem, z, pitch=None, x_first=None, x_last=None):
+    """Boolean (N, N) mask with n_elem points on row z.  Either a pitch (in
+    cells, array centred) or the end columns are given."""
+    mask = np.zeros((N, N), dtype=bool)
+    if pitch is not None:
+        span = (n_elem - 1) * pitch
+        x0 = (N - 1 - span) // 2
+        xs = x0 + pitch * np.arange(n_elem)
+    else:
+        xs = np.round(np.linspace(x_first, x_last, n_elem)).astype(int)
+    mask[z, xs] = True
+    return mask, xs
+
+
+def draw_defects(c, rng, pml, n_min=1, n_max=5, semi_min=6.0, semi_max=60.0,
+                 v_lo=1480.0, v_hi=3200.0, z_frac=0.2):
+    """Paint 1..5 circles/ellipses (probability 1/2 each) into the normalized
+    model ``c`` (modified in place); semi-axes uniform in cells, velocity
+    uniform in [v_lo, v_hi] m/s, centres uniform in the lower 80 %."""
+    Z, X = c.shape
+    zz, xx = np.mgrid[0:Z, 0:X]
+    k = int(rng.integers(n_min, n_max + 1))
+    for _ in range(k):
+        a = rng.uniform(semi_min, semi_max)
+        b = a if rng.random() < 0.5 else rng.uniform(semi_min, semi_max)
+        v = rng.uniform(v_lo, v_hi)
+        cz = rng.uniform(z_frac * Z, Z - pml)
+        cx = rng.uniform(pml, X - pml)
+        inside = ((xx - cx) / a) ** 2 + ((zz - cz) / b) ** 2 <= 1.0
+        c[inside] = courant(v)
+    return c
+
+
+def config1(seed=0):
+    """Plain single shot: 256x256, dx = 0.1 mm, background 5900 m/s with one
+    1480 m/s circular inclusion of radius 1.5 mm (15 cells) at a seeded
+    position, 20-cell PML, 5 MHz Ricker at the top centre, 32 receivers along
+    the top row, CFL 0.5, 1500 steps."""
+    N, T, pml = 256, 1500, 20
+    dx = 1e-4
+    dt = CFL * dx / V_MAX
+    rng = np.random.default_rng(seed)
+    c_bg = np.full((N, N), float(courant(V_BG)))
+    c_true = c_bg.copy()
+    r = 15
+    cz = rng.uniform(pml + r + 40, N - pml - r - 5)
+    cx = rng.uniform(pml + r + 5, N - pml - r - 5)
+    zz, xx = np.mgrid[0:N, 0:N]
+    c_true[(xx - cx) ** 2 + (zz - cz) ** 2 <= r * r] = float(courant(1480.0))
+    src_mask = np.zeros((N, N), dtype=bool)
+    src_mask[pml, N // 2] = True
+    rx_mask, _ = linear_array(N, 32, pml, x_first=pml, x_last=N - pml - 1)
+    src = ricker(5e6, dt, T)[None, :]
+    return dict(X=N, Z=N, T=T, pml=pml, R=1e-3, c_true=c_true, c_bg=c_bg,
+                src_mask=src_mask, rx_mask=rx_mask, src=src, dx=dx, dt=dt)
+
+
+def config2(model_seed=0, N=1024, T=4000, n_elem=64, pitch=12, pml=20):
+    """Full-matrix capture: N x N, dx = 0.05 mm, 64-element array (pitch
+    0.6 mm = 12 cells) on the top row, each element fires in turn and all
+    receive; 1-5 random defects; CFL 0.5; 20-cell PML."""
+    dx = 5e-5
+    dt = CFL * dx / V_MAX
+    rng = np.random.default_rng(model_seed)
+    c_bg = np.full((N, N), float(courant(V_BG)))
+    c_true = draw_defects(c_bg.copy(), rng, pml)
+    mask, xs = linear_array(N, n_elem, pml, pitch=pitch)
+    w = ricker(5e6, dt, T)
+    src = np.tile(w, (n_elem, 1))
+    return dict(X=N, Z=N, T=T, pml=pml, R=1e-3, c_true=c_true, c_bg=c_bg,
+                src_mask=mask, rx_mask=mask.copy(), src=src, dx=dx, dt=dt,
+                n_shots=n_elem)

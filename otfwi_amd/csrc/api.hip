@@ -1,0 +1,1000 @@
+// Host orchestration and C ABIs of libotfwi_pml.so (see include/otfwi_pml.h).
+//
+// A model owns its device buffers and one HIP stream.  A forward run of a
+// batch of shots is T-1 launches of the fused forward step (every shot of
+// the batch in each launch); a gradient adds the misfit kernels and T-1
+// launches of the fused adjoint step, then an FP64 reduction over shots.
+// The host loop only enqueues work; nothing synchronizes inside it.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "otfwi_internal.h"
+#include "otfwi_pml.h"
+
+// Host-side arithmetic that must round like the reference (1 - d*dt,
+// (c^2*dt)/dx, running source sums) stays unfused.
+#pragma clang fp contract(off)
+
+using namespace otfwi;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCK(expr)                                                                  \
+    do {                                                                             \
+        hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess)                                                        \
+            return fail(OTFWI_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                         \
+    } while (0)
+
+#define RCK(expr)                        \
+    do {                                 \
+        int r_ = (expr);                 \
+        if (r_ != OTFWI_OK) return r_;   \
+    } while (0)
+
+template <typename T>
+int dalloc(T **p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) return OTFWI_OK;
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(OTFWI_E_MEM, "hipMalloc of %zu bytes failed: %s", n * sizeof(T),
+                    hipGetErrorString(e));
+    }
+    return OTFWI_OK;
+}
+
+template <typename T>
+void dfree(T *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+}  // namespace
+
+struct Batch {
+    int cap = 0;
+    void *P[2] = {}, *Ax[2] = {}, *Az[2] = {}, *Px[2] = {};
+    void *LP[2] = {}, *LAx[2] = {}, *LAz[2] = {}, *LPx[2] = {};
+    void *grad = nullptr, *ring = nullptr, *rec = nullptr, *adj = nullptr, *jcum = nullptr;
+    double *mis_tr = nullptr, *mis_shot = nullptr, *scratch = nullptr;
+    int *shot_src = nullptr;
+    float *obs = nullptr;   // obs of the shots in this batch
+    size_t ring_elems = 0;
+};
+
+struct otfwi_model {
+    int X = 0, Z = 0, T = 0, ld = 0, prec = 4;
+    float dx = 1, dz = 1, dt = 1;
+    hipStream_t stream = nullptr;
+    void *cm = nullptr, *kx = nullptr, *kz = nullptr, *icum = nullptr;
+    float *cq32 = nullptr;          // c^2 as given (FP32)
+    bool have_cq = false, have_damp = false;
+    std::vector<float> dxs_h, dzs_h;  // shifted damping profiles d_x[.][x+1], d_z[z+1][.]
+    std::vector<float> kx_h, kz_h;    // 1 where the cell is undamped along that axis, else 0
+    std::vector<int> sx, sz, rxx, rxz, vx, vz;
+    std::vector<float> src_h;
+    int n_src = 0, n_rx = 0, n_ring = 0;
+    int *evt = nullptr;
+    PointEvt *pts = nullptr;
+    bool points_dirty = true;
+    int th_fwd = 32, th_adj = 16, max_batch = 0;
+    uint8_t *tf_fwd = nullptr, *tf_adj = nullptr;
+    int tf_fwd_th = 0, tf_adj_th = 0;
+    std::vector<float> obs_h;
+    int obs_shots = 0;
+    Batch b;
+    double *gacc = nullptr;          // FP64 sum of per-shot gradients across batches
+    float *gout = nullptr;           // FP32 mean gradient on device
+    // state of a pending two-stage gradient
+    int pend_n = 0;
+};
+
+namespace {
+
+size_t esize(const otfwi_model *m) { return m->prec == 8 ? 8 : 4; }
+
+int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+void free_batch(Batch &b)
+{
+    for (int i = 0; i < 2; i++) {
+        dfree(b.P[i]); dfree(b.Ax[i]); dfree(b.Az[i]); dfree(b.Px[i]);
+        dfree(b.LP[i]); dfree(b.LAx[i]); dfree(b.LAz[i]); dfree(b.LPx[i]);
+    }
+    dfree(b.grad); dfree(b.ring); dfree(b.rec); dfree(b.adj); dfree(b.jcum);
+    dfree(b.mis_tr); dfree(b.mis_shot); dfree(b.scratch); dfree(b.shot_src); dfree(b.obs);
+    b.cap = 0;
+    b.ring_elems = 0;
+}
+
+int ensure_batch(otfwi_model *m, int n, bool need_adj)
+{
+    Batch &b = m->b;
+    const size_t es = esize(m);
+    const size_t plane = (size_t)m->Z * m->ld;
+    const size_t ring_need = (size_t)n * m->T * std::max(m->n_ring, 1);
+    const bool adj_ok = b.LP[0] != nullptr || !need_adj;
+    if (b.cap >= n && adj_ok && (!need_adj || b.ring_elems >= ring_need)) return OTFWI_OK;
+    const int cap = std::max(n, b.cap);
+    free_batch(b);
+    auto fa = [&](void **p, size_t elems) -> int {
+        char *q = nullptr;
+        RCK(dalloc(&q, elems * es));
+        *p = q;
+        return OTFWI_OK;
+    };
+    for (int i = 0; i < 2; i++) {
+        RCK(fa(&b.P[i], plane * cap));
+        RCK(fa(&b.Ax[i], plane * cap));
+        RCK(fa(&b.Az[i], plane * cap));
+        RCK(fa(&b.Px[i], plane * cap));
+        if (need_adj) {
+            RCK(fa(&b.LP[i], plane * cap));
+            RCK(fa(&b.LAx[i], plane * cap));
+            RCK(fa(&b.LAz[i], plane * cap));
+            RCK(fa(&b.LPx[i], plane * cap));
+        }
+    }
+    const size_t ntr = (size_t)cap * std::max(m->n_rx, 1);
+    RCK(fa(&b.rec, ntr * m->T));
+    if (need_adj) {
+        RCK(fa(&b.grad, plane * cap));
+        b.ring_elems = (size_t)cap * m->T * std::max(m->n_ring, 1);
+        RCK(fa(&b.ring, b.ring_elems));
+        RCK(fa(&b.adj, ntr * m->T));
+        RCK(fa(&b.jcum, ntr * m->T));
+        RCK(dalloc(&b.mis_tr, ntr));
+        RCK(dalloc(&b.mis_shot, (size_t)cap));
+        RCK(dalloc(&b.scratch, ntr * 3 * m->T + ntr));
+        RCK(dalloc(&b.obs, ntr * m->T));
+    }
+    RCK(dalloc(&b.shot_src, (size_t)cap));
+    b.cap = cap;
+    return OTFWI_OK;
+}
+
+// Tile flags for tile height th: STRIP if the E box touches a damped cell,
+// EVENT if it holds a source, receiver or ring point.
+int build_flags(otfwi_model *m, int th, uint8_t **dst, int *dst_th)
+{
+    if (*dst && *dst_th == th && !m->points_dirty) return OTFWI_OK;
+    const int ntx = ceil_div(m->X, TW), ntz = ceil_div(m->Z, th);
+    std::vector<uint8_t> f((size_t)ntx * ntz, 0);
+    std::vector<char> ev((size_t)m->X * m->Z, 0);
+    auto mark = [&](const std::vector<int> &xs, const std::vector<int> &zs) {
+        for (size_t i = 0; i < xs.size(); i++) ev[(size_t)zs[i] * m->X + xs[i]] = 1;
+    };
+    mark(m->sx, m->sz);
+    mark(m->rxx, m->rxz);
+    mark(m->vx, m->vz);
+    for (int tz = 0; tz < ntz; tz++)
+        for (int tx = 0; tx < ntx; tx++) {
+            uint8_t fl = 0;
+            const int xa = std::max(tx * TW - 4, BOX), xb = std::min(tx * TW + TW + 4, m->X - BOX);
+            const int za = std::max(tz * th - 4, BOX), zb = std::min(tz * th + th + 4, m->Z - BOX);
+            for (int z = za; z < zb; z++)
+                for (int x = xa; x < xb; x++) {
+                    if (m->kx_h[x] != 1.0f || m->kz_h[z] != 1.0f) fl |= TF_STRIP;
+                    if (ev[(size_t)z * m->X + x]) fl |= TF_EVENT;
+                }
+            f[(size_t)tz * ntx + tx] = fl;
+        }
+    dfree(*dst);
+    RCK(dalloc(dst, f.size()));
+    HIPCK(hipMemcpy(*dst, f.data(), f.size(), hipMemcpyHostToDevice));
+    *dst_th = th;
+    return OTFWI_OK;
+}
+
+int check_points(const otfwi_model *m, int n, const int *xs, const int *zs, const char *what)
+{
+    if (n < 0 || (n > 0 && (!xs || !zs))) return fail(OTFWI_E_ARG, "%s: bad list", what);
+    std::vector<char> seen((size_t)m->X * m->Z, 0);
+    for (int i = 0; i < n; i++) {
+        const int x = xs[i], z = zs[i];
+        if (x < BOX || z < BOX || x >= m->X - BOX || z >= m->Z - BOX)
+            return fail(OTFWI_E_ARG,
+                        "%s %d at (x=%d, z=%d) lies outside the updated interior "
+                        "4 <= x < %d, 4 <= z < %d",
+                        what, i, x, z, m->X - BOX, m->Z - BOX);
+        char &s = seen[(size_t)z * m->X + x];
+        if (s) return fail(OTFWI_E_ARG, "%s %d at (x=%d, z=%d) repeats a cell", what, i, x, z);
+        s = 1;
+    }
+    return OTFWI_OK;
+}
+
+// Event map and point table from the source, receiver and ring lists.
+int build_points(otfwi_model *m)
+{
+    if (!m->points_dirty && m->evt) return OTFWI_OK;
+    std::vector<int> map((size_t)m->Z * m->ld, -1);
+    std::vector<PointEvt> pts;
+    auto get = [&](int x, int z) -> PointEvt & {
+        int &e = map[(size_t)z * m->ld + x];
+        if (e < 0) {
+            e = (int)pts.size();
+            pts.push_back(PointEvt{-1, -1, -1, 0});
+        }
+        return pts[e];
+    };
+    for (int i = 0; i < m->n_src; i++) get(m->sx[i], m->sz[i]).src = i;
+    for (int i = 0; i < m->n_rx; i++) get(m->rxx[i], m->rxz[i]).rx = i;
+    for (int i = 0; i < m->n_ring; i++) get(m->vx[i], m->vz[i]).ring = i;
+    dfree(m->evt);
+    dfree(m->pts);
+    RCK(dalloc(&m->evt, map.size()));
+    HIPCK(hipMemcpy(m->evt, map.data(), map.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!pts.empty()) {
+        RCK(dalloc(&m->pts, pts.size()));
+        HIPCK(hipMemcpy(m->pts, pts.data(), pts.size() * sizeof(PointEvt), hipMemcpyHostToDevice));
+    }
+    m->points_dirty = false;
+    dfree(m->tf_fwd);
+    dfree(m->tf_adj);
+    return OTFWI_OK;
+}
+
+template <typename R>
+int upload_model_arrays(otfwi_model *m)
+{
+    // cm = (c^2 dt)/dx, rounded as the reference forms it (regression_PML_single.cu:512).
+    const size_t n = (size_t)m->Z * m->ld;
+    std::vector<float> cq((size_t)m->Z * m->X);
+    HIPCK(hipMemcpy(cq.data(), m->cq32, cq.size() * sizeof(float), hipMemcpyDeviceToHost));
+    std::vector<R> cm(n, R(0));
+    for (int z = 0; z < m->Z; z++)
+        for (int x = 0; x < m->X; x++) {
+            const R c = (R)cq[(size_t)z * m->X + x];
+            cm[(size_t)z * m->ld + x] = (c * (R)m->dt) / (R)m->dx;
+        }
+    if (!m->cm) RCK(dalloc((char **)&m->cm, n * sizeof(R)));
+    HIPCK(hipMemcpy(m->cm, cm.data(), n * sizeof(R), hipMemcpyHostToDevice));
+    return OTFWI_OK;
+}
+
+int check_cq(const float *cq, size_t n)
+{
+    for (size_t i = 0; i < n; i++)
+        if (!std::isfinite(cq[i]) || cq[i] < 0.0f)
+            return fail(OTFWI_E_NUMERIC, "c^2 at flat index %zu is %g (must be finite and >= 0)",
+                        i, (double)cq[i]);
+    return OTFWI_OK;
+}
+
+int set_cq_host(otfwi_model *m, const float *cq)
+{
+    if (!cq) return fail(OTFWI_E_ARG, "cquad is NULL");
+    const size_t n = (size_t)m->X * m->Z;
+    RCK(check_cq(cq, n));
+    if (!m->cq32) RCK(dalloc(&m->cq32, n));
+    HIPCK(hipMemcpy(m->cq32, cq, n * sizeof(float), hipMemcpyHostToDevice));
+    m->have_cq = true;
+    return m->prec == 8 ? upload_model_arrays<double>(m) : upload_model_arrays<float>(m);
+}
+
+template <typename R>
+int upload_sources(otfwi_model *m)
+{
+    // running sum of each wavelet in the state precision, sequential as in
+    // somaFonteIntegral (regression_PML_single.cu:286-291): icum[n][t] =
+    // sum_{tau=1..t} s[n][tau]; s[n][0] is never injected.
+    std::vector<R> ic((size_t)std::max(m->n_src, 1) * m->T, R(0));
+    for (int n = 0; n < m->n_src; n++) {
+        R acc = R(0);
+        for (int t = 1; t < m->T; t++) {
+            acc = acc + (R)m->src_h[(size_t)n * m->T + t];
+            ic[(size_t)n * m->T + t] = acc;
+        }
+    }
+    dfree(m->icum);
+    RCK(dalloc((char **)&m->icum, ic.size() * sizeof(R)));
+    HIPCK(hipMemcpy(m->icum, ic.data(), ic.size() * sizeof(R), hipMemcpyHostToDevice));
+    return OTFWI_OK;
+}
+
+// k = 1 - d*dt in the state precision from the one-cell-shifted profiles
+// d_x[z][x+1] and d_z[z+1][x] (regression_PML_single.cu:505-506, 512-513).
+template <typename R>
+int upload_damping(otfwi_model *m)
+{
+    std::vector<R> kx(m->X), kz(m->Z);
+    m->kx_h.assign(m->X, 1.0f);
+    m->kz_h.assign(m->Z, 1.0f);
+    for (int x = 0; x < m->X; x++) {
+        kx[x] = R(1) - (R)m->dxs_h[x] * (R)m->dt;
+        m->kx_h[x] = kx[x] == R(1) ? 1.0f : 0.0f;
+    }
+    for (int z = 0; z < m->Z; z++) {
+        kz[z] = R(1) - (R)m->dzs_h[z] * (R)m->dt;
+        m->kz_h[z] = kz[z] == R(1) ? 1.0f : 0.0f;
+    }
+    dfree(m->kx);
+    dfree(m->kz);
+    RCK(dalloc((char **)&m->kx, kx.size() * sizeof(R)));
+    RCK(dalloc((char **)&m->kz, kz.size() * sizeof(R)));
+    HIPCK(hipMemcpy(m->kx, kx.data(), kx.size() * sizeof(R), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(m->kz, kz.data(), kz.size() * sizeof(R), hipMemcpyHostToDevice));
+    dfree(m->tf_fwd);
+    dfree(m->tf_adj);
+    return OTFWI_OK;
+}
+
+int ready(otfwi_model *m, bool need_ring)
+{
+    if (!m->have_cq) return fail(OTFWI_E_STATE, "c^2 not set");
+    if (!m->have_damp) return fail(OTFWI_E_STATE, "damping not set");
+    if (need_ring && m->n_ring == 0) return fail(OTFWI_E_STATE, "boundary ring not set");
+    RCK(build_points(m));
+    RCK(build_flags(m, m->th_fwd, &m->tf_fwd, &m->tf_fwd_th));
+    RCK(build_flags(m, m->th_adj, &m->tf_adj, &m->tf_adj_th));
+    return OTFWI_OK;
+}
+
+template <typename R>
+int run_forward(otfwi_model *m, int n, bool save_ring)
+{
+    Batch &b = m->b;
+    const size_t es = sizeof(R);
+    const size_t plane = (size_t)m->Z * m->ld;
+    hipStream_t st = m->stream;
+    for (int i = 0; i < 2; i++) {
+        HIPCK(hipMemsetAsync(b.P[i], 0, plane * n * es, st));
+        HIPCK(hipMemsetAsync(b.Ax[i], 0, plane * n * es, st));
+        HIPCK(hipMemsetAsync(b.Az[i], 0, plane * n * es, st));
+        HIPCK(hipMemsetAsync(b.Px[i], 0, plane * n * es, st));
+    }
+    HIPCK(hipMemsetAsync(b.rec, 0, (size_t)n * m->n_rx * m->T * es, st));
+    if (save_ring) {
+        // sample 0 of the ring is never written by the steps (reference memsets it)
+        for (int s = 0; s < n; s++)
+            HIPCK(hipMemsetAsync((char *)b.ring + (size_t)s * m->T * m->n_ring * es, 0,
+                                 (size_t)m->n_ring * es, st));
+    }
+    FwdArgs<R> a{};
+    a.X = m->X; a.Z = m->Z; a.ld = m->ld;
+    a.ntx = ceil_div(m->X, TW); a.ntz = ceil_div(m->Z, m->th_fwd);
+    a.nshots = n; a.T = m->T;
+    a.sstride = (long long)plane;
+    a.cm = (const R *)m->cm;
+    a.cA = (R)m->dt / (R)m->dx;
+    a.kx = (const R *)m->kx; a.kz = (const R *)m->kz;
+    a.tflags = m->tf_fwd;
+    a.evt = m->evt; a.pts = m->pts;
+    a.shot_src = b.shot_src;
+    a.icum = (const R *)m->icum;
+    a.rec = (R *)b.rec; a.n_rx = m->n_rx;
+    a.ring = save_ring ? (R *)b.ring : nullptr;
+    a.n_ring = m->n_ring;
+    for (int t = 1; t < m->T; t++) {
+        const int p = (t - 1) & 1, c = t & 1;
+        a.t = t;
+        a.P0 = (const R *)b.P[p]; a.P1 = (R *)b.P[c];
+        a.Ax0 = (const R *)b.Ax[p]; a.Ax1 = (R *)b.Ax[c];
+        a.Az0 = (const R *)b.Az[p]; a.Az1 = (R *)b.Az[c];
+        a.Px0 = (const R *)b.Px[p]; a.Px1 = (R *)b.Px[c];
+        HIPCK(launch_fwd_step<R>(a, m->th_fwd, st));
+    }
+    return OTFWI_OK;
+}
+
+template <typename R>
+int run_adjoint(otfwi_model *m, int n)
+{
+    Batch &b = m->b;
+    const size_t es = sizeof(R);
+    const size_t plane = (size_t)m->Z * m->ld;
+    hipStream_t st = m->stream;
+    for (int i = 0; i < 2; i++) {
+        HIPCK(hipMemsetAsync(b.LP[i], 0, plane * n * es, st));
+        HIPCK(hipMemsetAsync(b.LAx[i], 0, plane * n * es, st));
+        HIPCK(hipMemsetAsync(b.LAz[i], 0, plane * n * es, st));
+        HIPCK(hipMemsetAsync(b.LPx[i], 0, plane * n * es, st));
+    }
+    HIPCK(hipMemsetAsync(b.grad, 0, plane * n * es, st));
+    AdjArgs<R> a{};
+    a.X = m->X; a.Z = m->Z; a.ld = m->ld;
+    a.ntx = ceil_div(m->X, TW); a.ntz = ceil_div(m->Z, m->th_adj);
+    a.nshots = n; a.T = m->T;
+    a.sstride = (long long)plane;
+    a.grad = (R *)b.grad;
+    a.cm = (const R *)m->cm;
+    a.cA = (R)m->dt / (R)m->dx;
+    a.kx = (const R *)m->kx; a.kz = (const R *)m->kz;
+    a.tflags = m->tf_adj;
+    a.evt = m->evt; a.pts = m->pts;
+    a.jcum = (const R *)b.jcum; a.n_rx = m->n_rx;
+    a.ring = (const R *)b.ring; a.n_ring = m->n_ring;
+    const int T = m->T;
+    for (int t = 1; t < T; t++) {
+        const int lp = (t - 1) & 1, lc = t & 1;
+        // reconstructed field: U(t) lives in P slot (T-1+t)%2, A_b(t) in slot (T-2+t)%2,
+        // starting from the forward's P(T-1) and A(T-2) (regression_PML_single.cu:787-790)
+        const int up = (T - 2 + t) & 1, uc = (T - 1 + t) & 1;
+        const int ap = (T - 1 + t) & 1, ac = (T - 2 + t) & 1;
+        a.t = t;
+        a.LP0 = (const R *)b.LP[lp]; a.LP1 = (R *)b.LP[lc];
+        a.LAx0 = (const R *)b.LAx[lp]; a.LAx1 = (R *)b.LAx[lc];
+        a.LAz0 = (const R *)b.LAz[lp]; a.LAz1 = (R *)b.LAz[lc];
+        a.LPx0 = (const R *)b.LPx[lp]; a.LPx1 = (R *)b.LPx[lc];
+        a.U0 = (const R *)b.P[up]; a.U1 = (R *)b.P[uc];
+        a.UAx0 = (const R *)b.Ax[ap]; a.UAx1 = (R *)b.Ax[ac];
+        a.UAz0 = (const R *)b.Az[ap]; a.UAz1 = (R *)b.Az[ac];
+        HIPCK(launch_adj_step<R>(a, m->th_adj, st));
+    }
+    return OTFWI_OK;
+}
+
+int upload_shot_src(otfwi_model *m, int n, const int *shot_src)
+{
+    std::vector<int> ss(n, -1);
+    for (int i = 0; i < n; i++) {
+        ss[i] = shot_src ? shot_src[i] : -1;
+        if (ss[i] < -1 || ss[i] >= m->n_src)
+            return fail(OTFWI_E_ARG, "shot %d fires source %d (have %d sources)", i, ss[i], m->n_src);
+    }
+    HIPCK(hipMemcpyAsync(m->b.shot_src, ss.data(), n * sizeof(int), hipMemcpyHostToDevice,
+                         m->stream));
+    HIPCK(hipStreamSynchronize(m->stream));
+    return OTFWI_OK;
+}
+
+template <typename R>
+int copy_rec_out(otfwi_model *m, int n, float *rec_out)
+{
+    const size_t cnt = (size_t)n * m->n_rx * m->T;
+    if (sizeof(R) == 4) {
+        HIPCK(hipMemcpyAsync(rec_out, m->b.rec, cnt * sizeof(float), hipMemcpyDeviceToHost,
+                             m->stream));
+        HIPCK(hipStreamSynchronize(m->stream));
+    } else {
+        std::vector<double> tmp(cnt);
+        HIPCK(hipMemcpyAsync(tmp.data(), m->b.rec, cnt * sizeof(double), hipMemcpyDeviceToHost,
+                             m->stream));
+        HIPCK(hipStreamSynchronize(m->stream));
+        for (size_t i = 0; i < cnt; i++) rec_out[i] = (float)tmp[i];
+    }
+    return OTFWI_OK;
+}
+
+int batch_size(otfwi_model *m, int n)
+{
+    return m->max_batch > 0 ? std::min(n, m->max_batch) : n;
+}
+
+template <typename R>
+int forward_impl(otfwi_model *m, int n, const int *shot_src, float *rec_out)
+{
+    RCK(ready(m, false));
+    const int B = batch_size(m, n);
+    RCK(ensure_batch(m, B, false));
+    for (int s0 = 0; s0 < n; s0 += B) {
+        const int nb = std::min(B, n - s0);
+        RCK(upload_shot_src(m, nb, shot_src ? shot_src + s0 : nullptr));
+        RCK(run_forward<R>(m, nb, false));
+        if (rec_out) RCK(copy_rec_out<R>(m, nb, rec_out + (size_t)s0 * m->n_rx * m->T));
+    }
+    HIPCK(hipStreamSynchronize(m->stream));
+    return OTFWI_OK;
+}
+
+template <typename R>
+int upload_obs_batch(otfwi_model *m, int s0, int nb)
+{
+    if (m->obs_shots < s0 + nb)
+        return fail(OTFWI_E_STATE, "observed data holds %d shots, need %d", m->obs_shots, s0 + nb);
+    const size_t per = (size_t)m->n_rx * m->T;
+    HIPCK(hipMemcpyAsync(m->b.obs, m->obs_h.data() + (size_t)s0 * per, nb * per * sizeof(float),
+                         hipMemcpyHostToDevice, m->stream));
+    return OTFWI_OK;
+}
+
+
+// One batch of a gradient: forward with ring save, misfit + adjoint source
+// (device kind) or host adjoint source already in b.adj (kind < 0), adjoint
+// run, and accumulation of the batch's per-shot gradients into m->gacc.
+template <typename R>
+int grad_batch_device(otfwi_model *m, int s0, int nb, const int *shot_src, int kind,
+                      double *mis_out, float *rec_out)
+{
+    Batch &b = m->b;
+    RCK(upload_shot_src(m, nb, shot_src ? shot_src + s0 : nullptr));
+    RCK(upload_obs_batch<R>(m, s0, nb));
+    RCK(run_forward<R>(m, nb, true));
+    if (rec_out) RCK(copy_rec_out<R>(m, nb, rec_out + (size_t)s0 * m->n_rx * m->T));
+    const int ntr = nb * m->n_rx;
+    int *flags = reinterpret_cast<int *>(b.scratch + (size_t)ntr * 3 * m->T);
+    if (kind == OTFWI_MISFIT_W2) HIPCK(hipMemsetAsync(flags, 0, ntr * sizeof(int), m->stream));
+    HIPCK((launch_misfit<R, float>(kind, (const R *)b.rec, b.obs, (R *)b.adj, (R *)b.jcum,
+                                   b.mis_tr, ntr, m->T, b.scratch, m->stream)));
+    HIPCK(launch_reduce_misfit(b.mis_tr, nb, m->n_rx, b.mis_shot, m->stream));
+    RCK(run_adjoint<R>(m, nb));
+    const size_t plane = (size_t)m->Z * m->ld;
+    HIPCK(launch_accum_grad<R>((const R *)b.grad, nb, m->X, m->Z, m->ld, (long long)plane, m->gacc,
+                               m->stream));
+    std::vector<double> hmis(nb);
+    HIPCK(hipMemcpyAsync(hmis.data(), b.mis_shot, nb * sizeof(double), hipMemcpyDeviceToHost,
+                         m->stream));
+    std::vector<int> fl(kind == OTFWI_MISFIT_W2 ? ntr : 0);
+    if (kind == OTFWI_MISFIT_W2)
+        HIPCK(hipMemcpyAsync(fl.data(), flags, ntr * sizeof(int), hipMemcpyDeviceToHost, m->stream));
+    HIPCK(hipStreamSynchronize(m->stream));
+    for (size_t i = 0; i < fl.size(); i++)
+        if (fl[i])
+            return fail(OTFWI_E_NUMERIC, "W2: observed trace %zu of shot %d has a non-finite CDF",
+                        i % m->n_rx, s0 + (int)(i / m->n_rx));
+    if (mis_out)
+        for (int i = 0; i < nb; i++) mis_out[s0 + i] = hmis[i];
+    return OTFWI_OK;
+}
+
+int ensure_gacc(otfwi_model *m)
+{
+    const size_t n = (size_t)m->X * m->Z;
+    if (!m->gacc) RCK(dalloc(&m->gacc, n));
+    if (!m->gout) RCK(dalloc(&m->gout, n));
+    HIPCK(hipMemsetAsync(m->gacc, 0, n * sizeof(double), m->stream));
+    return OTFWI_OK;
+}
+
+template <typename R>
+int gradient_impl(otfwi_model *m, int n, const int *shot_src, int kind, double *mis_out,
+                  float *grad_out, bool out_dev, float *rec_out)
+{
+    if (kind != OTFWI_MISFIT_L2 && kind != OTFWI_MISFIT_W2)
+        return fail(OTFWI_E_ARG, "unknown misfit kind %d", kind);
+    if (n <= 0) return fail(OTFWI_E_ARG, "n_shots must be positive");
+    if (m->n_rx <= 0) return fail(OTFWI_E_STATE, "receivers not set");
+    if (kind == OTFWI_MISFIT_W2 && m->T < 2) return fail(OTFWI_E_ARG, "W2 needs T >= 2");
+    RCK(ready(m, true));
+    const int B = batch_size(m, n);
+    RCK(ensure_batch(m, B, true));
+    RCK(ensure_gacc(m));
+    for (int s0 = 0; s0 < n; s0 += B)
+        RCK(grad_batch_device<R>(m, s0, std::min(B, n - s0), shot_src, kind, mis_out, rec_out));
+    const int cells = m->X * m->Z;
+    HIPCK(launch_finalize_grad(m->gacc, cells, 1.0 / n, m->gout, m->stream));
+    if (grad_out) {
+        HIPCK(hipMemcpyAsync(grad_out, m->gout, cells * sizeof(float),
+                             out_dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, m->stream));
+    }
+    HIPCK(hipStreamSynchronize(m->stream));
+    return OTFWI_OK;
+}
+
+template <typename R>
+int forward_for_grad_impl(otfwi_model *m, int n, const int *shot_src, float *rec_out)
+{
+    if (n <= 0) return fail(OTFWI_E_ARG, "n_shots must be positive");
+    RCK(ready(m, true));
+    if (m->max_batch > 0 && n > m->max_batch)
+        return fail(OTFWI_E_ARG, "two-stage gradient takes at most max_batch=%d shots per call",
+                    m->max_batch);
+    RCK(ensure_batch(m, n, true));
+    RCK(upload_shot_src(m, n, shot_src));
+    RCK(run_forward<R>(m, n, true));
+    if (rec_out) RCK(copy_rec_out<R>(m, n, rec_out));
+    HIPCK(hipStreamSynchronize(m->stream));
+    m->pend_n = n;
+    return OTFWI_OK;
+}
+
+template <typename R>
+int adjoint_from_host_impl(otfwi_model *m, const float *adj, float *grad_out)
+{
+    const int n = m->pend_n;
+    if (n <= 0) return fail(OTFWI_E_STATE, "no pending forward run (call otfwi_forward_for_grad)");
+    if (!adj) return fail(OTFWI_E_ARG, "adjoint source is NULL");
+    Batch &b = m->b;
+    const size_t cnt = (size_t)n * m->n_rx * m->T;
+    std::vector<R> tmp(cnt);
+    for (size_t i = 0; i < cnt; i++) tmp[i] = (R)adj[i];
+    HIPCK(hipMemcpyAsync(b.adj, tmp.data(), cnt * sizeof(R), hipMemcpyHostToDevice, m->stream));
+    HIPCK(launch_jcum<R>((const R *)b.adj, (R *)b.jcum, n * m->n_rx, m->T, m->stream));
+    RCK(run_adjoint<R>(m, n));
+    RCK(ensure_gacc(m));
+    const size_t plane = (size_t)m->Z * m->ld;
+    HIPCK(launch_accum_grad<R>((const R *)b.grad, n, m->X, m->Z, m->ld, (long long)plane, m->gacc,
+                               m->stream));
+    const int cells = m->X * m->Z;
+    HIPCK(launch_finalize_grad(m->gacc, cells, 1.0 / n, m->gout, m->stream));
+    if (grad_out)
+        HIPCK(hipMemcpyAsync(grad_out, m->gout, cells * sizeof(float), hipMemcpyDeviceToHost,
+                             m->stream));
+    HIPCK(hipStreamSynchronize(m->stream));
+    m->pend_n = 0;
+    return OTFWI_OK;
+}
+
+#define DISPATCH(m, call) ((m)->prec == 8 ? call<double> : call<float>)
+
+int check_model(const otfwi_model *m)
+{
+    if (!m) return fail(OTFWI_E_ARG, "model handle is NULL");
+    return OTFWI_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// Handle API
+// ===========================================================================
+extern "C" {
+
+const char *otfwi_last_error(void) { return g_err.c_str(); }
+
+int otfwi_device_info(char *buf, int len)
+{
+    int dev = 0;
+    hipDeviceProp_t p;
+    HIPCK(hipGetDevice(&dev));
+    HIPCK(hipGetDeviceProperties(&p, dev));
+    if (buf && len > 0)
+        snprintf(buf, len, "%s | %s | %d CUs | %.1f GB", p.name, p.gcnArchName,
+                 p.multiProcessorCount, p.totalGlobalMem / 1e9);
+    return OTFWI_OK;
+}
+
+int otfwi_create(otfwi_model **out, int X, int Z, int T, int precision, const float *constvec)
+{
+    if (!out) return fail(OTFWI_E_ARG, "out is NULL");
+    *out = nullptr;
+    if (X < 2 * BOX + 1 || Z < 2 * BOX + 1 || T < 2)
+        return fail(OTFWI_E_ARG, "grid %dx%d with T=%d is too small", X, Z, T);
+    if ((long long)X * Z > (1LL << 30)) return fail(OTFWI_E_ARG, "grid too large");
+    if (precision != 4 && precision != 8)
+        return fail(OTFWI_E_ARG, "precision must be 4 (FP32) or 8 (FP64)");
+    otfwi_model *m = new otfwi_model();
+    m->X = X; m->Z = Z; m->T = T; m->prec = precision;
+    m->ld = ceil_div(X, 64) * 64;
+    if (constvec) { m->dx = constvec[0]; m->dz = constvec[1]; m->dt = constvec[2]; }
+    if (!(m->dx > 0) || !(m->dt > 0)) {
+        delete m;
+        return fail(OTFWI_E_ARG, "dx and dt must be positive");
+    }
+    if (precision == 8) { m->th_fwd = 16; m->th_adj = 16; }
+    m->dxs_h.assign(X, 0.0f);
+    m->dzs_h.assign(Z, 0.0f);
+    m->kx_h.assign(X, 1.0f);
+    m->kz_h.assign(Z, 1.0f);
+    hipError_t e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete m;
+        return fail(OTFWI_E_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
+    }
+    *out = m;
+    return OTFWI_OK;
+}
+
+int otfwi_destroy(otfwi_model *m)
+{
+    if (!m) return OTFWI_OK;
+    (void)hipStreamSynchronize(m->stream);
+    free_batch(m->b);
+    dfree(m->cm); dfree(m->kx); dfree(m->kz); dfree(m->icum); dfree(m->cq32);
+    dfree(m->evt); dfree(m->pts); dfree(m->tf_fwd); dfree(m->tf_adj);
+    dfree(m->gacc); dfree(m->gout);
+    (void)hipStreamDestroy(m->stream);
+    delete m;
+    return OTFWI_OK;
+}
+
+int otfwi_set_damping(otfwi_model *m, const float *d_x, const float *d_z)
+{
+    RCK(check_model(m));
+    if (!d_x || !d_z) return fail(OTFWI_E_ARG, "damping arrays are NULL");
+    const int X = m->X, Z = m->Z;
+    for (int z = 1; z < Z; z++)
+        for (int x = 0; x < X; x++)
+            if (d_x[(size_t)z * X + x] != d_x[x])
+                return fail(OTFWI_E_ARG, "d_x varies along z at (x=%d, z=%d): only separable "
+                                         "profiles are supported", x, z);
+    for (int z = 0; z < Z; z++)
+        for (int x = 1; x < X; x++)
+            if (d_z[(size_t)z * X + x] != d_z[(size_t)z * X])
+                return fail(OTFWI_E_ARG, "d_z varies along x at (x=%d, z=%d): only separable "
+                                         "profiles are supported", x, z);
+    for (int x = 0; x < X; x++) {
+        const float v = x + 1 < X ? d_x[x + 1] : 0.0f;
+        if (!std::isfinite(v) || v < 0.0f) return fail(OTFWI_E_NUMERIC, "bad d_x value %g", v);
+        m->dxs_h[x] = v;
+    }
+    for (int z = 0; z < Z; z++) {
+        const float v = z + 1 < Z ? d_z[(size_t)(z + 1) * X] : 0.0f;
+        if (!std::isfinite(v) || v < 0.0f) return fail(OTFWI_E_NUMERIC, "bad d_z value %g", v);
+        m->dzs_h[z] = v;
+    }
+    RCK(m->prec == 8 ? upload_damping<double>(m) : upload_damping<float>(m));
+    m->have_damp = true;
+    return OTFWI_OK;
+}
+
+int otfwi_set_cquad(otfwi_model *m, const float *cquad)
+{
+    RCK(check_model(m));
+    return set_cq_host(m, cquad);
+}
+
+int otfwi_set_cquad_device(otfwi_model *m, const float *cquad_dev)
+{
+    RCK(check_model(m));
+    if (!cquad_dev) return fail(OTFWI_E_ARG, "cquad is NULL");
+    const size_t n = (size_t)m->X * m->Z;
+    if (!m->cq32) RCK(dalloc(&m->cq32, n));
+    HIPCK(hipMemcpy(m->cq32, cquad_dev, n * sizeof(float), hipMemcpyDeviceToDevice));
+    std::vector<float> h(n);
+    HIPCK(hipMemcpy(h.data(), m->cq32, n * sizeof(float), hipMemcpyDeviceToHost));
+    RCK(check_cq(h.data(), n));
+    m->have_cq = true;
+    return m->prec == 8 ? upload_model_arrays<double>(m) : upload_model_arrays<float>(m);
+}
+
+int otfwi_set_sources(otfwi_model *m, int n_src, const int *src_x, const int *src_z,
+                      const float *wavelets)
+{
+    RCK(check_model(m));
+    RCK(check_points(m, n_src, src_x, src_z, "source"));
+    if (n_src > 0 && !wavelets) return fail(OTFWI_E_ARG, "wavelets are NULL");
+    m->n_src = n_src;
+    m->sx.assign(src_x, src_x + n_src);
+    m->sz.assign(src_z, src_z + n_src);
+    m->src_h.assign(wavelets, wavelets + (size_t)n_src * m->T);
+    for (float v : m->src_h)
+        if (!std::isfinite(v)) return fail(OTFWI_E_NUMERIC, "non-finite source sample");
+    m->points_dirty = true;
+    return m->prec == 8 ? upload_sources<double>(m) : upload_sources<float>(m);
+}
+
+int otfwi_set_receivers(otfwi_model *m, int n_rx, const int *rx_x, const int *rx_z)
+{
+    RCK(check_model(m));
+    RCK(check_points(m, n_rx, rx_x, rx_z, "receiver"));
+    m->n_rx = n_rx;
+    m->rxx.assign(rx_x, rx_x + n_rx);
+    m->rxz.assign(rx_z, rx_z + n_rx);
+    m->points_dirty = true;
+    free_batch(m->b);
+    return OTFWI_OK;
+}
+
+int otfwi_set_ring(otfwi_model *m, int n_ring, const int *ring_x, const int *ring_z)
+{
+    RCK(check_model(m));
+    RCK(check_points(m, n_ring, ring_x, ring_z, "ring point"));
+    m->n_ring = n_ring;
+    m->vx.assign(ring_x, ring_x + n_ring);
+    m->vz.assign(ring_z, ring_z + n_ring);
+    m->points_dirty = true;
+    free_batch(m->b);
+    return OTFWI_OK;
+}
+
+int otfwi_set_observed(otfwi_model *m, int n_shots, const float *obs)
+{
+    RCK(check_model(m));
+    if (n_shots <= 0 || !obs) return fail(OTFWI_E_ARG, "bad observed data");
+    if (m->n_rx <= 0) return fail(OTFWI_E_STATE, "set receivers before observed data");
+    const size_t cnt = (size_t)n_shots * m->n_rx * m->T;
+    m->obs_h.assign(obs, obs + cnt);
+    m->obs_shots = n_shots;
+    return OTFWI_OK;
+}
+
+int otfwi_set_tuning(otfwi_model *m, int th_fwd, int th_adj, int max_batch)
+{
+    RCK(check_model(m));
+    auto ok = [&](int th, bool adj) {
+        if (th == 0) return true;
+        if (m->prec == 8) return th == 16 || (!adj && th == 32);
+        return th == 16 || th == 32 || th == 64;
+    };
+    if (!ok(th_fwd, false) || !ok(th_adj, true))
+        return fail(OTFWI_E_ARG, "unsupported tile height (fwd %d, adj %d) for precision %d",
+                    th_fwd, th_adj, m->prec);
+    if (max_batch < 0) return fail(OTFWI_E_ARG, "max_batch must be >= 0");
+    if (th_fwd) m->th_fwd = th_fwd;
+    if (th_adj) m->th_adj = th_adj;
+    if (max_batch != m->max_batch) free_batch(m->b);
+    m->max_batch = max_batch;
+    return OTFWI_OK;
+}
+
+int otfwi_get_tuning(otfwi_model *m, int *th_fwd, int *th_adj, int *max_batch)
+{
+    RCK(check_model(m));
+    if (th_fwd) *th_fwd = m->th_fwd;
+    if (th_adj) *th_adj = m->th_adj;
+    if (max_batch) *max_batch = m->max_batch;
+    return OTFWI_OK;
+}
+
+int otfwi_forward(otfwi_model *m, int n_shots, const int *shot_src, float *rec_out)
+{
+    RCK(check_model(m));
+    if (n_shots <= 0) return fail(OTFWI_E_ARG, "n_shots must be positive");
+    return DISPATCH(m, forward_impl)(m, n_shots, shot_src, rec_out);
+}
+
+int otfwi_gradient(otfwi_model *m, int n_shots, const int *shot_src, int misfit_kind,
+                   double *misfit_out, float *grad_out, float *rec_out)
+{
+    RCK(check_model(m));
+    return DISPATCH(m, gradient_impl)(m, n_shots, shot_src, misfit_kind, misfit_out, grad_out,
+                                      false, rec_out);
+}
+
+int otfwi_gradient_device(otfwi_model *m, int n_shots, const int *shot_src, int misfit_kind,
+                          double *misfit_out, float *grad_out_dev)
+{
+    RCK(check_model(m));
+    return DISPATCH(m, gradient_impl)(m, n_shots, shot_src, misfit_kind, misfit_out, grad_out_dev,
+                                      true, nullptr);
+}
+
+int otfwi_forward_for_grad(otfwi_model *m, int n_shots, const int *shot_src, float *rec_out)
+{
+    RCK(check_model(m));
+    return DISPATCH(m, forward_for_grad_impl)(m, n_shots, shot_src, rec_out);
+}
+
+int otfwi_adjoint_from_host(otfwi_model *m, const float *adj_src, float *grad_out)
+{
+    RCK(check_model(m));
+    return DISPATCH(m, adjoint_from_host_impl)(m, adj_src, grad_out);
+}
+
+int otfwi_misfit(int kind, int n_traces, int T, const double *sim, const double *obs,
+                 double *mis_out, double *adj_out)
+{
+    if (kind != OTFWI_MISFIT_L2 && kind != OTFWI_MISFIT_W2)
+        return fail(OTFWI_E_ARG, "unknown misfit kind %d", kind);
+    if (n_traces <= 0 || T < 2 || !sim || !obs)
+        return fail(OTFWI_E_ARG, "bad misfit arguments (n_traces=%d, T=%d)", n_traces, T);
+    if (kind == OTFWI_MISFIT_W2 && (size_t)2 * T * sizeof(double) > 160 * 1024)
+        return fail(OTFWI_E_ARG, "W2 supports T <= %d", (int)(160 * 1024 / 16));
+    const size_t cnt = (size_t)n_traces * T;
+    double *dsim = nullptr, *dobs = nullptr, *dadj = nullptr, *dmis = nullptr, *dscr = nullptr;
+    int rc = OTFWI_OK;
+    auto cleanup = [&] { dfree(dsim); dfree(dobs); dfree(dadj); dfree(dmis); dfree(dscr); };
+    if ((rc = dalloc(&dsim, cnt)) || (rc = dalloc(&dobs, cnt)) || (rc = dalloc(&dadj, cnt)) ||
+        (rc = dalloc(&dmis, (size_t)n_traces)) || (rc = dalloc(&dscr, cnt * 3 + n_traces))) {
+        cleanup();
+        return rc;
+    }
+    hipError_t e = hipMemcpy(dsim, sim, cnt * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dobs, obs, cnt * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(dscr, 0, (cnt * 3 + n_traces) * sizeof(double));
+    if (e == hipSuccess)
+        e = launch_misfit<double, double>(kind, dsim, dobs, dadj, nullptr, dmis, n_traces, T, dscr,
+                                          nullptr);
+    std::vector<int> fl(n_traces, 0);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess && mis_out)
+        e = hipMemcpy(mis_out, dmis, n_traces * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && adj_out)
+        e = hipMemcpy(adj_out, dadj, cnt * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && kind == OTFWI_MISFIT_W2)
+        e = hipMemcpy(fl.data(), dscr + cnt * 3, n_traces * sizeof(int), hipMemcpyDeviceToHost);
+    cleanup();
+    if (e != hipSuccess) return fail(OTFWI_E_HIP, "misfit: %s", hipGetErrorString(e));
+    for (int i = 0; i < n_traces; i++)
+        if (fl[i]) return fail(OTFWI_E_NUMERIC, "W2: observed trace %d has a non-finite CDF", i);
+    return OTFWI_OK;
+}
+
+// ===========================================================================
+// Reference ABI (regression_PML_single.cu), on one process-global FP32 model.
+// ===========================================================================
+
+static otfwi_model *g_ref = nullptr;
+static float *g_rec_h = nullptr, *g_grad_h = nullptr, *g_sim_h = nullptr, *g_adj_h = nullptr;
+static void *g_py_adj = nullptr;
+
+static void ref_report(int rc, const char *where)
+{
+    if (rc != OTFWI_OK) fprintf(stderr, "otfwi: %s: %s\n", where, g_err.c_str());
+}
+
+void init_memory_sim(int X, int Z, int T, float *cquad, float *const_vec, float *d_x, float *d_z,
+                     int n_src, int *src_x, int *src_z, int n_rx, int *rx_x, int *rx_z, float *src,
+                     float *init, float **rec)
+{
+    (void)init;
+    if (g_ref) otfwi_destroy(g_ref);
+    g_ref = nullptr;
+    if (g_rec_h) (void)hipHostFree(g_rec_h);
+    g_rec_h = nullptr;
+    int rc = otfwi_create(&g_ref, X, Z, T, 4, const_vec);
+    if (!rc) rc = otfwi_set_damping(g_ref, d_x, d_z);
+    if (!rc) rc = otfwi_set_cquad(g_ref, cquad);
+    if (!rc) rc = otfwi_set_sources(g_ref, n_src, src_x, src_z, src);
+    if (!rc) rc = otfwi_set_receivers(g_ref, n_rx, rx_x, rx_z);
+    if (!rc && hipHostMalloc((void **)&g_rec_h, (size_t)std::max(n_rx, 1) * T * sizeof(float)) !=
+                   hipSuccess)
+        rc = fail(OTFWI_E_MEM, "pinned recording buffer");
+    if (!rc && g_rec_h) memset(g_rec_h, 0, (size_t)std::max(n_rx, 1) * T * sizeof(float));
+    ref_report(rc, "init_memory_sim");
+    if (rec) *rec = g_rec_h;
+}
+
+void init_memory_reg(float *obs, float **grad, int *rev_x, int *rev_z, int n_rev)
+{
+    (void)obs;
+    int rc = g_ref ? OTFWI_OK : fail(OTFWI_E_STATE, "init_memory_sim was not called");
+    if (!rc) rc = otfwi_set_ring(g_ref, n_rev, rev_x, rev_z);
+    if (!rc) {
+        if (g_grad_h) (void)hipHostFree(g_grad_h);
+        if (g_sim_h) (void)hipHostFree(g_sim_h);
+        if (g_adj_h) (void)hipHostFree(g_adj_h);
+        g_grad_h = g_sim_h = g_adj_h = nullptr;
+        const size_t tr = (size_t)std::max(g_ref->n_rx, 1) * g_ref->T;
+        if (hipHostMalloc((void **)&g_grad_h, (size_t)g_ref->X * g_ref->Z * sizeof(float)) !=
+                hipSuccess ||
+            hipHostMalloc((void **)&g_sim_h, tr * sizeof(float)) != hipSuccess ||
+            hipHostMalloc((void **)&g_adj_h, tr * sizeof(float)) != hipSuccess)
+            rc = fail(OTFWI_E_MEM, "pinned gradient buffers");
+        else
+            memset(g_grad_h, 0, (size_t)g_ref->X * g_ref->Z * sizeof(float));
+    }
+    ref_report(rc, "init_memory_reg");
+    if (grad) *grad = g_grad_h;
+}
+
+void setCquad(float *cquad)
+{
+    int rc = g_ref ? otfwi_set_cquad(g_ref, cquad)
+                   : fail(OTFWI_E_STATE, "init_memory_sim was not called");
+    ref_report(rc, "setCquad");
+}
+
+void cuda_simulate(int en_out, int idx_source)
+{
+    (void)en_out;
+    int rc = g_ref ? OTFWI_OK : fail(OTFWI_E_STATE, "init_memory_sim was not called");
+    if (!rc) rc = otfwi_forward(g_ref, 1, &idx_source, g_rec_h);
+    ref_report(rc, "cuda_simulate");
+}
+
+void cuda_grad_ext(float *mse, otfwi_adj_fn adj, int idx_source)
+{
+    int rc = (g_ref && g_grad_h) ? OTFWI_OK
+                                 : fail(OTFWI_E_STATE, "init_memory_sim/init_memory_reg not called");
+    if (!rc) rc = otfwi_forward_for_grad(g_ref, 1, &idx_source, g_sim_h);
+    if (!rc) {
+        const size_t tr = (size_t)g_ref->n_rx * g_ref->T;
+        memset(g_adj_h, 0, tr * sizeof(float));
+        if (adj) adj(g_sim_h, g_adj_h);
+        rc = otfwi_adjoint_from_host(g_ref, g_adj_h, g_grad_h);
+    }
+    if (mse) *mse = 0.0f;
+    ref_report(rc, "cuda_grad_ext");
+}
+
+void def_python_adj(void *fn) { g_py_adj = fn; }
+
+}  // extern "C"

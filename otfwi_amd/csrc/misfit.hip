@@ -1,0 +1,381 @@
+// Misfit kernels: per-trace L2 and 1D Wasserstein-2 misfits with their
+// adjoint sources, the running sum of the time-flipped adjoint source that
+// the adjoint step injects, and the FP64 cross-shot reductions.
+//
+// W2 follows adjoint_functions.wasserstein (adjoint_functions.py:25-67) and
+// the scipy semantics it relies on:
+//   t_k = k/(T-1) (numpy linspace, last sample exactly 1), dt = t_1 - t_0;
+//   c = max(-1.1 min(obs), 0.1 max(obs)); g = obs + c, f = sim + c;
+//   g /= sum(g dt), f /= sum(f dt);
+//   G = [0, cumsum(diff(t) (g[1:] + g[:-1]) / 2)], F likewise;
+//   interp1d(xs, ys, 'linear', extrapolate): stable sort of xs,
+//     i = clip(searchsorted_left(xs, q), 1, n-1),
+//     y = (ys[i]-ys[i-1]) / (xs[i]-xs[i-1]) * (q - xs[i-1]) + ys[i-1];
+//   q = G^-1(F(t)); D = 2 [0, cumsum((h[1:] + h[:-1]) / 2)] with h = t - q
+//     (unit spacing); adj = D(t) - D(1); W2 = sum (F^-1(t) - G^-1(t))^2.
+// One wavefront owns one trace; CDFs are built with FP64 wavefront prefix
+// scans and kept in LDS; inverse CDFs are binary searches in LDS.  A CDF that
+// is not monotone (sim + c < 0 somewhere) is stably sorted into a global
+// scratch area first (rank sort: O(T^2) but only for such traces).
+#include "otfwi_internal.h"
+
+namespace otfwi {
+
+constexpr int WAVE = 64;
+
+__device__ __forceinline__ double tval(int k, int T, double step)
+{
+    return k == T - 1 ? 1.0 : (double)k * step;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int o = WAVE / 2; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+__device__ __forceinline__ double wave_min(double v)
+{
+    for (int o = WAVE / 2; o > 0; o >>= 1)
+        v = fmin(v, __shfl_xor(v, o, WAVE));
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int o = WAVE / 2; o > 0; o >>= 1)
+        v = fmax(v, __shfl_xor(v, o, WAVE));
+    return v;
+}
+// exclusive prefix sum over the wavefront (lane order)
+__device__ __forceinline__ double wave_excl_scan(double v)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    double incl = v;
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const double u = __shfl_up(incl, o, WAVE);
+        if (lane >= o) incl += u;
+    }
+    return incl - v;
+}
+
+// Trapezoid running integral, in place: y[k] <- [0, cumsum(d_k (y[k+1]+y[k])/2)]
+// with d_k = t_{k+1} - t_k (spacing from tval) or 1 (unit).  Lane l owns the
+// contiguous chunk [l*C, (l+1)*C); the only cross-chunk operand, y[l*C-1], is
+// read before any lane writes.
+template <bool UNIT>
+__device__ void cumtrapz_inplace(double *y, int T, double step)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int C = (T + WAVE - 1) / WAVE;
+    const int k0 = lane * C, k1 = min(k0 + C, T);
+    const double left = (k0 > 0 && k0 < T) ? y[k0 - 1] : 0.0;
+    double local = 0.0;
+    for (int k = max(k0, 1); k < k1; k++) {
+        const double d = UNIT ? 1.0 : tval(k, T, step) - tval(k - 1, T, step);
+        const double ym = (k == k0) ? left : y[k - 1];
+        local += d * (y[k] + ym) / 2.0;
+    }
+    const double off = wave_excl_scan(local);
+    __syncthreads();
+    double run = off, prev = left;
+    for (int k = k0; k < k1; k++) {
+        const double yk = y[k];
+        if (k > 0) {
+            const double d = UNIT ? 1.0 : tval(k, T, step) - tval(k - 1, T, step);
+            run += d * (yk + prev) / 2.0;
+        }
+        prev = yk;
+        y[k] = (k == 0) ? 0.0 : run;
+    }
+    __syncthreads();
+}
+
+// searchsorted(xs, q, side='left') on a sorted array of n values.
+template <typename XS>
+__device__ __forceinline__ int lower_bound(const XS &xs, int n, double q)
+{
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (xs(mid) < q) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+template <typename XS, typename YS>
+__device__ __forceinline__ double interp_lin(const XS &xs, const YS &ys, int n, double q)
+{
+    int i = lower_bound(xs, n, q);
+    i = i < 1 ? 1 : (i > n - 1 ? n - 1 : i);
+    const double xl = xs(i - 1), xh = xs(i), yl = ys(i - 1), yh = ys(i);
+    const double slope = (yh - yl) / (xh - xl);
+    return slope * (q - xl) + yl;
+}
+
+// Value of interp1d(t, Y) at t_k itself (searchsorted gives k).
+__device__ __forceinline__ double interp_at_grid(const double *Y, int k, int T, double step)
+{
+    const int i = k < 1 ? 1 : k;
+    const double xl = tval(i - 1, T, step), xh = tval(i, T, step);
+    const double slope = (Y[i] - Y[i - 1]) / (xh - xl);
+    return slope * (tval(k, T, step) - xl) + Y[i - 1];
+}
+
+// Stable sort of (xs, t) by xs into (sx, st): element j goes to position
+// #{m : xs_m < xs_j} + #{m < j : xs_m == xs_j}, the order numpy's mergesort
+// argsort produces.
+__device__ void rank_sort(const double *xs, int T, double step, double *sx, double *st)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    for (int j = lane; j < T; j += WAVE) {
+        const double v = xs[j];
+        int r = 0;
+        for (int m = 0; m < T; m++) {
+            const double u = xs[m];
+            r += (u < v) || (u == v && m < j);
+        }
+        sx[r] = v;
+        st[r] = tval(j, T, step);
+    }
+    __syncthreads();
+}
+
+__device__ bool is_monotone(const double *Y, int T)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    int bad = 0;
+    for (int k = lane + 1; k < T; k += WAVE)
+        bad |= Y[k] < Y[k - 1];
+    return __any(bad) == 0;
+}
+
+template <typename R, typename O>
+__global__ void __launch_bounds__(WAVE)
+w2_kernel(const R *sim, const O *obs, R *adj, double *mis, int T, double *scratch, int *flags)
+{
+    extern __shared__ double lds[];
+    double *Gs = lds;          // g, then G
+    double *Fs = lds + T;      // f, then F
+    const int tr = blockIdx.x;
+    const int lane = threadIdx.x;
+    const R *f0 = sim + (size_t)tr * T;
+    const O *g0 = obs + (size_t)tr * T;
+    double *h = scratch + (size_t)tr * 3 * T;    // h, then D
+    double *sx = h + T, *st = h + 2 * T;          // sorted copies when needed
+    const double step = 1.0 / (double)(T - 1);
+    const double dt = tval(1, T, step) - tval(0, T, step);
+
+    double mn = INFINITY, mx = -INFINITY;
+    for (int k = lane; k < T; k += WAVE) {
+        const double v = (double)g0[k];
+        mn = fmin(mn, v);
+        mx = fmax(mx, v);
+    }
+    mn = wave_min(mn);
+    mx = wave_max(mx);
+    const double c = fmax(-1.1 * mn, 0.1 * mx);
+
+    double sg = 0.0, sf = 0.0;
+    for (int k = lane; k < T; k += WAVE) {
+        const double g = (double)g0[k] + c, f = (double)f0[k] + c;
+        Gs[k] = g;
+        Fs[k] = f;
+        sg += g * dt;
+        sf += f * dt;
+    }
+    sg = wave_sum(sg);
+    sf = wave_sum(sf);
+    __syncthreads();
+    for (int k = lane; k < T; k += WAVE) {
+        Gs[k] = Gs[k] / sg;
+        Fs[k] = Fs[k] / sf;
+    }
+    __syncthreads();
+    cumtrapz_inplace<false>(Gs, T, step);
+    cumtrapz_inplace<false>(Fs, T, step);
+
+    const bool gmono = is_monotone(Gs, T), fmono = is_monotone(Fs, T);
+    const double *gx = Gs, *gy = nullptr, *fx = Fs, *fy = nullptr;
+    if (!fmono) {
+        rank_sort(Fs, T, step, sx, st);
+        fx = sx;
+        fy = st;
+    }
+    // c >= -1.1 min(obs) and c >= 0.1 max(obs) give obs + c >= 0 on every
+    // sample, so G never decreases for finite input; the flag reports the
+    // non-finite case to the host.
+    if (!gmono && lane == 0) flags[tr] = 1;
+
+    auto tgrid = [=](int i) { return tval(i, T, step); };
+    auto Gx = [=](int i) { return gx[i]; };
+    auto Fx = [=](int i) { return fx[i]; };
+    auto Fy = [=](int i) { return fy ? fy[i] : tval(i, T, step); };
+    (void)gy;
+
+    double w = 0.0;
+    for (int k = lane; k < T; k += WAVE) {
+        const double tk = tgrid(k);
+        const double Fi = interp_at_grid(Fs, k, T, step);
+        const double q = interp_lin(Gx, tgrid, T, Fi);
+        const double F1 = interp_lin(Fx, Fy, T, tk);
+        const double G1 = interp_lin(Gx, tgrid, T, tk);
+        h[k] = tk - q;
+        const double e = F1 - G1;
+        w += e * e;
+    }
+    w = wave_sum(w);
+    __syncthreads();
+    // D = 2 * cumtrapz(h) with unit spacing, in the h buffer
+    cumtrapz_inplace<true>(h, T, step);
+    for (int k = lane; k < T; k += WAVE)
+        h[k] = 2.0 * h[k];
+    __syncthreads();
+    const double D1 = [&] {
+        const double xl = tval(T - 2, T, step), xh = tval(T - 1, T, step);
+        return (h[T - 1] - h[T - 2]) / (xh - xl) * (1.0 - xl) + h[T - 2];
+    }();
+    R *a = adj + (size_t)tr * T;
+    for (int k = lane; k < T; k += WAVE)
+        a[k] = (R)(interp_at_grid(h, k, T, step) - D1);
+    if (lane == 0) mis[tr] = w;
+    __syncthreads();
+}
+
+template <typename R, typename O>
+__global__ void __launch_bounds__(WAVE)
+l2_kernel(const R *sim, const O *obs, R *adj, double *mis, int T)
+{
+    const int tr = blockIdx.x, lane = threadIdx.x;
+    double w = 0.0;
+    for (int k = lane; k < T; k += WAVE) {
+        const double r = (double)sim[(size_t)tr * T + k] - (double)obs[(size_t)tr * T + k];
+        adj[(size_t)tr * T + k] = (R)r;
+        w += r * r;
+    }
+    w = wave_sum(w);
+    if (lane == 0) mis[tr] = w;
+}
+
+// jcum[t] = sum_{tau=1..t} adj[T-1-tau] (jcum[0] = 0): the running integral of
+// the flipped adjoint source (somaFonteIntegral with flip=1,
+// regression_PML_single.cu:276-292, 803), accumulated in FP64 then rounded.
+template <typename R>
+__global__ void __launch_bounds__(WAVE)
+jcum_kernel(const R *adj, R *jcum, int T)
+{
+    const int tr = blockIdx.x, lane = threadIdx.x;
+    const R *a = adj + (size_t)tr * T;
+    R *J = jcum + (size_t)tr * T;
+    const int C = (T + WAVE - 1) / WAVE;
+    const int k0 = lane * C, k1 = min(k0 + C, T);
+    double local = 0.0;
+    for (int k = max(k0, 1); k < k1; k++)
+        local += (double)a[T - 1 - k];
+    double run = wave_excl_scan(local);
+    for (int k = k0; k < k1; k++) {
+        if (k > 0) run += (double)a[T - 1 - k];
+        J[k] = (R)run;
+    }
+}
+
+template <typename R>
+hipError_t launch_jcum(const R *adj, R *jcum, int ntraces, int T, hipStream_t s)
+{
+    if (ntraces <= 0) return hipSuccess;
+    hipLaunchKernelGGL((jcum_kernel<R>), dim3(ntraces), dim3(WAVE), 0, s, adj, jcum, T);
+    return hipGetLastError();
+}
+template hipError_t launch_jcum<float>(const float *, float *, int, int, hipStream_t);
+template hipError_t launch_jcum<double>(const double *, double *, int, int, hipStream_t);
+
+template <typename R, typename O>
+hipError_t launch_misfit(int kind, const R *sim, const O *obs, R *adj, R *jcum, double *mis,
+                         int ntraces, int T, double *scratch, hipStream_t s)
+{
+    if (ntraces <= 0) return hipSuccess;
+    if (kind == 0) {
+        hipLaunchKernelGGL((l2_kernel<R, O>), dim3(ntraces), dim3(WAVE), 0, s, sim, obs, adj, mis,
+                           T);
+    } else {
+        const size_t lds = (size_t)2 * T * sizeof(double);
+        hipError_t e = hipFuncSetAttribute((const void *)w2_kernel<R, O>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        int *flags = reinterpret_cast<int *>(scratch + (size_t)ntraces * 3 * T);
+        hipLaunchKernelGGL((w2_kernel<R, O>), dim3(ntraces), dim3(WAVE), lds, s, sim, obs, adj,
+                           mis, T, scratch, flags);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !jcum) return e;
+    hipLaunchKernelGGL((jcum_kernel<R>), dim3(ntraces), dim3(WAVE), 0, s, adj, jcum, T);
+    return hipGetLastError();
+}
+template hipError_t launch_misfit<float, float>(int, const float *, const float *, float *,
+                                                float *, double *, int, int, double *,
+                                                hipStream_t);
+template hipError_t launch_misfit<double, float>(int, const double *, const float *, double *,
+                                                 double *, double *, int, int, double *,
+                                                 hipStream_t);
+template hipError_t launch_misfit<double, double>(int, const double *, const double *, double *,
+                                                  double *, double *, int, int, double *,
+                                                  hipStream_t);
+
+template <typename R>
+__global__ void accum_grad_kernel(const R *grad, int nshots, int X, int Z, int ld,
+                                  long long sstride, double *acc)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= X * Z) return;
+    const int z = i / X, x = i - z * X;
+    const R *g = grad + (size_t)z * ld + x;
+    double a = 0.0;
+    for (int s = 0; s < nshots; s++)
+        a += (double)g[(size_t)s * sstride];
+    acc[i] += a;
+}
+
+template <typename R>
+hipError_t launch_accum_grad(const R *grad, int nshots, int X, int Z, int ld, long long sstride,
+                             double *acc, hipStream_t s)
+{
+    const int n = X * Z;
+    hipLaunchKernelGGL((accum_grad_kernel<R>), dim3((n + 255) / 256), dim3(256), 0, s, grad,
+                       nshots, X, Z, ld, sstride, acc);
+    return hipGetLastError();
+}
+template hipError_t launch_accum_grad<float>(const float *, int, int, int, int, long long,
+                                             double *, hipStream_t);
+template hipError_t launch_accum_grad<double>(const double *, int, int, int, int, long long,
+                                              double *, hipStream_t);
+
+__global__ void finalize_grad_kernel(const double *acc, int n, double scale, float *o32)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) o32[i] = (float)(acc[i] * scale);
+}
+
+hipError_t launch_finalize_grad(const double *acc, int n, double scale, float *o32, hipStream_t s)
+{
+    hipLaunchKernelGGL(finalize_grad_kernel, dim3((n + 255) / 256), dim3(256), 0, s, acc, n, scale,
+                       o32);
+    return hipGetLastError();
+}
+
+__global__ void reduce_misfit_kernel(const double *mis, int nshots, int n_rx, double *out)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nshots) return;
+    double acc = 0.0;
+    for (int r = 0; r < n_rx; r++)
+        acc += mis[(size_t)s * n_rx + r];
+    out[s] = acc;
+}
+
+hipError_t launch_reduce_misfit(const double *mis, int nshots, int n_rx, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(reduce_misfit_kernel, dim3((nshots + 63) / 64), dim3(64), 0, s, mis, nshots,
+                       n_rx, out);
+    return hipGetLastError();
+}
+
+}  // namespace otfwi

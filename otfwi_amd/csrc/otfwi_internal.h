@@ -1,0 +1,99 @@
+// Internal declarations shared by the HIP kernels and the host orchestration
+// of libotfwi_pml.so.  Not part of the public C ABI (include/otfwi_pml.h).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace otfwi {
+
+// Tile geometry of the fused step kernels.  A work-group owns a TW x TH tile
+// of cells and recomputes the P half-step on the tile plus a 4-cell halo
+// ("E box", EW x (TH+8)) so that the A half-step of the same time step needs
+// no exchange with neighbouring tiles.
+constexpr int TW = 64;                  // owned columns (one wavefront row)
+constexpr int EW = TW + 8;              // E box columns  x0-4 .. x0+67
+constexpr int AXW = TW + 16;            // staged Ax columns x0-8 .. x0+71
+constexpr int NTHREADS = 320;           // 5 wavefronts: 288 busy in E phase, 256 in owned phase
+constexpr int BOX = 4;                  // never-updated outer ring width
+
+// Tile flags (per tile of a model, for one tile height).
+enum : uint8_t {
+    TF_STRIP = 1,   // E box touches a damped (split-field) cell
+    TF_EVENT = 2,   // E box holds a source, receiver or ring point
+};
+
+// Point table entry: what happens at a cell flagged in the event map.
+struct PointEvt {
+    int ring;   // slot in the boundary ring, or -1
+    int rx;     // receiver index, or -1
+    int src;    // source index, or -1
+    int pad;
+};
+
+template <typename R>
+struct FwdArgs {
+    int X, Z, ld, ntx, ntz, nshots, t, T;
+    long long sstride;            // elements between shots in a field (Z*ld)
+    const R *P0, *Ax0, *Az0, *Px0;  // state at t-1
+    R *P1, *Ax1, *Az1, *Px1;        // state at t
+    const R *cm;                  // (c^2*dt)/dx per cell, [Z][ld]
+    R cA;                         // dt/dx
+    const R *kx, *kz;             // 1 - d*dt along x ([X]) and z ([Z]), shifted by one cell
+    const uint8_t *tflags;        // [ntz*ntx]
+    const int *evt;               // [Z][ld] index into pts, or -1
+    const PointEvt *pts;
+    const int *shot_src;          // [nshots] active source (-1: all)
+    const R *icum;                // [n_src][T] running sum of each source wavelet
+    R *rec;                       // [nshots][n_rx][T] (may be null)
+    int n_rx;
+    R *ring;                      // [nshots][T][n_ring] (may be null)
+    int n_ring;
+};
+
+template <typename R>
+struct AdjArgs {
+    int X, Z, ld, ntx, ntz, nshots, t, T;
+    long long sstride;
+    // adjoint (lambda) field, damped forward operator
+    const R *LP0, *LAx0, *LAz0, *LPx0;
+    R *LP1, *LAx1, *LAz1, *LPx1;
+    // reconstructed forward field (undamped reverse operator)
+    const R *U0, *UAx0, *UAz0;
+    R *U1, *UAx1, *UAz1;
+    R *grad;                      // [nshots][Z][ld] accumulated imaging condition
+    const R *cm;
+    R cA;
+    const R *kx, *kz;
+    const uint8_t *tflags;
+    const int *evt;
+    const PointEvt *pts;
+    const R *jcum;                // [nshots][n_rx][T] running sum of the flipped adjoint source
+    int n_rx;
+    const R *ring;                // [nshots][T][n_ring]
+    int n_ring;
+};
+
+// Launchers (prop.hip).  th = tile height (16, 32 or 64).
+template <typename R>
+hipError_t launch_fwd_step(const FwdArgs<R> &a, int th, hipStream_t s);
+template <typename R>
+hipError_t launch_adj_step(const AdjArgs<R> &a, int th, hipStream_t s);
+
+// Misfit / reduction launchers (misfit.hip).
+// kind: 0 = L2, 1 = W2.  sim/obs/adj/jcum: [nshots][n_rx][T]; mis: [nshots*n_rx] FP64 per trace.
+// jcum may be null (no running sum wanted).  O is the observed-trace type.
+template <typename R, typename O>
+hipError_t launch_misfit(int kind, const R *sim, const O *obs, R *adj, R *jcum, double *mis,
+                         int ntraces, int T, double *scratch, hipStream_t s);
+// jcum of host-provided adjoint sources.
+template <typename R>
+hipError_t launch_jcum(const R *adj, R *jcum, int ntraces, int T, hipStream_t s);
+// acc[z*X+x] += sum_s grad[s][z][x] (FP64), and out = (float)(acc*scale).
+template <typename R>
+hipError_t launch_accum_grad(const R *grad, int nshots, int X, int Z, int ld, long long sstride,
+                             double *acc, hipStream_t s);
+hipError_t launch_finalize_grad(const double *acc, int n, double scale, float *o32, hipStream_t s);
+// Per-shot misfit sum over its traces in trace order (FP64).
+hipError_t launch_reduce_misfit(const double *mis, int nshots, int n_rx, double *out, hipStream_t s);
+
+}  // namespace otfwi

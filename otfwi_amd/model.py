@@ -1,0 +1,156 @@
+"""Batched propagator handle: the native model behind Cuda_interface_PML.
+
+One :class:`PMLModel` holds a velocity model, its damping profiles, sources,
+receivers and boundary ring on the GPU, and runs every shot of a batch in one
+launch per time step.  All arrays use the reference's layouts
+(regression_PML_single.cu:9, cuda_interface_pml.py:40-53): models (Z, X),
+traces (n_shots, n_rx, T), point lists in caller order.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, f32, i32, ptr
+
+
+class PMLModel:
+    """Handle on one native model (precision 4 = FP32, 8 = FP64 validation)."""
+
+    def __init__(self, X, Z, T, precision=4, constvec=(1.0, 1.0, 1.0)):
+        self._lib = _lib.load()
+        self.X, self.Z, self.T = int(X), int(Z), int(T)
+        self.precision = precision
+        h = ctypes.c_void_p()
+        cv = f32(constvec)
+        check(self._lib.otfwi_create(ctypes.byref(h), self.X, self.Z, self.T, precision, ptr(cv)))
+        self._h = h
+        self.n_src = self.n_rx = self.n_ring = 0
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.otfwi_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- set-up ---------------------------------------------------------
+    def set_damping(self, d_x, d_z):
+        dx, dz = f32(d_x), f32(d_z)
+        if dx.shape != (self.Z, self.X) or dz.shape != (self.Z, self.X):
+            raise ValueError("damping arrays must be (Z, X)")
+        check(self._lib.otfwi_set_damping(self._h, ptr(dx), ptr(dz)))
+
+    def set_cquad(self, cquad):
+        cq = f32(cquad).reshape(-1)
+        if cq.size != self.X * self.Z:
+            raise ValueError(f"cquad has {cq.size} values, expected {self.X * self.Z}")
+        check(self._lib.otfwi_set_cquad(self._h, ptr(cq)))
+
+    def set_cquad_device(self, dev_ptr):
+        """c^2 from a device address (FP32, Z*X contiguous), e.g. a torch tensor."""
+        check(self._lib.otfwi_set_cquad_device(self._h, ctypes.c_void_p(int(dev_ptr))))
+
+    def set_sources(self, xs, zs, wavelets):
+        xs, zs = i32(xs), i32(zs)
+        w = f32(wavelets).reshape(len(xs), self.T)
+        check(self._lib.otfwi_set_sources(self._h, len(xs), ptr(xs), ptr(zs), ptr(w)))
+        self.n_src = len(xs)
+
+    def set_receivers(self, xs, zs):
+        xs, zs = i32(xs), i32(zs)
+        check(self._lib.otfwi_set_receivers(self._h, len(xs), ptr(xs), ptr(zs)))
+        self.n_rx = len(xs)
+
+    def set_ring(self, xs, zs):
+        xs, zs = i32(xs), i32(zs)
+        check(self._lib.otfwi_set_ring(self._h, len(xs), ptr(xs), ptr(zs)))
+        self.n_ring = len(xs)
+
+    def set_observed(self, obs):
+        o = f32(obs)
+        if o.ndim == 2:
+            o = o[None]
+        if o.shape[1:] != (self.n_rx, self.T):
+            raise ValueError(f"observed must be (n_shots, {self.n_rx}, {self.T})")
+        check(self._lib.otfwi_set_observed(self._h, o.shape[0], ptr(o)))
+
+    def set_tuning(self, th_fwd=0, th_adj=0, max_batch=0):
+        check(self._lib.otfwi_set_tuning(self._h, th_fwd, th_adj, max_batch))
+
+    def get_tuning(self):
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(self._lib.otfwi_get_tuning(self._h, ctypes.byref(a), ctypes.byref(b),
+                                          ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    # ---- runs -------------------------------------------------------------
+    @staticmethod
+    def _shots(shot_src):
+        s = i32(np.atleast_1d(shot_src))
+        return s, len(s)
+
+    def forward(self, shot_src=(-1,)):
+        """Traces (n_shots, n_rx, T) FP32; shot k fires source shot_src[k]
+        (-1: all sources at once)."""
+        s, n = self._shots(shot_src)
+        rec = np.zeros((n, self.n_rx, self.T), np.float32)
+        check(self._lib.otfwi_forward(self._h, n, ptr(s), ptr(rec)))
+        return rec
+
+    def gradient(self, shot_src=(-1,), misfit="w2", want_rec=False):
+        """Device misfit + adjoint gradient.  Returns (grad (Z,X) FP32 mean
+        over shots, per-shot misfit FP64 [, traces])."""
+        kind = {"l2": _lib.MISFIT_L2, "w2": _lib.MISFIT_W2}[misfit]
+        s, n = self._shots(shot_src)
+        mis = np.zeros(n, np.float64)
+        grad = np.zeros((self.Z, self.X), np.float32)
+        rec = np.zeros((n, self.n_rx, self.T), np.float32) if want_rec else None
+        check(self._lib.otfwi_gradient(self._h, n, ptr(s), kind, ptr(mis), ptr(grad), ptr(rec)))
+        return (grad, mis, rec) if want_rec else (grad, mis)
+
+    def gradient_device(self, shot_src, misfit, grad_dev_ptr):
+        """Same as gradient(), writing the (Z,X) FP32 mean gradient to device memory."""
+        kind = {"l2": _lib.MISFIT_L2, "w2": _lib.MISFIT_W2}[misfit]
+        s, n = self._shots(shot_src)
+        mis = np.zeros(n, np.float64)
+        check(self._lib.otfwi_gradient_device(self._h, n, ptr(s), kind, ptr(mis),
+                                              ctypes.c_void_p(int(grad_dev_ptr))))
+        return mis
+
+    def forward_for_grad(self, shot_src=(-1,)):
+        """Stage 1 of a host-misfit gradient: traces of a forward run with the
+        boundary ring saved on the device."""
+        s, n = self._shots(shot_src)
+        rec = np.zeros((n, self.n_rx, self.T), np.float32)
+        check(self._lib.otfwi_forward_for_grad(self._h, n, ptr(s), ptr(rec)))
+        return rec
+
+    def adjoint_from_host(self, adj):
+        """Stage 2: mean gradient from host adjoint sources (n_shots, n_rx, T)."""
+        a = f32(adj)
+        grad = np.zeros((self.Z, self.X), np.float32)
+        check(self._lib.otfwi_adjoint_from_host(self._h, ptr(a), ptr(grad)))
+        return grad
+
+
+def misfit(kind, sim, obs):
+    """Per-trace misfit and adjoint source of (n, T) traces on the GPU (FP64)."""
+    k = {"l2": _lib.MISFIT_L2, "w2": _lib.MISFIT_W2}[kind]
+    s = np.ascontiguousarray(sim, dtype=np.float64)
+    o = np.ascontiguousarray(obs, dtype=np.float64)
+    if s.ndim == 1:
+        s, o = s[None], o[None]
+    if s.shape != o.shape:
+        raise ValueError("sim and obs shapes differ")
+    n, T = s.shape
+    mis = np.zeros(n, np.float64)
+    adj = np.zeros((n, T), np.float64)
+    check(_lib.load().otfwi_misfit(k, n, T, ptr(s), ptr(o), ptr(mis), ptr(adj)))
+    return mis, adj

@@ -1,0 +1,81 @@
+"""Test problems shared by the CPU and GPU tests.
+
+Every array is rounded to FP32 once here, so the GPU library (which takes
+FP32 inputs) and both oracle precisions see identical input values.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from oracle import oracle as O
+from otfwi_amd import configs
+
+
+def _round(a):
+    return np.asarray(a, dtype=np.float32)
+
+
+def small_problem(X=96, Z=80, T=240, pml=10, n_src=2, n_rx=6, seed=0, R=1e-3, f0=0.04,
+                  prec_deriv=4):
+    """Heterogeneous model with a slow blob, sources and receivers on row
+    pml+2, a Ricker wavelet of f0 cycles per sample."""
+    rng = np.random.default_rng(seed)
+    zz, xx = np.mgrid[0:Z, 0:X]
+    c_bg = np.full((Z, X), 0.45)
+    c_true = c_bg.copy()
+    cz, cx = rng.uniform(0.55 * Z, 0.75 * Z), rng.uniform(0.35 * X, 0.65 * X)
+    c_true[(xx - cx) ** 2 + (zz - cz) ** 2 <= (0.12 * min(X, Z)) ** 2] = 0.3
+    src_mask = np.zeros((Z, X), bool)
+    src_mask[pml + 2, np.round(np.linspace(pml + 6, X - pml - 7, n_src)).astype(int)] = True
+    rx_mask = np.zeros((Z, X), bool)
+    rx_mask[pml + 2, np.round(np.linspace(pml + 3, X - pml - 4, n_rx)).astype(int)] = True
+    w = configs.ricker(f0, 1.0, T, t0=1.2 / f0)
+    src = np.tile(w, (n_src, 1)) * (1.0 + 0.1 * np.arange(n_src))[:, None]
+    d_x, d_z = O.calc_pml(X, Z, pml, R=R)
+    ring = O.revert_mask(X, Z, pml, prec_deriv, src_mask)
+    return dict(
+        X=X, Z=Z, T=T, pml=pml,
+        cq_true=_round(c_true ** 2), cq_bg=_round(c_bg ** 2),
+        d_x=_round(d_x), d_z=_round(d_z),
+        src_mask=src_mask, rx_mask=rx_mask,
+        src_xz=O.point_order(src_mask), rx_xz=O.point_order(rx_mask),
+        ring_xz=O.point_order(ring),
+        src=_round(src),
+    )
+
+
+def config1_problem(seed=0, prec_deriv=4):
+    """Benchmark configuration 1 (256^2, T=1500) in the same form."""
+    cfg = configs.config1(seed)
+    X, Z = cfg["X"], cfg["Z"]
+    d_x, d_z = O.calc_pml(X, Z, cfg["pml"], R=cfg["R"])
+    ring = O.revert_mask(X, Z, cfg["pml"], prec_deriv, cfg["src_mask"])
+    return dict(
+        X=X, Z=Z, T=cfg["T"], pml=cfg["pml"],
+        cq_true=_round(cfg["c_true"] ** 2), cq_bg=_round(cfg["c_bg"] ** 2),
+        d_x=_round(d_x), d_z=_round(d_z),
+        src_mask=cfg["src_mask"], rx_mask=cfg["rx_mask"],
+        src_xz=O.point_order(cfg["src_mask"]), rx_xz=O.point_order(cfg["rx_mask"]),
+        ring_xz=O.point_order(ring),
+        src=_round(cfg["src"]),
+    )
+
+
+def make_model(p, precision=4, cq="cq_true", th_fwd=0, th_adj=0, max_batch=0):
+    from otfwi_amd import PMLModel
+
+    m = PMLModel(p["X"], p["Z"], p["T"], precision=precision)
+    m.set_damping(p["d_x"], p["d_z"])
+    m.set_cquad(p[cq])
+    m.set_sources(*p["src_xz"], p["src"])
+    m.set_receivers(*p["rx_xz"])
+    m.set_ring(*p["ring_xz"])
+    if th_fwd or th_adj or max_batch:
+        m.set_tuning(th_fwd, th_adj, max_batch)
+    return m
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))

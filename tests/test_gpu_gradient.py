@@ -1,0 +1,109 @@
+"""Adjoint gradient on the GPU against the CPU oracle (reference phase 2,
+regression_PML_single.cu:704-846, with adjoint_functions.L2 / wasserstein).
+
+Bars: L2 misfit within 1e-6 and L2 gradient within 1e-4 of the
+reference-precision oracle; the FP64 build within 1e-9 of the FP64 oracle.
+W2 amplifies FP32 trace rounding (about 300x in the adjoint source), so the
+FP32 W2 gradient is held to the reference-vs-FP64 spread measured in the same
+test instead (see the printed numbers), and the W2 stage itself is checked on
+identical traces in test_gpu_misfit.py.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.problems import config1_problem, make_model, rel, small_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def observed(p, shots):
+    return np.stack([O.forward(p["cq_true"], p["d_x"], p["d_z"], p["src_xz"], p["src"],
+                               p["rx_xz"], p["T"], idx=i)["rec"] for i in shots])
+
+
+def oracle_grad(p, obs, shots, misfit, precision="ref"):
+    return O.gradient(p["cq_bg"], p["d_x"], p["d_z"], p["src_xz"], p["src"], p["rx_xz"],
+                      p["ring_xz"], obs, p["T"], misfit, shots=shots, precision=precision)
+
+
+@pytest.mark.parametrize("kind", ["l2", "w2"])
+def test_small_gradient(gpu, kind):
+    p = small_problem(seed=11)
+    shots = [0, 1]
+    obs = observed(p, shots)
+    m = make_model(p, cq="cq_bg")
+    m.set_observed(obs)
+    grad, mis = m.gradient(shots, misfit=kind)
+    fn = O.l2 if kind == "l2" else O.wasserstein
+    g_ref, mse_ref, _ = oracle_grad(p, obs, shots, fn)
+    g64, mse64, _ = oracle_grad(p, obs.astype(np.float64), shots, fn, precision="f64")
+    eg, em = rel(grad, g_ref), abs(mis.mean() - mse_ref) / abs(mse_ref)
+    print(f"{kind}: grad {eg:.3e} (oracle ref vs f64 {rel(g_ref, g64):.3e}), "
+          f"misfit {em:.3e} (ref vs f64 {abs(mse_ref - mse64) / abs(mse64):.3e})")
+    if kind == "l2":
+        assert eg < 1e-4 and em < 1e-6
+    else:
+        assert eg < max(1e-4, 5 * rel(g_ref, g64))
+        assert em < max(1e-6, 5 * abs(mse_ref - mse64) / abs(mse64))
+
+
+@pytest.mark.parametrize("kind", ["l2", "w2"])
+def test_small_gradient_fp64(gpu, kind):
+    p = small_problem(seed=12)
+    shots = [1, 0]
+    obs = observed(p, shots)
+    m = make_model(p, precision=8, cq="cq_bg")
+    m.set_observed(obs)
+    grad, mis = m.gradient(shots, misfit=kind)
+    fn = O.l2 if kind == "l2" else O.wasserstein
+    g64, mse64, _ = oracle_grad(p, obs.astype(np.float64), shots, fn, precision="f64")
+    eg, em = rel(grad, g64), abs(mis.mean() - mse64) / abs(mse64)
+    print(f"fp64 {kind}: grad {eg:.3e} misfit {em:.3e}")
+    assert eg < 1e-6 and em < 1e-9
+
+
+def test_config1_l2_and_w2(gpu):
+    p = config1_problem(seed=0)
+    obs = observed(p, [-1])
+    m = make_model(p, cq="cq_bg")
+    m.set_observed(obs)
+    for kind, fn in (("l2", O.l2), ("w2", O.wasserstein)):
+        grad, mis = m.gradient([-1], misfit=kind)
+        g_ref, mse_ref, _ = oracle_grad(p, obs, [-1], fn)
+        g64, mse64, _ = oracle_grad(p, obs.astype(np.float64), [-1], fn, precision="f64")
+        eg, em = rel(grad, g_ref), abs(mis[0] - mse_ref) / abs(mse_ref)
+        print(f"config1 {kind}: grad vs ref-oracle {eg:.3e}, vs f64 {rel(grad, g64):.3e} "
+              f"(ref vs f64 {rel(g_ref, g64):.3e}); misfit vs ref {em:.3e}, vs f64 "
+              f"{abs(mis[0] - mse64) / abs(mse64):.3e}")
+        if kind == "l2":
+            assert eg < 1e-4 and em < 1e-6
+        else:
+            assert eg < max(1e-4, 5 * rel(g_ref, g64))
+
+
+def test_host_adjoint_matches_device(gpu):
+    p = small_problem(seed=13)
+    shots = [0, 1]
+    obs = observed(p, shots)
+    m = make_model(p, cq="cq_bg")
+    m.set_observed(obs)
+    g_dev, mis = m.gradient(shots, misfit="l2")
+    sim = m.forward_for_grad(shots)
+    adj = sim.astype(np.float64) - obs
+    g_host = m.adjoint_from_host(adj)
+    assert rel(g_host, g_dev) < 1e-6
+
+
+def test_batched_gradient_equals_chunked(gpu):
+    p = small_problem(seed=14, n_src=3)
+    shots = [0, 1, 2]
+    obs = observed(p, shots)
+    a = make_model(p, cq="cq_bg")
+    a.set_observed(obs)
+    b = make_model(p, cq="cq_bg", max_batch=1)
+    b.set_observed(obs)
+    ga, ma = a.gradient(shots, misfit="w2")
+    gb, mb = b.gradient(shots, misfit="w2")
+    assert np.array_equal(ma, mb)
+    assert rel(ga, gb) < 1e-6   # FP64 cross-shot sums in another grouping

@@ -1,0 +1,57 @@
+"""Device W2 / L2 against the numpy oracle on identical traces
+(adjoint_functions.py:8-11, 25-67)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from otfwi_amd import adjoint_functions as AF
+
+pytestmark = pytest.mark.gpu
+
+
+def traces(n, T, seed, neg=False):
+    rng = np.random.default_rng(seed)
+    t = np.arange(T)
+    obs = np.zeros((n, T))
+    sim = np.zeros((n, T))
+    for i in range(n):
+        for arr in (obs, sim):
+            k0 = rng.uniform(0.1, 0.7) * T
+            w = rng.uniform(0.01, 0.05) * T
+            arr[i] = np.exp(-((t - k0) / w) ** 2) * np.cos((t - k0) / (0.3 * w)) * rng.uniform(0.5, 2)
+    if neg:   # make sim + shift negative somewhere: non-monotone CDF
+        sim[:, T // 3] = -5.0 * np.abs(obs).max(axis=1)
+    return sim.astype(np.float32).astype(np.float64), obs.astype(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("n,T", [(1, 64), (4, 1500), (3, 4000)])
+def test_w2_matches_oracle(gpu, n, T):
+    sim, obs = traces(n, T, seed=T + n)
+    w, a = AF.wasserstein(sim, obs)
+    w0, a0 = O.wasserstein(sim, obs)
+    print(f"n={n} T={T}: W2 rel {abs(w - w0) / abs(w0):.2e}, adj rel "
+          f"{np.linalg.norm(a - a0) / np.linalg.norm(a0):.2e}")
+    assert abs(w - w0) <= 1e-10 * abs(w0)
+    assert np.linalg.norm(a - a0) <= 1e-9 * np.linalg.norm(a0)
+
+
+def test_w2_non_monotone_cdf(gpu):
+    sim, obs = traces(2, 700, seed=9, neg=True)
+    w, a = AF.wasserstein(sim, obs)
+    w0, a0 = O.wasserstein(sim, obs)
+    assert abs(w - w0) <= 1e-9 * abs(w0)
+    assert np.linalg.norm(a - a0) <= 1e-8 * np.linalg.norm(a0)
+
+
+def test_w2_identical_traces(gpu):
+    sim, obs = traces(3, 500, seed=3)
+    w, a = AF.wasserstein(obs, obs)
+    assert abs(w) < 1e-20 and np.abs(a).max() < 1e-12
+
+
+def test_l2(gpu):
+    sim, obs = traces(5, 333, seed=4)
+    w, a = AF.L2(sim, obs)
+    w0, a0 = O.l2(sim, obs)
+    assert abs(w - w0) <= 1e-13 * abs(w0)
+    assert np.array_equal(a, a0)
