@@ -126,6 +126,14 @@ int otfwi_set_observed(otfwi_model *m, int n_shots, const float *obs);
 int otfwi_set_tuning(otfwi_model *m, int th_fwd, int th_adj, int max_batch);
 int otfwi_get_tuning(otfwi_model *m, int *th_fwd, int *th_adj, int *max_batch);
 
+/* Device time of the last otfwi_gradient / otfwi_gradient_device call,
+ * from HIP events on the model's stream: out[0] forward time loops (ms),
+ * out[1] misfit kernels (ms), out[2] adjoint time loops (ms), out[3] number
+ * of forward-step launches, out[4] number of adjoint-step launches.  No
+ * reference counterpart (the reference prints wall time only,
+ * cuda_interface_pml.py:233). */
+int otfwi_get_timing(otfwi_model *m, double *out);
+
 /* Forward runs of n_shots shots, shot k firing source shot_src[k] (-1: all
  * sources).  rec_out (may be NULL) receives (n_shots, n_rx, T) FP32 traces.
  * Replaces the per-shot loop of Cuda_interface_PML.simulate

@@ -112,6 +112,11 @@ struct otfwi_model {
     float *gout = nullptr;           // FP32 mean gradient on device
     // state of a pending two-stage gradient
     int pend_n = 0;
+    // HIP events around the time loops of the last call (forward, misfit,
+    // adjoint), accumulated over its batches
+    hipEvent_t ev[4] = {};
+    double t_fwd_ms = 0, t_mis_ms = 0, t_adj_ms = 0;
+    long long n_fwd_launch = 0, n_adj_launch = 0;
 };
 
 namespace {
@@ -525,7 +530,9 @@ int grad_batch_device(otfwi_model *m, int s0, int nb, const int *shot_src, int k
     Batch &b = m->b;
     RCK(upload_shot_src(m, nb, shot_src ? shot_src + s0 : nullptr));
     RCK(upload_obs_batch<R>(m, s0, nb));
+    HIPCK(hipEventRecord(m->ev[0], m->stream));
     RCK(run_forward<R>(m, nb, true));
+    HIPCK(hipEventRecord(m->ev[1], m->stream));
     if (rec_out) RCK(copy_rec_out<R>(m, nb, rec_out + (size_t)s0 * m->n_rx * m->T));
     const int ntr = nb * m->n_rx;
     int *flags = reinterpret_cast<int *>(b.scratch + (size_t)ntr * 3 * m->T);
@@ -533,7 +540,9 @@ int grad_batch_device(otfwi_model *m, int s0, int nb, const int *shot_src, int k
     HIPCK((launch_misfit<R, float>(kind, (const R *)b.rec, b.obs, (R *)b.adj, (R *)b.jcum,
                                    b.mis_tr, ntr, m->T, b.scratch, m->stream)));
     HIPCK(launch_reduce_misfit(b.mis_tr, nb, m->n_rx, b.mis_shot, m->stream));
+    HIPCK(hipEventRecord(m->ev[2], m->stream));
     RCK(run_adjoint<R>(m, nb));
+    HIPCK(hipEventRecord(m->ev[3], m->stream));
     const size_t plane = (size_t)m->Z * m->ld;
     HIPCK(launch_accum_grad<R>((const R *)b.grad, nb, m->X, m->Z, m->ld, (long long)plane, m->gacc,
                                m->stream));
@@ -544,6 +553,15 @@ int grad_batch_device(otfwi_model *m, int s0, int nb, const int *shot_src, int k
     if (kind == OTFWI_MISFIT_W2)
         HIPCK(hipMemcpyAsync(fl.data(), flags, ntr * sizeof(int), hipMemcpyDeviceToHost, m->stream));
     HIPCK(hipStreamSynchronize(m->stream));
+    float ms[3];
+    HIPCK(hipEventElapsedTime(&ms[0], m->ev[0], m->ev[1]));
+    HIPCK(hipEventElapsedTime(&ms[1], m->ev[1], m->ev[2]));
+    HIPCK(hipEventElapsedTime(&ms[2], m->ev[2], m->ev[3]));
+    m->t_fwd_ms += ms[0];
+    m->t_mis_ms += ms[1];
+    m->t_adj_ms += ms[2];
+    m->n_fwd_launch += m->T - 1;
+    m->n_adj_launch += m->T - 1;
     for (size_t i = 0; i < fl.size(); i++)
         if (fl[i])
             return fail(OTFWI_E_NUMERIC, "W2: observed trace %zu of shot %d has a non-finite CDF",
@@ -575,6 +593,8 @@ int gradient_impl(otfwi_model *m, int n, const int *shot_src, int kind, double *
     const int B = batch_size(m, n);
     RCK(ensure_batch(m, B, true));
     RCK(ensure_gacc(m));
+    m->t_fwd_ms = m->t_mis_ms = m->t_adj_ms = 0;
+    m->n_fwd_launch = m->n_adj_launch = 0;
     for (int s0 = 0; s0 < n; s0 += B)
         RCK(grad_batch_device<R>(m, s0, std::min(B, n - s0), shot_src, kind, mis_out, rec_out));
     const int cells = m->X * m->Z;
@@ -683,9 +703,10 @@ int otfwi_create(otfwi_model **out, int X, int Z, int T, int precision, const fl
     m->kx_h.assign(X, 1.0f);
     m->kz_h.assign(Z, 1.0f);
     hipError_t e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+    for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipEventCreate(&m->ev[i]);
     if (e != hipSuccess) {
         delete m;
-        return fail(OTFWI_E_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
+        return fail(OTFWI_E_HIP, "stream/event creation failed: %s", hipGetErrorString(e));
     }
     *out = m;
     return OTFWI_OK;
@@ -699,6 +720,8 @@ int otfwi_destroy(otfwi_model *m)
     dfree(m->cm); dfree(m->kx); dfree(m->kz); dfree(m->icum); dfree(m->cq32);
     dfree(m->evt); dfree(m->pts); dfree(m->tf_fwd); dfree(m->tf_adj);
     dfree(m->gacc); dfree(m->gout);
+    for (int i = 0; i < 4; i++)
+        if (m->ev[i]) (void)hipEventDestroy(m->ev[i]);
     (void)hipStreamDestroy(m->stream);
     delete m;
     return OTFWI_OK;
@@ -830,6 +853,18 @@ int otfwi_get_tuning(otfwi_model *m, int *th_fwd, int *th_adj, int *max_batch)
     if (th_fwd) *th_fwd = m->th_fwd;
     if (th_adj) *th_adj = m->th_adj;
     if (max_batch) *max_batch = m->max_batch;
+    return OTFWI_OK;
+}
+
+int otfwi_get_timing(otfwi_model *m, double *out)
+{
+    RCK(check_model(m));
+    if (!out) return fail(OTFWI_E_ARG, "out is NULL");
+    out[0] = m->t_fwd_ms;
+    out[1] = m->t_mis_ms;
+    out[2] = m->t_adj_ms;
+    out[3] = (double)m->n_fwd_launch;
+    out[4] = (double)m->n_adj_launch;
     return OTFWI_OK;
 }
 

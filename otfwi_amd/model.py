@@ -90,6 +90,14 @@ class PMLModel:
                                           ctypes.byref(c)))
         return a.value, b.value, c.value
 
+    def timing(self):
+        """Device time of the last gradient call from HIP events on the
+        model's stream: dict(fwd_ms, misfit_ms, adj_ms, n_fwd, n_adj)."""
+        out = np.zeros(5, np.float64)
+        check(self._lib.otfwi_get_timing(self._h, ptr(out)))
+        return dict(fwd_ms=out[0], misfit_ms=out[1], adj_ms=out[2], n_fwd=int(out[3]),
+                    n_adj=int(out[4]))
+
     # ---- runs -------------------------------------------------------------
     @staticmethod
     def _shots(shot_src):
