@@ -99,6 +99,24 @@ def cpu_baseline(N, T, n_shots, n_rx, sample_steps):
                 sample_seconds=round(t4 - t0, 2))
 
 
+def calib_copy(nbytes):
+    """One device-to-device hipMemcpy of nbytes (the runtime's copy kernel):
+    the reference point that scales HBM byte counters."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(f"[bench] copied {nbytes} bytes device-to-device", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,8 +127,11 @@ def main():
     ap.add_argument("--shots", type=int, default=64)
     ap.add_argument("--th-fwd", type=int, default=0)
     ap.add_argument("--th-adj", type=int, default=0)
+    ap.add_argument("--misfit", default="w2", choices=["w2", "l2"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=300)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="after the run, copy 1 GiB device-to-device (scale for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
@@ -139,12 +160,12 @@ def main():
           file=sys.stderr, flush=True)
 
     for i in range(args.warmup):
-        m.gradient(shots, misfit="w2")
+        m.gradient(shots, misfit=args.misfit)
         print(f"[bench] warmup {i} done", file=sys.stderr, flush=True)
     t_ev = dict(fwd_ms=0.0, misfit_ms=0.0, adj_ms=0.0, n_fwd=0, n_adj=0)
     t0 = time.perf_counter()
     for i in range(args.steps):
-        grad, mis = m.gradient(shots, misfit="w2")
+        grad, mis = m.gradient(shots, misfit=args.misfit)
         tm = m.timing()
         for k in t_ev:
             t_ev[k] += tm[k]
@@ -172,8 +193,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded random-defect velocity model; observed traces simulated)",
-        "config": {"workload": f"fmc{S}_{N}x{N}_T{T}_w2_gradient", "grid": [N, N], "T": T,
-                   "shots": S, "receivers": S, "misfit": "w2", "pml": 20,
+        "config": {"workload": f"fmc{S}_{N}x{N}_T{T}_{args.misfit}_gradient", "grid": [N, N],
+                   "T": T, "shots": S, "receivers": S, "misfit": args.misfit, "pml": 20,
                    "tiles": list(m.get_tuning()[:2])},
         "roofline": {"bound": "hbm", "kernel": "adj_step_kernel",
                      "achieved": adj_ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
@@ -193,6 +214,8 @@ def main():
     if not args.no_cpu:
         cb = cpu_baseline(N, T, S, S, args.cpu_steps)
         out["cpu_baseline"] = cb
+    if args.calib_copy:
+        calib_copy(1 << 30)
     print(json.dumps(out))
 
 

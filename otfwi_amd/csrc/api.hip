@@ -88,7 +88,7 @@ struct Batch {
 };
 
 struct otfwi_model {
-    int X = 0, Z = 0, T = 0, ld = 0, prec = 4;
+    int X = 0, Z = 0, T = 0, ld = 0, zp = 0, prec = 4;   // ld, zp: padded plane shape
     float dx = 1, dz = 1, dt = 1;
     hipStream_t stream = nullptr;
     void *cm = nullptr, *kx = nullptr, *kz = nullptr, *icum = nullptr;
@@ -141,7 +141,7 @@ int ensure_batch(otfwi_model *m, int n, bool need_adj)
 {
     Batch &b = m->b;
     const size_t es = esize(m);
-    const size_t plane = (size_t)m->Z * m->ld;
+    const size_t plane = (size_t)m->zp * m->ld;
     const size_t ring_need = (size_t)n * m->T * std::max(m->n_ring, 1);
     const bool adj_ok = b.LP[0] != nullptr || !need_adj;
     if (b.cap >= n && adj_ok && (!need_adj || b.ring_elems >= ring_need)) return OTFWI_OK;
@@ -207,6 +207,9 @@ int build_flags(otfwi_model *m, int th, uint8_t **dst, int *dst_th)
                     if (m->kx_h[x] != 1.0f || m->kz_h[z] != 1.0f) fl |= TF_STRIP;
                     if (ev[(size_t)z * m->X + x]) fl |= TF_EVENT;
                 }
+            if (tx * TW - 4 < BOX || tx * TW + TW + 4 > m->X - BOX || tz * th - 4 < BOX ||
+                tz * th + th + 4 > m->Z - BOX)
+                fl |= TF_EDGE;
             f[(size_t)tz * ntx + tx] = fl;
         }
     dfree(*dst);
@@ -238,10 +241,10 @@ int check_points(const otfwi_model *m, int n, const int *xs, const int *zs, cons
 int build_points(otfwi_model *m)
 {
     if (!m->points_dirty && m->evt) return OTFWI_OK;
-    std::vector<int> map((size_t)m->Z * m->ld, -1);
+    std::vector<int> map((size_t)m->zp * m->ld, -1);
     std::vector<PointEvt> pts;
     auto get = [&](int x, int z) -> PointEvt & {
-        int &e = map[(size_t)z * m->ld + x];
+        int &e = map[(size_t)(z + ZOFF) * m->ld + x + XOFF];
         if (e < 0) {
             e = (int)pts.size();
             pts.push_back(PointEvt{-1, -1, -1, 0});
@@ -269,14 +272,14 @@ template <typename R>
 int upload_model_arrays(otfwi_model *m)
 {
     // cm = (c^2 dt)/dx, rounded as the reference forms it (regression_PML_single.cu:512).
-    const size_t n = (size_t)m->Z * m->ld;
+    const size_t n = (size_t)m->zp * m->ld;
     std::vector<float> cq((size_t)m->Z * m->X);
     HIPCK(hipMemcpy(cq.data(), m->cq32, cq.size() * sizeof(float), hipMemcpyDeviceToHost));
     std::vector<R> cm(n, R(0));
     for (int z = 0; z < m->Z; z++)
         for (int x = 0; x < m->X; x++) {
             const R c = (R)cq[(size_t)z * m->X + x];
-            cm[(size_t)z * m->ld + x] = (c * (R)m->dt) / (R)m->dx;
+            cm[(size_t)(z + ZOFF) * m->ld + x + XOFF] = (c * (R)m->dt) / (R)m->dx;
         }
     if (!m->cm) RCK(dalloc((char **)&m->cm, n * sizeof(R)));
     HIPCK(hipMemcpy(m->cm, cm.data(), n * sizeof(R), hipMemcpyHostToDevice));
@@ -328,16 +331,17 @@ int upload_sources(otfwi_model *m)
 template <typename R>
 int upload_damping(otfwi_model *m)
 {
-    std::vector<R> kx(m->X), kz(m->Z);
+    // padded: kx[x + XOFF] over ld entries, kz[z + ZOFF] over zp entries, 1 off the grid
+    std::vector<R> kx(m->ld, R(1)), kz(m->zp, R(1));
     m->kx_h.assign(m->X, 1.0f);
     m->kz_h.assign(m->Z, 1.0f);
     for (int x = 0; x < m->X; x++) {
-        kx[x] = R(1) - (R)m->dxs_h[x] * (R)m->dt;
-        m->kx_h[x] = kx[x] == R(1) ? 1.0f : 0.0f;
+        kx[x + XOFF] = R(1) - (R)m->dxs_h[x] * (R)m->dt;
+        m->kx_h[x] = kx[x + XOFF] == R(1) ? 1.0f : 0.0f;
     }
     for (int z = 0; z < m->Z; z++) {
-        kz[z] = R(1) - (R)m->dzs_h[z] * (R)m->dt;
-        m->kz_h[z] = kz[z] == R(1) ? 1.0f : 0.0f;
+        kz[z + ZOFF] = R(1) - (R)m->dzs_h[z] * (R)m->dt;
+        m->kz_h[z] = kz[z + ZOFF] == R(1) ? 1.0f : 0.0f;
     }
     dfree(m->kx);
     dfree(m->kz);
@@ -366,7 +370,7 @@ int run_forward(otfwi_model *m, int n, bool save_ring)
 {
     Batch &b = m->b;
     const size_t es = sizeof(R);
-    const size_t plane = (size_t)m->Z * m->ld;
+    const size_t plane = (size_t)m->zp * m->ld;
     hipStream_t st = m->stream;
     for (int i = 0; i < 2; i++) {
         HIPCK(hipMemsetAsync(b.P[i], 0, plane * n * es, st));
@@ -413,7 +417,7 @@ int run_adjoint(otfwi_model *m, int n)
 {
     Batch &b = m->b;
     const size_t es = sizeof(R);
-    const size_t plane = (size_t)m->Z * m->ld;
+    const size_t plane = (size_t)m->zp * m->ld;
     hipStream_t st = m->stream;
     for (int i = 0; i < 2; i++) {
         HIPCK(hipMemsetAsync(b.LP[i], 0, plane * n * es, st));
@@ -543,7 +547,7 @@ int grad_batch_device(otfwi_model *m, int s0, int nb, const int *shot_src, int k
     HIPCK(hipEventRecord(m->ev[2], m->stream));
     RCK(run_adjoint<R>(m, nb));
     HIPCK(hipEventRecord(m->ev[3], m->stream));
-    const size_t plane = (size_t)m->Z * m->ld;
+    const size_t plane = (size_t)m->zp * m->ld;
     HIPCK(launch_accum_grad<R>((const R *)b.grad, nb, m->X, m->Z, m->ld, (long long)plane, m->gacc,
                                m->stream));
     std::vector<double> hmis(nb);
@@ -564,7 +568,11 @@ int grad_batch_device(otfwi_model *m, int s0, int nb, const int *shot_src, int k
     m->n_adj_launch += m->T - 1;
     for (size_t i = 0; i < fl.size(); i++)
         if (fl[i])
-            return fail(OTFWI_E_NUMERIC, "W2: observed trace %zu of shot %d has a non-finite CDF",
+            return fail(OTFWI_E_NUMERIC,
+                        fl[i] == 2 ? "W2: observed trace of receiver %zu, shot %d is all zero "
+                                     "(the W2 normalization is 0/0)"
+                                   : "W2: observed trace of receiver %zu, shot %d has a "
+                                     "non-finite CDF",
                         i % m->n_rx, s0 + (int)(i / m->n_rx));
     if (mis_out)
         for (int i = 0; i < nb; i++) mis_out[s0 + i] = hmis[i];
@@ -638,7 +646,7 @@ int adjoint_from_host_impl(otfwi_model *m, const float *adj, float *grad_out)
     HIPCK(launch_jcum<R>((const R *)b.adj, (R *)b.jcum, n * m->n_rx, m->T, m->stream));
     RCK(run_adjoint<R>(m, n));
     RCK(ensure_gacc(m));
-    const size_t plane = (size_t)m->Z * m->ld;
+    const size_t plane = (size_t)m->zp * m->ld;
     HIPCK(launch_accum_grad<R>((const R *)b.grad, n, m->X, m->Z, m->ld, (long long)plane, m->gacc,
                                m->stream));
     const int cells = m->X * m->Z;
@@ -691,7 +699,8 @@ int otfwi_create(otfwi_model **out, int X, int Z, int T, int precision, const fl
         return fail(OTFWI_E_ARG, "precision must be 4 (FP32) or 8 (FP64)");
     otfwi_model *m = new otfwi_model();
     m->X = X; m->Z = Z; m->T = T; m->prec = precision;
-    m->ld = ceil_div(X, 64) * 64;
+    m->ld = padded_ld(X);
+    m->zp = padded_rows(Z);
     if (constvec) { m->dx = constvec[0]; m->dz = constvec[1]; m->dt = constvec[2]; }
     if (!(m->dx > 0) || !(m->dt > 0)) {
         delete m;
@@ -937,8 +946,9 @@ int otfwi_misfit(int kind, int n_traces, int T, const double *sim, const double 
         e = hipMemcpy(fl.data(), dscr + cnt * 3, n_traces * sizeof(int), hipMemcpyDeviceToHost);
     cleanup();
     if (e != hipSuccess) return fail(OTFWI_E_HIP, "misfit: %s", hipGetErrorString(e));
+    // an all-zero observed trace (flag 2) yields NaN, as the reference does
     for (int i = 0; i < n_traces; i++)
-        if (fl[i]) return fail(OTFWI_E_NUMERIC, "W2: observed trace %d has a non-finite CDF", i);
+        if (fl[i] == 1) return fail(OTFWI_E_NUMERIC, "W2: observed trace %d has a non-finite CDF", i);
     return OTFWI_OK;
 }
 

@@ -185,6 +185,9 @@ w2_kernel(const R *sim, const O *obs, R *adj, double *mis, int T, double *scratc
     }
     sg = wave_sum(sg);
     sf = wave_sum(sf);
+    // an all-zero observed trace gives c = 0 and a 0/0 normalization (NaN, as
+    // in the reference); report it
+    if (!(sg > 0.0) && lane == 0) flags[tr] = 2;
     __syncthreads();
     for (int k = lane; k < T; k += WAVE) {
         Gs[k] = Gs[k] / sg;
@@ -204,7 +207,7 @@ w2_kernel(const R *sim, const O *obs, R *adj, double *mis, int T, double *scratc
     // c >= -1.1 min(obs) and c >= 0.1 max(obs) give obs + c >= 0 on every
     // sample, so G never decreases for finite input; the flag reports the
     // non-finite case to the host.
-    if (!gmono && lane == 0) flags[tr] = 1;
+    if (!gmono && lane == 0 && flags[tr] == 0) flags[tr] = 1;
 
     auto tgrid = [=](int i) { return tval(i, T, step); };
     auto Gx = [=](int i) { return gx[i]; };
@@ -327,7 +330,7 @@ __global__ void accum_grad_kernel(const R *grad, int nshots, int X, int Z, int l
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= X * Z) return;
     const int z = i / X, x = i - z * X;
-    const R *g = grad + (size_t)z * ld + x;
+    const R *g = grad + (size_t)(z + ZOFF) * ld + x + XOFF;
     double a = 0.0;
     for (int s = 0; s < nshots; s++)
         a += (double)g[(size_t)s * sstride];

@@ -10,16 +10,26 @@ namespace otfwi {
 // of cells and recomputes the P half-step on the tile plus a 4-cell halo
 // ("E box", EW x (TH+8)) so that the A half-step of the same time step needs
 // no exchange with neighbouring tiles.
-constexpr int TW = 64;                  // owned columns (one wavefront row)
+constexpr int TW = 64;                  // owned columns
 constexpr int EW = TW + 8;              // E box columns  x0-4 .. x0+67
 constexpr int AXW = TW + 16;            // staged Ax columns x0-8 .. x0+71
-constexpr int NTHREADS = 320;           // 5 wavefronts: 288 busy in E phase, 256 in owned phase
+constexpr int NTHREADS = 256;           // 16 column quads x 16 row groups
 constexpr int BOX = 4;                  // never-updated outer ring width
+
+// Padded field layout.  Every field of every shot is a (ZP x LD) plane with
+// the cell (x, z) at (z + ZOFF) * LD + x + XOFF; the padding and every cell
+// outside the updated box hold 0 at all times, so stencil loads need neither
+// bounds checks nor masks.  XOFF = 64 keeps each tile row 256-byte aligned.
+constexpr int XOFF = 64, ZOFF = 8;
+inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
+inline int padded_ld(int X) { return round_up(X, 64) + 2 * XOFF; }
+inline int padded_rows(int Z) { return round_up(Z, 64) + 2 * ZOFF; }
 
 // Tile flags (per tile of a model, for one tile height).
 enum : uint8_t {
     TF_STRIP = 1,   // E box touches a damped (split-field) cell
     TF_EVENT = 2,   // E box holds a source, receiver or ring point
+    TF_EDGE = 4,    // E box reaches outside the updated box (results need masking)
 };
 
 // Point table entry: what happens at a cell flagged in the event map.
@@ -33,14 +43,14 @@ struct PointEvt {
 template <typename R>
 struct FwdArgs {
     int X, Z, ld, ntx, ntz, nshots, t, T;
-    long long sstride;            // elements between shots in a field (Z*ld)
-    const R *P0, *Ax0, *Az0, *Px0;  // state at t-1
+    long long sstride;            // elements between shots in a field (ZP*ld)
+    const R *P0, *Ax0, *Az0, *Px0;  // state at t-1 (padded planes)
     R *P1, *Ax1, *Az1, *Px1;        // state at t
-    const R *cm;                  // (c^2*dt)/dx per cell, [Z][ld]
+    const R *cm;                  // (c^2*dt)/dx per cell, padded plane, 0 outside the grid
     R cA;                         // dt/dx
-    const R *kx, *kz;             // 1 - d*dt along x ([X]) and z ([Z]), shifted by one cell
+    const R *kx, *kz;             // 1 - d*dt at x + XOFF ([ld]) and z + ZOFF ([ZP]), shifted by one cell
     const uint8_t *tflags;        // [ntz*ntx]
-    const int *evt;               // [Z][ld] index into pts, or -1
+    const int *evt;               // padded plane: index into pts, or -1
     const PointEvt *pts;
     const int *shot_src;          // [nshots] active source (-1: all)
     const R *icum;                // [n_src][T] running sum of each source wavelet

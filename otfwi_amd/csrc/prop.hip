@@ -5,25 +5,53 @@
 //   P half-step  Px(t) = Px(t-1)*kx + m*D-x Ax(t-1),  Pz(t) = Pz(t-1)*kz + m*D-z Az(t-1)
 //   A half-step  Ax(t) = Ax(t-1)*kx + cA*D+x P(t),    Az(t) = Az(t-1)*kz + cA*D+z P(t)
 // with P = Px+Pz, m = (c^2 dt)/dx, cA = dt/dx, k = 1 - d*dt looked up one
-// cell further along its axis, and every stencil operand outside the box
-// 4 <= x < X-4, 4 <= z < Z-4 reading 0.
+// cell further along its axis, and every cell outside the box
+// 4 <= x < X-4, 4 <= z < Z-4 held at 0.
 //
 // Storage: P = Px+Pz is kept on every cell; the split component Px (the
-// boundary memory variable "psi") is read and written only on damped cells,
-// where kx != 1 or kz != 1.  Ax and Az are full fields.  Each field is
-// [slot][shot][Z][ld] FP32 (or FP64 for the validation build), x fastest.
+// boundary memory variable "psi") is non-zero only on damped cells, where
+// kx != 1 or kz != 1, and only strip tiles touch it.  Fields are padded
+// planes (otfwi_internal.h) whose padding and dead ring are always 0, so all
+// loads are unconditional, 16-byte vectors.
 //
 // One launch advances every shot of a batch by one full time step.  A
-// work-group owns a 64 x TH tile: it stages A(t-1) with an 8-cell halo in
-// LDS, recomputes P(t) on the tile plus a 4-cell halo (the "E box"), keeps
-// that in LDS and finishes A(t) on the tile, so the per-step HBM traffic is
-// the algorithmic minimum (read P, Ax, Az, m; write P, Ax, Az) plus halo
-// re-reads that hit in L2.  Source injection, receiver sampling and the
-// boundary-ring save happen on the cells where they occur, inside the same
-// kernel.
+// 256-thread work-group owns a 64 x TH tile:
+//   1. it loads A(t-1) with an 8-cell halo into LDS and P(t-1), m for the
+//      tile plus a 4-cell halo (the "E box") into registers -- every global
+//      load of the tile is issued before the first barrier;
+//   2. E phase: P(t) on the E box, in column quads with a register window
+//      along z, into LDS (source injection, receiver sampling and the
+//      boundary-ring save happen here, on the cells where they occur);
+//   3. owned phase: A(t) on the tile.
+// The per-step HBM traffic is the algorithmic minimum (read P, Ax, Az, m;
+// write P, Ax, Az) plus halo re-reads that hit in L2.
 #include "otfwi_internal.h"
 
 namespace otfwi {
+
+template <typename R> struct Vec4T;
+template <> struct Vec4T<float> { using T = float4; };
+template <> struct Vec4T<double> { using T = double4; };
+template <typename R> using V4 = typename Vec4T<R>::T;
+
+template <typename R>
+__device__ __forceinline__ V4<R> ld4(const R *p) { return *reinterpret_cast<const V4<R> *>(p); }
+template <typename R>
+__device__ __forceinline__ void st4(R *p, const V4<R> &v) { *reinterpret_cast<V4<R> *>(p) = v; }
+template <typename R>
+__device__ __forceinline__ void unpack(const V4<R> &v, R *a) { a[0] = v.x; a[1] = v.y; a[2] = v.z; a[3] = v.w; }
+template <typename R>
+__device__ __forceinline__ V4<R> pack(const R *a) { V4<R> v; v.x = a[0]; v.y = a[1]; v.z = a[2]; v.w = a[3]; return v; }
+
+// Correctly rounded product that the compiler may not fuse into a following
+// add.  Every update below is written as fma(coef, derivative, mul_rn(old,
+// k)): when the damping factor k is the constant 1 (interior tiles) and when
+// it is loaded (strip tiles) the same roundings happen, so results do not
+// depend on which tiles are strip tiles, i.e. on the tile height.
+__device__ __forceinline__ float mul_rn(float a, float b) { return __fmul_rn(a, b); }
+__device__ __forceinline__ double mul_rn(double a, double b) { return __dmul_rn(a, b); }
+__device__ __forceinline__ float add_rn(float a, float b) { return __fadd_rn(a, b); }
+__device__ __forceinline__ double add_rn(double a, double b) { return __dadd_rn(a, b); }
 
 // Coefficients of the 8th-order staggered first derivative exactly as the
 // reference writes them (double literals, deriv_macros.h:87-94).
@@ -83,11 +111,6 @@ __device__ __forceinline__ R lap9(R m4, R m3, R m2, R m1, R c, R p1, R p2, R p3,
     return s;
 }
 
-__device__ __forceinline__ bool in_box(int x, int z, int X, int Z)
-{
-    return x >= BOX && z >= BOX && x < X - BOX && z < Z - BOX;
-}
-
 // Work-groups are dispatched round-robin over the 8 XCDs (block b runs on
 // XCD b % 8).  Give each XCD a contiguous range of logical tiles so that
 // spatial neighbours, whose halos overlap, share one L2.
@@ -99,314 +122,578 @@ __device__ __forceinline__ int xcd_remap(int b, int nb)
     return base + k;
 }
 
-#define ROW8(A, r, c) (A)[r][(c) + 0], (A)[r][(c) + 1], (A)[r][(c) + 2], (A)[r][(c) + 3], \
-                      (A)[r][(c) + 4], (A)[r][(c) + 5], (A)[r][(c) + 6], (A)[r][(c) + 7]
+// Staging of a (ROWS x 4*NQ) block of a padded plane into LDS through
+// registers: every thread issues all of its loads before any store.
+template <typename R, int ROWS, int NQ>
+struct Stage {
+    static constexpr int N = ROWS * NQ;
+    static constexpr int K = (N + NTHREADS - 1) / NTHREADS;
+    V4<R> r[K];
+    __device__ __forceinline__ void load(const R *org, int ld, int z_first, int x_first, int tid)
+    {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int i = min(tid + k * NTHREADS, N - 1);
+            const int row = i / NQ, q = i - row * NQ;
+            r[k] = ld4(org + (z_first + row) * ld + x_first + 4 * q);
+        }
+    }
+    template <int W>
+    __device__ __forceinline__ void store(R (*lds)[W], int tid) const
+    {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int i = tid + k * NTHREADS;
+            if (i < N) {
+                const int row = i / NQ, q = i - row * NQ;
+                st4(&lds[row][4 * q], r[k]);
+            }
+        }
+    }
+};
+
+// d[i] = stag(v[o+i .. o+i+7]), i = 0..3, from a 12-value row segment.
+template <typename R, int O>
+__device__ __forceinline__ void stag_row(const R (&v)[12], R (&d)[4])
+{
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+        d[i] = stag<R>(v[O + i], v[O + i + 1], v[O + i + 2], v[O + i + 3], v[O + i + 4],
+                       v[O + i + 5], v[O + i + 6], v[O + i + 7]);
+}
+
+template <typename R>
+__device__ __forceinline__ void row12(const R *row, R (&v)[12])
+{
+    unpack<R>(ld4(row), v);
+    unpack<R>(ld4(row + 4), v + 4);
+    unpack<R>(ld4(row + 8), v + 8);
+}
+
+// d[i] = stag over a z window of 8 quads (column i of each).
+template <typename R>
+__device__ __forceinline__ void stag_col(const V4<R> (&w)[8], R (&d)[4])
+{
+    R a[8][4];
+#pragma unroll
+    for (int j = 0; j < 8; j++) unpack<R>(w[j], a[j]);
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+        d[i] = stag<R>(a[0][i], a[1][i], a[2][i], a[3][i], a[4][i], a[5][i], a[6][i], a[7][i]);
+}
+
+__device__ __forceinline__ bool in_box(int x, int z, int X, int Z)
+{
+    return x >= BOX && z >= BOX && x < X - BOX && z < Z - BOX;
+}
+
+// Row assignment of the E phase: the EH rows of the 16 central column quads
+// are split over 16 row groups as evenly as possible; the 2*TH side cells
+// (columns x0-4..x0-1 and x0+64..x0+67 on the tile rows) go one quad-row
+// each to the last 2*TH threads, whose groups hold the shorter runs.
+template <int TH>
+struct EMap {
+    static constexpr int EH = TH + 8;
+    static constexpr int RB = EH / 16, RX = EH % 16;
+    static constexpr int RMAX = RB + (RX ? 1 : 0);
+    static constexpr int NSIDE = 2 * TH;
+    static_assert(NSIDE <= NTHREADS, "side cells exceed the work-group");
+    static __device__ __forceinline__ int start(int g) { return g * RB + min(g, RX); }
+    static __device__ __forceinline__ int count(int g) { return RB + (g < RX ? 1 : 0); }
+};
+
+// ---------------------------------------------------------------------------
+// forward step
+// ---------------------------------------------------------------------------
+
+template <typename R>
+struct FwdCtx {
+    const FwdArgs<R> &a;
+    int s, X, Z, ld;
+    uint8_t fl;
+    const R *psi0;
+    R *psi1, *P1;
+    const int *evt;   // event map at the plane origin (cell (0,0))
+};
+
+// P(t) for one E quad (cells x..x+3 on row z), given D-x Ax and D-z Az.
+// Returns the four values (0 outside the updated box) and performs the
+// source injection, receiver sampling, ring save and psi update there.
+template <typename R>
+__device__ __forceinline__ void p_quad(const FwdCtx<R> &c, int x, int z, bool owned,
+                                       const R (&dAx)[4], const R (&dAz)[4], const V4<R> &pp4,
+                                       const V4<R> &mm4, R (&p)[4])
+{
+    const FwdArgs<R> &a = c.a;
+    R pp[4], mm[4], kx[4] = {R(1), R(1), R(1), R(1)}, psi[4] = {R(0), R(0), R(0), R(0)};
+    unpack<R>(pp4, pp);
+    unpack<R>(mm4, mm);
+    R kz = R(1);
+    const int gi = z * c.ld + x;
+    if (c.fl & TF_STRIP) {
+        unpack<R>(ld4(a.kx + XOFF + x), kx);
+        kz = a.kz[ZOFF + z];
+        unpack<R>(ld4(c.psi0 + gi), psi);
+    }
+    R nx[4], nz[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        nx[i] = fma(mm[i], dAx[i], mul_rn(psi[i], kx[i]));
+        nz[i] = fma(mm[i], dAz[i], mul_rn(pp[i] - psi[i], kz));
+    }
+    if (c.fl & TF_EVENT) {
+        int ev[4];
+        {
+            const int4 e4 = *reinterpret_cast<const int4 *>(c.evt + gi);
+            ev[0] = e4.x; ev[1] = e4.y; ev[2] = e4.z; ev[3] = e4.w;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if (ev[i] < 0) continue;
+            const PointEvt pe = a.pts[ev[i]];
+            if (pe.src >= 0) {
+                const int ss = a.shot_src[c.s];
+                if (ss < 0 || ss == pe.src) {
+                    const R h = a.icum[(size_t)pe.src * a.T + a.t] * R(0.5);
+                    nx[i] = add_rn(nx[i], h);
+                    nz[i] = add_rn(nz[i], h);
+                }
+            }
+            if (owned) {
+                const R pv = add_rn(nx[i], nz[i]);
+                if (pe.rx >= 0 && a.rec) a.rec[((size_t)c.s * a.n_rx + pe.rx) * a.T + a.t] = pv;
+                if (pe.ring >= 0 && a.ring)
+                    a.ring[((size_t)c.s * a.T + a.t) * a.n_ring + pe.ring] = pv;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) p[i] = add_rn(nx[i], nz[i]);
+    if (c.fl & TF_EDGE) {
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            if (!in_box(x + i, z, c.X, c.Z)) { p[i] = R(0); nx[i] = R(0); }
+    }
+    if (owned) {
+        st4(c.P1 + gi, pack<R>(p));
+        if (c.fl & TF_STRIP) {
+            R ps[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) ps[i] = (kx[i] != R(1) || kz != R(1)) ? nx[i] : R(0);
+            st4(c.psi1 + gi, pack<R>(ps));
+        }
+    }
+}
 
 template <typename R, int TH>
 __global__ void __launch_bounds__(NTHREADS)
 fwd_step_kernel(const FwdArgs<R> a)
 {
-    constexpr int EH = TH + 8, AZH = TH + 16, RE = EH / 4, RO = TH / 4;
-    static_assert(TH % 4 == 0, "tile height must be a multiple of 4");
-    __shared__ R sAx[EH][AXW];   // Ax(t-1): rows z0-4.., cols x0-8..
-    __shared__ R sAz[AZH][EW];   // Az(t-1): rows z0-8.., cols x0-4..
-    __shared__ R sP[EH][EW];     // P(t) on the E box: rows z0-4.., cols x0-4..
+    using M = EMap<TH>;
+    constexpr int EH = M::EH, AZH = TH + 16, RO = TH / 16;
+    static_assert(TH % 16 == 0, "tile height must be a multiple of 16");
+    __shared__ __align__(16) R sAx[EH][AXW];   // Ax(t-1): rows z0-4.., cols x0-8..
+    __shared__ __align__(16) R sAz[AZH][EW];   // Az(t-1): rows z0-8.., cols x0-4..
+    __shared__ __align__(16) R sP[EH][EW];     // P(t) on the E box: rows z0-4.., cols x0-4..
 
     const int tiles = a.ntx * a.ntz;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const int s = L / tiles, tl = L - s * tiles;
     const int tz = tl / a.ntx, tx = tl - tz * a.ntx;
     const int x0 = tx * TW, z0 = tz * TH;
-    const int X = a.X, Z = a.Z, ld = a.ld;
-    const size_t so = (size_t)s * (size_t)a.sstride;
+    const int ld = a.ld;
+    const size_t org = (size_t)s * (size_t)a.sstride + (size_t)ZOFF * ld + XOFF;
     const int tid = threadIdx.x;
-    const uint8_t fl = a.tflags[tl];
+    const int q = tid & 15, g = tid >> 4;
 
+    const size_t porg = (size_t)ZOFF * ld + XOFF;   // origin of the shared (per-model) planes
+    FwdCtx<R> c{a, s, a.X, a.Z, ld, a.tflags[tl], a.Px0 + org, a.Px1 + org, a.P1 + org,
+                a.evt + porg};
+
+    // ---- 1. all global loads of the tile ---------------------------------
+    Stage<R, EH, AXW / 4> stx;
+    Stage<R, AZH, EW / 4> stz;
+    stx.load(a.Ax0 + org, ld, z0 - 4, x0 - 8, tid);
+    stz.load(a.Az0 + org, ld, z0 - 8, x0 - 4, tid);
+    const R *P0 = a.P0 + org, *cm = a.cm + porg;
+    const int er0 = M::start(g), ern = M::count(g);
+    V4<R> pp[M::RMAX], mm[M::RMAX];
+#pragma unroll
+    for (int j = 0; j < M::RMAX; j++) {
+        const int er = min(er0 + j, EH - 1);
+        const int gi = (z0 - 4 + er) * ld + x0 + 4 * q;
+        pp[j] = ld4(P0 + gi);
+        mm[j] = ld4(cm + gi);
+    }
+    const int sidx = tid - (NTHREADS - M::NSIDE);
+    const bool has_side = sidx >= 0;
+    const int s_er = 4 + (has_side ? sidx % TH : 0);
+    const int s_q = (has_side && sidx >= TH) ? 17 : 0;   // E quad: 0 (left) or 17 (right)
+    V4<R> spp, smm;
     {
-        const R *Ax0 = a.Ax0 + so, *Az0 = a.Az0 + so;
-        for (int i = tid; i < EH * AXW; i += NTHREADS) {
-            const int r = i / AXW, c = i - r * AXW;
-            const int z = z0 - 4 + r, x = x0 - 8 + c;
-            sAx[r][c] = in_box(x, z, X, Z) ? Ax0[z * ld + x] : R(0);
-        }
-        for (int i = tid; i < AZH * EW; i += NTHREADS) {
-            const int r = i / EW, c = i - r * EW;
-            const int z = z0 - 8 + r, x = x0 - 4 + c;
-            sAz[r][c] = in_box(x, z, X, Z) ? Az0[z * ld + x] : R(0);
-        }
+        const int gi = (z0 - 4 + s_er) * ld + x0 - 4 + 4 * s_q;
+        spp = ld4(P0 + gi);
+        smm = ld4(cm + gi);
     }
+    stx.store(sAx, tid);
+    stz.store(sAz, tid);
+    __syncthreads();
 
-    // ---- E phase: P(t) on the tile plus halo -------------------------------
-    const bool eact = tid < EW * 4;
-    const int ex = tid % EW, ey = tid / EW;
-    const int xe = x0 - 4 + ex;
-    const bool xin = xe >= BOX && xe < X - BOX;
-    const int er0 = ey * RE;
-    R pp[RE], mm[RE];
-    if (eact) {
+    // ---- 2. E phase: P(t) ------------------------------------------------
+    {
+        const int Q = q + 1;   // central E quads 1..16
+        V4<R> w[8];
 #pragma unroll
-        for (int j = 0; j < RE; j++) {
-            const int z = z0 - 4 + er0 + j;
-            const bool in = xin && z >= BOX && z < Z - BOX;
-            pp[j] = in ? a.P0[so + z * ld + xe] : R(0);
-            mm[j] = in ? a.cm[z * ld + xe] : R(0);
+        for (int j = 0; j < 8; j++) w[j] = ld4(&sAz[min(er0 + j, AZH - 1)][4 * Q]);
+#pragma unroll
+        for (int j = 0; j < M::RMAX; j++) {
+            if (j < ern) {
+                const int er = er0 + j;
+                R v[12], dAx[4], dAz[4], p[4];
+                row12<R>(&sAx[er][4 * Q], v);
+                stag_row<R, 0>(v, dAx);
+                stag_col<R>(w, dAz);
+                p_quad<R>(c, x0 + 4 * q, z0 - 4 + er, er >= 4 && er < TH + 4, dAx, dAz, pp[j],
+                          mm[j], p);
+                st4(&sP[er][4 * Q], pack<R>(p));
+            }
+            if (j + 1 < M::RMAX) {
+#pragma unroll
+                for (int k = 0; k < 7; k++) w[k] = w[k + 1];
+                w[7] = ld4(&sAz[min(er0 + j + 8, AZH - 1)][4 * Q]);
+            }
+        }
+        if (has_side) {
+            R v[12], dAx[4], dAz[4], p[4];
+            row12<R>(&sAx[s_er][4 * s_q], v);
+            stag_row<R, 0>(v, dAx);
+            V4<R> ws[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) ws[j] = ld4(&sAz[s_er + j][4 * s_q]);
+            stag_col<R>(ws, dAz);
+            p_quad<R>(c, x0 - 4 + 4 * s_q, z0 - 4 + s_er, false, dAx, dAz, spp, smm, p);
+            st4(&sP[s_er][4 * s_q], pack<R>(p));
         }
     }
     __syncthreads();
 
-    if (eact) {
-        const R kxv = ((fl & TF_STRIP) && xin) ? a.kx[xe] : R(1);
-        R w0 = sAz[er0 + 0][ex], w1 = sAz[er0 + 1][ex], w2 = sAz[er0 + 2][ex], w3 = sAz[er0 + 3][ex];
-        R w4 = sAz[er0 + 4][ex], w5 = sAz[er0 + 5][ex], w6 = sAz[er0 + 6][ex], w7 = sAz[er0 + 7][ex];
+    // ---- 3. owned phase: A(t) -------------------------------------------
+    {
+        const int x = x0 + 4 * q;
+        const int or0 = g * RO;
+        R kx[4] = {R(1), R(1), R(1), R(1)};
+        if (c.fl & TF_STRIP) unpack<R>(ld4(a.kx + XOFF + x), kx);
+        V4<R> w[8];
 #pragma unroll
-        for (int j = 0; j < RE; j++) {
-            const int er = er0 + j;
-            const int z = z0 - 4 + er;
-            const bool in = xin && z >= BOX && z < Z - BOX;
-            const R dAx = stag<R>(ROW8(sAx, er, ex));
-            const R dAz = stag<R>(w0, w1, w2, w3, w4, w5, w6, w7);
-            w0 = w1; w1 = w2; w2 = w3; w3 = w4; w4 = w5; w5 = w6; w6 = w7;
-            w7 = sAz[er + 8][ex];
-            const int gi = z * ld + xe;
-            R psi = R(0), kzv = R(1);
-            bool strip = false;
-            if (fl & TF_STRIP) {
-                if (in) kzv = a.kz[z];
-                strip = in && (kxv != R(1) || kzv != R(1));
-                if (strip) psi = a.Px0[so + gi];
-            }
-            R nx = fma(psi, kxv, mm[j] * dAx);
-            R nz = fma(pp[j] - psi, kzv, mm[j] * dAz);
-            const bool owned = er >= 4 && er < TH + 4 && ex >= 4 && ex < TW + 4 && xe < X && z < Z;
-            if ((fl & TF_EVENT) && in) {
-                const int e = a.evt[gi];
-                if (e >= 0) {
-                    const PointEvt pe = a.pts[e];
-                    if (pe.src >= 0) {
-                        const int ss = a.shot_src[s];
-                        if (ss < 0 || ss == pe.src) {
-                            const R h = a.icum[(size_t)pe.src * a.T + a.t] * R(0.5);
-                            nx += h;
-                            nz += h;
-                        }
-                    }
-                    if (owned) {
-                        const R pv = nx + nz;
-                        if (pe.rx >= 0 && a.rec)
-                            a.rec[((size_t)s * a.n_rx + pe.rx) * a.T + a.t] = pv;
-                        if (pe.ring >= 0 && a.ring)
-                            a.ring[((size_t)s * a.T + a.t) * a.n_ring + pe.ring] = pv;
-                    }
-                }
-            }
-            const R p = in ? nx + nz : R(0);
-            sP[er][ex] = p;
-            if (owned) {
-                a.P1[so + gi] = p;
-                if (strip) a.Px1[so + gi] = nx;
-            }
-        }
-    }
-    __syncthreads();
-
-    // ---- owned phase: A(t) on the tile ------------------------------------
-    if (tid < TW * 4) {
-        const int ox = tid & (TW - 1), oy = tid / TW;
-        const int x = x0 + ox;
-        const bool xin2 = x >= BOX && x < X - BOX;
-        const R kxv = ((fl & TF_STRIP) && xin2) ? a.kx[x] : R(1);
-        const int or0 = oy * RO;
-        R w0 = sP[or0 + 1][ox + 4], w1 = sP[or0 + 2][ox + 4], w2 = sP[or0 + 3][ox + 4], w3 = sP[or0 + 4][ox + 4];
-        R w4 = sP[or0 + 5][ox + 4], w5 = sP[or0 + 6][ox + 4], w6 = sP[or0 + 7][ox + 4], w7 = sP[or0 + 8][ox + 4];
+        for (int j = 0; j < 8; j++) w[j] = ld4(&sP[or0 + 1 + j][4 * q + 4]);
+        R *Ax1 = a.Ax1 + org, *Az1 = a.Az1 + org;
 #pragma unroll
         for (int j = 0; j < RO; j++) {
-            const int orr = or0 + j;
-            const int z = z0 + orr;
-            const R dPx = stag<R>(ROW8(sP, orr + 4, ox + 1));
-            const R dPz = stag<R>(w0, w1, w2, w3, w4, w5, w6, w7);
+            const int orr = or0 + j, z = z0 + orr;
+            R v[12], dPx[4], dPz[4], ax[4], az[4], nax[4], naz[4];
+            row12<R>(&sP[orr + 4][4 * q], v);
+            stag_row<R, 1>(v, dPx);
+            stag_col<R>(w, dPz);
             if (j + 1 < RO) {
-                w0 = w1; w1 = w2; w2 = w3; w3 = w4; w4 = w5; w5 = w6; w6 = w7;
-                w7 = sP[orr + 9][ox + 4];
+#pragma unroll
+                for (int k = 0; k < 7; k++) w[k] = w[k + 1];
+                w[7] = ld4(&sP[orr + 9][4 * q + 4]);
             }
-            if (x < X && z < Z) {
-                const bool in = xin2 && z >= BOX && z < Z - BOX;
-                const R kzv = ((fl & TF_STRIP) && in) ? a.kz[z] : R(1);
-                const int gi = z * ld + x;
-                const R ax = sAx[orr + 4][ox + 8], az = sAz[orr + 8][ox + 4];
-                a.Ax1[so + gi] = in ? fma(ax, kxv, a.cA * dPx) : R(0);
-                a.Az1[so + gi] = in ? fma(az, kzv, a.cA * dPz) : R(0);
+            unpack<R>(ld4(&sAx[orr + 4][4 * q + 8]), ax);
+            unpack<R>(ld4(&sAz[orr + 8][4 * q + 4]), az);
+            const R kz = (c.fl & TF_STRIP) ? a.kz[ZOFF + z] : R(1);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                nax[i] = fma(a.cA, dPx[i], mul_rn(ax[i], kx[i]));
+                naz[i] = fma(a.cA, dPz[i], mul_rn(az[i], kz));
             }
+            if (c.fl & TF_EDGE) {
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    if (!in_box(x + i, z, c.X, c.Z)) { nax[i] = R(0); naz[i] = R(0); }
+            }
+            const int gi = z * ld + x;
+            st4(Ax1 + gi, pack<R>(nax));
+            st4(Az1 + gi, pack<R>(naz));
         }
     }
 }
 
-// Adjoint step (reference phase 2, regression_PML_single.cu:793-823): the
-// adjoint field lambda advances with the damped forward operator and takes the
-// running sum of the time-flipped adjoint source at the receivers; the forward
-// field is reconstructed backwards with the undamped reverse operator and the
-// saved boundary ring; grad += lambda_P(t) * lap8(U(t)) on the tile.
+// ---------------------------------------------------------------------------
+// adjoint step
+// ---------------------------------------------------------------------------
+// Reference phase 2 (regression_PML_single.cu:793-823): the adjoint field
+// lambda advances with the damped forward operator and takes the running sum
+// of the time-flipped adjoint source at the receivers; the forward field is
+// reconstructed backwards with the undamped reverse operator and the saved
+// boundary ring; grad += lambda_P(t) * lap8(U(t)) on the tile.
+
+template <typename R>
+struct AdjCtx {
+    const AdjArgs<R> &a;
+    int s, X, Z, ld, trev;
+    uint8_t fl;
+    const R *psi0;
+    R *psi1, *LP1, *U1;
+    const int *evt;   // event map at the plane origin (cell (0,0))
+};
+
+template <typename R>
+__device__ __forceinline__ void lu_quad(const AdjCtx<R> &c, int x, int z, bool owned,
+                                        const R (&dLx)[4], const R (&dLz)[4], const R (&dUx)[4],
+                                        const R (&dUz)[4], const V4<R> &lp4, const V4<R> &up4,
+                                        const V4<R> &mm4, R (&lp)[4], R (&up)[4])
+{
+    const AdjArgs<R> &a = c.a;
+    R lpp[4], upp[4], mm[4], kx[4] = {R(1), R(1), R(1), R(1)}, psi[4] = {R(0), R(0), R(0), R(0)};
+    unpack<R>(lp4, lpp);
+    unpack<R>(up4, upp);
+    unpack<R>(mm4, mm);
+    R kz = R(1);
+    const int gi = z * c.ld + x;
+    if (c.fl & TF_STRIP) {
+        unpack<R>(ld4(a.kx + XOFF + x), kx);
+        kz = a.kz[ZOFF + z];
+        unpack<R>(ld4(c.psi0 + gi), psi);
+    }
+    R nx[4], nz[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        nx[i] = fma(mm[i], dLx[i], mul_rn(psi[i], kx[i]));
+        nz[i] = fma(mm[i], dLz[i], mul_rn(lpp[i] - psi[i], kz));
+        up[i] = fma(-mm[i], dUx[i] + dUz[i], upp[i]);
+    }
+    if (c.fl & TF_EVENT) {
+        int ev[4];
+        {
+            const int4 e4 = *reinterpret_cast<const int4 *>(c.evt + gi);
+            ev[0] = e4.x; ev[1] = e4.y; ev[2] = e4.z; ev[3] = e4.w;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if (ev[i] < 0) continue;
+            const PointEvt pe = a.pts[ev[i]];
+            if (pe.rx >= 0) {
+                const R h = a.jcum[((size_t)c.s * a.n_rx + pe.rx) * a.T + a.t] * R(0.5);
+                nx[i] = add_rn(nx[i], h);
+                nz[i] = add_rn(nz[i], h);
+            }
+            if (pe.ring >= 0) up[i] = a.ring[((size_t)c.s * a.T + c.trev) * a.n_ring + pe.ring];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) lp[i] = add_rn(nx[i], nz[i]);
+    if (c.fl & TF_EDGE) {
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            if (!in_box(x + i, z, c.X, c.Z)) { lp[i] = R(0); up[i] = R(0); nx[i] = R(0); }
+    }
+    if (owned) {
+        st4(c.LP1 + gi, pack<R>(lp));
+        st4(c.U1 + gi, pack<R>(up));
+        if (c.fl & TF_STRIP) {
+            R ps[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) ps[i] = (kx[i] != R(1) || kz != R(1)) ? nx[i] : R(0);
+            st4(c.psi1 + gi, pack<R>(ps));
+        }
+    }
+}
+
 template <typename R, int TH>
 __global__ void __launch_bounds__(NTHREADS)
 adj_step_kernel(const AdjArgs<R> a)
 {
-    constexpr int EH = TH + 8, AZH = TH + 16, RE = EH / 4, RO = TH / 4;
-    __shared__ R sLAx[EH][AXW];
-    __shared__ R sLAz[AZH][EW];
-    __shared__ R sLP[EH][EW];
-    __shared__ R sUAx[EH][AXW];
-    __shared__ R sUAz[AZH][EW];
-    __shared__ R sU[EH][EW];
+    using M = EMap<TH>;
+    constexpr int EH = M::EH, AZH = TH + 16, RO = TH / 16;
+    static_assert(TH % 16 == 0, "tile height must be a multiple of 16");
+    __shared__ __align__(16) R sLAx[EH][AXW];
+    __shared__ __align__(16) R sLAz[AZH][EW];
+    __shared__ __align__(16) R sLP[EH][EW];
+    __shared__ __align__(16) R sUAx[EH][AXW];
+    __shared__ __align__(16) R sUAz[AZH][EW];
+    __shared__ __align__(16) R sU[EH][EW];
 
     const int tiles = a.ntx * a.ntz;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const int s = L / tiles, tl = L - s * tiles;
     const int tz = tl / a.ntx, tx = tl - tz * a.ntx;
     const int x0 = tx * TW, z0 = tz * TH;
-    const int X = a.X, Z = a.Z, ld = a.ld;
-    const size_t so = (size_t)s * (size_t)a.sstride;
+    const int ld = a.ld;
+    const size_t org = (size_t)s * (size_t)a.sstride + (size_t)ZOFF * ld + XOFF;
     const int tid = threadIdx.x;
-    const uint8_t fl = a.tflags[tl];
-    const int trev = a.T - 1 - a.t;   // forward time of U(t)
+    const int q = tid & 15, g = tid >> 4;
+
+    const size_t porg = (size_t)ZOFF * ld + XOFF;   // origin of the shared (per-model) planes
+    AdjCtx<R> c{a, s, a.X, a.Z, ld, a.T - 1 - a.t, a.tflags[tl], a.LPx0 + org, a.LPx1 + org,
+                a.LP1 + org, a.U1 + org, a.evt + porg};
+
+    Stage<R, EH, AXW / 4> slx, sux;
+    Stage<R, AZH, EW / 4> slz, suz;
+    slx.load(a.LAx0 + org, ld, z0 - 4, x0 - 8, tid);
+    sux.load(a.UAx0 + org, ld, z0 - 4, x0 - 8, tid);
+    slz.load(a.LAz0 + org, ld, z0 - 8, x0 - 4, tid);
+    suz.load(a.UAz0 + org, ld, z0 - 8, x0 - 4, tid);
+    const R *LP0 = a.LP0 + org, *U0 = a.U0 + org, *cm = a.cm + porg;
+    const int er0 = M::start(g), ern = M::count(g);
+    V4<R> lp[M::RMAX], up[M::RMAX], mm[M::RMAX];
+#pragma unroll
+    for (int j = 0; j < M::RMAX; j++) {
+        const int er = min(er0 + j, EH - 1);
+        const int gi = (z0 - 4 + er) * ld + x0 + 4 * q;
+        lp[j] = ld4(LP0 + gi);
+        up[j] = ld4(U0 + gi);
+        mm[j] = ld4(cm + gi);
+    }
+    const int sidx = tid - (NTHREADS - M::NSIDE);
+    const bool has_side = sidx >= 0;
+    const int s_er = 4 + (has_side ? sidx % TH : 0);
+    const int s_q = (has_side && sidx >= TH) ? 17 : 0;
+    V4<R> slp, sup, smm;
+    {
+        const int gi = (z0 - 4 + s_er) * ld + x0 - 4 + 4 * s_q;
+        slp = ld4(LP0 + gi);
+        sup = ld4(U0 + gi);
+        smm = ld4(cm + gi);
+    }
+    // gradient of the owned quads, read early
+    R *G = a.grad + org;
+    const int or0 = g * RO;
+    V4<R> gr[RO];
+#pragma unroll
+    for (int j = 0; j < RO; j++) gr[j] = ld4(G + (z0 + or0 + j) * ld + x0 + 4 * q);
+    slx.store(sLAx, tid);
+    sux.store(sUAx, tid);
+    slz.store(sLAz, tid);
+    suz.store(sUAz, tid);
+    __syncthreads();
 
     {
-        const R *LAx0 = a.LAx0 + so, *LAz0 = a.LAz0 + so;
-        const R *UAx0 = a.UAx0 + so, *UAz0 = a.UAz0 + so;
-        for (int i = tid; i < EH * AXW; i += NTHREADS) {
-            const int r = i / AXW, c = i - r * AXW;
-            const int z = z0 - 4 + r, x = x0 - 8 + c;
-            const bool in = in_box(x, z, X, Z);
-            sLAx[r][c] = in ? LAx0[z * ld + x] : R(0);
-            sUAx[r][c] = in ? UAx0[z * ld + x] : R(0);
-        }
-        for (int i = tid; i < AZH * EW; i += NTHREADS) {
-            const int r = i / EW, c = i - r * EW;
-            const int z = z0 - 8 + r, x = x0 - 4 + c;
-            const bool in = in_box(x, z, X, Z);
-            sLAz[r][c] = in ? LAz0[z * ld + x] : R(0);
-            sUAz[r][c] = in ? UAz0[z * ld + x] : R(0);
-        }
-    }
-
-    const bool eact = tid < EW * 4;
-    const int ex = tid % EW, ey = tid / EW;
-    const int xe = x0 - 4 + ex;
-    const bool xin = xe >= BOX && xe < X - BOX;
-    const int er0 = ey * RE;
-    R lp[RE], up[RE], mm[RE];
-    if (eact) {
+        const int Q = q + 1;
+        V4<R> wl[8], wu[8];
 #pragma unroll
-        for (int j = 0; j < RE; j++) {
-            const int z = z0 - 4 + er0 + j;
-            const bool in = xin && z >= BOX && z < Z - BOX;
-            lp[j] = in ? a.LP0[so + z * ld + xe] : R(0);
-            up[j] = in ? a.U0[so + z * ld + xe] : R(0);
-            mm[j] = in ? a.cm[z * ld + xe] : R(0);
+        for (int j = 0; j < 8; j++) {
+            wl[j] = ld4(&sLAz[min(er0 + j, AZH - 1)][4 * Q]);
+            wu[j] = ld4(&sUAz[min(er0 + j, AZH - 1)][4 * Q]);
+        }
+#pragma unroll
+        for (int j = 0; j < M::RMAX; j++) {
+            if (j < ern) {
+                const int er = er0 + j;
+                R v[12], dLx[4], dLz[4], dUx[4], dUz[4], lo[4], uo[4];
+                row12<R>(&sLAx[er][4 * Q], v);
+                stag_row<R, 0>(v, dLx);
+                row12<R>(&sUAx[er][4 * Q], v);
+                stag_row<R, 0>(v, dUx);
+                stag_col<R>(wl, dLz);
+                stag_col<R>(wu, dUz);
+                lu_quad<R>(c, x0 + 4 * q, z0 - 4 + er, er >= 4 && er < TH + 4, dLx, dLz, dUx, dUz,
+                           lp[j], up[j], mm[j], lo, uo);
+                st4(&sLP[er][4 * Q], pack<R>(lo));
+                st4(&sU[er][4 * Q], pack<R>(uo));
+            }
+            if (j + 1 < M::RMAX) {
+#pragma unroll
+                for (int k = 0; k < 7; k++) { wl[k] = wl[k + 1]; wu[k] = wu[k + 1]; }
+                wl[7] = ld4(&sLAz[min(er0 + j + 8, AZH - 1)][4 * Q]);
+                wu[7] = ld4(&sUAz[min(er0 + j + 8, AZH - 1)][4 * Q]);
+            }
+        }
+        if (has_side) {
+            R v[12], dLx[4], dLz[4], dUx[4], dUz[4], lo[4], uo[4];
+            row12<R>(&sLAx[s_er][4 * s_q], v);
+            stag_row<R, 0>(v, dLx);
+            row12<R>(&sUAx[s_er][4 * s_q], v);
+            stag_row<R, 0>(v, dUx);
+            V4<R> ws[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) ws[j] = ld4(&sLAz[s_er + j][4 * s_q]);
+            stag_col<R>(ws, dLz);
+#pragma unroll
+            for (int j = 0; j < 8; j++) ws[j] = ld4(&sUAz[s_er + j][4 * s_q]);
+            stag_col<R>(ws, dUz);
+            lu_quad<R>(c, x0 - 4 + 4 * s_q, z0 - 4 + s_er, false, dLx, dLz, dUx, dUz, slp, sup,
+                       smm, lo, uo);
+            st4(&sLP[s_er][4 * s_q], pack<R>(lo));
+            st4(&sU[s_er][4 * s_q], pack<R>(uo));
         }
     }
     __syncthreads();
 
-    if (eact) {
-        const R kxv = ((fl & TF_STRIP) && xin) ? a.kx[xe] : R(1);
-        R w0 = sLAz[er0 + 0][ex], w1 = sLAz[er0 + 1][ex], w2 = sLAz[er0 + 2][ex], w3 = sLAz[er0 + 3][ex];
-        R w4 = sLAz[er0 + 4][ex], w5 = sLAz[er0 + 5][ex], w6 = sLAz[er0 + 6][ex], w7 = sLAz[er0 + 7][ex];
-        R v0 = sUAz[er0 + 0][ex], v1 = sUAz[er0 + 1][ex], v2 = sUAz[er0 + 2][ex], v3 = sUAz[er0 + 3][ex];
-        R v4 = sUAz[er0 + 4][ex], v5 = sUAz[er0 + 5][ex], v6 = sUAz[er0 + 6][ex], v7 = sUAz[er0 + 7][ex];
+    {
+        const int x = x0 + 4 * q;
+        R kx[4] = {R(1), R(1), R(1), R(1)};
+        if (c.fl & TF_STRIP) unpack<R>(ld4(a.kx + XOFF + x), kx);
+        // lambda window: rows orr+1..orr+8; U window: rows orr..orr+8
+        V4<R> wl[8], wu[9];
 #pragma unroll
-        for (int j = 0; j < RE; j++) {
-            const int er = er0 + j;
-            const int z = z0 - 4 + er;
-            const bool in = xin && z >= BOX && z < Z - BOX;
-            const R dLx = stag<R>(ROW8(sLAx, er, ex));
-            const R dLz = stag<R>(w0, w1, w2, w3, w4, w5, w6, w7);
-            const R dUx = stag<R>(ROW8(sUAx, er, ex));
-            const R dUz = stag<R>(v0, v1, v2, v3, v4, v5, v6, v7);
-            w0 = w1; w1 = w2; w2 = w3; w3 = w4; w4 = w5; w5 = w6; w6 = w7;
-            w7 = sLAz[er + 8][ex];
-            v0 = v1; v1 = v2; v2 = v3; v3 = v4; v4 = v5; v5 = v6; v6 = v7;
-            v7 = sUAz[er + 8][ex];
-            const int gi = z * ld + xe;
-            R psi = R(0), kzv = R(1);
-            bool strip = false;
-            if (fl & TF_STRIP) {
-                if (in) kzv = a.kz[z];
-                strip = in && (kxv != R(1) || kzv != R(1));
-                if (strip) psi = a.LPx0[so + gi];
-            }
-            R nx = fma(psi, kxv, mm[j] * dLx);
-            R nz = fma(lp[j] - psi, kzv, mm[j] * dLz);
-            R u = fma(-mm[j], dUx + dUz, up[j]);
-            const bool owned = er >= 4 && er < TH + 4 && ex >= 4 && ex < TW + 4 && xe < X && z < Z;
-            if ((fl & TF_EVENT) && in) {
-                const int e = a.evt[gi];
-                if (e >= 0) {
-                    const PointEvt pe = a.pts[e];
-                    if (pe.rx >= 0) {
-                        const R h = a.jcum[((size_t)s * a.n_rx + pe.rx) * a.T + a.t] * R(0.5);
-                        nx += h;
-                        nz += h;
-                    }
-                    if (pe.ring >= 0)
-                        u = a.ring[((size_t)s * a.T + trev) * a.n_ring + pe.ring];
-                }
-            }
-            const R lpv = in ? nx + nz : R(0);
-            const R uv = in ? u : R(0);
-            sLP[er][ex] = lpv;
-            sU[er][ex] = uv;
-            if (owned) {
-                a.LP1[so + gi] = lpv;
-                a.U1[so + gi] = uv;
-                if (strip) a.LPx1[so + gi] = nx;
-            }
-        }
-    }
-    __syncthreads();
-
-    if (tid < TW * 4) {
-        const int ox = tid & (TW - 1), oy = tid / TW;
-        const int x = x0 + ox;
-        const bool xin2 = x >= BOX && x < X - BOX;
-        const R kxv = ((fl & TF_STRIP) && xin2) ? a.kx[x] : R(1);
-        const int or0 = oy * RO;
-        R w0 = sLP[or0 + 1][ox + 4], w1 = sLP[or0 + 2][ox + 4], w2 = sLP[or0 + 3][ox + 4], w3 = sLP[or0 + 4][ox + 4];
-        R w4 = sLP[or0 + 5][ox + 4], w5 = sLP[or0 + 6][ox + 4], w6 = sLP[or0 + 7][ox + 4], w7 = sLP[or0 + 8][ox + 4];
-        // U window covers rows orr .. orr+8 (the 9-point Laplacian); the
-        // staggered derivative uses the upper eight.
-        R u0 = sU[or0 + 0][ox + 4], u1 = sU[or0 + 1][ox + 4], u2 = sU[or0 + 2][ox + 4], u3 = sU[or0 + 3][ox + 4];
-        R u4 = sU[or0 + 4][ox + 4], u5 = sU[or0 + 5][ox + 4], u6 = sU[or0 + 6][ox + 4], u7 = sU[or0 + 7][ox + 4];
-        R u8 = sU[or0 + 8][ox + 4];
-        R *G = a.grad + so;
+        for (int j = 0; j < 8; j++) wl[j] = ld4(&sLP[or0 + 1 + j][4 * q + 4]);
+#pragma unroll
+        for (int j = 0; j < 9; j++) wu[j] = ld4(&sU[or0 + j][4 * q + 4]);
+        R *LAx1 = a.LAx1 + org, *LAz1 = a.LAz1 + org, *UAx1 = a.UAx1 + org, *UAz1 = a.UAz1 + org;
+        constexpr R inv5040 = R(1) / R(5040);
 #pragma unroll
         for (int j = 0; j < RO; j++) {
-            const int orr = or0 + j;
-            const int z = z0 + orr;
-            const R dLPx = stag<R>(ROW8(sLP, orr + 4, ox + 1));
-            const R dLPz = stag<R>(w0, w1, w2, w3, w4, w5, w6, w7);
-            const R dUx = stag<R>(ROW8(sU, orr + 4, ox + 1));
-            const R dUz = stag<R>(u1, u2, u3, u4, u5, u6, u7, u8);
-            const R lx = lap9<R>(sU[orr + 4][ox + 0], sU[orr + 4][ox + 1], sU[orr + 4][ox + 2],
-                                 sU[orr + 4][ox + 3], u4, sU[orr + 4][ox + 5], sU[orr + 4][ox + 6],
-                                 sU[orr + 4][ox + 7], sU[orr + 4][ox + 8]);
-            const R lz = lap9<R>(u0, u1, u2, u3, u4, u5, u6, u7, u8);
-            const R lam = sLP[orr + 4][ox + 4];
+            const int orr = or0 + j, z = z0 + orr;
+            R v[12], dLx[4], dLz[4], dUx[4], dUz[4], lx[4], lz[4];
+            row12<R>(&sLP[orr + 4][4 * q], v);
+            stag_row<R, 1>(v, dLx);
+            row12<R>(&sU[orr + 4][4 * q], v);
+            stag_row<R, 1>(v, dUx);
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                lx[i] = lap9<R>(v[i], v[i + 1], v[i + 2], v[i + 3], v[i + 4], v[i + 5], v[i + 6],
+                                v[i + 7], v[i + 8]);
+            stag_col<R>(wl, dLz);
+            {
+                V4<R> w8[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) w8[k] = wu[k + 1];
+                stag_col<R>(w8, dUz);
+                R u[9][4];
+#pragma unroll
+                for (int k = 0; k < 9; k++) unpack<R>(wu[k], u[k]);
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    lz[i] = lap9<R>(u[0][i], u[1][i], u[2][i], u[3][i], u[4][i], u[5][i], u[6][i],
+                                    u[7][i], u[8][i]);
+            }
+            R lam[4], lax[4], laz[4], uax[4], uaz[4], gg[4];
+            unpack<R>(ld4(&sLP[orr + 4][4 * q + 4]), lam);
+            unpack<R>(ld4(&sLAx[orr + 4][4 * q + 8]), lax);
+            unpack<R>(ld4(&sLAz[orr + 8][4 * q + 4]), laz);
+            unpack<R>(ld4(&sUAx[orr + 4][4 * q + 8]), uax);
+            unpack<R>(ld4(&sUAz[orr + 8][4 * q + 4]), uaz);
+            unpack<R>(gr[j], gg);
             if (j + 1 < RO) {
-                w0 = w1; w1 = w2; w2 = w3; w3 = w4; w4 = w5; w5 = w6; w6 = w7;
-                w7 = sLP[orr + 9][ox + 4];
-                u0 = u1; u1 = u2; u2 = u3; u3 = u4; u4 = u5; u5 = u6; u6 = u7; u7 = u8;
-                u8 = sU[orr + 9][ox + 4];
+#pragma unroll
+                for (int k = 0; k < 7; k++) wl[k] = wl[k + 1];
+                wl[7] = ld4(&sLP[orr + 9][4 * q + 4]);
+#pragma unroll
+                for (int k = 0; k < 8; k++) wu[k] = wu[k + 1];
+                wu[8] = ld4(&sU[orr + 9][4 * q + 4]);
             }
-            if (x < X && z < Z) {
-                const bool in = xin2 && z >= BOX && z < Z - BOX;
-                const R kzv = ((fl & TF_STRIP) && in) ? a.kz[z] : R(1);
-                const int gi = z * ld + x;
-                const R lax = sLAx[orr + 4][ox + 8], laz = sLAz[orr + 8][ox + 4];
-                const R uax = sUAx[orr + 4][ox + 8], uaz = sUAz[orr + 8][ox + 4];
-                a.LAx1[so + gi] = in ? fma(lax, kxv, a.cA * dLPx) : R(0);
-                a.LAz1[so + gi] = in ? fma(laz, kzv, a.cA * dLPz) : R(0);
-                a.UAx1[so + gi] = in ? fma(-a.cA, dUx, uax) : R(0);
-                a.UAz1[so + gi] = in ? fma(-a.cA, dUz, uaz) : R(0);
-                G[gi] = fma(lam, (lx + lz) * (R(1) / R(5040)), G[gi]);
+            const R kz = (c.fl & TF_STRIP) ? a.kz[ZOFF + z] : R(1);
+            R o1[4], o2[4], o3[4], o4[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                o1[i] = fma(a.cA, dLx[i], mul_rn(lax[i], kx[i]));
+                o2[i] = fma(a.cA, dLz[i], mul_rn(laz[i], kz));
+                o3[i] = fma(-a.cA, dUx[i], uax[i]);
+                o4[i] = fma(-a.cA, dUz[i], uaz[i]);
+                gg[i] = fma(lam[i], (lx[i] + lz[i]) * inv5040, gg[i]);
             }
+            if (c.fl & TF_EDGE) {
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    if (!in_box(x + i, z, c.X, c.Z)) { o1[i] = o2[i] = o3[i] = o4[i] = R(0); }
+            }
+            const int gi = z * ld + x;
+            st4(LAx1 + gi, pack<R>(o1));
+            st4(LAz1 + gi, pack<R>(o2));
+            st4(UAx1 + gi, pack<R>(o3));
+            st4(UAz1 + gi, pack<R>(o4));
+            st4(G + gi, pack<R>(gg));
         }
     }
 }

@@ -55,3 +55,11 @@ def test_l2(gpu):
     w0, a0 = O.l2(sim, obs)
     assert abs(w - w0) <= 1e-13 * abs(w0)
     assert np.array_equal(a, a0)
+
+
+def test_w2_all_zero_observed_is_nan_like_reference(gpu):
+    # adjoint_functions.py:38-43: obs == 0 gives c = 0 and g /= 0 -> NaN
+    sim, obs = traces(2, 300, seed=5)
+    obs[1] = 0.0
+    w, a = AF.wasserstein(sim, obs)
+    assert np.isnan(w) and np.isnan(a[1]).all() and np.isfinite(a[0]).all()
