@@ -126,6 +126,14 @@ int otfwi_set_observed(otfwi_model *m, int n_shots, const float *obs);
 int otfwi_set_tuning(otfwi_model *m, int th_fwd, int th_adj, int max_batch);
 int otfwi_get_tuning(otfwi_model *m, int *th_fwd, int *th_adj, int *max_batch);
 
+/* Launch style of the forward / adjoint step kernels: 0 = one work-group per
+ * tile, 1 = persistent work-groups (a CU-filling grid that walks the tiles and
+ * prefetches the next tile while computing the current one).  Both give
+ * bitwise-identical results.  No reference counterpart (the reference
+ * launches fixed 32x32 blocks, regression_PML_single.cu:120-135). */
+int otfwi_set_launch(otfwi_model *m, int persistent_fwd, int persistent_adj);
+int otfwi_get_launch(otfwi_model *m, int *persistent_fwd, int *persistent_adj);
+
 /* Device time of the last otfwi_gradient / otfwi_gradient_device call,
  * from HIP events on the model's stream: out[0] forward time loops (ms),
  * out[1] misfit kernels (ms), out[2] adjoint time loops (ms), out[3] number

@@ -103,6 +103,8 @@ struct otfwi_model {
     PointEvt *pts = nullptr;
     bool points_dirty = true;
     int th_fwd = 32, th_adj = 16, max_batch = 0;
+    int pers_fwd = 0, pers_adj = 0;   // persistent-work-group launches (0 = one work-group per tile)
+    int n_cu = 0;
     uint8_t *tf_fwd = nullptr, *tf_adj = nullptr;
     int tf_fwd_th = 0, tf_adj_th = 0;
     std::vector<float> obs_h;
@@ -365,6 +367,16 @@ int ready(otfwi_model *m, bool need_ring)
     return OTFWI_OK;
 }
 
+// Work-groups of a persistent launch: every CU filled to the kernel's
+// occupancy, rounded down to a multiple of the 8 XCDs (0 = per-tile launch).
+int persistent_wgs(const otfwi_model *m, bool adj)
+{
+    if (!(adj ? m->pers_adj : m->pers_fwd) || m->prec == 8) return 0;
+    const int occ = step_occupancy(adj, m->prec, adj ? m->th_adj : m->th_fwd, true);
+    const int n = std::max(8, (occ * m->n_cu) / 8 * 8);
+    return n;
+}
+
 template <typename R>
 int run_forward(otfwi_model *m, int n, bool save_ring)
 {
@@ -400,6 +412,7 @@ int run_forward(otfwi_model *m, int n, bool save_ring)
     a.rec = (R *)b.rec; a.n_rx = m->n_rx;
     a.ring = save_ring ? (R *)b.ring : nullptr;
     a.n_ring = m->n_ring;
+    const int pw = persistent_wgs(m, false);
     for (int t = 1; t < m->T; t++) {
         const int p = (t - 1) & 1, c = t & 1;
         a.t = t;
@@ -407,7 +420,7 @@ int run_forward(otfwi_model *m, int n, bool save_ring)
         a.Ax0 = (const R *)b.Ax[p]; a.Ax1 = (R *)b.Ax[c];
         a.Az0 = (const R *)b.Az[p]; a.Az1 = (R *)b.Az[c];
         a.Px0 = (const R *)b.Px[p]; a.Px1 = (R *)b.Px[c];
-        HIPCK(launch_fwd_step<R>(a, m->th_fwd, st));
+        HIPCK(launch_fwd_step<R>(a, m->th_fwd, pw, st));
     }
     return OTFWI_OK;
 }
@@ -440,6 +453,7 @@ int run_adjoint(otfwi_model *m, int n)
     a.jcum = (const R *)b.jcum; a.n_rx = m->n_rx;
     a.ring = (const R *)b.ring; a.n_ring = m->n_ring;
     const int T = m->T;
+    const int pw = persistent_wgs(m, true);
     for (int t = 1; t < T; t++) {
         const int lp = (t - 1) & 1, lc = t & 1;
         // reconstructed field: U(t) lives in P slot (T-1+t)%2, A_b(t) in slot (T-2+t)%2,
@@ -454,7 +468,7 @@ int run_adjoint(otfwi_model *m, int n)
         a.U0 = (const R *)b.P[up]; a.U1 = (R *)b.P[uc];
         a.UAx0 = (const R *)b.Ax[ap]; a.UAx1 = (R *)b.Ax[ac];
         a.UAz0 = (const R *)b.Az[ap]; a.UAz1 = (R *)b.Az[ac];
-        HIPCK(launch_adj_step<R>(a, m->th_adj, st));
+        HIPCK(launch_adj_step<R>(a, m->th_adj, pw, st));
     }
     return OTFWI_OK;
 }
@@ -707,6 +721,15 @@ int otfwi_create(otfwi_model **out, int X, int Z, int T, int precision, const fl
         return fail(OTFWI_E_ARG, "dx and dt must be positive");
     }
     if (precision == 8) { m->th_fwd = 16; m->th_adj = 16; }
+    {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+            delete m;
+            return fail(OTFWI_E_HIP, "no usable HIP device");
+        }
+        m->n_cu = prop.multiProcessorCount;
+    }
     m->dxs_h.assign(X, 0.0f);
     m->dzs_h.assign(Z, 0.0f);
     m->kx_h.assign(X, 1.0f);
@@ -853,6 +876,22 @@ int otfwi_set_tuning(otfwi_model *m, int th_fwd, int th_adj, int max_batch)
     if (th_adj) m->th_adj = th_adj;
     if (max_batch != m->max_batch) free_batch(m->b);
     m->max_batch = max_batch;
+    return OTFWI_OK;
+}
+
+int otfwi_set_launch(otfwi_model *m, int persistent_fwd, int persistent_adj)
+{
+    RCK(check_model(m));
+    m->pers_fwd = persistent_fwd ? 1 : 0;
+    m->pers_adj = persistent_adj ? 1 : 0;
+    return OTFWI_OK;
+}
+
+int otfwi_get_launch(otfwi_model *m, int *persistent_fwd, int *persistent_adj)
+{
+    RCK(check_model(m));
+    if (persistent_fwd) *persistent_fwd = m->pers_fwd;
+    if (persistent_adj) *persistent_adj = m->pers_adj;
     return OTFWI_OK;
 }
 

@@ -83,11 +83,14 @@ struct AdjArgs {
     int n_ring;
 };
 
-// Launchers (prop.hip).  th = tile height (16, 32 or 64).
+// Launchers (prop.hip).  th = tile height (16, 32 or 64); persistent_wgs > 0
+// launches that many persistent work-groups instead of one per tile.
 template <typename R>
-hipError_t launch_fwd_step(const FwdArgs<R> &a, int th, hipStream_t s);
+hipError_t launch_fwd_step(const FwdArgs<R> &a, int th, int persistent_wgs, hipStream_t s);
 template <typename R>
-hipError_t launch_adj_step(const AdjArgs<R> &a, int th, hipStream_t s);
+hipError_t launch_adj_step(const AdjArgs<R> &a, int th, int persistent_wgs, hipStream_t s);
+// Resident work-groups per CU of a step kernel (0 if the variant does not exist).
+int step_occupancy(bool adj, int prec, int th, bool persistent);
 
 // Misfit / reduction launchers (misfit.hip).
 // kind: 0 = L2, 1 = W2.  sim/obs/adj/jcum: [nshots][n_rx][T]; mis: [nshots*n_rx] FP64 per trace.

@@ -182,6 +182,16 @@ __device__ __forceinline__ void stag_col(const V4<R> (&w)[8], R (&d)[4])
         d[i] = stag<R>(a[0][i], a[1][i], a[2][i], a[3][i], a[4][i], a[5][i], a[6][i], a[7][i]);
 }
 
+// Work-group barrier ordering LDS only.  The step kernels exchange data
+// through LDS alone, so their barriers need not wait for outstanding global
+// loads (the next tile's prefetch) or stores (results already computed).
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ __forceinline__ bool in_box(int x, int z, int X, int Z)
 {
     return x >= BOX && z >= BOX && x < X - BOX && z < Z - BOX;
@@ -285,61 +295,80 @@ __device__ __forceinline__ void p_quad(const FwdCtx<R> &c, int x, int z, bool ow
     }
 }
 
+// Per-thread register image of everything one tile loads from HBM, so the
+// loads of the next tile can be in flight while the current one computes.
 template <typename R, int TH>
-__global__ void __launch_bounds__(NTHREADS)
-fwd_step_kernel(const FwdArgs<R> a)
-{
+struct FwdLoads {
     using M = EMap<TH>;
-    constexpr int EH = M::EH, AZH = TH + 16, RO = TH / 16;
-    static_assert(TH % 16 == 0, "tile height must be a multiple of 16");
-    __shared__ __align__(16) R sAx[EH][AXW];   // Ax(t-1): rows z0-4.., cols x0-8..
-    __shared__ __align__(16) R sAz[AZH][EW];   // Az(t-1): rows z0-8.., cols x0-4..
-    __shared__ __align__(16) R sP[EH][EW];     // P(t) on the E box: rows z0-4.., cols x0-4..
-
-    const int tiles = a.ntx * a.ntz;
-    const int L = xcd_remap(blockIdx.x, gridDim.x);
-    const int s = L / tiles, tl = L - s * tiles;
-    const int tz = tl / a.ntx, tx = tl - tz * a.ntx;
-    const int x0 = tx * TW, z0 = tz * TH;
-    const int ld = a.ld;
-    const size_t org = (size_t)s * (size_t)a.sstride + (size_t)ZOFF * ld + XOFF;
-    const int tid = threadIdx.x;
-    const int q = tid & 15, g = tid >> 4;
-
-    const size_t porg = (size_t)ZOFF * ld + XOFF;   // origin of the shared (per-model) planes
-    FwdCtx<R> c{a, s, a.X, a.Z, ld, a.tflags[tl], a.Px0 + org, a.Px1 + org, a.P1 + org,
-                a.evt + porg};
-
-    // ---- 1. all global loads of the tile ---------------------------------
+    static constexpr int EH = M::EH, AZH = TH + 16;
     Stage<R, EH, AXW / 4> stx;
     Stage<R, AZH, EW / 4> stz;
-    stx.load(a.Ax0 + org, ld, z0 - 4, x0 - 8, tid);
-    stz.load(a.Az0 + org, ld, z0 - 8, x0 - 4, tid);
-    const R *P0 = a.P0 + org, *cm = a.cm + porg;
-    const int er0 = M::start(g), ern = M::count(g);
-    V4<R> pp[M::RMAX], mm[M::RMAX];
-#pragma unroll
-    for (int j = 0; j < M::RMAX; j++) {
-        const int er = min(er0 + j, EH - 1);
-        const int gi = (z0 - 4 + er) * ld + x0 + 4 * q;
-        pp[j] = ld4(P0 + gi);
-        mm[j] = ld4(cm + gi);
-    }
-    const int sidx = tid - (NTHREADS - M::NSIDE);
-    const bool has_side = sidx >= 0;
-    const int s_er = 4 + (has_side ? sidx % TH : 0);
-    const int s_q = (has_side && sidx >= TH) ? 17 : 0;   // E quad: 0 (left) or 17 (right)
-    V4<R> spp, smm;
+    V4<R> pp[M::RMAX], mm[M::RMAX], spp, smm;
+
+    __device__ __forceinline__ void load(const FwdArgs<R> &a, int x0, int z0, size_t org, int tid)
     {
+        const int ld = a.ld, q = tid & 15, g = tid >> 4;
+        stx.load(a.Ax0 + org, ld, z0 - 4, x0 - 8, tid);
+        stz.load(a.Az0 + org, ld, z0 - 8, x0 - 4, tid);
+        const R *P0 = a.P0 + org, *cm = a.cm + (size_t)ZOFF * ld + XOFF;
+        const int er0 = M::start(g);
+#pragma unroll
+        for (int j = 0; j < M::RMAX; j++) {
+            const int er = min(er0 + j, EH - 1);
+            const int gi = (z0 - 4 + er) * ld + x0 + 4 * q;
+            pp[j] = ld4(P0 + gi);
+            mm[j] = ld4(cm + gi);
+        }
+        const int sidx = tid - (NTHREADS - M::NSIDE);
+        const int s_er = 4 + (sidx >= 0 ? sidx % TH : 0);
+        const int s_q = (sidx >= TH) ? 17 : 0;
         const int gi = (z0 - 4 + s_er) * ld + x0 - 4 + 4 * s_q;
         spp = ld4(P0 + gi);
         smm = ld4(cm + gi);
     }
-    stx.store(sAx, tid);
-    stz.store(sAz, tid);
-    __syncthreads();
+};
 
-    // ---- 2. E phase: P(t) ------------------------------------------------
+// Tile (s, tl) from the logical tile index L.
+struct TileId {
+    int s, tl, x0, z0;
+    size_t org;
+};
+template <int TH>
+__device__ __forceinline__ TileId tile_id(int L, int ntx, int ntz, int ld, long long sstride)
+{
+    TileId t;
+    const int tiles = ntx * ntz;
+    t.s = L / tiles;
+    t.tl = L - t.s * tiles;
+    const int tz = t.tl / ntx, tx = t.tl - tz * ntx;
+    t.x0 = tx * TW;
+    t.z0 = tz * TH;
+    t.org = (size_t)t.s * (size_t)sstride + (size_t)ZOFF * ld + XOFF;
+    return t;
+}
+
+// E phase + owned phase of one tile whose A(t-1) is staged in LDS and whose
+// P(t-1), m quads are in `ld`.  Ends after the owned-phase stores (callers
+// that reuse the LDS must synchronize first).
+template <typename R, int TH>
+__device__ __forceinline__ void fwd_compute(const FwdArgs<R> &a, const TileId &T,
+                                            const FwdLoads<R, TH> &L, int tid, R (*sAx)[AXW],
+                                            R (*sAz)[EW], R (*sP)[EW])
+{
+    using M = EMap<TH>;
+    constexpr int EH = M::EH, AZH = TH + 16, RO = TH / 16;
+    const int ld = a.ld, q = tid & 15, g = tid >> 4;
+    const int x0 = T.x0, z0 = T.z0;
+    const size_t porg = (size_t)ZOFF * ld + XOFF;
+    FwdCtx<R> c{a, T.s, a.X, a.Z, ld, a.tflags[T.tl], a.Px0 + T.org, a.Px1 + T.org,
+                a.P1 + T.org, a.evt + porg};
+    const int er0 = M::start(g), ern = M::count(g);
+    const int sidx = tid - (NTHREADS - M::NSIDE);
+    const bool has_side = sidx >= 0;
+    const int s_er = 4 + (has_side ? sidx % TH : 0);
+    const int s_q = (has_side && sidx >= TH) ? 17 : 0;
+
+    // ---- E phase: P(t) ------------------------------------------------
     {
         const int Q = q + 1;   // central E quads 1..16
         V4<R> w[8];
@@ -353,8 +382,8 @@ fwd_step_kernel(const FwdArgs<R> a)
                 row12<R>(&sAx[er][4 * Q], v);
                 stag_row<R, 0>(v, dAx);
                 stag_col<R>(w, dAz);
-                p_quad<R>(c, x0 + 4 * q, z0 - 4 + er, er >= 4 && er < TH + 4, dAx, dAz, pp[j],
-                          mm[j], p);
+                p_quad<R>(c, x0 + 4 * q, z0 - 4 + er, er >= 4 && er < TH + 4, dAx, dAz, L.pp[j],
+                          L.mm[j], p);
                 st4(&sP[er][4 * Q], pack<R>(p));
             }
             if (j + 1 < M::RMAX) {
@@ -371,13 +400,13 @@ fwd_step_kernel(const FwdArgs<R> a)
 #pragma unroll
             for (int j = 0; j < 8; j++) ws[j] = ld4(&sAz[s_er + j][4 * s_q]);
             stag_col<R>(ws, dAz);
-            p_quad<R>(c, x0 - 4 + 4 * s_q, z0 - 4 + s_er, false, dAx, dAz, spp, smm, p);
+            p_quad<R>(c, x0 - 4 + 4 * s_q, z0 - 4 + s_er, false, dAx, dAz, L.spp, L.smm, p);
             st4(&sP[s_er][4 * s_q], pack<R>(p));
         }
     }
-    __syncthreads();
+    lds_barrier();
 
-    // ---- 3. owned phase: A(t) -------------------------------------------
+    // ---- owned phase: A(t) ---------------------------------------------
     {
         const int x = x0 + 4 * q;
         const int or0 = g * RO;
@@ -386,7 +415,7 @@ fwd_step_kernel(const FwdArgs<R> a)
         V4<R> w[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) w[j] = ld4(&sP[or0 + 1 + j][4 * q + 4]);
-        R *Ax1 = a.Ax1 + org, *Az1 = a.Az1 + org;
+        R *Ax1 = a.Ax1 + T.org, *Az1 = a.Az1 + T.org;
 #pragma unroll
         for (int j = 0; j < RO; j++) {
             const int orr = or0 + j, z = z0 + orr;
@@ -416,6 +445,76 @@ fwd_step_kernel(const FwdArgs<R> a)
             st4(Ax1 + gi, pack<R>(nax));
             st4(Az1 + gi, pack<R>(naz));
         }
+    }
+}
+
+#define FWD_LDS(TH)                                                   \
+    __shared__ __align__(16) R sAx[EMap<TH>::EH][AXW];                \
+    __shared__ __align__(16) R sAz[TH + 16][EW];                      \
+    __shared__ __align__(16) R sP[EMap<TH>::EH][EW]
+
+// One tile per work-group.
+template <typename R, int TH>
+__global__ void __launch_bounds__(NTHREADS)
+fwd_step_kernel(const FwdArgs<R> a)
+{
+    static_assert(TH % 16 == 0, "tile height must be a multiple of 16");
+    FWD_LDS(TH);
+    const int tid = threadIdx.x;
+    const TileId T = tile_id<TH>(xcd_remap(blockIdx.x, gridDim.x), a.ntx, a.ntz, a.ld, a.sstride);
+    FwdLoads<R, TH> L;
+    L.load(a, T.x0, T.z0, T.org, tid);
+    L.stx.store(sAx, tid);
+    L.stz.store(sAz, tid);
+    lds_barrier();
+    fwd_compute<R, TH>(a, T, L, tid, sAx, sAz, sP);
+}
+
+// Range of logical tiles of XCD `xcd` when nb tiles are split over 8 XCDs in
+// contiguous chunks (the same split xcd_remap makes).
+__device__ __forceinline__ void xcd_chunk(int xcd, int nb, int &lo, int &hi)
+{
+    const int q = nb >> 3, r = nb & 7;
+    lo = (xcd < r) ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q;
+    hi = lo + q + (xcd < r ? 1 : 0);
+}
+
+// Persistent work-groups: a grid of a few work-groups per CU walks the tiles
+// of its XCD's chunk with stride (work-groups on that XCD), so the tiles in
+// flight at one time are neighbours sharing halos in L2.  The loads of the
+// next tile are issued right after the current tile is in LDS and overlap
+// its computation.
+template <typename R, int TH>
+__global__ void __launch_bounds__(NTHREADS)
+fwd_step_persistent(const FwdArgs<R> a)
+{
+    FWD_LDS(TH);
+    const int tid = threadIdx.x;
+    const int nb = a.ntx * a.ntz * a.nshots;
+    const int xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
+    const int nk = (gridDim.x - xcd + 7) >> 3;   // work-groups on this XCD
+    int lo, hi;
+    xcd_chunk(xcd, nb, lo, hi);
+    int Lt = lo + k;
+    FwdLoads<R, TH> L;
+    TileId T = tile_id<TH>(min(Lt, nb - 1), a.ntx, a.ntz, a.ld, a.sstride);
+    if (Lt < hi) L.load(a, T.x0, T.z0, T.org, tid);
+    for (; Lt < hi; Lt += nk) {
+        L.stx.store(sAx, tid);
+        L.stz.store(sAz, tid);
+        FwdLoads<R, TH> C;   // this tile's register part
+#pragma unroll
+        for (int j = 0; j < EMap<TH>::RMAX; j++) { C.pp[j] = L.pp[j]; C.mm[j] = L.mm[j]; }
+        C.spp = L.spp;
+        C.smm = L.smm;
+        const TileId Tc = T;
+        lds_barrier();
+        if (Lt + nk < hi) {
+            T = tile_id<TH>(Lt + nk, a.ntx, a.ntz, a.ld, a.sstride);
+            L.load(a, T.x0, T.z0, T.org, tid);
+        }
+        fwd_compute<R, TH>(a, Tc, C, tid, sAx, sAz, sP);
+        lds_barrier();
     }
 }
 
@@ -501,72 +600,73 @@ __device__ __forceinline__ void lu_quad(const AdjCtx<R> &c, int x, int z, bool o
 }
 
 template <typename R, int TH>
-__global__ void __launch_bounds__(NTHREADS)
-adj_step_kernel(const AdjArgs<R> a)
-{
+struct AdjLoads {
     using M = EMap<TH>;
-    constexpr int EH = M::EH, AZH = TH + 16, RO = TH / 16;
-    static_assert(TH % 16 == 0, "tile height must be a multiple of 16");
-    __shared__ __align__(16) R sLAx[EH][AXW];
-    __shared__ __align__(16) R sLAz[AZH][EW];
-    __shared__ __align__(16) R sLP[EH][EW];
-    __shared__ __align__(16) R sUAx[EH][AXW];
-    __shared__ __align__(16) R sUAz[AZH][EW];
-    __shared__ __align__(16) R sU[EH][EW];
-
-    const int tiles = a.ntx * a.ntz;
-    const int L = xcd_remap(blockIdx.x, gridDim.x);
-    const int s = L / tiles, tl = L - s * tiles;
-    const int tz = tl / a.ntx, tx = tl - tz * a.ntx;
-    const int x0 = tx * TW, z0 = tz * TH;
-    const int ld = a.ld;
-    const size_t org = (size_t)s * (size_t)a.sstride + (size_t)ZOFF * ld + XOFF;
-    const int tid = threadIdx.x;
-    const int q = tid & 15, g = tid >> 4;
-
-    const size_t porg = (size_t)ZOFF * ld + XOFF;   // origin of the shared (per-model) planes
-    AdjCtx<R> c{a, s, a.X, a.Z, ld, a.T - 1 - a.t, a.tflags[tl], a.LPx0 + org, a.LPx1 + org,
-                a.LP1 + org, a.U1 + org, a.evt + porg};
-
+    static constexpr int EH = M::EH, AZH = TH + 16, RO = TH / 16;
     Stage<R, EH, AXW / 4> slx, sux;
     Stage<R, AZH, EW / 4> slz, suz;
-    slx.load(a.LAx0 + org, ld, z0 - 4, x0 - 8, tid);
-    sux.load(a.UAx0 + org, ld, z0 - 4, x0 - 8, tid);
-    slz.load(a.LAz0 + org, ld, z0 - 8, x0 - 4, tid);
-    suz.load(a.UAz0 + org, ld, z0 - 8, x0 - 4, tid);
-    const R *LP0 = a.LP0 + org, *U0 = a.U0 + org, *cm = a.cm + porg;
-    const int er0 = M::start(g), ern = M::count(g);
-    V4<R> lp[M::RMAX], up[M::RMAX], mm[M::RMAX];
-#pragma unroll
-    for (int j = 0; j < M::RMAX; j++) {
-        const int er = min(er0 + j, EH - 1);
-        const int gi = (z0 - 4 + er) * ld + x0 + 4 * q;
-        lp[j] = ld4(LP0 + gi);
-        up[j] = ld4(U0 + gi);
-        mm[j] = ld4(cm + gi);
-    }
-    const int sidx = tid - (NTHREADS - M::NSIDE);
-    const bool has_side = sidx >= 0;
-    const int s_er = 4 + (has_side ? sidx % TH : 0);
-    const int s_q = (has_side && sidx >= TH) ? 17 : 0;
-    V4<R> slp, sup, smm;
+    V4<R> lp[M::RMAX], up[M::RMAX], mm[M::RMAX], slp, sup, smm, gr[RO];
+
+    __device__ __forceinline__ void load(const AdjArgs<R> &a, int x0, int z0, size_t org, int tid)
     {
+        const int ld = a.ld, q = tid & 15, g = tid >> 4;
+        slx.load(a.LAx0 + org, ld, z0 - 4, x0 - 8, tid);
+        sux.load(a.UAx0 + org, ld, z0 - 4, x0 - 8, tid);
+        slz.load(a.LAz0 + org, ld, z0 - 8, x0 - 4, tid);
+        suz.load(a.UAz0 + org, ld, z0 - 8, x0 - 4, tid);
+        const R *LP0 = a.LP0 + org, *U0 = a.U0 + org, *cm = a.cm + (size_t)ZOFF * ld + XOFF;
+        const int er0 = M::start(g);
+#pragma unroll
+        for (int j = 0; j < M::RMAX; j++) {
+            const int er = min(er0 + j, EH - 1);
+            const int gi = (z0 - 4 + er) * ld + x0 + 4 * q;
+            lp[j] = ld4(LP0 + gi);
+            up[j] = ld4(U0 + gi);
+            mm[j] = ld4(cm + gi);
+        }
+        const int sidx = tid - (NTHREADS - M::NSIDE);
+        const int s_er = 4 + (sidx >= 0 ? sidx % TH : 0);
+        const int s_q = (sidx >= TH) ? 17 : 0;
         const int gi = (z0 - 4 + s_er) * ld + x0 - 4 + 4 * s_q;
         slp = ld4(LP0 + gi);
         sup = ld4(U0 + gi);
         smm = ld4(cm + gi);
-    }
-    // gradient of the owned quads, read early
-    R *G = a.grad + org;
-    const int or0 = g * RO;
-    V4<R> gr[RO];
+        const R *G = a.grad + org;
+        const int or0 = g * RO;
 #pragma unroll
-    for (int j = 0; j < RO; j++) gr[j] = ld4(G + (z0 + or0 + j) * ld + x0 + 4 * q);
-    slx.store(sLAx, tid);
-    sux.store(sUAx, tid);
-    slz.store(sLAz, tid);
-    suz.store(sUAz, tid);
-    __syncthreads();
+        for (int j = 0; j < RO; j++) gr[j] = ld4(G + (z0 + or0 + j) * ld + x0 + 4 * q);
+    }
+    template <int W1, int W2>
+    __device__ __forceinline__ void store(R (*lax)[W1], R (*uax)[W1], R (*laz)[W2], R (*uaz)[W2],
+                                          int tid) const
+    {
+        slx.store(lax, tid);
+        sux.store(uax, tid);
+        slz.store(laz, tid);
+        suz.store(uaz, tid);
+    }
+};
+
+template <typename R, int TH>
+__device__ __forceinline__ void adj_compute(const AdjArgs<R> &a, const TileId &T,
+                                            const AdjLoads<R, TH> &L, int tid, R (*sLAx)[AXW],
+                                            R (*sLAz)[EW], R (*sLP)[EW], R (*sUAx)[AXW],
+                                            R (*sUAz)[EW], R (*sU)[EW])
+{
+    using M = EMap<TH>;
+    constexpr int EH = M::EH, AZH = TH + 16, RO = TH / 16;
+    (void)EH;
+    const int ld = a.ld, q = tid & 15, g = tid >> 4;
+    const int x0 = T.x0, z0 = T.z0;
+    const size_t porg = (size_t)ZOFF * ld + XOFF;
+    AdjCtx<R> c{a, T.s, a.X, a.Z, ld, a.T - 1 - a.t, a.tflags[T.tl], a.LPx0 + T.org,
+                a.LPx1 + T.org, a.LP1 + T.org, a.U1 + T.org, a.evt + porg};
+    const int er0 = M::start(g), ern = M::count(g);
+    const int sidx = tid - (NTHREADS - M::NSIDE);
+    const bool has_side = sidx >= 0;
+    const int s_er = 4 + (has_side ? sidx % TH : 0);
+    const int s_q = (has_side && sidx >= TH) ? 17 : 0;
+    const int or0 = g * RO;
 
     {
         const int Q = q + 1;
@@ -588,7 +688,7 @@ adj_step_kernel(const AdjArgs<R> a)
                 stag_col<R>(wl, dLz);
                 stag_col<R>(wu, dUz);
                 lu_quad<R>(c, x0 + 4 * q, z0 - 4 + er, er >= 4 && er < TH + 4, dLx, dLz, dUx, dUz,
-                           lp[j], up[j], mm[j], lo, uo);
+                           L.lp[j], L.up[j], L.mm[j], lo, uo);
                 st4(&sLP[er][4 * Q], pack<R>(lo));
                 st4(&sU[er][4 * Q], pack<R>(uo));
             }
@@ -612,13 +712,13 @@ adj_step_kernel(const AdjArgs<R> a)
 #pragma unroll
             for (int j = 0; j < 8; j++) ws[j] = ld4(&sUAz[s_er + j][4 * s_q]);
             stag_col<R>(ws, dUz);
-            lu_quad<R>(c, x0 - 4 + 4 * s_q, z0 - 4 + s_er, false, dLx, dLz, dUx, dUz, slp, sup,
-                       smm, lo, uo);
+            lu_quad<R>(c, x0 - 4 + 4 * s_q, z0 - 4 + s_er, false, dLx, dLz, dUx, dUz, L.slp, L.sup,
+                       L.smm, lo, uo);
             st4(&sLP[s_er][4 * s_q], pack<R>(lo));
             st4(&sU[s_er][4 * s_q], pack<R>(uo));
         }
     }
-    __syncthreads();
+    lds_barrier();
 
     {
         const int x = x0 + 4 * q;
@@ -630,7 +730,8 @@ adj_step_kernel(const AdjArgs<R> a)
         for (int j = 0; j < 8; j++) wl[j] = ld4(&sLP[or0 + 1 + j][4 * q + 4]);
 #pragma unroll
         for (int j = 0; j < 9; j++) wu[j] = ld4(&sU[or0 + j][4 * q + 4]);
-        R *LAx1 = a.LAx1 + org, *LAz1 = a.LAz1 + org, *UAx1 = a.UAx1 + org, *UAz1 = a.UAz1 + org;
+        R *LAx1 = a.LAx1 + T.org, *LAz1 = a.LAz1 + T.org, *UAx1 = a.UAx1 + T.org,
+          *UAz1 = a.UAz1 + T.org, *G = a.grad + T.org;
         constexpr R inv5040 = R(1) / R(5040);
 #pragma unroll
         for (int j = 0; j < RO; j++) {
@@ -664,7 +765,7 @@ adj_step_kernel(const AdjArgs<R> a)
             unpack<R>(ld4(&sLAz[orr + 8][4 * q + 4]), laz);
             unpack<R>(ld4(&sUAx[orr + 4][4 * q + 8]), uax);
             unpack<R>(ld4(&sUAz[orr + 8][4 * q + 4]), uaz);
-            unpack<R>(gr[j], gg);
+            unpack<R>(L.gr[j], gg);
             if (j + 1 < RO) {
 #pragma unroll
                 for (int k = 0; k < 7; k++) wl[k] = wl[k + 1];
@@ -698,57 +799,163 @@ adj_step_kernel(const AdjArgs<R> a)
     }
 }
 
+#define ADJ_LDS(TH)                                                    \
+    __shared__ __align__(16) R sLAx[EMap<TH>::EH][AXW];                \
+    __shared__ __align__(16) R sLAz[TH + 16][EW];                      \
+    __shared__ __align__(16) R sLP[EMap<TH>::EH][EW];                  \
+    __shared__ __align__(16) R sUAx[EMap<TH>::EH][AXW];                \
+    __shared__ __align__(16) R sUAz[TH + 16][EW];                      \
+    __shared__ __align__(16) R sU[EMap<TH>::EH][EW]
+
 template <typename R, int TH>
-static hipError_t fwd_launch_th(const FwdArgs<R> &a, hipStream_t st)
+__global__ void __launch_bounds__(NTHREADS)
+adj_step_kernel(const AdjArgs<R> a)
+{
+    static_assert(TH % 16 == 0, "tile height must be a multiple of 16");
+    ADJ_LDS(TH);
+    const int tid = threadIdx.x;
+    const TileId T = tile_id<TH>(xcd_remap(blockIdx.x, gridDim.x), a.ntx, a.ntz, a.ld, a.sstride);
+    AdjLoads<R, TH> L;
+    L.load(a, T.x0, T.z0, T.org, tid);
+    L.store(sLAx, sUAx, sLAz, sUAz, tid);
+    lds_barrier();
+    adj_compute<R, TH>(a, T, L, tid, sLAx, sLAz, sLP, sUAx, sUAz, sU);
+}
+
+template <typename R, int TH>
+__global__ void __launch_bounds__(NTHREADS)
+adj_step_persistent(const AdjArgs<R> a)
+{
+    ADJ_LDS(TH);
+    const int tid = threadIdx.x;
+    const int nb = a.ntx * a.ntz * a.nshots;
+    const int xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
+    const int nk = (gridDim.x - xcd + 7) >> 3;
+    int lo, hi;
+    xcd_chunk(xcd, nb, lo, hi);
+    int Lt = lo + k;
+    AdjLoads<R, TH> L;
+    TileId T = tile_id<TH>(min(Lt, nb - 1), a.ntx, a.ntz, a.ld, a.sstride);
+    if (Lt < hi) L.load(a, T.x0, T.z0, T.org, tid);
+    for (; Lt < hi; Lt += nk) {
+        L.store(sLAx, sUAx, sLAz, sUAz, tid);
+        AdjLoads<R, TH> C;
+#pragma unroll
+        for (int j = 0; j < EMap<TH>::RMAX; j++) { C.lp[j] = L.lp[j]; C.up[j] = L.up[j]; C.mm[j] = L.mm[j]; }
+#pragma unroll
+        for (int j = 0; j < TH / 16; j++) C.gr[j] = L.gr[j];
+        C.slp = L.slp;
+        C.sup = L.sup;
+        C.smm = L.smm;
+        const TileId Tc = T;
+        lds_barrier();
+        if (Lt + nk < hi) {
+            T = tile_id<TH>(Lt + nk, a.ntx, a.ntz, a.ld, a.sstride);
+            L.load(a, T.x0, T.z0, T.org, tid);
+        }
+        adj_compute<R, TH>(a, Tc, C, tid, sLAx, sLAz, sLP, sUAx, sUAz, sU);
+        lds_barrier();
+    }
+}
+
+template <typename R, int TH>
+static hipError_t fwd_launch_th(const FwdArgs<R> &a, int persistent_wgs, hipStream_t st)
 {
     const int nb = a.ntx * a.ntz * a.nshots;
-    hipLaunchKernelGGL((fwd_step_kernel<R, TH>), dim3(nb), dim3(NTHREADS), 0, st, a);
+    if (persistent_wgs > 0) {
+        hipLaunchKernelGGL((fwd_step_persistent<R, TH>), dim3(min(persistent_wgs, nb)),
+                           dim3(NTHREADS), 0, st, a);
+    } else {
+        hipLaunchKernelGGL((fwd_step_kernel<R, TH>), dim3(nb), dim3(NTHREADS), 0, st, a);
+    }
     return hipGetLastError();
 }
 
 template <typename R, int TH>
-static hipError_t adj_launch_th(const AdjArgs<R> &a, hipStream_t st)
+static hipError_t adj_launch_th(const AdjArgs<R> &a, int persistent_wgs, hipStream_t st)
 {
     const int nb = a.ntx * a.ntz * a.nshots;
-    hipLaunchKernelGGL((adj_step_kernel<R, TH>), dim3(nb), dim3(NTHREADS), 0, st, a);
+    if (persistent_wgs > 0) {
+        hipLaunchKernelGGL((adj_step_persistent<R, TH>), dim3(min(persistent_wgs, nb)),
+                           dim3(NTHREADS), 0, st, a);
+    } else {
+        hipLaunchKernelGGL((adj_step_kernel<R, TH>), dim3(nb), dim3(NTHREADS), 0, st, a);
+    }
     return hipGetLastError();
 }
 
+template <typename R, int TH>
+static int occ_fwd(bool persistent)
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &n, persistent ? (const void *)fwd_step_persistent<R, TH> : (const void *)fwd_step_kernel<R, TH>,
+            NTHREADS, 0) != hipSuccess)
+        return 0;
+    return n;
+}
+template <typename R, int TH>
+static int occ_adj(bool persistent)
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &n, persistent ? (const void *)adj_step_persistent<R, TH> : (const void *)adj_step_kernel<R, TH>,
+            NTHREADS, 0) != hipSuccess)
+        return 0;
+    return n;
+}
+
 template <>
-hipError_t launch_fwd_step<float>(const FwdArgs<float> &a, int th, hipStream_t s)
+hipError_t launch_fwd_step<float>(const FwdArgs<float> &a, int th, int pw, hipStream_t s)
 {
     switch (th) {
-    case 16: return fwd_launch_th<float, 16>(a, s);
-    case 32: return fwd_launch_th<float, 32>(a, s);
-    case 64: return fwd_launch_th<float, 64>(a, s);
+    case 16: return fwd_launch_th<float, 16>(a, pw, s);
+    case 32: return fwd_launch_th<float, 32>(a, pw, s);
+    case 64: return fwd_launch_th<float, 64>(a, pw, s);
     default: return hipErrorInvalidValue;
     }
 }
 template <>
-hipError_t launch_fwd_step<double>(const FwdArgs<double> &a, int th, hipStream_t s)
+hipError_t launch_fwd_step<double>(const FwdArgs<double> &a, int th, int pw, hipStream_t s)
 {
     switch (th) {
-    case 16: return fwd_launch_th<double, 16>(a, s);
-    case 32: return fwd_launch_th<double, 32>(a, s);
+    case 16: return fwd_launch_th<double, 16>(a, pw, s);
+    case 32: return fwd_launch_th<double, 32>(a, pw, s);
     default: return hipErrorInvalidValue;
     }
 }
 template <>
-hipError_t launch_adj_step<float>(const AdjArgs<float> &a, int th, hipStream_t s)
+hipError_t launch_adj_step<float>(const AdjArgs<float> &a, int th, int pw, hipStream_t s)
 {
     switch (th) {
-    case 16: return adj_launch_th<float, 16>(a, s);
-    case 32: return adj_launch_th<float, 32>(a, s);
-    case 64: return adj_launch_th<float, 64>(a, s);
+    case 16: return adj_launch_th<float, 16>(a, pw, s);
+    case 32: return adj_launch_th<float, 32>(a, pw, s);
+    case 64: return adj_launch_th<float, 64>(a, pw, s);
     default: return hipErrorInvalidValue;
     }
 }
 template <>
-hipError_t launch_adj_step<double>(const AdjArgs<double> &a, int th, hipStream_t s)
+hipError_t launch_adj_step<double>(const AdjArgs<double> &a, int th, int pw, hipStream_t s)
 {
     switch (th) {
-    case 16: return adj_launch_th<double, 16>(a, s);
+    case 16: return adj_launch_th<double, 16>(a, pw, s);
     default: return hipErrorInvalidValue;
+    }
+}
+
+int step_occupancy(bool adj, int prec, int th, bool persistent)
+{
+    if (prec == 8) {
+        if (!adj && th == 16) return occ_fwd<double, 16>(persistent);
+        if (!adj && th == 32) return occ_fwd<double, 32>(persistent);
+        if (adj && th == 16) return occ_adj<double, 16>(persistent);
+        return 0;
+    }
+    switch (th) {
+    case 16: return adj ? occ_adj<float, 16>(persistent) : occ_fwd<float, 16>(persistent);
+    case 32: return adj ? occ_adj<float, 32>(persistent) : occ_fwd<float, 32>(persistent);
+    case 64: return adj ? occ_adj<float, 64>(persistent) : occ_fwd<float, 64>(persistent);
+    default: return 0;
     }
 }
 

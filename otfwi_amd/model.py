@@ -84,6 +84,16 @@ class PMLModel:
     def set_tuning(self, th_fwd=0, th_adj=0, max_batch=0):
         check(self._lib.otfwi_set_tuning(self._h, th_fwd, th_adj, max_batch))
 
+    def set_launch(self, persistent_fwd=False, persistent_adj=False):
+        """Persistent (prefetching) work-groups for the forward / adjoint steps."""
+        check(self._lib.otfwi_set_launch(self._h, int(bool(persistent_fwd)),
+                                         int(bool(persistent_adj))))
+
+    def get_launch(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        check(self._lib.otfwi_get_launch(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return bool(a.value), bool(b.value)
+
     def get_tuning(self):
         a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         check(self._lib.otfwi_get_tuning(self._h, ctypes.byref(a), ctypes.byref(b),

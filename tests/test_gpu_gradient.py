@@ -151,3 +151,19 @@ def test_all_zero_observed_trace_is_reported(gpu):
         m.gradient([0], misfit="w2")
     g, mis = m.gradient([0], misfit="l2")     # L2 is well defined there
     assert np.isfinite(g).all()
+
+
+@pytest.mark.parametrize("thf,tha", [(16, 16), (32, 32)])
+def test_persistent_launch_bitwise(gpu, thf, tha):
+    """Persistent work-groups compute the same cells with the same arithmetic."""
+    p = small_problem(X=330, Z=160, T=300, pml=10, seed=17, n_src=3, n_rx=5)
+    shots = [0, 1, 2]
+    obs = observed(p, shots)
+    out = []
+    for pers in (False, True):
+        m = make_model(p, cq="cq_bg", th_fwd=thf, th_adj=tha)
+        m.set_launch(pers, pers)
+        m.set_observed(obs)
+        out.append(m.gradient(shots, misfit="l2", want_rec=True))
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)

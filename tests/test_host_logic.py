@@ -1,0 +1,83 @@
+"""Host-side logic of the drop-in wrapper (no GPU): PML profile, point
+ordering, boundary ring, and the C-ABI export set."""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_pml_profile_closed_form():
+    # cuda_interface_pml.py:116-132 with R = 1e-5, Vmax = 0.5, L = 20:
+    # d0 = 3*0.5*ln(1e5)/(2*20^3); edge value d0*L^2 = 0.4317...
+    d_x, d_z = O.calc_pml(64, 48, 20)
+    edge = 3 * 0.5 * np.log(1e5) / (2 * 20)
+    assert abs(d_x[0, 0] - edge) < 1e-12 and abs(edge - 0.4317) < 1e-4
+    assert d_x[5, 0] == d_x[0, 0] and d_x[0, 19] == 0 and d_x[0, 63] == d_x[0, 0]
+    assert d_z[0, 7] == d_z[0, 0] and abs(d_z[0, 0] - edge) < 1e-12
+    assert d_z[19, 0] == 0 and d_z[47, 3] == d_z[0, 0]
+    # the profile is d0 * x^2 on linspace(0, L, L)
+    x = np.linspace(0, 20, 20)
+    assert np.allclose(d_x[0, 44:], 3 * 0.5 * np.log(1e5) / (2 * 20 ** 3) * x ** 2, rtol=0, atol=0)
+
+
+def test_wrapper_profile_equals_oracle():
+    from otfwi_amd.cuda_interface_pml import Cuda_interface_PML
+
+    class Probe(Cuda_interface_PML):
+        def __init__(self, X, Z, L, R):   # host part only, no library call
+            self.X, self.Z, self.PML_size, self.R, self.Vmax = X, Z, L, R, 0.5
+
+    for R in (1e-5, 1e-3):
+        a = Probe(70, 50, 12, R)._calcPML()
+        b = O.calc_pml(70, 50, 12, R=R)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_ring_size_config1():
+    # 256^2, PML 20, prec_deriv 4 (margin 5): 4 bands minus corners = 4220 points
+    src = np.zeros((256, 256), bool)
+    src[20, 128] = True
+    mask = O.revert_mask(256, 256, 20, 4, src)
+    assert mask.sum() == 4220          # the source cell lies in the top band
+    x, z = O.point_order(mask)
+    assert np.all(np.diff(z * 256 + x) > 0)   # np.where row-major order
+
+
+def test_point_order_row_major():
+    m = np.zeros((5, 6), bool)
+    m[3, 1] = m[1, 4] = m[1, 2] = True
+    x, z = O.point_order(m)
+    assert list(zip(z, x)) == [(1, 2), (1, 4), (3, 1)]
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "otfwi_pml.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    txt = "\n".join(line for line in txt.split("\n") if not line.strip().startswith("typedef"))
+    names = re.findall(r"^\s*(?:const\s+)?[a-z_]+\s*\*?\s*([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt, re.M)
+    return sorted(set(n for n in names if n not in ("typedef",)))
+
+
+def test_library_exports_every_declared_symbol():
+    from otfwi_amd import _lib
+
+    names = header_functions()
+    assert "otfwi_gradient" in names and "cuda_grad_ext" in names and len(names) >= 25
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    for n in names:
+        assert hasattr(lib, n), n
+    assert sorted(_lib.EXPORTED) == names
+
+
+def test_reference_abi_symbols_present():
+    from otfwi_amd import _lib
+
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    for n in ("init_memory_sim", "init_memory_reg", "setCquad", "cuda_simulate", "cuda_grad_ext",
+              "def_python_adj"):
+        assert hasattr(lib, n)
