@@ -60,11 +60,12 @@ constexpr double K1 = 1.196289e+00, K2 = -7.975260e-02, K3 = 9.570313e-03, K4 = 
 // sum_k c_k (v[3+k] - v[4-k]) for taps v[j] = f(x0 + j): the backward
 // operator D- at x = x0+4 and the forward operator D+ at x = x0+3.
 //
-// FP32: each coefficient is carried as hi + lo FP32 parts.  Rounding the
-// coefficients to FP32 alone changes the stencil systematically, which moves
-// the adjoint gradient by ~1e-4 relative over a 1500-step run; with the lo
-// parts folded in first and the hi terms accumulated smallest first, the
-// FP32 derivative is as accurate as the reference's FP64-evaluated one.
+// FP32: the coefficients are the FP32 roundings of the reference's literals
+// plus the low part of c1 folded in first, and the terms are accumulated
+// smallest first.  Rounding the coefficients to FP32 alone changes the stencil
+// systematically (the C1 adjoint gradient moves by ~1.7e-4 relative); the
+// c1 low part restores it to the reference's own FP32 level (~2e-5 from FP64);
+// the low parts of c2..c4 lie below FP32 resolution of the derivative.
 template <typename R>
 __device__ __forceinline__ R stag(R v0, R v1, R v2, R v3, R v4, R v5, R v6, R v7);
 
@@ -73,13 +74,9 @@ __device__ __forceinline__ float stag<float>(float v0, float v1, float v2, float
                                              float v5, float v6, float v7)
 {
     constexpr float h1 = (float)K1, h2 = (float)K2, h3 = (float)K3, h4 = (float)K4;
-    constexpr float l1 = (float)(K1 - (double)h1), l2 = (float)(K2 - (double)h2);
-    constexpr float l3 = (float)(K3 - (double)h3), l4 = (float)(K4 - (double)h4);
+    constexpr float l1 = (float)(K1 - (double)h1);
     const float d1 = v4 - v3, d2 = v5 - v2, d3 = v6 - v1, d4 = v7 - v0;
     float s = l1 * d1;
-    s = fmaf(l2, d2, s);
-    s = fmaf(l3, d3, s);
-    s = fmaf(l4, d4, s);
     s = fmaf(h4, d4, s);
     s = fmaf(h3, d3, s);
     s = fmaf(h2, d2, s);
@@ -129,13 +126,15 @@ struct Stage {
     static constexpr int N = ROWS * NQ;
     static constexpr int K = (N + NTHREADS - 1) / NTHREADS;
     V4<R> r[K];
-    __device__ __forceinline__ void load(const R *org, int ld, int z_first, int x_first, int tid)
+    // base: the padded plane; (z_first, x_first) in cell coordinates
+    __device__ __forceinline__ void load(const R *base, int ld, int z_first, int x_first, int tid)
     {
+        const unsigned o0 = (unsigned)((z_first + ZOFF) * ld + x_first + XOFF);
 #pragma unroll
         for (int k = 0; k < K; k++) {
             const int i = min(tid + k * NTHREADS, N - 1);
             const int row = i / NQ, q = i - row * NQ;
-            r[k] = ld4(org + (z_first + row) * ld + x_first + 4 * q);
+            r[k] = ld4(base + (o0 + (unsigned)(row * ld + 4 * q)));
         }
     }
     template <int W>
@@ -239,9 +238,9 @@ __device__ __forceinline__ void p_quad(const FwdCtx<R> &c, int x, int z, bool ow
     unpack<R>(pp4, pp);
     unpack<R>(mm4, mm);
     R kz = R(1);
-    const int gi = z * c.ld + x;
+    const unsigned gi = (unsigned)((z + ZOFF) * c.ld + x + XOFF);
     if (c.fl & TF_STRIP) {
-        unpack<R>(ld4(a.kx + XOFF + x), kx);
+        unpack<R>(ld4(a.kx + (unsigned)(XOFF + x)), kx);
         kz = a.kz[ZOFF + z];
         unpack<R>(ld4(c.psi0 + gi), psi);
     }
@@ -305,24 +304,25 @@ struct FwdLoads {
     Stage<R, AZH, EW / 4> stz;
     V4<R> pp[M::RMAX], mm[M::RMAX], spp, smm;
 
-    __device__ __forceinline__ void load(const FwdArgs<R> &a, int x0, int z0, size_t org, int tid)
+    // sb: element offset of the shot's planes
+    __device__ __forceinline__ void load(const FwdArgs<R> &a, int x0, int z0, size_t sb, int tid)
     {
         const int ld = a.ld, q = tid & 15, g = tid >> 4;
-        stx.load(a.Ax0 + org, ld, z0 - 4, x0 - 8, tid);
-        stz.load(a.Az0 + org, ld, z0 - 8, x0 - 4, tid);
-        const R *P0 = a.P0 + org, *cm = a.cm + (size_t)ZOFF * ld + XOFF;
+        stx.load(a.Ax0 + sb, ld, z0 - 4, x0 - 8, tid);
+        stz.load(a.Az0 + sb, ld, z0 - 8, x0 - 4, tid);
+        const R *P0 = a.P0 + sb, *cm = a.cm;
         const int er0 = M::start(g);
 #pragma unroll
         for (int j = 0; j < M::RMAX; j++) {
             const int er = min(er0 + j, EH - 1);
-            const int gi = (z0 - 4 + er) * ld + x0 + 4 * q;
+            const unsigned gi = (unsigned)((z0 - 4 + er + ZOFF) * ld + x0 + 4 * q + XOFF);
             pp[j] = ld4(P0 + gi);
             mm[j] = ld4(cm + gi);
         }
         const int sidx = tid - (NTHREADS - M::NSIDE);
         const int s_er = 4 + (sidx >= 0 ? sidx % TH : 0);
         const int s_q = (sidx >= TH) ? 17 : 0;
-        const int gi = (z0 - 4 + s_er) * ld + x0 - 4 + 4 * s_q;
+        const unsigned gi = (unsigned)((z0 - 4 + s_er + ZOFF) * ld + x0 - 4 + 4 * s_q + XOFF);
         spp = ld4(P0 + gi);
         smm = ld4(cm + gi);
     }
@@ -331,7 +331,7 @@ struct FwdLoads {
 // Tile (s, tl) from the logical tile index L.
 struct TileId {
     int s, tl, x0, z0;
-    size_t org;
+    size_t sb;   // element offset of the shot's planes
 };
 template <int TH>
 __device__ __forceinline__ TileId tile_id(int L, int ntx, int ntz, int ld, long long sstride)
@@ -343,7 +343,8 @@ __device__ __forceinline__ TileId tile_id(int L, int ntx, int ntz, int ld, long 
     const int tz = t.tl / ntx, tx = t.tl - tz * ntx;
     t.x0 = tx * TW;
     t.z0 = tz * TH;
-    t.org = (size_t)t.s * (size_t)sstride + (size_t)ZOFF * ld + XOFF;
+    t.sb = (size_t)t.s * (size_t)sstride;
+    (void)ld;
     return t;
 }
 
@@ -359,9 +360,8 @@ __device__ __forceinline__ void fwd_compute(const FwdArgs<R> &a, const TileId &T
     constexpr int EH = M::EH, AZH = TH + 16, RO = TH / 16;
     const int ld = a.ld, q = tid & 15, g = tid >> 4;
     const int x0 = T.x0, z0 = T.z0;
-    const size_t porg = (size_t)ZOFF * ld + XOFF;
-    FwdCtx<R> c{a, T.s, a.X, a.Z, ld, a.tflags[T.tl], a.Px0 + T.org, a.Px1 + T.org,
-                a.P1 + T.org, a.evt + porg};
+    FwdCtx<R> c{a, T.s, a.X, a.Z, ld, a.tflags[T.tl], a.Px0 + T.sb, a.Px1 + T.sb, a.P1 + T.sb,
+                a.evt};
     const int er0 = M::start(g), ern = M::count(g);
     const int sidx = tid - (NTHREADS - M::NSIDE);
     const bool has_side = sidx >= 0;
@@ -411,11 +411,11 @@ __device__ __forceinline__ void fwd_compute(const FwdArgs<R> &a, const TileId &T
         const int x = x0 + 4 * q;
         const int or0 = g * RO;
         R kx[4] = {R(1), R(1), R(1), R(1)};
-        if (c.fl & TF_STRIP) unpack<R>(ld4(a.kx + XOFF + x), kx);
+        if (c.fl & TF_STRIP) unpack<R>(ld4(a.kx + (unsigned)(XOFF + x)), kx);
         V4<R> w[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) w[j] = ld4(&sP[or0 + 1 + j][4 * q + 4]);
-        R *Ax1 = a.Ax1 + T.org, *Az1 = a.Az1 + T.org;
+        R *Ax1 = a.Ax1 + T.sb, *Az1 = a.Az1 + T.sb;
 #pragma unroll
         for (int j = 0; j < RO; j++) {
             const int orr = or0 + j, z = z0 + orr;
@@ -441,7 +441,7 @@ __device__ __forceinline__ void fwd_compute(const FwdArgs<R> &a, const TileId &T
                 for (int i = 0; i < 4; i++)
                     if (!in_box(x + i, z, c.X, c.Z)) { nax[i] = R(0); naz[i] = R(0); }
             }
-            const int gi = z * ld + x;
+            const unsigned gi = (unsigned)((z + ZOFF) * ld + x + XOFF);
             st4(Ax1 + gi, pack<R>(nax));
             st4(Az1 + gi, pack<R>(naz));
         }
@@ -463,7 +463,7 @@ fwd_step_kernel(const FwdArgs<R> a)
     const int tid = threadIdx.x;
     const TileId T = tile_id<TH>(xcd_remap(blockIdx.x, gridDim.x), a.ntx, a.ntz, a.ld, a.sstride);
     FwdLoads<R, TH> L;
-    L.load(a, T.x0, T.z0, T.org, tid);
+    L.load(a, T.x0, T.z0, T.sb, tid);
     L.stx.store(sAx, tid);
     L.stz.store(sAz, tid);
     lds_barrier();
@@ -498,7 +498,7 @@ fwd_step_persistent(const FwdArgs<R> a)
     int Lt = lo + k;
     FwdLoads<R, TH> L;
     TileId T = tile_id<TH>(min(Lt, nb - 1), a.ntx, a.ntz, a.ld, a.sstride);
-    if (Lt < hi) L.load(a, T.x0, T.z0, T.org, tid);
+    if (Lt < hi) L.load(a, T.x0, T.z0, T.sb, tid);
     for (; Lt < hi; Lt += nk) {
         L.stx.store(sAx, tid);
         L.stz.store(sAz, tid);
@@ -511,7 +511,7 @@ fwd_step_persistent(const FwdArgs<R> a)
         lds_barrier();
         if (Lt + nk < hi) {
             T = tile_id<TH>(Lt + nk, a.ntx, a.ntz, a.ld, a.sstride);
-            L.load(a, T.x0, T.z0, T.org, tid);
+            L.load(a, T.x0, T.z0, T.sb, tid);
         }
         fwd_compute<R, TH>(a, Tc, C, tid, sAx, sAz, sP);
         lds_barrier();
@@ -549,9 +549,9 @@ __device__ __forceinline__ void lu_quad(const AdjCtx<R> &c, int x, int z, bool o
     unpack<R>(up4, upp);
     unpack<R>(mm4, mm);
     R kz = R(1);
-    const int gi = z * c.ld + x;
+    const unsigned gi = (unsigned)((z + ZOFF) * c.ld + x + XOFF);
     if (c.fl & TF_STRIP) {
-        unpack<R>(ld4(a.kx + XOFF + x), kx);
+        unpack<R>(ld4(a.kx + (unsigned)(XOFF + x)), kx);
         kz = a.kz[ZOFF + z];
         unpack<R>(ld4(c.psi0 + gi), psi);
     }
@@ -607,19 +607,19 @@ struct AdjLoads {
     Stage<R, AZH, EW / 4> slz, suz;
     V4<R> lp[M::RMAX], up[M::RMAX], mm[M::RMAX], slp, sup, smm, gr[RO];
 
-    __device__ __forceinline__ void load(const AdjArgs<R> &a, int x0, int z0, size_t org, int tid)
+    __device__ __forceinline__ void load(const AdjArgs<R> &a, int x0, int z0, size_t sb, int tid)
     {
         const int ld = a.ld, q = tid & 15, g = tid >> 4;
-        slx.load(a.LAx0 + org, ld, z0 - 4, x0 - 8, tid);
-        sux.load(a.UAx0 + org, ld, z0 - 4, x0 - 8, tid);
-        slz.load(a.LAz0 + org, ld, z0 - 8, x0 - 4, tid);
-        suz.load(a.UAz0 + org, ld, z0 - 8, x0 - 4, tid);
-        const R *LP0 = a.LP0 + org, *U0 = a.U0 + org, *cm = a.cm + (size_t)ZOFF * ld + XOFF;
+        slx.load(a.LAx0 + sb, ld, z0 - 4, x0 - 8, tid);
+        sux.load(a.UAx0 + sb, ld, z0 - 4, x0 - 8, tid);
+        slz.load(a.LAz0 + sb, ld, z0 - 8, x0 - 4, tid);
+        suz.load(a.UAz0 + sb, ld, z0 - 8, x0 - 4, tid);
+        const R *LP0 = a.LP0 + sb, *U0 = a.U0 + sb, *cm = a.cm;
         const int er0 = M::start(g);
 #pragma unroll
         for (int j = 0; j < M::RMAX; j++) {
             const int er = min(er0 + j, EH - 1);
-            const int gi = (z0 - 4 + er) * ld + x0 + 4 * q;
+            const unsigned gi = (unsigned)((z0 - 4 + er + ZOFF) * ld + x0 + 4 * q + XOFF);
             lp[j] = ld4(LP0 + gi);
             up[j] = ld4(U0 + gi);
             mm[j] = ld4(cm + gi);
@@ -627,14 +627,15 @@ struct AdjLoads {
         const int sidx = tid - (NTHREADS - M::NSIDE);
         const int s_er = 4 + (sidx >= 0 ? sidx % TH : 0);
         const int s_q = (sidx >= TH) ? 17 : 0;
-        const int gi = (z0 - 4 + s_er) * ld + x0 - 4 + 4 * s_q;
+        const unsigned gi = (unsigned)((z0 - 4 + s_er + ZOFF) * ld + x0 - 4 + 4 * s_q + XOFF);
         slp = ld4(LP0 + gi);
         sup = ld4(U0 + gi);
         smm = ld4(cm + gi);
-        const R *G = a.grad + org;
+        const R *G = a.grad + sb;
         const int or0 = g * RO;
 #pragma unroll
-        for (int j = 0; j < RO; j++) gr[j] = ld4(G + (z0 + or0 + j) * ld + x0 + 4 * q);
+        for (int j = 0; j < RO; j++)
+            gr[j] = ld4(G + (unsigned)((z0 + or0 + j + ZOFF) * ld + x0 + 4 * q + XOFF));
     }
     template <int W1, int W2>
     __device__ __forceinline__ void store(R (*lax)[W1], R (*uax)[W1], R (*laz)[W2], R (*uaz)[W2],
@@ -658,9 +659,8 @@ __device__ __forceinline__ void adj_compute(const AdjArgs<R> &a, const TileId &T
     (void)EH;
     const int ld = a.ld, q = tid & 15, g = tid >> 4;
     const int x0 = T.x0, z0 = T.z0;
-    const size_t porg = (size_t)ZOFF * ld + XOFF;
-    AdjCtx<R> c{a, T.s, a.X, a.Z, ld, a.T - 1 - a.t, a.tflags[T.tl], a.LPx0 + T.org,
-                a.LPx1 + T.org, a.LP1 + T.org, a.U1 + T.org, a.evt + porg};
+    AdjCtx<R> c{a, T.s, a.X, a.Z, ld, a.T - 1 - a.t, a.tflags[T.tl], a.LPx0 + T.sb,
+                a.LPx1 + T.sb, a.LP1 + T.sb, a.U1 + T.sb, a.evt};
     const int er0 = M::start(g), ern = M::count(g);
     const int sidx = tid - (NTHREADS - M::NSIDE);
     const bool has_side = sidx >= 0;
@@ -723,15 +723,15 @@ __device__ __forceinline__ void adj_compute(const AdjArgs<R> &a, const TileId &T
     {
         const int x = x0 + 4 * q;
         R kx[4] = {R(1), R(1), R(1), R(1)};
-        if (c.fl & TF_STRIP) unpack<R>(ld4(a.kx + XOFF + x), kx);
+        if (c.fl & TF_STRIP) unpack<R>(ld4(a.kx + (unsigned)(XOFF + x)), kx);
         // lambda window: rows orr+1..orr+8; U window: rows orr..orr+8
         V4<R> wl[8], wu[9];
 #pragma unroll
         for (int j = 0; j < 8; j++) wl[j] = ld4(&sLP[or0 + 1 + j][4 * q + 4]);
 #pragma unroll
         for (int j = 0; j < 9; j++) wu[j] = ld4(&sU[or0 + j][4 * q + 4]);
-        R *LAx1 = a.LAx1 + T.org, *LAz1 = a.LAz1 + T.org, *UAx1 = a.UAx1 + T.org,
-          *UAz1 = a.UAz1 + T.org, *G = a.grad + T.org;
+        R *LAx1 = a.LAx1 + T.sb, *LAz1 = a.LAz1 + T.sb, *UAx1 = a.UAx1 + T.sb,
+          *UAz1 = a.UAz1 + T.sb, *G = a.grad + T.sb;
         constexpr R inv5040 = R(1) / R(5040);
 #pragma unroll
         for (int j = 0; j < RO; j++) {
@@ -789,7 +789,7 @@ __device__ __forceinline__ void adj_compute(const AdjArgs<R> &a, const TileId &T
                 for (int i = 0; i < 4; i++)
                     if (!in_box(x + i, z, c.X, c.Z)) { o1[i] = o2[i] = o3[i] = o4[i] = R(0); }
             }
-            const int gi = z * ld + x;
+            const unsigned gi = (unsigned)((z + ZOFF) * ld + x + XOFF);
             st4(LAx1 + gi, pack<R>(o1));
             st4(LAz1 + gi, pack<R>(o2));
             st4(UAx1 + gi, pack<R>(o3));
@@ -816,7 +816,7 @@ adj_step_kernel(const AdjArgs<R> a)
     const int tid = threadIdx.x;
     const TileId T = tile_id<TH>(xcd_remap(blockIdx.x, gridDim.x), a.ntx, a.ntz, a.ld, a.sstride);
     AdjLoads<R, TH> L;
-    L.load(a, T.x0, T.z0, T.org, tid);
+    L.load(a, T.x0, T.z0, T.sb, tid);
     L.store(sLAx, sUAx, sLAz, sUAz, tid);
     lds_barrier();
     adj_compute<R, TH>(a, T, L, tid, sLAx, sLAz, sLP, sUAx, sUAz, sU);
@@ -836,7 +836,7 @@ adj_step_persistent(const AdjArgs<R> a)
     int Lt = lo + k;
     AdjLoads<R, TH> L;
     TileId T = tile_id<TH>(min(Lt, nb - 1), a.ntx, a.ntz, a.ld, a.sstride);
-    if (Lt < hi) L.load(a, T.x0, T.z0, T.org, tid);
+    if (Lt < hi) L.load(a, T.x0, T.z0, T.sb, tid);
     for (; Lt < hi; Lt += nk) {
         L.store(sLAx, sUAx, sLAz, sUAz, tid);
         AdjLoads<R, TH> C;
@@ -851,7 +851,7 @@ adj_step_persistent(const AdjArgs<R> a)
         lds_barrier();
         if (Lt + nk < hi) {
             T = tile_id<TH>(Lt + nk, a.ntx, a.ntz, a.ld, a.sstride);
-            L.load(a, T.x0, T.z0, T.org, tid);
+            L.load(a, T.x0, T.z0, T.sb, tid);
         }
         adj_compute<R, TH>(a, Tc, C, tid, sLAx, sLAz, sLP, sUAx, sUAz, sU);
         lds_barrier();
