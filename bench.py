@@ -129,6 +129,7 @@ def main():
     ap.add_argument("--th-adj", type=int, default=0)
     ap.add_argument("--misfit", default="w2", choices=["w2", "l2"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the run, copy 1 GiB device-to-device (scale for HBM byte counters)")
@@ -149,15 +150,18 @@ def main():
     m.set_sources(prob["src_x"], prob["src_z"], cfg["src"])
     m.set_receivers(prob["src_x"], prob["src_z"])
     m.set_ring(prob["ring_x"], prob["ring_z"])
-    if args.th_fwd or args.th_adj:
-        m.set_tuning(args.th_fwd, args.th_adj, 0)
     shots = np.arange(S)
     m.set_cquad((cfg["c_true"] ** 2).astype(np.float32))
+    tune = None
+    if args.th_fwd or args.th_adj:
+        m.set_tuning(args.th_fwd, args.th_adj, 0)
+    elif not args.no_autotune:
+        tune = m.autotune(S, steps=20)
     obs = m.forward(shots)
     m.set_observed(obs)
     m.set_cquad((cfg["c_bg"] ** 2).astype(np.float32))
-    print(f"[bench] device {_lib.device_info()}; {S} shots {N}x{N} T={T}; tiles {m.get_tuning()}",
-          file=sys.stderr, flush=True)
+    print(f"[bench] device {_lib.device_info()}; {S} shots {N}x{N} T={T}; tiles {m.get_tuning()}"
+          f" persistent {m.get_launch()} autotune {tune}", file=sys.stderr, flush=True)
 
     for i in range(args.warmup):
         m.gradient(shots, misfit=args.misfit)
@@ -195,7 +199,8 @@ def main():
         "data": "synthetic (seeded random-defect velocity model; observed traces simulated)",
         "config": {"workload": f"fmc{S}_{N}x{N}_T{T}_{args.misfit}_gradient", "grid": [N, N],
                    "T": T, "shots": S, "receivers": S, "misfit": args.misfit, "pml": 20,
-                   "tiles": list(m.get_tuning()[:2])},
+                   "tiles": list(m.get_tuning()[:2]), "persistent": list(m.get_launch()),
+                   "autotune": tune},
         "roofline": {"bound": "hbm", "kernel": "adj_step_kernel",
                      "achieved": adj_ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": adj_ach / HBM_PEAK, "traffic": None,

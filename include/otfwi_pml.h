@@ -134,6 +134,20 @@ int otfwi_get_tuning(otfwi_model *m, int *th_fwd, int *th_adj, int *max_batch);
 int otfwi_set_launch(otfwi_model *m, int persistent_fwd, int persistent_adj);
 int otfwi_get_launch(otfwi_model *m, int *persistent_fwd, int *persistent_adj);
 
+/* Replay the per-step launch loops as captured hipGraphs (default 1); a
+ * graph is re-captured only when shots, tiles, launch style or device buffers
+ * change.  No reference counterpart (regression_PML_single.cu:661-678
+ * launches and synchronizes four kernels per step). */
+int otfwi_set_graph(otfwi_model *m, int use_graph);
+
+/* Autotuner: times `steps` launches of every tile height and launch style of
+ * the forward and adjoint step kernels on a batch of n_shots shots of this
+ * model and keeps the fastest.  choice[0..3] (may be NULL) = th_fwd,
+ * persistent_fwd, th_adj, persistent_adj; ms[0..1] (may be NULL) = best
+ * average launch time of the forward / adjoint step in milliseconds.  No
+ * reference counterpart (fixed 32x32 blocks). */
+int otfwi_autotune(otfwi_model *m, int n_shots, int steps, int *choice, float *ms);
+
 /* Device time of the last otfwi_gradient / otfwi_gradient_device call,
  * from HIP events on the model's stream: out[0] forward time loops (ms),
  * out[1] misfit kernels (ms), out[2] adjoint time loops (ms), out[3] number

@@ -105,6 +105,13 @@ struct otfwi_model {
     int th_fwd = 32, th_adj = 16, max_batch = 0;
     int pers_fwd = 0, pers_adj = 0;   // persistent-work-group launches (0 = one work-group per tile)
     int n_cu = 0;
+    // hipGraph replay of the step loops: slot 0 forward, 1 forward with ring save,
+    // 2 adjoint; a graph is reused while its key (shots, tiles, launch style and
+    // the epoch of the device buffers it captured) is unchanged
+    bool use_graph = true;
+    unsigned long long epoch = 1;
+    hipGraphExec_t gexec[3] = {};
+    unsigned long long gkey[3] = {};
     uint8_t *tf_fwd = nullptr, *tf_adj = nullptr;
     int tf_fwd_th = 0, tf_adj_th = 0;
     std::vector<float> obs_h;
@@ -149,6 +156,7 @@ int ensure_batch(otfwi_model *m, int n, bool need_adj)
     if (b.cap >= n && adj_ok && (!need_adj || b.ring_elems >= ring_need)) return OTFWI_OK;
     const int cap = std::max(n, b.cap);
     free_batch(b);
+    m->epoch++;
     auto fa = [&](void **p, size_t elems) -> int {
         char *q = nullptr;
         RCK(dalloc(&q, elems * es));
@@ -218,6 +226,7 @@ int build_flags(otfwi_model *m, int th, uint8_t **dst, int *dst_th)
     RCK(dalloc(dst, f.size()));
     HIPCK(hipMemcpy(*dst, f.data(), f.size(), hipMemcpyHostToDevice));
     *dst_th = th;
+    m->epoch++;
     return OTFWI_OK;
 }
 
@@ -267,6 +276,7 @@ int build_points(otfwi_model *m)
     m->points_dirty = false;
     dfree(m->tf_fwd);
     dfree(m->tf_adj);
+    m->epoch++;
     return OTFWI_OK;
 }
 
@@ -323,6 +333,7 @@ int upload_sources(otfwi_model *m)
         }
     }
     dfree(m->icum);
+    m->epoch++;
     RCK(dalloc((char **)&m->icum, ic.size() * sizeof(R)));
     HIPCK(hipMemcpy(m->icum, ic.data(), ic.size() * sizeof(R), hipMemcpyHostToDevice));
     return OTFWI_OK;
@@ -347,6 +358,7 @@ int upload_damping(otfwi_model *m)
     }
     dfree(m->kx);
     dfree(m->kz);
+    m->epoch++;
     RCK(dalloc((char **)&m->kx, kx.size() * sizeof(R)));
     RCK(dalloc((char **)&m->kz, kz.size() * sizeof(R)));
     HIPCK(hipMemcpy(m->kx, kx.data(), kx.size() * sizeof(R), hipMemcpyHostToDevice));
@@ -377,6 +389,49 @@ int persistent_wgs(const otfwi_model *m, bool adj)
     return n;
 }
 
+// Runs body() (which enqueues kernel launches only) directly, or captures it
+// once into a hipGraph for slot `slot` and replays that.
+template <typename F>
+int run_loop(otfwi_model *m, int slot, unsigned long long key, F body)
+{
+    if (!m->use_graph) return body();
+    if (!m->gexec[slot] || m->gkey[slot] != key) {
+        if (m->gexec[slot]) (void)hipGraphExecDestroy(m->gexec[slot]);
+        m->gexec[slot] = nullptr;
+        hipGraph_t graph = nullptr;
+        HIPCK(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = body();
+        const hipError_t e = hipStreamEndCapture(m->stream, &graph);
+        if (rc != OTFWI_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        HIPCK(e);
+        const hipError_t ei = hipGraphInstantiate(&m->gexec[slot], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPCK(ei);
+        m->gkey[slot] = key;
+    }
+    HIPCK(hipGraphLaunch(m->gexec[slot], m->stream));
+    return OTFWI_OK;
+}
+
+unsigned long long loop_key(const otfwi_model *m, int n, int th, int pw, int t0, int t1)
+{
+    unsigned long long k = m->epoch;
+    k = k * 1000003ULL + (unsigned long long)n;
+    k = k * 1000003ULL + (unsigned long long)th;
+    k = k * 1000003ULL + (unsigned long long)pw;
+    k = k * 1000003ULL + (unsigned long long)t0;
+    k = k * 1000003ULL + (unsigned long long)t1;
+    return k;
+}
+
+template <typename R>
+int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1);
+template <typename R>
+int adj_loop(otfwi_model *m, int n, int t0, int t1);
+
 template <typename R>
 int run_forward(otfwi_model *m, int n, bool save_ring)
 {
@@ -397,6 +452,16 @@ int run_forward(otfwi_model *m, int n, bool save_ring)
             HIPCK(hipMemsetAsync((char *)b.ring + (size_t)s * m->T * m->n_ring * es, 0,
                                  (size_t)m->n_ring * es, st));
     }
+    return fwd_loop<R>(m, n, save_ring, 1, m->T);
+}
+
+// Launches of forward steps t0 .. t1-1 on the current batch (no memsets).
+template <typename R>
+int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1)
+{
+    Batch &b = m->b;
+    const size_t plane = (size_t)m->zp * m->ld;
+    hipStream_t st = m->stream;
     FwdArgs<R> a{};
     a.X = m->X; a.Z = m->Z; a.ld = m->ld;
     a.ntx = ceil_div(m->X, TW); a.ntz = ceil_div(m->Z, m->th_fwd);
@@ -413,16 +478,19 @@ int run_forward(otfwi_model *m, int n, bool save_ring)
     a.ring = save_ring ? (R *)b.ring : nullptr;
     a.n_ring = m->n_ring;
     const int pw = persistent_wgs(m, false);
-    for (int t = 1; t < m->T; t++) {
-        const int p = (t - 1) & 1, c = t & 1;
-        a.t = t;
-        a.P0 = (const R *)b.P[p]; a.P1 = (R *)b.P[c];
-        a.Ax0 = (const R *)b.Ax[p]; a.Ax1 = (R *)b.Ax[c];
-        a.Az0 = (const R *)b.Az[p]; a.Az1 = (R *)b.Az[c];
-        a.Px0 = (const R *)b.Px[p]; a.Px1 = (R *)b.Px[c];
-        HIPCK(launch_fwd_step<R>(a, m->th_fwd, pw, st));
-    }
-    return OTFWI_OK;
+    auto body = [&]() -> int {
+        for (int t = t0; t < t1; t++) {
+            const int p = (t - 1) & 1, c = t & 1;
+            a.t = t;
+            a.P0 = (const R *)b.P[p]; a.P1 = (R *)b.P[c];
+            a.Ax0 = (const R *)b.Ax[p]; a.Ax1 = (R *)b.Ax[c];
+            a.Az0 = (const R *)b.Az[p]; a.Az1 = (R *)b.Az[c];
+            a.Px0 = (const R *)b.Px[p]; a.Px1 = (R *)b.Px[c];
+            HIPCK(launch_fwd_step<R>(a, m->th_fwd, pw, st));
+        }
+        return OTFWI_OK;
+    };
+    return run_loop(m, save_ring ? 1 : 0, loop_key(m, n, m->th_fwd, pw, t0, t1), body);
 }
 
 template <typename R>
@@ -439,6 +507,16 @@ int run_adjoint(otfwi_model *m, int n)
         HIPCK(hipMemsetAsync(b.LPx[i], 0, plane * n * es, st));
     }
     HIPCK(hipMemsetAsync(b.grad, 0, plane * n * es, st));
+    return adj_loop<R>(m, n, 1, m->T);
+}
+
+// Launches of adjoint steps t0 .. t1-1 on the current batch (no memsets).
+template <typename R>
+int adj_loop(otfwi_model *m, int n, int t0, int t1)
+{
+    Batch &b = m->b;
+    const size_t plane = (size_t)m->zp * m->ld;
+    hipStream_t st = m->stream;
     AdjArgs<R> a{};
     a.X = m->X; a.Z = m->Z; a.ld = m->ld;
     a.ntx = ceil_div(m->X, TW); a.ntz = ceil_div(m->Z, m->th_adj);
@@ -454,7 +532,8 @@ int run_adjoint(otfwi_model *m, int n)
     a.ring = (const R *)b.ring; a.n_ring = m->n_ring;
     const int T = m->T;
     const int pw = persistent_wgs(m, true);
-    for (int t = 1; t < T; t++) {
+    auto body = [&]() -> int {
+    for (int t = t0; t < t1; t++) {
         const int lp = (t - 1) & 1, lc = t & 1;
         // reconstructed field: U(t) lives in P slot (T-1+t)%2, A_b(t) in slot (T-2+t)%2,
         // starting from the forward's P(T-1) and A(T-2) (regression_PML_single.cu:787-790)
@@ -471,6 +550,8 @@ int run_adjoint(otfwi_model *m, int n)
         HIPCK(launch_adj_step<R>(a, m->th_adj, pw, st));
     }
     return OTFWI_OK;
+    };
+    return run_loop(m, 2, loop_key(m, n, m->th_adj, pw, t0, t1), body);
 }
 
 int upload_shot_src(otfwi_model *m, int n, const int *shot_src)
@@ -673,6 +754,69 @@ int adjoint_from_host_impl(otfwi_model *m, const float *adj, float *grad_out)
     return OTFWI_OK;
 }
 
+// Times `steps` launches of each tile height / launch style of the forward
+// and adjoint step kernels on the live batch of n shots and keeps the
+// fastest.  out[0..3] = th_fwd, persistent_fwd, th_adj, persistent_adj;
+// ms[0], ms[1] = best average launch time (ms) of the forward / adjoint.
+template <typename R>
+int autotune_impl(otfwi_model *m, int n, int steps, int *out, float *ms)
+{
+    if (n <= 0 || steps <= 0) return fail(OTFWI_E_ARG, "autotune: n_shots and steps must be > 0");
+    steps = std::min(steps, m->T - 1);
+    RCK(ready(m, true));
+    RCK(ensure_batch(m, n, true));
+    RCK(upload_shot_src(m, n, nullptr));
+    Batch &b = m->b;
+    const size_t es = sizeof(R), plane = (size_t)m->zp * m->ld;
+    for (int i = 0; i < 2; i++)
+        for (void *p : {b.P[i], b.Ax[i], b.Az[i], b.Px[i], b.LP[i], b.LAx[i], b.LAz[i], b.LPx[i]})
+            HIPCK(hipMemsetAsync(p, 0, plane * n * es, m->stream));
+    HIPCK(hipMemsetAsync(b.grad, 0, plane * n * es, m->stream));
+    HIPCK(hipMemsetAsync(b.jcum, 0, (size_t)n * m->n_rx * m->T * es, m->stream));
+    HIPCK(hipMemsetAsync(b.ring, 0, (size_t)n * m->T * m->n_ring * es, m->stream));
+    const bool graph = m->use_graph;
+    m->use_graph = false;
+    const int ths[3] = {16, 32, 64};
+    int best[4] = {m->th_fwd, m->pers_fwd, m->th_adj, m->pers_adj};
+    float bms[2] = {1e30f, 1e30f};
+    int rc = OTFWI_OK;
+    for (int adj = 0; adj < 2 && rc == OTFWI_OK; adj++)
+        for (int th : ths)
+            for (int pers = 0; pers < (m->prec == 8 ? 1 : 2) && rc == OTFWI_OK; pers++) {
+                if (m->prec == 8 && (th == 64 || (adj && th != 16))) continue;
+                if (adj) { m->th_adj = th; m->pers_adj = pers; }
+                else { m->th_fwd = th; m->pers_fwd = pers; }
+                if ((rc = ready(m, true)) != OTFWI_OK) break;
+                auto launch = [&](int k) {
+                    return adj ? adj_loop<R>(m, n, 1, 1 + k) : fwd_loop<R>(m, n, true, 1, 1 + k);
+                };
+                if ((rc = launch(std::min(2, steps))) != OTFWI_OK) break;
+                float t = 0;
+                if (hipEventRecord(m->ev[0], m->stream) != hipSuccess ||
+                    (rc = launch(steps)) != OTFWI_OK ||
+                    hipEventRecord(m->ev[1], m->stream) != hipSuccess ||
+                    hipEventSynchronize(m->ev[1]) != hipSuccess ||
+                    hipEventElapsedTime(&t, m->ev[0], m->ev[1]) != hipSuccess) {
+                    if (rc == OTFWI_OK) rc = fail(OTFWI_E_HIP, "autotune timing failed");
+                    break;
+                }
+                t /= steps;
+                if (t < bms[adj]) {
+                    bms[adj] = t;
+                    best[2 * adj] = th;
+                    best[2 * adj + 1] = pers;
+                }
+            }
+    m->use_graph = graph;
+    m->th_fwd = best[0]; m->pers_fwd = best[1];
+    m->th_adj = best[2]; m->pers_adj = best[3];
+    RCK(rc);
+    RCK(ready(m, true));
+    if (out) for (int i = 0; i < 4; i++) out[i] = best[i];
+    if (ms) { ms[0] = bms[0]; ms[1] = bms[1]; }
+    return OTFWI_OK;
+}
+
 #define DISPATCH(m, call) ((m)->prec == 8 ? call<double> : call<float>)
 
 int check_model(const otfwi_model *m)
@@ -754,6 +898,8 @@ int otfwi_destroy(otfwi_model *m)
     dfree(m->gacc); dfree(m->gout);
     for (int i = 0; i < 4; i++)
         if (m->ev[i]) (void)hipEventDestroy(m->ev[i]);
+    for (int i = 0; i < 3; i++)
+        if (m->gexec[i]) (void)hipGraphExecDestroy(m->gexec[i]);
     (void)hipStreamDestroy(m->stream);
     delete m;
     return OTFWI_OK;
@@ -893,6 +1039,19 @@ int otfwi_get_launch(otfwi_model *m, int *persistent_fwd, int *persistent_adj)
     if (persistent_fwd) *persistent_fwd = m->pers_fwd;
     if (persistent_adj) *persistent_adj = m->pers_adj;
     return OTFWI_OK;
+}
+
+int otfwi_set_graph(otfwi_model *m, int use_graph)
+{
+    RCK(check_model(m));
+    m->use_graph = use_graph != 0;
+    return OTFWI_OK;
+}
+
+int otfwi_autotune(otfwi_model *m, int n_shots, int steps, int *choice, float *ms)
+{
+    RCK(check_model(m));
+    return DISPATCH(m, autotune_impl)(m, n_shots, steps, choice, ms);
 }
 
 int otfwi_get_tuning(otfwi_model *m, int *th_fwd, int *th_adj, int *max_batch)

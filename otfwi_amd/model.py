@@ -89,6 +89,19 @@ class PMLModel:
         check(self._lib.otfwi_set_launch(self._h, int(bool(persistent_fwd)),
                                          int(bool(persistent_adj))))
 
+    def set_graph(self, use_graph=True):
+        """Replay the step loops as captured hipGraphs (default on)."""
+        check(self._lib.otfwi_set_graph(self._h, int(bool(use_graph))))
+
+    def autotune(self, n_shots, steps=20):
+        """Time every tile height / launch style on a batch of n_shots and keep
+        the fastest.  Returns the choice and the best per-launch times (ms)."""
+        ch = np.zeros(4, np.int32)
+        ms = np.zeros(2, np.float32)
+        check(self._lib.otfwi_autotune(self._h, int(n_shots), int(steps), ptr(ch), ptr(ms)))
+        return dict(th_fwd=int(ch[0]), persistent_fwd=bool(ch[1]), th_adj=int(ch[2]),
+                    persistent_adj=bool(ch[3]), fwd_ms=float(ms[0]), adj_ms=float(ms[1]))
+
     def get_launch(self):
         a, b = ctypes.c_int(), ctypes.c_int()
         check(self._lib.otfwi_get_launch(self._h, ctypes.byref(a), ctypes.byref(b)))
