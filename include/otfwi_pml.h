@@ -183,6 +183,23 @@ int otfwi_gradient_device(otfwi_model *m, int n_shots, const int *shot_src, int 
 int otfwi_forward_for_grad(otfwi_model *m, int n_shots, const int *shot_src, float *rec_out);
 int otfwi_adjoint_from_host(otfwi_model *m, const float *adj_src, float *grad_out);
 
+/* Misfit of the current model without the adjoint run: forward runs (no
+ * ring save) and the device misfit; misfit_out receives n_shots FP64
+ * per-shot sums.  Used by the misfit-decrease acceptance test of the
+ * inversion driver; the reference evaluates the objective only through the
+ * full gradient (regression.py:45-49 over cuda_interface_pml.py:208-236). */
+int otfwi_objective(otfwi_model *m, int n_shots, const int *shot_src, int misfit_kind,
+                    double *misfit_out);
+
+/* Source-side illumination collected by the next gradient calls: every
+ * stride-th forward step adds sum over the call's shots of P(t)^2 (times
+ * stride) to a (Z,X) FP32 map (stride 0 = off).  otfwi_get_illum copies the
+ * map of the last gradient call to host memory, or to device memory when
+ * out_on_device is nonzero.  No reference counterpart (the reference
+ * preconditions with a fixed mask and scale, regression.py:45-49). */
+int otfwi_set_illum(otfwi_model *m, int stride);
+int otfwi_get_illum(otfwi_model *m, float *out, int out_on_device);
+
 /* Stand-alone misfit on host arrays (n_traces of T samples, FP64 in and
  * out, as adj_wrapper hands them over): per-trace misfit and adjoint source.
  * Replaces adjoint_functions.L2 / .wasserstein (adjoint_functions.py:8-11,

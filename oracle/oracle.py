@@ -233,6 +233,25 @@ def wasserstein(sim, obs):
     return W2, out
 
 
+def illumination(cquad, d_x, d_z, src_xz, src, T, shots, stride, precision="ref"):
+    """Source-side illumination of the build's plain inversion step (no
+    reference counterpart): sum over shots and over the steps t = 1..T-1 with
+    t % stride == 0 of stride * P(t)^2, where P(t) = Px + Pz after the source
+    injection of step t (the field the reference records,
+    regression_PML_single.cu:295-309).  The fields come from forward() with a
+    receiver on every cell.  FP64 sum."""
+    Z, X = cquad.shape
+    zz, xx = np.mgrid[0:Z, 0:X]
+    every = (xx.ravel().astype(np.int32), zz.ravel().astype(np.int32))
+    ill = np.zeros(Z * X)
+    ts = [t for t in range(1, T) if t % stride == 0]
+    for idx in shots:
+        rec = forward(cquad, d_x, d_z, src_xz, src, every, T, idx=idx,
+                      precision=precision)["rec"].astype(np.float64)
+        ill += stride * (rec[:, ts] ** 2).sum(axis=1)
+    return ill.reshape(Z, X)
+
+
 # --------------------------------------------------------------------------
 # whole gradient, as Cuda_interface_PML.calc_grad drives it
 # --------------------------------------------------------------------------

@@ -50,6 +50,9 @@ _SIGS = [
     ("otfwi_gradient_device", _i, [_vp, _i, _vp, _i, _vp, _vp]),
     ("otfwi_forward_for_grad", _i, [_vp, _i, _vp, _vp]),
     ("otfwi_adjoint_from_host", _i, [_vp, _vp, _vp]),
+    ("otfwi_objective", _i, [_vp, _i, _vp, _i, _vp]),
+    ("otfwi_set_illum", _i, [_vp, _i]),
+    ("otfwi_get_illum", _i, [_vp, _vp, _i]),
     ("otfwi_misfit", _i, [_i, _i, _i, _vp, _vp, _vp, _vp]),
 ]
 
@@ -106,6 +109,53 @@ def f32(a):
 
 def i32(a):
     return np.ascontiguousarray(a, dtype=np.int32)
+
+
+UNET_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libotfwi_unet.so")
+_ll = ctypes.c_longlong
+_f = ctypes.c_float
+UNET_SIGS = [
+    ("otfwi_unet_last_error", ctypes.c_char_p, []),
+    ("otfwi_unet_conv3x3", _i, [_vp, _i, _vp, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
+    ("otfwi_unet_wgrad", _i, [_vp, _i, _vp, _i, _i, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
+    ("otfwi_unet_flip_weights", _i, [_vp, _i, _i, _vp, _vp]),
+    ("otfwi_unet_pool2", _i, [_vp, _i, _i, _i, _i, _vp, _vp]),
+    ("otfwi_unet_upsum_mask", _i, [_vp, _i, _i, _vp, _i, _i, _i, _i, _vp, _vp]),
+    ("otfwi_unet_skip_combine", _i, [_vp, _i, _i, _vp, _vp, _i, _i, _i, _i, _vp, _vp]),
+    ("otfwi_unet_pack_input", _i, [_vp, _i, _i, _i, _vp, _vp]),
+    ("otfwi_unet_head_fwd", _i, [_vp, _vp, _vp, _ll, _vp, _vp]),
+    ("otfwi_unet_head_bwd", _i, [_vp, _vp, _vp, _vp, _ll, _f, _vp, _vp, _vp, _vp, _vp]),
+    ("otfwi_unet_bias_grad", _i, [_vp, _ll, _i, _vp, _vp]),
+    ("otfwi_unet_adam", _i, [_vp, _vp, _vp, _vp, _vp, _ll, _f, _f, _f, _f, _i, _vp]),
+    ("otfwi_unet_to_bf16", _i, [_vp, _ll, _vp, _vp]),
+]
+UNET_EXPORTED = [s[0] for s in UNET_SIGS]
+_ULIB = None
+
+
+def load_unet():
+    """Load libotfwi_unet.so (the learned preconditioner's kernels).  Raises
+    ImportError when it has not been built; there is no fallback."""
+    global _ULIB
+    if _ULIB is None:
+        if not os.path.exists(UNET_LIB_PATH):
+            raise ImportError(
+                f"{UNET_LIB_PATH} is missing: build it with __graft_entry__.build() "
+                "(make -C otfwi_amd/csrc); there is no CPU fallback")
+        lib = ctypes.CDLL(UNET_LIB_PATH)
+        for name, res, args in UNET_SIGS:
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _ULIB = lib
+    return _ULIB
+
+
+def ucheck(rc):
+    if rc != 0:
+        msg = load_unet().otfwi_unet_last_error()
+        raise OtfwiError(rc, msg.decode() if msg else "")
+    return rc
 
 
 def device_info():

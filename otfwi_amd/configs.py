@@ -108,3 +108,26 @@ def config2(model_seed=0, N=1024, T=4000, n_elem=64, pitch=12, pml=20):
     return dict(X=N, Z=N, T=T, pml=pml, R=1e-3, c_true=c_true, c_bg=c_bg,
                 src_mask=mask, rx_mask=mask.copy(), src=src, dx=dx, dt=dt,
                 n_shots=n_elem)
+
+
+def learn_config(model_seed=0, N=256, T=1200, n_elem=32, pitch=6, pml=20, n_max=3,
+                 semi_min=6.0, semi_max=30.0):
+    """Full-matrix capture block for the inversion comparisons and the learned
+    preconditioner's training pairs: the 1024^2 full-matrix-capture block
+    scaled to N x N at the same dx = 0.05 mm and 5 MHz (so the same cells per
+    wavelength), an n_elem array at pitch 0.3 mm on the top row, 1..n_max
+    circles/ellipses with semi-axes 0.3-1.5 mm, c uniform in [1480, 3200]
+    m/s in the lower 80 %, T long enough for the two-way travel to the
+    bottom."""
+    dx = 5e-5
+    dt = CFL * dx / V_MAX
+    rng = np.random.default_rng(model_seed)
+    c_bg = np.full((N, N), float(courant(V_BG)))
+    c_true = draw_defects(c_bg.copy(), rng, pml, n_max=n_max, semi_min=semi_min,
+                          semi_max=semi_max)
+    mask, xs = linear_array(N, n_elem, pml, pitch=pitch)
+    w = ricker(5e6, dt, T)
+    src = np.tile(w, (n_elem, 1))
+    return dict(X=N, Z=N, T=T, pml=pml, R=1e-3, c_true=c_true, c_bg=c_bg,
+                src_mask=mask, rx_mask=mask.copy(), src=src, dx=dx, dt=dt,
+                n_shots=n_elem)

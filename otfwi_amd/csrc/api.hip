@@ -118,6 +118,11 @@ struct otfwi_model {
     int obs_shots = 0;
     Batch b;
     double *gacc = nullptr;          // FP64 sum of per-shot gradients across batches
+    // source-side illumination of the last gradient call: sum over its shots of
+    // P(t)^2 at every ill_stride-th step, times ill_stride (0 = not collected)
+    int ill_stride = 0;
+    float *ill = nullptr;
+    bool ill_valid = false;
     float *gout = nullptr;           // FP32 mean gradient on device
     // state of a pending two-stage gradient
     int pend_n = 0;
@@ -428,12 +433,12 @@ unsigned long long loop_key(const otfwi_model *m, int n, int th, int pw, int t0,
 }
 
 template <typename R>
-int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1);
+int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum = false);
 template <typename R>
 int adj_loop(otfwi_model *m, int n, int t0, int t1);
 
 template <typename R>
-int run_forward(otfwi_model *m, int n, bool save_ring)
+int run_forward(otfwi_model *m, int n, bool save_ring, bool illum = false)
 {
     Batch &b = m->b;
     const size_t es = sizeof(R);
@@ -452,12 +457,12 @@ int run_forward(otfwi_model *m, int n, bool save_ring)
             HIPCK(hipMemsetAsync((char *)b.ring + (size_t)s * m->T * m->n_ring * es, 0,
                                  (size_t)m->n_ring * es, st));
     }
-    return fwd_loop<R>(m, n, save_ring, 1, m->T);
+    return fwd_loop<R>(m, n, save_ring, 1, m->T, illum);
 }
 
 // Launches of forward steps t0 .. t1-1 on the current batch (no memsets).
 template <typename R>
-int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1)
+int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum)
 {
     Batch &b = m->b;
     const size_t plane = (size_t)m->zp * m->ld;
@@ -478,6 +483,7 @@ int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1)
     a.ring = save_ring ? (R *)b.ring : nullptr;
     a.n_ring = m->n_ring;
     const int pw = persistent_wgs(m, false);
+    const int ist = (illum && m->ill) ? m->ill_stride : 0;
     auto body = [&]() -> int {
         for (int t = t0; t < t1; t++) {
             const int p = (t - 1) & 1, c = t & 1;
@@ -487,10 +493,14 @@ int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1)
             a.Az0 = (const R *)b.Az[p]; a.Az1 = (R *)b.Az[c];
             a.Px0 = (const R *)b.Px[p]; a.Px1 = (R *)b.Px[c];
             HIPCK(launch_fwd_step<R>(a, m->th_fwd, pw, st));
+            if (ist > 0 && t % ist == 0)
+                HIPCK(launch_illum<R>((const R *)b.P[c], n, m->X, m->Z, m->ld, (long long)plane,
+                                      (float)ist, m->ill, st));
         }
         return OTFWI_OK;
     };
-    return run_loop(m, save_ring ? 1 : 0, loop_key(m, n, m->th_fwd, pw, t0, t1), body);
+    return run_loop(m, save_ring ? 1 : 0,
+                    loop_key(m, n, m->th_fwd, pw, t0, t1) * 31ULL + (unsigned long long)ist, body);
 }
 
 template <typename R>
@@ -619,6 +629,19 @@ int upload_obs_batch(otfwi_model *m, int s0, int nb)
 }
 
 
+int check_w2_flags(const otfwi_model *m, const std::vector<int> &fl, int s0)
+{
+    for (size_t i = 0; i < fl.size(); i++)
+        if (fl[i])
+            return fail(OTFWI_E_NUMERIC,
+                        fl[i] == 2 ? "W2: observed trace of receiver %zu, shot %d is all zero "
+                                     "(the W2 normalization is 0/0)"
+                                   : "W2: observed trace of receiver %zu, shot %d has a "
+                                     "non-finite CDF",
+                        i % m->n_rx, s0 + (int)(i / m->n_rx));
+    return OTFWI_OK;
+}
+
 // One batch of a gradient: forward with ring save, misfit + adjoint source
 // (device kind) or host adjoint source already in b.adj (kind < 0), adjoint
 // run, and accumulation of the batch's per-shot gradients into m->gacc.
@@ -630,7 +653,7 @@ int grad_batch_device(otfwi_model *m, int s0, int nb, const int *shot_src, int k
     RCK(upload_shot_src(m, nb, shot_src ? shot_src + s0 : nullptr));
     RCK(upload_obs_batch<R>(m, s0, nb));
     HIPCK(hipEventRecord(m->ev[0], m->stream));
-    RCK(run_forward<R>(m, nb, true));
+    RCK(run_forward<R>(m, nb, true, true));
     HIPCK(hipEventRecord(m->ev[1], m->stream));
     if (rec_out) RCK(copy_rec_out<R>(m, nb, rec_out + (size_t)s0 * m->n_rx * m->T));
     const int ntr = nb * m->n_rx;
@@ -661,16 +684,22 @@ int grad_batch_device(otfwi_model *m, int s0, int nb, const int *shot_src, int k
     m->t_adj_ms += ms[2];
     m->n_fwd_launch += m->T - 1;
     m->n_adj_launch += m->T - 1;
-    for (size_t i = 0; i < fl.size(); i++)
-        if (fl[i])
-            return fail(OTFWI_E_NUMERIC,
-                        fl[i] == 2 ? "W2: observed trace of receiver %zu, shot %d is all zero "
-                                     "(the W2 normalization is 0/0)"
-                                   : "W2: observed trace of receiver %zu, shot %d has a "
-                                     "non-finite CDF",
-                        i % m->n_rx, s0 + (int)(i / m->n_rx));
+    RCK(check_w2_flags(m, fl, s0));
     if (mis_out)
         for (int i = 0; i < nb; i++) mis_out[s0 + i] = hmis[i];
+    return OTFWI_OK;
+}
+
+int ensure_ill(otfwi_model *m)
+{
+    m->ill_valid = false;
+    if (m->ill_stride <= 0) return OTFWI_OK;
+    const size_t n = (size_t)m->X * m->Z;
+    if (!m->ill) {
+        RCK(dalloc(&m->ill, n));
+        m->epoch++;
+    }
+    HIPCK(hipMemsetAsync(m->ill, 0, n * sizeof(float), m->stream));
     return OTFWI_OK;
 }
 
@@ -696,6 +725,7 @@ int gradient_impl(otfwi_model *m, int n, const int *shot_src, int kind, double *
     const int B = batch_size(m, n);
     RCK(ensure_batch(m, B, true));
     RCK(ensure_gacc(m));
+    RCK(ensure_ill(m));
     m->t_fwd_ms = m->t_mis_ms = m->t_adj_ms = 0;
     m->n_fwd_launch = m->n_adj_launch = 0;
     for (int s0 = 0; s0 < n; s0 += B)
@@ -707,6 +737,46 @@ int gradient_impl(otfwi_model *m, int n, const int *shot_src, int kind, double *
                              out_dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, m->stream));
     }
     HIPCK(hipStreamSynchronize(m->stream));
+    m->ill_valid = m->ill_stride > 0;
+    return OTFWI_OK;
+}
+
+// Misfit of the current model without the adjoint: forward runs of the shots
+// (no ring save), device misfit, per-shot FP64 sums into mis_out.
+template <typename R>
+int objective_impl(otfwi_model *m, int n, const int *shot_src, int kind, double *mis_out)
+{
+    if (kind != OTFWI_MISFIT_L2 && kind != OTFWI_MISFIT_W2)
+        return fail(OTFWI_E_ARG, "unknown misfit kind %d", kind);
+    if (n <= 0) return fail(OTFWI_E_ARG, "n_shots must be positive");
+    if (m->n_rx <= 0) return fail(OTFWI_E_STATE, "receivers not set");
+    if (kind == OTFWI_MISFIT_W2 && m->T < 2) return fail(OTFWI_E_ARG, "W2 needs T >= 2");
+    RCK(ready(m, false));
+    const int B = batch_size(m, n);
+    RCK(ensure_batch(m, B, true));
+    Batch &b = m->b;
+    for (int s0 = 0; s0 < n; s0 += B) {
+        const int nb = std::min(B, n - s0);
+        RCK(upload_shot_src(m, nb, shot_src ? shot_src + s0 : nullptr));
+        RCK(upload_obs_batch<R>(m, s0, nb));
+        RCK(run_forward<R>(m, nb, false));
+        const int ntr = nb * m->n_rx;
+        int *flags = reinterpret_cast<int *>(b.scratch + (size_t)ntr * 3 * m->T);
+        if (kind == OTFWI_MISFIT_W2) HIPCK(hipMemsetAsync(flags, 0, ntr * sizeof(int), m->stream));
+        HIPCK((launch_misfit<R, float>(kind, (const R *)b.rec, b.obs, (R *)b.adj, nullptr,
+                                       b.mis_tr, ntr, m->T, b.scratch, m->stream)));
+        HIPCK(launch_reduce_misfit(b.mis_tr, nb, m->n_rx, b.mis_shot, m->stream));
+        std::vector<double> hmis(nb);
+        HIPCK(hipMemcpyAsync(hmis.data(), b.mis_shot, nb * sizeof(double), hipMemcpyDeviceToHost,
+                             m->stream));
+        std::vector<int> fl(kind == OTFWI_MISFIT_W2 ? ntr : 0);
+        if (kind == OTFWI_MISFIT_W2)
+            HIPCK(hipMemcpyAsync(fl.data(), flags, ntr * sizeof(int), hipMemcpyDeviceToHost,
+                                 m->stream));
+        HIPCK(hipStreamSynchronize(m->stream));
+        RCK(check_w2_flags(m, fl, s0));
+        for (int i = 0; i < nb; i++) mis_out[s0 + i] = hmis[i];
+    }
     return OTFWI_OK;
 }
 
@@ -895,7 +965,7 @@ int otfwi_destroy(otfwi_model *m)
     free_batch(m->b);
     dfree(m->cm); dfree(m->kx); dfree(m->kz); dfree(m->icum); dfree(m->cq32);
     dfree(m->evt); dfree(m->pts); dfree(m->tf_fwd); dfree(m->tf_adj);
-    dfree(m->gacc); dfree(m->gout);
+    dfree(m->gacc); dfree(m->gout); dfree(m->ill);
     for (int i = 0; i < 4; i++)
         if (m->ev[i]) (void)hipEventDestroy(m->ev[i]);
     for (int i = 0; i < 3; i++)
@@ -1108,6 +1178,37 @@ int otfwi_adjoint_from_host(otfwi_model *m, const float *adj_src, float *grad_ou
 {
     RCK(check_model(m));
     return DISPATCH(m, adjoint_from_host_impl)(m, adj_src, grad_out);
+}
+
+int otfwi_objective(otfwi_model *m, int n_shots, const int *shot_src, int misfit_kind,
+                    double *misfit_out)
+{
+    RCK(check_model(m));
+    if (!misfit_out) return fail(OTFWI_E_ARG, "misfit_out is NULL");
+    return DISPATCH(m, objective_impl)(m, n_shots, shot_src, misfit_kind, misfit_out);
+}
+
+int otfwi_set_illum(otfwi_model *m, int stride)
+{
+    RCK(check_model(m));
+    if (stride < 0 || stride > 1000000) return fail(OTFWI_E_ARG, "illumination stride %d", stride);
+    m->ill_stride = stride;
+    m->ill_valid = false;
+    return OTFWI_OK;
+}
+
+int otfwi_get_illum(otfwi_model *m, float *out, int out_on_device)
+{
+    RCK(check_model(m));
+    if (!out) return fail(OTFWI_E_ARG, "output is NULL");
+    if (!m->ill_valid)
+        return fail(OTFWI_E_STATE, "no illumination collected (set a stride, then run a gradient)");
+    const size_t bytes = (size_t)m->X * m->Z * sizeof(float);
+    HIPCK(hipMemcpyAsync(out, m->ill, bytes,
+                         out_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                         m->stream));
+    HIPCK(hipStreamSynchronize(m->stream));
+    return OTFWI_OK;
 }
 
 int otfwi_misfit(int kind, int n_traces, int T, const double *sim, const double *obs,

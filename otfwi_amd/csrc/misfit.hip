@@ -364,6 +364,38 @@ hipError_t launch_finalize_grad(const double *acc, int n, double scale, float *o
     return hipGetLastError();
 }
 
+// Source-side illumination: ill[z*X+x] += w * sum_s P_s(t)[z][x]^2 over the
+// shots of the batch (summed in shot order, FP32), one sampled step t.
+template <typename R>
+__global__ void illum_kernel(const R *__restrict__ P, int nshots, int X, int Z, int ld,
+                             long long sstride, float w, float *__restrict__ ill)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int z = blockIdx.y;
+    if (x >= X || z >= Z) return;
+    const R *p = P + (size_t)(z + ZOFF) * ld + x + XOFF;
+    float a = 0.0f;
+    for (int s = 0; s < nshots; s++) {
+        const float v = (float)p[(size_t)s * sstride];
+        a = fmaf(v, v, a);
+    }
+    float *o = ill + (size_t)z * X + x;
+    *o = fmaf(a, w, *o);
+}
+
+template <typename R>
+hipError_t launch_illum(const R *P, int nshots, int X, int Z, int ld, long long sstride, float w,
+                        float *ill, hipStream_t s)
+{
+    hipLaunchKernelGGL((illum_kernel<R>), dim3((X + 255) / 256, Z), dim3(256), 0, s, P, nshots, X,
+                       Z, ld, sstride, w, ill);
+    return hipGetLastError();
+}
+template hipError_t launch_illum<float>(const float *, int, int, int, int, long long, float,
+                                        float *, hipStream_t);
+template hipError_t launch_illum<double>(const double *, int, int, int, int, long long, float,
+                                         float *, hipStream_t);
+
 __global__ void reduce_misfit_kernel(const double *mis, int nshots, int n_rx, double *out)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;

@@ -106,6 +106,10 @@ template <typename R>
 hipError_t launch_accum_grad(const R *grad, int nshots, int X, int Z, int ld, long long sstride,
                              double *acc, hipStream_t s);
 hipError_t launch_finalize_grad(const double *acc, int n, double scale, float *o32, hipStream_t s);
+// ill[z*X+x] += w * sum_s P[s][z][x]^2 (padded planes P, unpadded FP32 ill).
+template <typename R>
+hipError_t launch_illum(const R *P, int nshots, int X, int Z, int ld, long long sstride, float w,
+                        float *ill, hipStream_t s);
 // Per-shot misfit sum over its traces in trace order (FP64).
 hipError_t launch_reduce_misfit(const double *mis, int nshots, int n_rx, double *out, hipStream_t s);
 

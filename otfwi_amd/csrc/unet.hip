@@ -1,0 +1,678 @@
+// Kernels of the learned preconditioner (libotfwi_unet.so): a 4-level U-Net
+// with BF16 weights and activations and FP32 accumulation on the gfx950
+// matrix cores, forward and backward, plus Adam.
+//
+// Tensors are NHWC ([n][y][x][c], channels innermost, BF16).  A 3x3 "same"
+// convolution is an implicit GEMM: M = 256 output pixels of a TY x TX tile,
+// N = TN output channels, K = 9 taps x Cin, on v_mfma_f32_32x32x16_bf16
+// (A = 32 pixels x 16 channels, B = 16 channels x 32 output channels).  The
+// input patch (tile + 1-pixel halo) and the weights of one CK-channel chunk
+// are staged in LDS once and read by all 9 taps.  The loader reads the
+// decoder's concatenation [skip, nearest-upsample(coarse)] directly, so
+// neither the concatenated nor the upsampled tensor is ever written.
+// The data gradient of a convolution is the same kernel with the weights
+// flipped and transposed; the weight gradient is a second MFMA kernel with
+// K = pixels.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <cmath>
+#include <cstdio>
+#include <string>
+
+#include "otfwi_unet.h"
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+thread_local std::string g_uerr;
+
+int ufail(const char *msg)
+{
+    g_uerr = msg;
+    return -1;
+}
+
+int hipck(hipError_t e)
+{
+    if (e == hipSuccess) return 0;
+    g_uerr = hipGetErrorString(e);
+    return -2;
+}
+
+// ---------------------------------------------------------------------------
+// input of a convolution: [A | B] along channels; B at half resolution when
+// b_half (nearest upsample), zero outside the image and beyond Cin
+// ---------------------------------------------------------------------------
+struct Src {
+    const bf16 *A; int CA;
+    const bf16 *B; int CB; int b_half;
+    int H, W;
+};
+
+__device__ inline u32x4 load8(const Src &s, int n, int y, int x, int ch)
+{
+    const u32x4 z = {0u, 0u, 0u, 0u};
+    if (y < 0 || y >= s.H || x < 0 || x >= s.W) return z;
+    if (ch < s.CA)
+        return *(const u32x4 *)(s.A + (((size_t)n * s.H + y) * s.W + x) * s.CA + ch);
+    const int c = ch - s.CA;
+    if (c >= s.CB) return z;
+    if (s.b_half) {
+        const int Hb = s.H >> 1, Wb = s.W >> 1;
+        return *(const u32x4 *)(s.B + (((size_t)n * Hb + (y >> 1)) * Wb + (x >> 1)) * s.CB + c);
+    }
+    return *(const u32x4 *)(s.B + (((size_t)n * s.H + y) * s.W + x) * s.CB + c);
+}
+
+struct ConvArgs {
+    Src in;
+    const bf16 *Wt;      // [Cout][9][Cin], tap = 3*dy + dx
+    const float *bias;   // [Cout] or null
+    const bf16 *mask;    // [N][H][W][Cout] or null: zero the output where mask <= 0
+    bf16 *Y;             // [N][H][W][Cout]
+    int Cout, relu;
+};
+
+template <int TX, int TN, int CK>
+struct ConvShape {
+    static constexpr int TY = 256 / TX;
+    static constexpr int PW = TX + 2, PH = TY + 2;
+    static constexpr int PS = CK + 8;   // LDS stride of one patch pixel (bf16)
+    static constexpr int WS = CK + 8;   // LDS stride of one weight row
+    static constexpr size_t lds_bytes = ((size_t)PH * PW * PS + (size_t)9 * TN * WS) * 2;
+};
+
+template <int TX, int TN, int CK>
+__global__ __launch_bounds__(256) void conv3x3_kernel(ConvArgs a)
+{
+    using S = ConvShape<TX, TN, CK>;
+    constexpr int TY = S::TY, PW = S::PW, PH = S::PH, PS = S::PS, WS = S::WS;
+    constexpr int NJ = TN / 32, PV = CK / 8;
+    extern __shared__ __attribute__((aligned(16))) bf16 lds[];
+    bf16 *patch = lds;
+    bf16 *wl = lds + PH * PW * PS;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int H = a.in.H, W = a.in.W;
+    const int tiles_x = W / TX;
+    const int tx0 = (blockIdx.x % tiles_x) * TX, ty0 = (blockIdx.x / tiles_x) * TY;
+    const int co0 = blockIdx.y * TN, n = blockIdx.z;
+    const int Cin = a.in.CA + a.in.CB;
+    const int r = lane & 31, kh8 = (lane >> 5) * 8;
+
+    f32x16 acc[2][NJ];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < NJ; j++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) acc[i][j][e] = 0.0f;
+
+    int pbase[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const int q = wave * 64 + i * 32 + r;
+        pbase[i] = (q / TX) * PW + (q % TX);
+    }
+
+    for (int c0 = 0; c0 < Cin; c0 += CK) {
+        for (int v = tid; v < PH * PW * PV; v += 256) {
+            const int pp = v / PV, part = v - pp * PV;
+            const int py = pp / PW, px = pp - py * PW;
+            *(u32x4 *)(patch + pp * PS + part * 8) =
+                load8(a.in, n, ty0 + py - 1, tx0 + px - 1, c0 + part * 8);
+        }
+        for (int v = tid; v < 9 * TN * PV; v += 256) {
+            const int row = v / PV, part = v - row * PV;   // row = tap*TN + co
+            const int tap = row / TN, co = row - tap * TN;
+            *(u32x4 *)(wl + row * WS + part * 8) =
+                *(const u32x4 *)(a.Wt + ((size_t)(co0 + co) * 9 + tap) * Cin + c0 + part * 8);
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int tap = 0; tap < 9; tap++) {
+            const int dy = tap / 3, dx = tap - 3 * (tap / 3);
+#pragma unroll
+            for (int kk = 0; kk < CK; kk += 16) {
+                bf16x8 af[2], bfm[NJ];
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+                    af[i] = *(const bf16x8 *)(patch + (pbase[i] + dy * PW + dx) * PS + kk + kh8);
+#pragma unroll
+                for (int j = 0; j < NJ; j++)
+                    bfm[j] = *(const bf16x8 *)(wl + (tap * TN + j * 32 + r) * WS + kk + kh8);
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+#pragma unroll
+                    for (int j = 0; j < NJ; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i], bfm[j],
+                                                                            acc[i][j], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+
+    // C layout of the 32x32 MFMA: lane holds column (lane % 32) and rows
+    // 8*(e/4) + 4*(lane/32) + e%4 for e = 0..15.
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+        const int co = co0 + j * 32 + r;
+        const float b = a.bias ? a.bias[co] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                const int row = 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+                const int q = wave * 64 + i * 32 + row;
+                const int y = ty0 + q / TX, x = tx0 + q % TX;
+                const size_t o = (((size_t)n * H + y) * W + x) * a.Cout + co;
+                float v = acc[i][j][e] + b;
+                if (a.relu) v = fmaxf(v, 0.0f);
+                if (a.mask && !((float)a.mask[o] > 0.0f)) v = 0.0f;
+                a.Y[o] = (bf16)v;
+            }
+    }
+}
+
+template <int TX, int TN, int CK>
+int launch_conv(const ConvArgs &a, int N, hipStream_t st)
+{
+    using S = ConvShape<TX, TN, CK>;
+    const void *fn = (const void *)conv3x3_kernel<TX, TN, CK>;
+    if (int rc = hipck(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)S::lds_bytes)))
+        return rc;
+    dim3 grid((a.in.W / TX) * (a.in.H / S::TY), a.Cout / TN, N);
+    hipLaunchKernelGGL((conv3x3_kernel<TX, TN, CK>), grid, dim3(256), S::lds_bytes, st, a);
+    return hipck(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// weight gradient: dW[co][tap][ci] += sum_p G[p][co] * X[p + tap][ci]
+// M = 32*MJ output channels, N = 32 input channels, K = pixels.  Wave w
+// (of 3) takes the taps of row dy = w.  Each work-group runs tiles_per_wg
+// pixel tiles and then adds its partial sums with FP32 atomics.
+// ---------------------------------------------------------------------------
+struct WgradArgs {
+    Src in;
+    const bf16 *G;   // [N][H][W][Cout]
+    float *dW;       // [Cout][9][Cin]
+    int N, Cout, tiles_per_wg;
+};
+
+template <int TX, int MJ>
+struct WgShape {
+    static constexpr int TY = 256 / TX;
+    static constexpr int PW = TX + 8, PH = TY + 2;   // transposed patch row (px contiguous)
+    static constexpr int GS = 256 + 8;               // transposed gradient row
+    static constexpr size_t lds_bytes = ((size_t)32 * MJ * GS + (size_t)32 * PH * PW) * 2;
+};
+
+template <int TX, int MJ>
+__global__ __launch_bounds__(192) void wgrad_kernel(WgradArgs a)
+{
+    using S = WgShape<TX, MJ>;
+    constexpr int TY = S::TY, PW = S::PW, PH = S::PH, GS = S::GS;
+    constexpr int CO = 32 * MJ;
+    extern __shared__ __attribute__((aligned(16))) bf16 lds[];
+    bf16 *gT = lds;                  // [CO][GS]: gradient tile, pixels contiguous
+    bf16 *xT = lds + CO * GS;        // [32][PH][PW]: input patch, pixels contiguous
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int H = a.in.H, W = a.in.W;
+    const int Cin = a.in.CA + a.in.CB;
+    const int co0 = blockIdx.y * CO, ci0 = blockIdx.z * 32;
+    const int r = lane & 31, kh8 = (lane >> 5) * 8;
+    const int tiles_x = W / TX, tiles_img = tiles_x * (H / TY);
+    const int tiles_all = tiles_img * a.N;
+
+    f32x16 acc[MJ][3];
+#pragma unroll
+    for (int m = 0; m < MJ; m++)
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) acc[m][d][e] = 0.0f;
+
+    const int t_begin = blockIdx.x * a.tiles_per_wg;
+    const int t_end = min(t_begin + a.tiles_per_wg, tiles_all);
+    for (int t = t_begin; t < t_end; t++) {
+        const int n = t / tiles_img, tt = t - n * tiles_img;
+        const int tx0 = (tt % tiles_x) * TX, ty0 = (tt / tiles_x) * TY;
+        // gradient tile -> gT[co][q]
+        for (int v = tid; v < 256 * (CO / 8); v += 192) {
+            const int q = v / (CO / 8), part = v - q * (CO / 8);
+            const int y = ty0 + q / TX, x = tx0 + q % TX;
+            const u32x4 g =
+                *(const u32x4 *)(a.G + (((size_t)n * H + y) * W + x) * a.Cout + co0 + part * 8);
+            const bf16 *gv = (const bf16 *)&g;
+#pragma unroll
+            for (int k = 0; k < 8; k++) gT[(part * 8 + k) * GS + q] = gv[k];
+        }
+        // input patch -> xT[ci][py][px]
+        for (int v = tid; v < PH * (TX + 2) * 4; v += 192) {
+            const int pp = v >> 2, part = v & 3;
+            const int py = pp / (TX + 2), px = pp - py * (TX + 2);
+            const u32x4 xv = load8(a.in, n, ty0 + py - 1, tx0 + px - 1, ci0 + part * 8);
+            const bf16 *xs = (const bf16 *)&xv;
+#pragma unroll
+            for (int k = 0; k < 8; k++) xT[((part * 8 + k) * PH + py) * PW + px] = xs[k];
+        }
+        __syncthreads();
+        const int dy = wave;
+#pragma unroll 2
+        for (int ks = 0; ks < 16; ks++) {
+            const int q0 = ks * 16 + kh8;             // 8 consecutive pixels of one tile row
+            const int ty = q0 / TX, tx = q0 % TX;
+            bf16x8 af[MJ];
+#pragma unroll
+            for (int m = 0; m < MJ; m++)
+                af[m] = *(const bf16x8 *)(gT + (m * 32 + r) * GS + q0);
+            const bf16 *xrow = xT + (r * PH + ty + dy) * PW + tx;
+#pragma unroll
+            for (int dx = 0; dx < 3; dx++) {
+                bf16x8 bv;
+#pragma unroll
+                for (int k = 0; k < 8; k++) bv[k] = xrow[dx + k];
+#pragma unroll
+                for (int m = 0; m < MJ; m++)
+                    acc[m][dx] =
+                        __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[m], bv, acc[m][dx], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    const int ci = ci0 + r;
+    if (ci >= Cin) return;
+#pragma unroll
+    for (int m = 0; m < MJ; m++)
+#pragma unroll
+        for (int dx = 0; dx < 3; dx++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                const int co = co0 + m * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+                atomicAdd(a.dW + ((size_t)co * 9 + 3 * wave + dx) * Cin + ci, acc[m][dx][e]);
+            }
+}
+
+template <int TX, int MJ>
+int launch_wgrad(const WgradArgs &a, hipStream_t st)
+{
+    using S = WgShape<TX, MJ>;
+    const void *fn = (const void *)wgrad_kernel<TX, MJ>;
+    if (int rc = hipck(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)S::lds_bytes)))
+        return rc;
+    const int tiles = (a.in.W / TX) * (a.in.H / S::TY) * a.N;
+    const int Cin = a.in.CA + a.in.CB;
+    dim3 grid((tiles + a.tiles_per_wg - 1) / a.tiles_per_wg, a.Cout / (32 * MJ), (Cin + 31) / 32);
+    hipLaunchKernelGGL((wgrad_kernel<TX, MJ>), grid, dim3(192), S::lds_bytes, st, a);
+    return hipck(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// small kernels
+// ---------------------------------------------------------------------------
+__global__ void flip_weights_kernel(const bf16 *W, int Cout, int Cin, bf16 *Wf)
+{
+    // Wf[ci][tap][co] = W[co][8 - tap][ci]
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n = (size_t)Cout * 9 * Cin;
+    if (i >= n) return;
+    const int co = (int)(i % Cout);
+    const int tap = (int)((i / Cout) % 9);
+    const int ci = (int)(i / ((size_t)Cout * 9));
+    Wf[i] = W[((size_t)co * 9 + (8 - tap)) * Cin + ci];
+}
+
+__global__ void pool2_kernel(const bf16 *X, int N, int H, int W, int C, bf16 *Y)
+{
+    // Y[n][y][x][c] = mean of X over the 2x2 block; 8 channels per thread
+    const int Ho = H / 2, Wo = W / 2, C8 = C / 8;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)N * Ho * Wo * C8) return;
+    const int c = (int)(i % C8) * 8;
+    size_t p = i / C8;
+    const int x = (int)(p % Wo); p /= Wo;
+    const int y = (int)(p % Ho);
+    const int n = (int)(p / Ho);
+    float s[8] = {};
+    for (int dy = 0; dy < 2; dy++)
+        for (int dx = 0; dx < 2; dx++) {
+            const u32x4 v =
+                *(const u32x4 *)(X + (((size_t)n * H + 2 * y + dy) * W + 2 * x + dx) * C + c);
+            const bf16 *b = (const bf16 *)&v;
+            for (int k = 0; k < 8; k++) s[k] += (float)b[k];
+        }
+    u32x4 o;
+    bf16 *ob = (bf16 *)&o;
+    for (int k = 0; k < 8; k++) ob[k] = (bf16)(0.25f * s[k]);
+    *(u32x4 *)(Y + (((size_t)n * Ho + y) * Wo + x) * C + c) = o;
+}
+
+// out[n][y][x][c] (coarse, C channels) = mask > 0 ? sum over the 2x2 fine
+// pixels of src[.][.][.][off + c] : 0   (backward of nearest upsample + ReLU)
+__global__ void upsum_mask_kernel(const bf16 *src, int Cs, int off, const bf16 *mask, int N,
+                                  int Ho, int Wo, int C, bf16 *out)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)N * Ho * Wo * C) return;
+    const int c = (int)(i % C);
+    size_t p = i / C;
+    const int x = (int)(p % Wo); p /= Wo;
+    const int y = (int)(p % Ho);
+    const int n = (int)(p / Ho);
+    float v = 0.0f;
+    if ((float)mask[i] > 0.0f) {
+        const int H = 2 * Ho, W = 2 * Wo;
+        for (int dy = 0; dy < 2; dy++)
+            for (int dx = 0; dx < 2; dx++)
+                v += (float)src[(((size_t)n * H + 2 * y + dy) * W + 2 * x + dx) * Cs + off + c];
+    }
+    out[i] = (bf16)v;
+}
+
+// out[n][y][x][c] = mask > 0 ? src[.][off + c] + below[n][y/2][x/2][c] / 4 : 0
+// (gradient of an encoder output: skip path + average-pool path, then ReLU)
+__global__ void skip_combine_kernel(const bf16 *src, int Cs, int off, const bf16 *below,
+                                    const bf16 *mask, int N, int H, int W, int C, bf16 *out)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)N * H * W * C) return;
+    const int c = (int)(i % C);
+    size_t p = i / C;
+    const int x = (int)(p % W); p /= W;
+    const int y = (int)(p % H);
+    const int n = (int)(p / H);
+    float v = 0.0f;
+    if ((float)mask[i] > 0.0f) {
+        v = (float)src[(((size_t)n * H + y) * W + x) * Cs + off + c];
+        v += 0.25f * (float)below[(((size_t)n * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c];
+    }
+    out[i] = (bf16)v;
+}
+
+// x0[n][y][x][0..15] = (in[n][0][y][x], in[n][1][y][x], 0, ...) in BF16
+__global__ void pack_input_kernel(const float *in, int N, int H, int W, bf16 *x0)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t hw = (size_t)H * W;
+    if (i >= (size_t)N * hw) return;
+    const size_t n = i / hw, p = i - n * hw;
+    u32x4 v0 = {0u, 0u, 0u, 0u}, v1 = {0u, 0u, 0u, 0u};
+    bf16 *b = (bf16 *)&v0;
+    b[0] = (bf16)in[(n * 2) * hw + p];
+    b[1] = (bf16)in[(n * 2 + 1) * hw + p];
+    *(u32x4 *)(x0 + i * 16) = v0;
+    *(u32x4 *)(x0 + i * 16 + 8) = v1;
+}
+
+// out[p] = b + sum_c X[p][c] * w[c]  (final 1x1 convolution, C = 32)
+__global__ void head_fwd_kernel(const bf16 *X, const float *w, const float *b, size_t np,
+                                float *out)
+{
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    float s = b[0];
+    for (int c8 = 0; c8 < 32; c8 += 8) {
+        const u32x4 v = *(const u32x4 *)(X + p * 32 + c8);
+        const bf16 *xb = (const bf16 *)&v;
+        for (int k = 0; k < 8; k++) s = fmaf((float)xb[k], w[c8 + k], s);
+    }
+    out[p] = s;
+}
+
+// MSE head backward: g = scale*(out - target); loss += sum g^2-like term
+// (sum (out-target)^2); gX[p][c] = X[p][c] > 0 ? g*w[c] : 0 (BF16);
+// dw[c] += sum_p g*X[p][c]; db += sum_p g.  256 threads per block.
+__global__ __launch_bounds__(256) void head_bwd_kernel(const bf16 *X, const float *w,
+                                                       const float *out, const float *target,
+                                                       size_t np, float scale, bf16 *gX,
+                                                       float *dw, float *db, double *loss)
+{
+    __shared__ float red[256][34];
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float part[34];
+    for (int k = 0; k < 34; k++) part[k] = 0.0f;
+    if (p < np) {
+        const float d = out[p] - target[p];
+        const float g = scale * d;
+        part[32] = g;
+        part[33] = d * d;
+        for (int c8 = 0; c8 < 32; c8 += 8) {
+            const u32x4 v = *(const u32x4 *)(X + p * 32 + c8);
+            const bf16 *xb = (const bf16 *)&v;
+            u32x4 o;
+            bf16 *ob = (bf16 *)&o;
+            for (int k = 0; k < 8; k++) {
+                const float xv = (float)xb[k];
+                part[c8 + k] = g * xv;
+                ob[k] = (bf16)(xv > 0.0f ? g * w[c8 + k] : 0.0f);
+            }
+            *(u32x4 *)(gX + p * 32 + c8) = o;
+        }
+    }
+    for (int k = 0; k < 34; k++) red[threadIdx.x][k] = part[k];
+    __syncthreads();
+    if (threadIdx.x < 34) {
+        float s = 0.0f;
+        for (int t = 0; t < 256; t++) s += red[t][threadIdx.x];
+        if (threadIdx.x < 32) atomicAdd(dw + threadIdx.x, s);
+        else if (threadIdx.x == 32) atomicAdd(db, s);
+        else atomicAdd(loss, (double)s);
+    }
+}
+
+// dB[c] += sum over pixels of G[p][c]; each block sums 1024 pixels
+__global__ __launch_bounds__(256) void bias_grad_kernel(const bf16 *G, size_t np, int C, float *dB)
+{
+    __shared__ float red[256];
+    const int per = 256 / C;                  // pixel lanes per channel (C divides 256)
+    const int c = threadIdx.x % C, lane = threadIdx.x / C;
+    const size_t p0 = (size_t)blockIdx.x * 1024;
+    float s = 0.0f;
+    if (lane < per)
+        for (size_t p = p0 + lane; p < p0 + 1024 && p < np; p += per) s += (float)G[p * C + c];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < C) {
+        float t = 0.0f;
+        for (int l = 0; l < per; l++) t += red[l * C + threadIdx.x];
+        atomicAdd(dB + threadIdx.x, t);
+    }
+}
+
+__global__ void adam_kernel(float *p, const float *g, float *m, float *v, bf16 *pb, size_t n,
+                            float lr, float b1, float b2, float eps, float bc1, float bc2)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float pi = p[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+    p[i] = pi;
+    if (pb) pb[i] = (bf16)pi;
+}
+
+__global__ void to_bf16_kernel(const float *x, size_t n, bf16 *y)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = (bf16)x[i];
+}
+
+inline unsigned blocks(size_t n, unsigned b = 256) { return (unsigned)((n + b - 1) / b); }
+
+bool al16(const void *p) { return ((uintptr_t)p & 15) == 0; }
+
+int check_src(const Src &s, int N)
+{
+    if (N <= 0 || s.H <= 0 || s.W <= 0) return ufail("bad tensor shape");
+    if (s.CA <= 0 || s.CA % 8 || s.CB < 0 || s.CB % 8)
+        return ufail("channel counts must be positive multiples of 8");
+    if (!s.A || !al16(s.A) || (s.CB && (!s.B || !al16(s.B))))
+        return ufail("input tensors must be non-null and 16-byte aligned");
+    if (s.CB && s.b_half && (s.H % 2 || s.W % 2)) return ufail("upsampled input needs even H, W");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *otfwi_unet_last_error(void) { return g_uerr.c_str(); }
+
+#define BF(p) reinterpret_cast<const bf16 *>(p)
+#define BFW(p) reinterpret_cast<bf16 *>(p)
+#define ST(s) reinterpret_cast<hipStream_t>(s)
+
+int otfwi_unet_conv3x3(const otfwi_bf16 *A, int CA, const otfwi_bf16 *B, int CB, int b_half,
+                       const otfwi_bf16 *Wt, const float *bias, const otfwi_bf16 *mask,
+                       otfwi_bf16 *Y, int N, int H, int W, int Cout, int relu, void *stream)
+{
+    ConvArgs a{};
+    a.in = Src{BF(A), CA, BF(B), CB, b_half, H, W};
+    a.Wt = BF(Wt); a.bias = bias; a.mask = BF(mask); a.Y = BFW(Y); a.Cout = Cout; a.relu = relu;
+    if (int rc = check_src(a.in, N)) return rc;
+    if (!Wt || !Y || !al16(Wt) || Cout <= 0 || Cout % 32)
+        return ufail("bad weights/output (Cout must be a multiple of 32)");
+    const int Cin = CA + CB;
+    const int CK = Cin % 32 == 0 ? 32 : (Cin == 16 ? 16 : 0);
+    if (!CK || (CB && CA % 32)) return ufail("Cin must be 16 or a multiple of 32 (CA too)");
+    const int TX = W % 64 == 0 ? 64 : (W % 32 == 0 ? 32 : 0);
+    if (!TX || H % (256 / TX)) return ufail("W must be a multiple of 32 and H of the tile height");
+    const int TN = Cout % 64 == 0 ? 64 : 32;
+    hipStream_t st = ST(stream);
+    if (CK == 16) {
+        if (TX == 64)
+            return TN == 64 ? launch_conv<64, 64, 16>(a, N, st) : launch_conv<64, 32, 16>(a, N, st);
+        return TN == 64 ? launch_conv<32, 64, 16>(a, N, st) : launch_conv<32, 32, 16>(a, N, st);
+    }
+    if (TX == 64)
+        return TN == 64 ? launch_conv<64, 64, 32>(a, N, st) : launch_conv<64, 32, 32>(a, N, st);
+    return TN == 64 ? launch_conv<32, 64, 32>(a, N, st) : launch_conv<32, 32, 32>(a, N, st);
+}
+
+int otfwi_unet_wgrad(const otfwi_bf16 *A, int CA, const otfwi_bf16 *B, int CB, int b_half,
+                     const otfwi_bf16 *G, float *dW, int N, int H, int W, int Cout,
+                     int tiles_per_wg, void *stream)
+{
+    WgradArgs a{};
+    a.in = Src{BF(A), CA, BF(B), CB, b_half, H, W};
+    a.G = BF(G); a.dW = dW; a.N = N; a.Cout = Cout; a.tiles_per_wg = tiles_per_wg;
+    if (int rc = check_src(a.in, N)) return rc;
+    if (!G || !dW || !al16(G) || Cout <= 0 || Cout % 32)
+        return ufail("bad gradient/output (Cout must be a multiple of 32)");
+    if (tiles_per_wg <= 0) return ufail("tiles_per_wg must be positive");
+    if (CB && CA % 32) return ufail("CA must be a multiple of 32 with a second input");
+    const int TX = W % 64 == 0 ? 64 : (W % 32 == 0 ? 32 : 0);
+    if (!TX || H % (256 / TX)) return ufail("W must be a multiple of 32 and H of the tile height");
+    hipStream_t st = ST(stream);
+    if (Cout % 64 == 0)
+        return TX == 64 ? launch_wgrad<64, 2>(a, st) : launch_wgrad<32, 2>(a, st);
+    return TX == 64 ? launch_wgrad<64, 1>(a, st) : launch_wgrad<32, 1>(a, st);
+}
+
+int otfwi_unet_flip_weights(const otfwi_bf16 *W, int Cout, int Cin, otfwi_bf16 *Wf, void *stream)
+{
+    if (!W || !Wf || Cout <= 0 || Cin <= 0) return ufail("bad weights");
+    const size_t n = (size_t)Cout * 9 * Cin;
+    hipLaunchKernelGGL(flip_weights_kernel, dim3(blocks(n)), dim3(256), 0, ST(stream), BF(W), Cout,
+                       Cin, BFW(Wf));
+    return hipck(hipGetLastError());
+}
+
+int otfwi_unet_pool2(const otfwi_bf16 *X, int N, int H, int W, int C, otfwi_bf16 *Y, void *stream)
+{
+    if (!X || !Y || C % 8 || H % 2 || W % 2 || !al16(X) || !al16(Y)) return ufail("bad pool args");
+    const size_t n = (size_t)N * (H / 2) * (W / 2) * (C / 8);
+    hipLaunchKernelGGL(pool2_kernel, dim3(blocks(n)), dim3(256), 0, ST(stream), BF(X), N, H, W, C,
+                       BFW(Y));
+    return hipck(hipGetLastError());
+}
+
+int otfwi_unet_upsum_mask(const otfwi_bf16 *src, int Cs, int off, const otfwi_bf16 *mask, int N,
+                          int Ho, int Wo, int C, otfwi_bf16 *out, void *stream)
+{
+    if (!src || !mask || !out || off < 0 || off + C > Cs) return ufail("bad upsum args");
+    const size_t n = (size_t)N * Ho * Wo * C;
+    hipLaunchKernelGGL(upsum_mask_kernel, dim3(blocks(n)), dim3(256), 0, ST(stream), BF(src), Cs,
+                       off, BF(mask), N, Ho, Wo, C, BFW(out));
+    return hipck(hipGetLastError());
+}
+
+int otfwi_unet_skip_combine(const otfwi_bf16 *src, int Cs, int off, const otfwi_bf16 *below,
+                            const otfwi_bf16 *mask, int N, int H, int W, int C, otfwi_bf16 *out,
+                            void *stream)
+{
+    if (!src || !below || !mask || !out || off < 0 || off + C > Cs || H % 2 || W % 2)
+        return ufail("bad skip-combine args");
+    const size_t n = (size_t)N * H * W * C;
+    hipLaunchKernelGGL(skip_combine_kernel, dim3(blocks(n)), dim3(256), 0, ST(stream), BF(src), Cs,
+                       off, BF(below), BF(mask), N, H, W, C, BFW(out));
+    return hipck(hipGetLastError());
+}
+
+int otfwi_unet_pack_input(const float *in, int N, int H, int W, otfwi_bf16 *x0, void *stream)
+{
+    if (!in || !x0 || !al16(x0)) return ufail("bad pack args");
+    const size_t n = (size_t)N * H * W;
+    hipLaunchKernelGGL(pack_input_kernel, dim3(blocks(n)), dim3(256), 0, ST(stream), in, N, H, W,
+                       BFW(x0));
+    return hipck(hipGetLastError());
+}
+
+int otfwi_unet_head_fwd(const otfwi_bf16 *X, const float *w, const float *b, long long np,
+                        float *out, void *stream)
+{
+    if (!X || !w || !b || !out || np <= 0 || !al16(X)) return ufail("bad head args");
+    hipLaunchKernelGGL(head_fwd_kernel, dim3(blocks((size_t)np)), dim3(256), 0, ST(stream), BF(X),
+                       w, b, (size_t)np, out);
+    return hipck(hipGetLastError());
+}
+
+int otfwi_unet_head_bwd(const otfwi_bf16 *X, const float *w, const float *out,
+                        const float *target, long long np, float scale, otfwi_bf16 *gX, float *dw,
+                        float *db, double *loss, void *stream)
+{
+    if (!X || !w || !out || !target || !gX || !dw || !db || !loss || np <= 0 || !al16(X) ||
+        !al16(gX))
+        return ufail("bad head-backward args");
+    hipLaunchKernelGGL(head_bwd_kernel, dim3(blocks((size_t)np)), dim3(256), 0, ST(stream), BF(X),
+                       w, out, target, (size_t)np, scale, BFW(gX), dw, db, loss);
+    return hipck(hipGetLastError());
+}
+
+int otfwi_unet_bias_grad(const otfwi_bf16 *G, long long np, int C, float *dB, void *stream)
+{
+    if (!G || !dB || np <= 0 || C <= 0 || C > 256 || 256 % C) return ufail("bad bias-grad args");
+    hipLaunchKernelGGL(bias_grad_kernel, dim3(blocks((size_t)np, 1024)), dim3(256), 0, ST(stream),
+                       BF(G), (size_t)np, C, dB);
+    return hipck(hipGetLastError());
+}
+
+int otfwi_unet_adam(float *p, const float *g, float *m, float *v, otfwi_bf16 *pb, long long n,
+                    float lr, float b1, float b2, float eps, int step, void *stream)
+{
+    if (!p || !g || !m || !v || n <= 0 || step <= 0) return ufail("bad adam args");
+    const float bc1 = 1.0f - powf(b1, (float)step), bc2 = 1.0f - powf(b2, (float)step);
+    hipLaunchKernelGGL(adam_kernel, dim3(blocks((size_t)n)), dim3(256), 0, ST(stream), p, g, m, v,
+                       BFW(pb), (size_t)n, lr, b1, b2, eps, bc1, bc2);
+    return hipck(hipGetLastError());
+}
+
+int otfwi_unet_to_bf16(const float *x, long long n, otfwi_bf16 *y, void *stream)
+{
+    if (!x || !y || n <= 0) return ufail("bad conversion args");
+    hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks((size_t)n)), dim3(256), 0, ST(stream), x,
+                       (size_t)n, BFW(y));
+    return hipck(hipGetLastError());
+}
+
+}  // extern "C"

@@ -155,6 +155,30 @@ class PMLModel:
                                               ctypes.c_void_p(int(grad_dev_ptr))))
         return mis
 
+    def objective(self, shot_src=(-1,), misfit="w2"):
+        """Per-shot misfit (FP64) of the current model: forward runs and the
+        device misfit, no adjoint."""
+        kind = {"l2": _lib.MISFIT_L2, "w2": _lib.MISFIT_W2}[misfit]
+        s, n = self._shots(shot_src)
+        mis = np.zeros(n, np.float64)
+        check(self._lib.otfwi_objective(self._h, n, ptr(s), kind, ptr(mis)))
+        return mis
+
+    def set_illum(self, stride):
+        """Collect the source-side illumination (sum of P^2 over shots and
+        every stride-th step, times stride) during the next gradient calls;
+        0 switches it off."""
+        check(self._lib.otfwi_set_illum(self._h, int(stride)))
+
+    def illum(self):
+        """(Z,X) FP32 illumination map of the last gradient call."""
+        out = np.zeros((self.Z, self.X), np.float32)
+        check(self._lib.otfwi_get_illum(self._h, ptr(out), 0))
+        return out
+
+    def illum_device(self, dev_ptr):
+        check(self._lib.otfwi_get_illum(self._h, ctypes.c_void_p(int(dev_ptr)), 1))
+
     def forward_for_grad(self, shot_src=(-1,)):
         """Stage 1 of a host-misfit gradient: traces of a forward run with the
         boundary ring saved on the device."""

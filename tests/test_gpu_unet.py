@@ -1,0 +1,203 @@
+"""Learned preconditioner kernels (libotfwi_unet.so) against the PyTorch-CPU
+FP32 oracle of the same operations (oracle/unet_ref.py).
+
+Bars: a single convolution within 5e-3 relative L2 of FP32 on the same BF16
+operands (its output is stored in BF16: 2^-9 relative rounding); the weight
+gradient (FP32 output) within 1e-4; the whole network's output within 1e-2
+and each layer's weight gradient within 3e-2 of the oracle that rounds to
+BF16 at the same points, since one rounding flip of an activation moves the
+result by one BF16 ulp (2^-8)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import unet_ref as R
+from otfwi_amd import _lib
+from otfwi_amd import unet as U
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def bf(x):
+    return torch.tensor(np.asarray(x, np.float32)).to(torch.bfloat16)
+
+
+def vp(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def st():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+@pytest.mark.parametrize("N,H,W,CA,CB,Cout", [
+    (2, 64, 64, 32, 64, 32),      # decoder concat, 64-wide tiles, TN 32
+    (1, 32, 64, 64, 0, 64),       # plain, 64-wide tiles, TN 64
+    (2, 32, 32, 128, 256, 128),   # concat at the coarsest width (32-wide tiles)
+    (1, 64, 128, 16, 0, 32),      # 16-channel input layer (CK 16)
+])
+def test_conv3x3_vs_oracle(gpu, N, H, W, CA, CB, Cout):
+    rng = np.random.default_rng(N * 1000 + H + CA + CB)
+    lib = _lib.load_unet()
+    A = bf(rng.standard_normal((N, H, W, CA)))
+    B = bf(rng.standard_normal((N, H // 2, W // 2, CB))) if CB else None
+    w = bf(rng.standard_normal((Cout, 9, CA + CB)) / np.sqrt(9 * (CA + CB)))
+    bias = rng.standard_normal(Cout).astype(np.float32) * 0.1
+    dA, dB, dw = A.cuda(), None if B is None else B.cuda(), w.cuda()
+    db = torch.tensor(bias).cuda()
+    Y = torch.empty((N, H, W, Cout), dtype=torch.bfloat16, device="cuda")
+    _lib.ucheck(lib.otfwi_unet_conv3x3(vp(dA), CA, vp(dB), CB, 1 if CB else 0, vp(dw), vp(db),
+                                       None, vp(Y), N, H, W, Cout, 1, st()))
+    torch.cuda.synchronize()
+    ref = R.conv3x3(A.float().numpy(), None if B is None else B.float().numpy(),
+                    w.float().numpy(), bias, relu=True)
+    e = rel(Y.float().cpu().numpy(), ref)
+    print(f"conv3x3 N{N} {H}x{W} {CA}+{CB}->{Cout}: rel {e:.2e}")
+    assert e < 5e-3
+
+
+@pytest.mark.parametrize("N,H,W,CA,CB,Cout", [
+    (2, 64, 64, 32, 64, 32),
+    (1, 32, 32, 128, 256, 128),
+    (1, 64, 64, 16, 0, 32),
+    (2, 32, 64, 64, 0, 64),
+])
+def test_wgrad_and_data_grad_vs_autograd(gpu, N, H, W, CA, CB, Cout):
+    rng = np.random.default_rng(7 + H + CA)
+    lib = _lib.load_unet()
+    A = bf(rng.standard_normal((N, H, W, CA)))
+    B = bf(rng.standard_normal((N, H // 2, W // 2, CB))) if CB else None
+    w = bf(rng.standard_normal((Cout, 9, CA + CB)) / np.sqrt(9 * (CA + CB)))
+    G = bf(rng.standard_normal((N, H, W, Cout)))
+    # oracle: autograd of sum(G * conv(x)) in FP32 on the same BF16 values
+    a = A.float().permute(0, 3, 1, 2)
+    if CB:
+        a = torch.cat([a, torch.nn.functional.interpolate(B.float().permute(0, 3, 1, 2),
+                                                          scale_factor=2, mode="nearest")], 1)
+    a.requires_grad_()
+    wt = w.float().reshape(Cout, 3, 3, CA + CB).permute(0, 3, 1, 2).contiguous().requires_grad_()
+    y = torch.nn.functional.conv2d(a, wt, padding=1)
+    (y * G.float().permute(0, 3, 1, 2)).sum().backward()
+    dw_ref = wt.grad.permute(0, 2, 3, 1).reshape(Cout, 9, CA + CB).numpy()
+    dx_ref = a.grad.permute(0, 2, 3, 1).numpy()
+
+    # device copies stay referenced until the kernels have run
+    Ad, Bd, Gc, dwc = A.cuda(), None if B is None else B.cuda(), G.cuda(), w.cuda()
+    dW = torch.zeros((Cout, 9, CA + CB), dtype=torch.float32, device="cuda")
+    _lib.ucheck(lib.otfwi_unet_wgrad(vp(Ad), CA, vp(Bd), CB, 1 if CB else 0, vp(Gc), vp(dW), N, H,
+                                     W, Cout, 3, st()))
+    wf = torch.empty((CA + CB) * 9 * Cout, dtype=torch.bfloat16, device="cuda")
+    _lib.ucheck(lib.otfwi_unet_flip_weights(vp(dwc), Cout, CA + CB, vp(wf), st()))
+    dX = torch.empty((N, H, W, CA + CB), dtype=torch.bfloat16, device="cuda")
+    if (CA + CB) % 32 == 0:
+        _lib.ucheck(lib.otfwi_unet_conv3x3(vp(Gc), Cout, None, 0, 0, vp(wf), None, None, vp(dX),
+                                           N, H, W, CA + CB, 0, st()))
+    torch.cuda.synchronize()
+    ew = rel(dW.cpu().numpy(), dw_ref)
+    print(f"wgrad {H}x{W} {CA}+{CB}->{Cout}: rel {ew:.2e}")
+    assert ew < 1e-4
+    if (CA + CB) % 32 == 0:
+        # the data gradient at full resolution (before the upsample's 2x2 sum)
+        ex = rel(dX.float().cpu().numpy(), dx_ref)
+        print(f"data grad: rel {ex:.2e}")
+        assert ex < 5e-3
+
+
+def test_network_forward_backward_vs_oracle(gpu):
+    N, H, W = 2, 256, 256
+    rng = np.random.default_rng(3)
+    p = U.init_params(seed=5)
+    lay, _ = U.param_layout()
+    off, nw, _, _ = lay["head"]
+    p[off:off + nw] = rng.standard_normal(nw).astype(np.float32) * 0.2
+    for name, ca, cb, co in U.CONVS:     # non-zero biases exercise the bias paths
+        _, _, boff, nb = lay[name]
+        p[boff:boff + nb] = rng.standard_normal(nb).astype(np.float32) * 0.05
+    zz, xx = np.mgrid[0:H, 0:W] / H
+    inp = np.stack([np.sin(7 * xx + 3 * zz + k) * np.cos(5 * zz - k) for k in range(2 * N)])
+    inp = (inp.reshape(N, 2, H, W) + 0.1 * rng.standard_normal((N, 2, H, W))).astype(np.float32)
+    inp = torch.tensor(inp).to(torch.bfloat16).float().numpy()
+    target = (0.3 * np.cos(4 * xx)[None] * np.ones((N, 1, 1))).astype(np.float32)
+
+    net = U.UNet(params=p)
+    out = net.forward(torch.tensor(inp).cuda())
+    sse = net.backward(out, torch.tensor(target).cuda())
+    torch.cuda.synchronize()
+    out = out.cpu().numpy()
+    g = net.g.cpu().numpy()
+    loss = float(sse.cpu().numpy()[0]) / out.size
+
+    o_ref, l_ref, g_ref = R.forward_backward(p, inp, target, bf16=True)
+    o32, l32, g32 = R.forward_backward(p, inp, target, bf16=False)
+    eo, el = rel(out, o_ref), abs(loss - l_ref) / l_ref
+    print(f"out rel {eo:.2e} (bf16 oracle vs fp32 {rel(o_ref, o32):.2e}), loss rel {el:.2e}")
+    assert eo < 1e-2 and el < 1e-2
+    worst = 0.0
+    for name, ca, cb, co in U.CONVS:
+        off, nw, boff, nb = lay[name]
+        w_idx = slice(off, off + nw)
+        if name == "enc0a":   # padded input channels carry no gradient
+            w_idx = np.arange(off, off + nw).reshape(co, 9, ca)[:, :, :U.IN_CH].ravel()
+        ew = rel(g[w_idx], g_ref[w_idx])
+        eb = rel(g[boff:boff + nb], g_ref[boff:boff + nb])
+        print(f"  {name}: dW rel {ew:.2e} (fp32 {rel(g_ref[w_idx], g32[w_idx]):.2e}), "
+              f"db rel {eb:.2e}")
+        worst = max(worst, ew, eb)
+    off, nw, boff, _ = lay["head"]
+    eh = rel(g[off:off + nw + 1], g_ref[off:off + nw + 1])
+    print(f"  head: rel {eh:.2e}")
+    assert worst < 3e-2 and eh < 1e-2
+
+
+def test_adam_step_matches_formula(gpu):
+    rng = np.random.default_rng(0)
+    net = U.UNet(seed=1)
+    p0 = net.params().copy()
+    g1 = rng.standard_normal(net.n).astype(np.float32)
+    g2 = rng.standard_normal(net.n).astype(np.float32)
+    m = np.zeros(net.n, np.float64)
+    v = np.zeros(net.n, np.float64)
+    p = p0.astype(np.float64)
+    lr, b1, b2, eps = 1e-3, 0.9, 0.999, 1e-8
+    for k, g in enumerate((g1, g2), 1):
+        net.g.copy_(torch.tensor(g))
+        net.adam(lr=lr)
+        m = b1 * m + (1 - b1) * g
+        v = b2 * v + (1 - b2) * g.astype(np.float64) ** 2
+        p = p - lr * (m / (1 - b1 ** k)) / (np.sqrt(v / (1 - b2 ** k)) + eps)
+    torch.cuda.synchronize()
+    got = net.params()
+    assert np.abs(got - p).max() < 1e-6
+    pb = net.pb.float().cpu().numpy()
+    assert np.array_equal(pb, torch.tensor(got).to(torch.bfloat16).float().numpy())
+
+
+def test_training_reduces_loss(gpu):
+    N, H, W = 4, 256, 256
+    rng = np.random.default_rng(1)
+    zz, xx = np.mgrid[0:H, 0:W] / H
+    inp = np.zeros((N, 2, H, W), np.float32)
+    tgt = np.zeros((N, H, W), np.float32)
+    for n in range(N):
+        cx, cz, r = rng.uniform(0.3, 0.7), rng.uniform(0.4, 0.8), rng.uniform(0.05, 0.12)
+        blob = (((xx - cx) ** 2 + (zz - cz) ** 2) < r * r).astype(np.float32)
+        inp[n, 0] = np.roll(blob, -8, axis=0) - np.roll(blob, 8, axis=0)   # edge-like "gradient"
+        inp[n, 1] = 1.0 - zz
+        tgt[n] = -0.5 * blob
+    net = U.UNet(seed=2)
+    x, t = torch.tensor(inp).cuda(), torch.tensor(tgt).cuda()
+    losses = []
+    for _ in range(40):
+        losses.append(float(net.train_step(x, t, lr=1e-3).item()) / tgt.size)
+    print("loss", losses[0], "->", losses[-1])
+    assert np.all(np.isfinite(losses))
+    assert losses[-1] < 0.5 * losses[0]

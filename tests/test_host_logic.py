@@ -81,3 +81,37 @@ def test_reference_abi_symbols_present():
     for n in ("init_memory_sim", "init_memory_reg", "setCquad", "cuda_simulate", "cuda_grad_ext",
               "def_python_adj"):
         assert hasattr(lib, n)
+
+
+def test_plain_direction_properties():
+    from otfwi_amd import inversion
+
+    rng = np.random.default_rng(0)
+    Z, X = 40, 50
+    g = rng.standard_normal((Z, X))
+    ill = rng.uniform(0.5, 2.0, (Z, X))
+    c = rng.uniform(0.2, 0.5, (Z, X))
+    mask = inversion.array_mask(Z, X, 5, 12, taper=4)
+    d = inversion.plain_direction(g, ill, c, mask)
+    assert np.isclose(np.abs(d).max(), 1.0)
+    assert np.all(d[mask == 0] == 0)
+    inside = mask > 0
+    # a descent direction: opposite sign to the gradient wherever it acts
+    assert np.all(np.sign(d[inside]) == -np.sign(g[inside]))
+    # positive rescaling of the gradient or of the illumination leaves it unchanged
+    assert np.allclose(inversion.plain_direction(7.0 * g, ill, c, mask), d)
+    assert np.allclose(inversion.plain_direction(g, 3.0 * ill, c, mask), d)
+    # preconditioning: with a uniform gradient the step is largest where the
+    # illumination is weakest
+    d1 = inversion.plain_direction(np.ones((Z, X)), ill, np.ones((Z, X)), np.ones((Z, X)))
+    k = np.unravel_index(np.argmax(np.abs(d1)), d1.shape)
+    assert ill[k] == ill.min()
+
+
+def test_array_mask_shape():
+    from otfwi_amd import inversion
+
+    m = inversion.array_mask(64, 64, 8, 20, taper=5)
+    assert m[:20].max() == 0 and m[:, :8].max() == 0 and m[-8:].max() == 0
+    assert m[30, 30] == 1.0
+    assert np.all(np.diff(m[20:26, 30]) > 0)

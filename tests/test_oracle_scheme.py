@@ -105,3 +105,22 @@ def test_pml_attenuates():
                        p["src"], p["rx_xz"], p["T"], precision="f64")["rec"]
     late = slice(350, None)
     assert np.abs(with_pml[:, late]).max() < 0.1 * np.abs(no_pml[:, late]).max()
+
+
+@pytest.mark.parametrize("stride", [1, 3])
+def test_illumination_matches_numpy_fields(stride):
+    p = small_problem(X=34, Z=30, T=70, pml=6, n_src=2, n_rx=3, seed=5, f0=0.08)
+    Z, X = p["Z"], p["X"]
+    zz, xx = np.mgrid[0:Z, 0:X]
+    every = (xx.ravel(), zz.ravel())
+    ill = np.zeros(Z * X)
+    for idx in (0, 1):
+        rec, _, _ = NP.forward(p["cq_bg"], p["d_x"], p["d_z"], p["src_xz"], p["src"], every,
+                               p["T"], idx=idx)
+        for t in range(1, p["T"]):
+            if t % stride == 0:
+                ill += stride * rec[:, t] ** 2
+    got = O.illumination(p["cq_bg"], p["d_x"], p["d_z"], p["src_xz"], p["src"], p["T"], [0, 1],
+                         stride, precision="f64")
+    assert np.abs(got.ravel() - ill).max() <= 1e-12 * np.abs(ill).max()
+    assert ill.max() > 0
