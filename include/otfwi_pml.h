@@ -183,6 +183,26 @@ int otfwi_gradient_device(otfwi_model *m, int n_shots, const int *shot_src, int 
 int otfwi_forward_for_grad(otfwi_model *m, int n_shots, const int *shot_src, float *rec_out);
 int otfwi_adjoint_from_host(otfwi_model *m, const float *adj_src, float *grad_out);
 
+/* Exact discrete adjoint gradient with snapshot checkpointing: the
+ * transpose of the forward recursion (not the reference's imaging
+ * condition), for J = mean over shots of the misfit value (L2: sum of
+ * (sim - obs)^2; W2: the reference W2 value, whose exact trace derivative is
+ * taken).  The forward state is saved every snap_every steps (<= 0:
+ * round(sqrt(T-1))) and each segment's A fields are recomputed before the
+ * adjoint runs back through it.  grad_out (FP32) and/or grad64_out (FP64)
+ * receive dJ/d(c^2) on the (Z,X) grid; misfit_out (may be NULL) the per-shot
+ * misfit.  No reference counterpart (the reference reconstructs the forward
+ * field by boundary saving, regression_PML_single.cu:758, 805, and images
+ * with the 9-point Laplacian, :390-445). */
+int otfwi_gradient_exact(otfwi_model *m, int n_shots, const int *shot_src, int misfit_kind,
+                         int snap_every, double *misfit_out, float *grad_out, double *grad64_out);
+
+/* c^2 in FP64 (for the FP64 build and finite-difference checks): the model
+ * is formed from these values directly instead of their FP32 rounding.
+ * Replaces setCquad (regression_PML_single.cu:253-257) when FP64 input is
+ * wanted. */
+int otfwi_set_cquad64(otfwi_model *m, const double *cquad);
+
 /* Misfit of the current model without the adjoint run: forward runs (no
  * ring save) and the device misfit; misfit_out receives n_shots FP64
  * per-shot sums.  Used by the misfit-decrease acceptance test of the
@@ -206,6 +226,14 @@ int otfwi_get_illum(otfwi_model *m, float *out, int out_on_device);
  * 25-67). */
 int otfwi_misfit(int misfit_kind, int n_traces, int T, const double *sim, const double *obs,
                  double *misfit_per_trace, double *adj_out);
+
+/* Exact derivative of the misfit value with respect to the simulated traces,
+ * on host arrays (FP64): L2 2 (sim - obs); W2 the derivative of the
+ * reference W2 value (adjoint_functions.py:25-67) through its interpolated
+ * inverse CDF, trapezoid CDF and normalization (needs sim + c >= 0).  The
+ * trace derivative used by otfwi_gradient_exact; no reference counterpart. */
+int otfwi_misfit_exact(int misfit_kind, int n_traces, int T, const double *sim,
+                       const double *obs, double *misfit_per_trace, double *rho_out);
 
 #ifdef __cplusplus
 }

@@ -219,6 +219,62 @@ def wasserstein_trace(f0, g0):
     return w, adj
 
 
+def wasserstein_exact_trace(f0, g0):
+    """Value of wasserstein_trace and its exact derivative with respect to the
+    simulated trace f0 (the build's exact-adjoint mode; the reference's own
+    adjoint source, adjoint_functions.py:58-62, is a different expression).
+    Follows :33-64 forward, then differentiates F^-1(t_k) on its
+    interpolation bracket, the trapezoid CDF and the normalization.  Needs a
+    monotone CDF of f0 + c.  FP64."""
+    T = len(f0)
+    t = np.linspace(0, 1, T)
+    dt = t[1] - t[0]
+    f0 = np.asarray(f0, dtype=np.float64)
+    g0 = np.asarray(g0, dtype=np.float64)
+    c = max(-1.1 * np.min(g0), 0.1 * np.max(g0))
+    g = g0 + c
+    h = f0 + c
+    g = g / np.sum(g * dt)
+    S = np.sum(h * dt)
+    f = h / S
+    G = _cumtrapz0(g, np.diff(t))
+    F = _cumtrapz0(f, np.diff(t))
+    if np.any(np.diff(F) < 0):
+        raise ValueError("CDF of sim + c is not monotone")
+    F1 = _interp_linear(F, t, t)
+    G1 = _interp_linear(G, t, t)
+    w = 0.0
+    for v in np.abs(F1 - G1) ** 2:
+        w += v
+    e = F1 - G1
+    i = np.clip(np.searchsorted(F, t, side="left"), 1, T - 1)
+    lo, hi = i - 1, i
+    d2 = (F[hi] - F[lo]) ** 2
+    dts = t[hi] - t[lo]
+    Fb = np.zeros(T)
+    np.add.at(Fb, lo, 2 * e * dts * (t - F[hi]) / d2)
+    np.add.at(Fb, hi, -2 * e * dts * (t - F[lo]) / d2)
+    suf = np.cumsum(Fb[::-1])[::-1]                    # suf[j] = sum_{i >= j} Fb[i]
+    d = np.diff(t)
+    fb = np.zeros(T)
+    fb[1:] += 0.5 * d * suf[1:]
+    fb[:-1] += 0.5 * d * suf[1:]
+    rho = fb / S - dt * np.dot(fb, h) / S ** 2
+    return w, rho
+
+
+def wasserstein_exact(sim, obs):
+    """Per-trace exact W2 derivative: (W2 sum over traces, dW2/dsim (n_rx, T))."""
+    sim = np.asarray(sim)
+    obs = np.asarray(obs)
+    out = np.zeros(sim.shape)
+    W2 = 0.0
+    for s in range(sim.shape[0]):
+        w, out[s] = wasserstein_exact_trace(sim[s], obs[s])
+        W2 += w
+    return W2, out
+
+
 def wasserstein(sim, obs):
     """adjoint_functions.wasserstein (:25-67): per-trace W2, summed in trace
     order; returns (W2, adjoint source (n_rx, T) float64)."""

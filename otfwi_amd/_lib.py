@@ -51,9 +51,12 @@ _SIGS = [
     ("otfwi_forward_for_grad", _i, [_vp, _i, _vp, _vp]),
     ("otfwi_adjoint_from_host", _i, [_vp, _vp, _vp]),
     ("otfwi_objective", _i, [_vp, _i, _vp, _i, _vp]),
+    ("otfwi_gradient_exact", _i, [_vp, _i, _vp, _i, _i, _vp, _vp, _vp]),
+    ("otfwi_set_cquad64", _i, [_vp, _vp]),
     ("otfwi_set_illum", _i, [_vp, _i]),
     ("otfwi_get_illum", _i, [_vp, _vp, _i]),
     ("otfwi_misfit", _i, [_i, _i, _i, _vp, _vp, _vp, _vp]),
+    ("otfwi_misfit_exact", _i, [_i, _i, _i, _vp, _vp, _vp, _vp]),
 ]
 
 EXPORTED = [s[0] for s in _SIGS]
@@ -69,11 +72,24 @@ class OtfwiError(RuntimeError):
         self.code = code
 
 
+def _runtime_first():
+    """Import PyTorch (when installed) before the first library load.  Its
+    wheel ships its own HIP and HSA runtimes; loaded first, they are the ones
+    this library binds to as well, so one process has one runtime.  Loaded
+    after this library has initialised /opt/rocm's runtime, PyTorch finds no
+    GPU."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load():
     """Load the library and declare every signature.  Raises ImportError when
     it has not been built."""
     global _LIB
     if _LIB is None:
+        _runtime_first()
         if not os.path.exists(LIB_PATH):
             raise ImportError(
                 f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
@@ -138,6 +154,7 @@ def load_unet():
     ImportError when it has not been built; there is no fallback."""
     global _ULIB
     if _ULIB is None:
+        _runtime_first()
         if not os.path.exists(UNET_LIB_PATH):
             raise ImportError(
                 f"{UNET_LIB_PATH} is missing: build it with __graft_entry__.build() "

@@ -123,6 +123,12 @@ struct otfwi_model {
     int ill_stride = 0;
     float *ill = nullptr;
     bool ill_valid = false;
+    // exact discrete adjoint: forward snapshots, the A fields of one segment,
+    // the receiver map; FP64 c^2 when given
+    void *xsnap = nullptr, *xseg = nullptr;
+    size_t xsnap_bytes = 0, xseg_bytes = 0;
+    int *rxmap = nullptr;
+    std::vector<double> cq64_h;
     float *gout = nullptr;           // FP32 mean gradient on device
     // state of a pending two-stage gradient
     int pend_n = 0;
@@ -151,12 +157,12 @@ void free_batch(Batch &b)
     b.ring_elems = 0;
 }
 
-int ensure_batch(otfwi_model *m, int n, bool need_adj)
+int ensure_batch(otfwi_model *m, int n, bool need_adj, bool need_ring = true)
 {
     Batch &b = m->b;
     const size_t es = esize(m);
     const size_t plane = (size_t)m->zp * m->ld;
-    const size_t ring_need = (size_t)n * m->T * std::max(m->n_ring, 1);
+    const size_t ring_need = need_ring ? (size_t)n * m->T * std::max(m->n_ring, 1) : 0;
     const bool adj_ok = b.LP[0] != nullptr || !need_adj;
     if (b.cap >= n && adj_ok && (!need_adj || b.ring_elems >= ring_need)) return OTFWI_OK;
     const int cap = std::max(n, b.cap);
@@ -184,8 +190,8 @@ int ensure_batch(otfwi_model *m, int n, bool need_adj)
     RCK(fa(&b.rec, ntr * m->T));
     if (need_adj) {
         RCK(fa(&b.grad, plane * cap));
-        b.ring_elems = (size_t)cap * m->T * std::max(m->n_ring, 1);
-        RCK(fa(&b.ring, b.ring_elems));
+        b.ring_elems = need_ring ? (size_t)cap * m->T * std::max(m->n_ring, 1) : 0;
+        if (b.ring_elems) RCK(fa(&b.ring, b.ring_elems));
         RCK(fa(&b.adj, ntr * m->T));
         RCK(fa(&b.jcum, ntr * m->T));
         RCK(dalloc(&b.mis_tr, ntr));
@@ -292,10 +298,12 @@ int upload_model_arrays(otfwi_model *m)
     const size_t n = (size_t)m->zp * m->ld;
     std::vector<float> cq((size_t)m->Z * m->X);
     HIPCK(hipMemcpy(cq.data(), m->cq32, cq.size() * sizeof(float), hipMemcpyDeviceToHost));
+    const bool c64 = m->cq64_h.size() == cq.size();
     std::vector<R> cm(n, R(0));
     for (int z = 0; z < m->Z; z++)
         for (int x = 0; x < m->X; x++) {
-            const R c = (R)cq[(size_t)z * m->X + x];
+            const size_t q = (size_t)z * m->X + x;
+            const R c = c64 ? (R)m->cq64_h[q] : (R)cq[q];
             cm[(size_t)(z + ZOFF) * m->ld + x + XOFF] = (c * (R)m->dt) / (R)m->dx;
         }
     if (!m->cm) RCK(dalloc((char **)&m->cm, n * sizeof(R)));
@@ -320,6 +328,25 @@ int set_cq_host(otfwi_model *m, const float *cq)
     if (!m->cq32) RCK(dalloc(&m->cq32, n));
     HIPCK(hipMemcpy(m->cq32, cq, n * sizeof(float), hipMemcpyHostToDevice));
     m->have_cq = true;
+    m->cq64_h.clear();
+    return m->prec == 8 ? upload_model_arrays<double>(m) : upload_model_arrays<float>(m);
+}
+
+int set_cq64_host(otfwi_model *m, const double *cq)
+{
+    if (!cq) return fail(OTFWI_E_ARG, "cquad is NULL");
+    const size_t n = (size_t)m->X * m->Z;
+    std::vector<float> c32(n);
+    for (size_t i = 0; i < n; i++) {
+        if (!std::isfinite(cq[i]) || cq[i] < 0.0)
+            return fail(OTFWI_E_NUMERIC, "c^2 at flat index %zu is %g (must be finite and >= 0)",
+                        i, cq[i]);
+        c32[i] = (float)cq[i];
+    }
+    if (!m->cq32) RCK(dalloc(&m->cq32, n));
+    HIPCK(hipMemcpy(m->cq32, c32.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    m->have_cq = true;
+    m->cq64_h.assign(cq, cq + n);
     return m->prec == 8 ? upload_model_arrays<double>(m) : upload_model_arrays<float>(m);
 }
 
@@ -461,17 +488,16 @@ int run_forward(otfwi_model *m, int n, bool save_ring, bool illum = false)
 }
 
 // Launches of forward steps t0 .. t1-1 on the current batch (no memsets).
+// Step-invariant arguments of the forward step on the current batch of n shots.
 template <typename R>
-int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum)
+FwdArgs<R> fwd_args(otfwi_model *m, int n)
 {
     Batch &b = m->b;
-    const size_t plane = (size_t)m->zp * m->ld;
-    hipStream_t st = m->stream;
     FwdArgs<R> a{};
     a.X = m->X; a.Z = m->Z; a.ld = m->ld;
     a.ntx = ceil_div(m->X, TW); a.ntz = ceil_div(m->Z, m->th_fwd);
     a.nshots = n; a.T = m->T;
-    a.sstride = (long long)plane;
+    a.sstride = (long long)m->zp * m->ld;
     a.cm = (const R *)m->cm;
     a.cA = (R)m->dt / (R)m->dx;
     a.kx = (const R *)m->kx; a.kz = (const R *)m->kz;
@@ -480,8 +506,19 @@ int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum)
     a.shot_src = b.shot_src;
     a.icum = (const R *)m->icum;
     a.rec = (R *)b.rec; a.n_rx = m->n_rx;
-    a.ring = save_ring ? (R *)b.ring : nullptr;
+    a.ring = (R *)b.ring;
     a.n_ring = m->n_ring;
+    return a;
+}
+
+template <typename R>
+int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum)
+{
+    Batch &b = m->b;
+    const size_t plane = (size_t)m->zp * m->ld;
+    hipStream_t st = m->stream;
+    FwdArgs<R> a = fwd_args<R>(m, n);
+    if (!save_ring) a.ring = nullptr;
     const int pw = persistent_wgs(m, false);
     const int ist = (illum && m->ill) ? m->ill_stride : 0;
     auto body = [&]() -> int {
@@ -753,7 +790,7 @@ int objective_impl(otfwi_model *m, int n, const int *shot_src, int kind, double 
     if (kind == OTFWI_MISFIT_W2 && m->T < 2) return fail(OTFWI_E_ARG, "W2 needs T >= 2");
     RCK(ready(m, false));
     const int B = batch_size(m, n);
-    RCK(ensure_batch(m, B, true));
+    RCK(ensure_batch(m, B, true, false));
     Batch &b = m->b;
     for (int s0 = 0; s0 < n; s0 += B) {
         const int nb = std::min(B, n - s0);
@@ -821,6 +858,205 @@ int adjoint_from_host_impl(otfwi_model *m, const float *adj, float *grad_out)
                              m->stream));
     HIPCK(hipStreamSynchronize(m->stream));
     m->pend_n = 0;
+    return OTFWI_OK;
+}
+
+// Padded receiver map for the exact adjoint: receiver index at its cell,
+// -1 elsewhere (receivers outside the updated box record 0 and get none).
+int build_rxmap(otfwi_model *m)
+{
+    const size_t plane = (size_t)m->zp * m->ld;
+    std::vector<int> map(plane, -1);
+    for (int r = 0; r < m->n_rx; r++) {
+        const int x = m->rxx[r], z = m->rxz[r];
+        if (x < BOX || x >= m->X - BOX || z < BOX || z >= m->Z - BOX) continue;
+        int &slot = map[(size_t)(z + ZOFF) * m->ld + x + XOFF];
+        if (slot >= 0)
+            return fail(OTFWI_E_ARG, "receivers %d and %d share cell (%d, %d)", slot, r, x, z);
+        slot = r;
+    }
+    dfree(m->rxmap);
+    RCK(dalloc(&m->rxmap, plane));
+    HIPCK(hipMemcpy(m->rxmap, map.data(), plane * sizeof(int), hipMemcpyHostToDevice));
+    return OTFWI_OK;
+}
+
+int ensure_xbuf(otfwi_model *m, size_t snap_bytes, size_t seg_bytes)
+{
+    if (m->xsnap_bytes < snap_bytes) {
+        dfree(m->xsnap);
+        m->xsnap_bytes = 0;
+        RCK(dalloc((char **)&m->xsnap, snap_bytes));
+        m->xsnap_bytes = snap_bytes;
+    }
+    if (m->xseg_bytes < seg_bytes) {
+        dfree(m->xseg);
+        m->xseg_bytes = 0;
+        RCK(dalloc((char **)&m->xseg, seg_bytes));
+        m->xseg_bytes = seg_bytes;
+    }
+    // padding and the never-updated ring of the segment A fields stay 0
+    HIPCK(hipMemsetAsync(m->xseg, 0, m->xseg_bytes, m->stream));
+    return OTFWI_OK;
+}
+
+// Gradient of J = mean over shots of the misfit value (L2: sum (sim-obs)^2;
+// W2: the reference W2 value) by the exact discrete adjoint, with the forward
+// A fields recomputed segment by segment from snapshots taken every k steps
+// (k <= 0: round(sqrt(T-1))).  Memory per shot: ceil((T-1)/k) snapshots of 4
+// planes + k planes of (Ax, Az), instead of T-1 of them.  Returns dJ/d(c^2)
+// (mean over shots) in FP32 and/or FP64.
+template <typename R>
+int exact_gradient_impl(otfwi_model *m, int n, const int *shot_src, int kind, int k_snap,
+                        double *mis_out, float *grad_out, double *grad64_out)
+{
+    if (kind != OTFWI_MISFIT_L2 && kind != OTFWI_MISFIT_W2)
+        return fail(OTFWI_E_ARG, "unknown misfit kind %d", kind);
+    if (n <= 0) return fail(OTFWI_E_ARG, "n_shots must be positive");
+    if (m->n_rx <= 0) return fail(OTFWI_E_STATE, "receivers not set");
+    if (m->T < 2) return fail(OTFWI_E_ARG, "exact gradient needs T >= 2");
+    RCK(ready(m, false));
+    const int T = m->T;
+    const int k = k_snap > 0 ? std::min(k_snap, T - 1)
+                             : std::max(1, (int)std::lround(std::sqrt((double)(T - 1))));
+    const int nseg = ceil_div(T - 1, k);
+    const int B = batch_size(m, n);
+    RCK(ensure_batch(m, B, true, false));
+    RCK(ensure_gacc(m));
+    RCK(build_rxmap(m));
+    Batch &b = m->b;
+    const size_t es = sizeof(R), plane = (size_t)m->zp * m->ld;
+    const size_t bp = plane * B;     // one field of the batch
+    RCK(ensure_xbuf(m, (size_t)nseg * 4 * bp * es, (size_t)k * 2 * bp * es));
+    hipStream_t st = m->stream;
+    R *snap = (R *)m->xsnap, *seg = (R *)m->xseg;
+    auto snap_f = [&](int j, int f) { return snap + ((size_t)j * 4 + f) * bp; };
+    auto seg_ax = [&](int i) { return seg + (size_t)i * 2 * bp; };
+    auto seg_az = [&](int i) { return seg + ((size_t)i * 2 + 1) * bp; };
+    const int pw = persistent_wgs(m, false);
+    m->t_fwd_ms = m->t_mis_ms = m->t_adj_ms = 0;
+    m->n_fwd_launch = m->n_adj_launch = 0;
+    for (int s0 = 0; s0 < n; s0 += B) {
+        const int nb = std::min(B, n - s0);
+        const size_t fb = plane * nb * es;   // bytes of one field of this batch
+        RCK(upload_shot_src(m, nb, shot_src ? shot_src + s0 : nullptr));
+        RCK(upload_obs_batch<R>(m, s0, nb));
+        for (int i = 0; i < 2; i++)
+            for (void *p : {b.P[i], b.Ax[i], b.Az[i], b.Px[i]}) HIPCK(hipMemsetAsync(p, 0, fb, st));
+        HIPCK(hipMemsetAsync(b.rec, 0, (size_t)nb * m->n_rx * T * es, st));
+        FwdArgs<R> fa = fwd_args<R>(m, nb);
+        fa.ring = nullptr;
+        HIPCK(hipEventRecord(m->ev[0], st));
+        // phase 1: forward with a snapshot of the state at t-1 before every
+        // segment start t = 1 + j k
+        for (int t = 1; t < T; t++) {
+            const int p = (t - 1) & 1, c = t & 1;
+            if ((t - 1) % k == 0) {
+                const int j = (t - 1) / k;
+                HIPCK(hipMemcpyAsync(snap_f(j, 0), b.P[p], fb, hipMemcpyDeviceToDevice, st));
+                HIPCK(hipMemcpyAsync(snap_f(j, 1), b.Px[p], fb, hipMemcpyDeviceToDevice, st));
+                HIPCK(hipMemcpyAsync(snap_f(j, 2), b.Ax[p], fb, hipMemcpyDeviceToDevice, st));
+                HIPCK(hipMemcpyAsync(snap_f(j, 3), b.Az[p], fb, hipMemcpyDeviceToDevice, st));
+            }
+            fa.t = t;
+            fa.P0 = (const R *)b.P[p]; fa.P1 = (R *)b.P[c];
+            fa.Ax0 = (const R *)b.Ax[p]; fa.Ax1 = (R *)b.Ax[c];
+            fa.Az0 = (const R *)b.Az[p]; fa.Az1 = (R *)b.Az[c];
+            fa.Px0 = (const R *)b.Px[p]; fa.Px1 = (R *)b.Px[c];
+            HIPCK(launch_fwd_step<R>(fa, m->th_fwd, pw, st));
+        }
+        HIPCK(hipEventRecord(m->ev[1], st));
+        // misfit value and its exact derivative rho = dJ/d(trace)
+        const int ntr = nb * m->n_rx;
+        int *flags = reinterpret_cast<int *>(b.scratch + (size_t)ntr * 3 * T);
+        HIPCK(hipMemsetAsync(flags, 0, ntr * sizeof(int), st));
+        HIPCK((launch_misfit_exact<R, float>(kind, (const R *)b.rec, b.obs, (R *)b.adj, b.mis_tr,
+                                             ntr, T, b.scratch, flags, st)));
+        HIPCK(launch_reduce_misfit(b.mis_tr, nb, m->n_rx, b.mis_shot, st));
+        HIPCK(hipEventRecord(m->ev[2], st));
+        // phase 2: segments from the last; recompute A, then run the adjoint back
+        for (int i = 0; i < 2; i++)
+            for (void *p : {b.LP[i], b.LPx[i], b.LAx[i], b.LAz[i]})
+                HIPCK(hipMemsetAsync(p, 0, fb, st));
+        HIPCK(hipMemsetAsync(b.grad, 0, fb, st));
+        XAdjArgs<R> xa{};
+        xa.X = m->X; xa.Z = m->Z; xa.ld = m->ld; xa.nshots = nb; xa.n_rx = m->n_rx; xa.T = T;
+        xa.sstride = (long long)plane;
+        xa.grad = (R *)b.grad;
+        xa.cm = (const R *)m->cm; xa.kx = (const R *)m->kx; xa.kz = (const R *)m->kz;
+        xa.cA = (R)m->dt / (R)m->dx;
+        xa.rxmap = m->rxmap;
+        xa.rho = (const R *)b.adj;
+        int cur = 0;   // LP/LPx[cur] hold u, w of step t+1; LAx/LAz[cur] hold a_t, b_t
+        for (int j = nseg - 1; j >= 0; j--) {
+            const int ts = 1 + j * k, te = std::min(ts + k, T);
+            const int p0 = (ts - 1) & 1;
+            HIPCK(hipMemcpyAsync(b.P[p0], snap_f(j, 0), fb, hipMemcpyDeviceToDevice, st));
+            HIPCK(hipMemcpyAsync(b.Px[p0], snap_f(j, 1), fb, hipMemcpyDeviceToDevice, st));
+            HIPCK(hipMemcpyAsync(seg_ax(0), snap_f(j, 2), fb, hipMemcpyDeviceToDevice, st));
+            HIPCK(hipMemcpyAsync(seg_az(0), snap_f(j, 3), fb, hipMemcpyDeviceToDevice, st));
+            for (int t = ts; t <= te - 2; t++) {
+                const int p = (t - 1) & 1, c = t & 1;
+                fa.t = t;
+                fa.P0 = (const R *)b.P[p]; fa.P1 = (R *)b.P[c];
+                fa.Px0 = (const R *)b.Px[p]; fa.Px1 = (R *)b.Px[c];
+                fa.Ax0 = seg_ax(t - ts); fa.Ax1 = seg_ax(t - ts + 1);
+                fa.Az0 = seg_az(t - ts); fa.Az1 = seg_az(t - ts + 1);
+                HIPCK(launch_fwd_step<R>(fa, m->th_fwd, pw, st));
+                m->n_fwd_launch++;
+            }
+            for (int t = te - 1; t >= ts; t--) {
+                xa.t = t;
+                xa.u0 = (const R *)b.LP[cur]; xa.w0 = (const R *)b.LPx[cur];
+                xa.a = (const R *)b.LAx[cur]; xa.b = (const R *)b.LAz[cur];
+                xa.u1 = (R *)b.LP[1 - cur]; xa.w1 = (R *)b.LPx[1 - cur];
+                xa.a1 = (R *)b.LAx[1 - cur]; xa.b1 = (R *)b.LAz[1 - cur];
+                xa.Ax = seg_ax(t - ts); xa.Az = seg_az(t - ts);
+                HIPCK(launch_xadj_step<R>(xa, st));
+                m->n_adj_launch++;
+                cur = 1 - cur;
+            }
+        }
+        HIPCK(hipEventRecord(m->ev[3], st));
+        HIPCK(launch_accum_grad<R>((const R *)b.grad, nb, m->X, m->Z, m->ld, (long long)plane,
+                                   m->gacc, st));
+        std::vector<double> hmis(nb);
+        std::vector<int> fl(ntr);
+        HIPCK(hipMemcpyAsync(hmis.data(), b.mis_shot, nb * sizeof(double), hipMemcpyDeviceToHost,
+                             st));
+        HIPCK(hipMemcpyAsync(fl.data(), flags, ntr * sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCK(hipStreamSynchronize(st));
+        float ms[3];
+        HIPCK(hipEventElapsedTime(&ms[0], m->ev[0], m->ev[1]));
+        HIPCK(hipEventElapsedTime(&ms[1], m->ev[1], m->ev[2]));
+        HIPCK(hipEventElapsedTime(&ms[2], m->ev[2], m->ev[3]));
+        m->t_fwd_ms += ms[0];
+        m->t_mis_ms += ms[1];
+        m->t_adj_ms += ms[2];
+        m->n_fwd_launch += T - 1;
+        for (size_t i = 0; i < fl.size(); i++)
+            if (fl[i] == 3)
+                return fail(OTFWI_E_NUMERIC,
+                            "exact W2: the CDF of simulated trace %zu of shot %d is not monotone "
+                            "(sim + c < 0 somewhere)", i % m->n_rx, s0 + (int)(i / m->n_rx));
+        RCK(check_w2_flags(m, fl, s0));
+        if (mis_out)
+            for (int i = 0; i < nb; i++) mis_out[s0 + i] = hmis[i];
+    }
+    const int cells = m->X * m->Z;
+    const double scale = ((double)m->dt / (double)m->dx) / n;   // m = c^2 dt/dx
+    if (grad_out) {
+        HIPCK(launch_finalize_grad(m->gacc, cells, scale, m->gout, st));
+        HIPCK(hipMemcpyAsync(grad_out, m->gout, cells * sizeof(float), hipMemcpyDeviceToHost, st));
+    }
+    if (grad64_out) {
+        std::vector<double> acc(cells);
+        HIPCK(hipMemcpyAsync(acc.data(), m->gacc, cells * sizeof(double), hipMemcpyDeviceToHost,
+                             st));
+        HIPCK(hipStreamSynchronize(st));
+        for (int i = 0; i < cells; i++) grad64_out[i] = acc[i] * scale;
+    }
+    HIPCK(hipStreamSynchronize(st));
     return OTFWI_OK;
 }
 
@@ -966,6 +1202,7 @@ int otfwi_destroy(otfwi_model *m)
     dfree(m->cm); dfree(m->kx); dfree(m->kz); dfree(m->icum); dfree(m->cq32);
     dfree(m->evt); dfree(m->pts); dfree(m->tf_fwd); dfree(m->tf_adj);
     dfree(m->gacc); dfree(m->gout); dfree(m->ill);
+    dfree(m->xsnap); dfree(m->xseg); dfree(m->rxmap);
     for (int i = 0; i < 4; i++)
         if (m->ev[i]) (void)hipEventDestroy(m->ev[i]);
     for (int i = 0; i < 3; i++)
@@ -1022,6 +1259,7 @@ int otfwi_set_cquad_device(otfwi_model *m, const float *cquad_dev)
     HIPCK(hipMemcpy(h.data(), m->cq32, n * sizeof(float), hipMemcpyDeviceToHost));
     RCK(check_cq(h.data(), n));
     m->have_cq = true;
+    m->cq64_h.clear();
     return m->prec == 8 ? upload_model_arrays<double>(m) : upload_model_arrays<float>(m);
 }
 
@@ -1188,6 +1426,20 @@ int otfwi_objective(otfwi_model *m, int n_shots, const int *shot_src, int misfit
     return DISPATCH(m, objective_impl)(m, n_shots, shot_src, misfit_kind, misfit_out);
 }
 
+int otfwi_gradient_exact(otfwi_model *m, int n_shots, const int *shot_src, int misfit_kind,
+                         int snap_every, double *misfit_out, float *grad_out, double *grad64_out)
+{
+    RCK(check_model(m));
+    return DISPATCH(m, exact_gradient_impl)(m, n_shots, shot_src, misfit_kind, snap_every,
+                                            misfit_out, grad_out, grad64_out);
+}
+
+int otfwi_set_cquad64(otfwi_model *m, const double *cquad)
+{
+    RCK(check_model(m));
+    return set_cq64_host(m, cquad);
+}
+
 int otfwi_set_illum(otfwi_model *m, int stride)
 {
     RCK(check_model(m));
@@ -1211,8 +1463,9 @@ int otfwi_get_illum(otfwi_model *m, float *out, int out_on_device)
     return OTFWI_OK;
 }
 
-int otfwi_misfit(int kind, int n_traces, int T, const double *sim, const double *obs,
-                 double *mis_out, double *adj_out)
+namespace {
+int misfit_host(int kind, bool exact, int n_traces, int T, const double *sim, const double *obs,
+                double *mis_out, double *adj_out)
 {
     if (kind != OTFWI_MISFIT_L2 && kind != OTFWI_MISFIT_W2)
         return fail(OTFWI_E_ARG, "unknown misfit kind %d", kind);
@@ -1229,12 +1482,15 @@ int otfwi_misfit(int kind, int n_traces, int T, const double *sim, const double 
         cleanup();
         return rc;
     }
+    int *flags = reinterpret_cast<int *>(dscr + cnt * 3);
     hipError_t e = hipMemcpy(dsim, sim, cnt * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(dobs, obs, cnt * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(dscr, 0, (cnt * 3 + n_traces) * sizeof(double));
     if (e == hipSuccess)
-        e = launch_misfit<double, double>(kind, dsim, dobs, dadj, nullptr, dmis, n_traces, T, dscr,
-                                          nullptr);
+        e = exact ? launch_misfit_exact<double, double>(kind, dsim, dobs, dadj, dmis, n_traces, T,
+                                                        dscr, flags, nullptr)
+                  : launch_misfit<double, double>(kind, dsim, dobs, dadj, nullptr, dmis, n_traces,
+                                                  T, dscr, nullptr);
     std::vector<int> fl(n_traces, 0);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e == hipSuccess && mis_out)
@@ -1242,13 +1498,29 @@ int otfwi_misfit(int kind, int n_traces, int T, const double *sim, const double 
     if (e == hipSuccess && adj_out)
         e = hipMemcpy(adj_out, dadj, cnt * sizeof(double), hipMemcpyDeviceToHost);
     if (e == hipSuccess && kind == OTFWI_MISFIT_W2)
-        e = hipMemcpy(fl.data(), dscr + cnt * 3, n_traces * sizeof(int), hipMemcpyDeviceToHost);
+        e = hipMemcpy(fl.data(), flags, n_traces * sizeof(int), hipMemcpyDeviceToHost);
     cleanup();
     if (e != hipSuccess) return fail(OTFWI_E_HIP, "misfit: %s", hipGetErrorString(e));
     // an all-zero observed trace (flag 2) yields NaN, as the reference does
-    for (int i = 0; i < n_traces; i++)
+    for (int i = 0; i < n_traces; i++) {
         if (fl[i] == 1) return fail(OTFWI_E_NUMERIC, "W2: observed trace %d has a non-finite CDF", i);
+        if (fl[i] == 3)
+            return fail(OTFWI_E_NUMERIC, "exact W2: CDF of simulated trace %d is not monotone", i);
+    }
     return OTFWI_OK;
+}
+}  // namespace
+
+int otfwi_misfit(int kind, int n_traces, int T, const double *sim, const double *obs,
+                 double *mis_out, double *adj_out)
+{
+    return misfit_host(kind, false, n_traces, T, sim, obs, mis_out, adj_out);
+}
+
+int otfwi_misfit_exact(int kind, int n_traces, int T, const double *sim, const double *obs,
+                       double *mis_out, double *rho_out)
+{
+    return misfit_host(kind, true, n_traces, T, sim, obs, mis_out, rho_out);
 }
 
 // ===========================================================================

@@ -244,6 +244,159 @@ w2_kernel(const R *sim, const O *obs, R *adj, double *mis, int T, double *scratc
     __syncthreads();
 }
 
+// In-place exclusive prefix sum over y[0..T) (lane-owned contiguous chunks):
+// y[k] <- sum_{i<k} y[i]; y[T] is not touched, the total is returned.
+__device__ double excl_prefix_inplace(double *y, int T)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int C = (T + WAVE - 1) / WAVE;
+    const int k0 = lane * C, k1 = min(k0 + C, T);
+    double local = 0.0;
+    for (int k = k0; k < k1; k++) local += y[k];
+    double run = wave_excl_scan(local);
+    const double total = wave_sum(local);
+    __syncthreads();
+    for (int k = k0; k < k1; k++) {
+        const double v = y[k];
+        y[k] = run;
+        run += v;
+    }
+    __syncthreads();
+    return total;
+}
+
+// Exact derivative of the reference W2 value W = sum_k (F^-1(t_k) - G^-1(t_k))^2
+// with respect to the simulated trace, for a monotone F (sim + c >= 0).
+// F^-1(t_k) = t_lo + (t_hi - t_lo) (t_k - F_lo) / (F_hi - F_lo) on the
+// bracket lo = clip(searchsorted_left(F, t_k), 1, T-1) - 1, hi = lo + 1, so
+//   dW/dF_lo += 2 e_k (t_hi - t_lo) (t_k - F_hi) / (F_hi - F_lo)^2 =: A_k,
+//   dW/dF_hi += -2 e_k (t_hi - t_lo) (t_k - F_lo) / (F_hi - F_lo)^2 =: B_k.
+// lo is non-decreasing in k, so bracket j collects the contiguous range of k
+// with t_k in (F_j, F_{j+1}] from prefix sums of A and B (no atomics).  Then
+// the transposes of the trapezoid CDF, F_j = sum_{l=1..j} d_l (f_l + f_{l-1})/2,
+// and of the normalization f = h / S, h = sim + c, S = sum(h dt).
+// flags: 1 G not monotone, 2 all-zero observed trace, 3 F not monotone.
+template <typename R, typename O>
+__global__ void __launch_bounds__(WAVE)
+w2x_kernel(const R *sim, const O *obs, R *rho, double *mis, int T, double *scratch, int *flags)
+{
+    extern __shared__ double lds[];
+    double *Gs = lds;
+    double *Fs = lds + T;
+    const int tr = blockIdx.x;
+    const int lane = threadIdx.x;
+    const R *f0 = sim + (size_t)tr * T;
+    const O *g0 = obs + (size_t)tr * T;
+    double *PA = scratch + (size_t)tr * 3 * T;   // A_k, then its exclusive prefix sums
+    double *PB = PA + T;                         // B_k, likewise
+    double *Fb = PB + T;                         // dW/dF, then its exclusive prefix sums
+    const double step = 1.0 / (double)(T - 1);
+    const double dt = tval(1, T, step) - tval(0, T, step);
+
+    double mn = INFINITY, mx = -INFINITY;
+    for (int k = lane; k < T; k += WAVE) {
+        const double v = (double)g0[k];
+        mn = fmin(mn, v);
+        mx = fmax(mx, v);
+    }
+    mn = wave_min(mn);
+    mx = wave_max(mx);
+    const double c = fmax(-1.1 * mn, 0.1 * mx);
+    double sg = 0.0, sf = 0.0;
+    for (int k = lane; k < T; k += WAVE) {
+        const double g = (double)g0[k] + c, f = (double)f0[k] + c;
+        Gs[k] = g;
+        Fs[k] = f;
+        sg += g * dt;
+        sf += f * dt;
+    }
+    sg = wave_sum(sg);
+    sf = wave_sum(sf);
+    if (!(sg > 0.0) && lane == 0) flags[tr] = 2;
+    __syncthreads();
+    for (int k = lane; k < T; k += WAVE) {
+        Gs[k] = Gs[k] / sg;
+        Fs[k] = Fs[k] / sf;
+    }
+    __syncthreads();
+    cumtrapz_inplace<false>(Gs, T, step);
+    cumtrapz_inplace<false>(Fs, T, step);
+    const bool gmono = is_monotone(Gs, T), fmono = is_monotone(Fs, T);
+    if (lane == 0 && flags[tr] == 0 && (!gmono || !fmono)) flags[tr] = gmono ? 3 : 1;
+    auto tgrid = [=](int i) { return tval(i, T, step); };
+    auto Gx = [=](int i) { return Gs[i]; };
+    auto Fx = [=](int i) { return Fs[i]; };
+
+    double w = 0.0;
+    for (int k = lane; k < T; k += WAVE) {
+        const double tk = tgrid(k);
+        int i = lower_bound(Fx, T, tk);
+        i = i < 1 ? 1 : (i > T - 1 ? T - 1 : i);
+        const double Fl = Fs[i - 1], Fh = Fs[i], tl = tgrid(i - 1), th = tgrid(i);
+        const double F1 = (th - tl) / (Fh - Fl) * (tk - Fl) + tl;
+        const double G1 = interp_lin(Gx, tgrid, T, tk);
+        const double e = F1 - G1;
+        w += e * e;
+        const double d2 = (Fh - Fl) * (Fh - Fl);
+        PA[k] = 2.0 * e * (th - tl) * (tk - Fh) / d2;
+        PB[k] = -2.0 * e * (th - tl) * (tk - Fl) / d2;
+    }
+    w = wave_sum(w);
+    __syncthreads();
+    const double totA = excl_prefix_inplace(PA, T);
+    const double totB = excl_prefix_inplace(PB, T);
+    // ks(j) = first k with t_k > F_j (bracket of k >= j); ks(0) = 0, ks(T-1) = T
+    auto ks = [&](int j) -> int {
+        if (j <= 0) return 0;
+        if (j >= T - 1) return T;
+        const double Fj = Fs[j];
+        int lo = 0, hi = T;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (Fj < tgrid(mid)) hi = mid; else lo = mid + 1;
+        }
+        return lo;
+    };
+    auto preA = [&](int k) { return k < T ? PA[k] : totA; };
+    auto preB = [&](int k) { return k < T ? PB[k] : totB; };
+    for (int j = lane; j < T; j += WAVE) {
+        double v = 0.0;
+        if (j <= T - 2) v += preA(ks(j + 1)) - preA(ks(j));
+        if (j >= 1) v += preB(ks(j)) - preB(ks(j - 1));
+        Fb[j] = v;
+    }
+    __syncthreads();
+    const double totF = excl_prefix_inplace(Fb, T);   // Fb[j] = sum_{i<j}; suffix S_j = totF - Fb[j]
+    auto fbar = [&](int m) -> double {
+        double v = 0.0;
+        if (m >= 1) v += 0.5 * (tgrid(m) - tgrid(m - 1)) * (totF - Fb[m]);
+        if (m + 1 <= T - 1) v += 0.5 * (tgrid(m + 1) - tgrid(m)) * (totF - Fb[m + 1]);
+        return v;
+    };
+    double dot = 0.0;
+    for (int m = lane; m < T; m += WAVE) dot += fbar(m) * ((double)f0[m] + c);
+    dot = wave_sum(dot);
+    R *o = rho + (size_t)tr * T;
+    for (int m = lane; m < T; m += WAVE) o[m] = (R)(fbar(m) / sf - dt * dot / (sf * sf));
+    if (lane == 0) mis[tr] = w;
+}
+
+// Exact L2: misfit sum (sim - obs)^2, derivative 2 (sim - obs).
+template <typename R, typename O>
+__global__ void __launch_bounds__(WAVE)
+l2x_kernel(const R *sim, const O *obs, R *rho, double *mis, int T)
+{
+    const int tr = blockIdx.x, lane = threadIdx.x;
+    double w = 0.0;
+    for (int k = lane; k < T; k += WAVE) {
+        const double r = (double)sim[(size_t)tr * T + k] - (double)obs[(size_t)tr * T + k];
+        rho[(size_t)tr * T + k] = (R)(2.0 * r);
+        w += r * r;
+    }
+    w = wave_sum(w);
+    if (lane == 0) mis[tr] = w;
+}
+
 template <typename R, typename O>
 __global__ void __launch_bounds__(WAVE)
 l2_kernel(const R *sim, const O *obs, R *adj, double *mis, int T)
@@ -322,6 +475,34 @@ template hipError_t launch_misfit<double, float>(int, const double *, const floa
 template hipError_t launch_misfit<double, double>(int, const double *, const double *, double *,
                                                   double *, double *, int, int, double *,
                                                   hipStream_t);
+
+template <typename R, typename O>
+hipError_t launch_misfit_exact(int kind, const R *sim, const O *obs, R *rho, double *mis,
+                               int ntraces, int T, double *scratch, int *flags, hipStream_t s)
+{
+    if (ntraces <= 0) return hipSuccess;
+    if (kind == 0) {
+        hipLaunchKernelGGL((l2x_kernel<R, O>), dim3(ntraces), dim3(WAVE), 0, s, sim, obs, rho, mis,
+                           T);
+    } else {
+        const size_t lds = (size_t)2 * T * sizeof(double);
+        hipError_t e = hipFuncSetAttribute((const void *)w2x_kernel<R, O>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((w2x_kernel<R, O>), dim3(ntraces), dim3(WAVE), lds, s, sim, obs, rho,
+                           mis, T, scratch, flags);
+    }
+    return hipGetLastError();
+}
+template hipError_t launch_misfit_exact<float, float>(int, const float *, const float *, float *,
+                                                      double *, int, int, double *, int *,
+                                                      hipStream_t);
+template hipError_t launch_misfit_exact<double, float>(int, const double *, const float *,
+                                                       double *, double *, int, int, double *,
+                                                       int *, hipStream_t);
+template hipError_t launch_misfit_exact<double, double>(int, const double *, const double *,
+                                                        double *, double *, int, int, double *,
+                                                        int *, hipStream_t);
 
 template <typename R>
 __global__ void accum_grad_kernel(const R *grad, int nshots, int X, int Z, int ld,

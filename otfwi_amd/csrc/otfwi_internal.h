@@ -83,6 +83,26 @@ struct AdjArgs {
     int n_ring;
 };
 
+// One step t of the exact discrete adjoint (exact.hip): u_t, w_t from
+// u_{t+1}, w_{t+1}, a_t, b_t and the forward A(t-1); then a_{t-1}, b_{t-1}.
+template <typename R>
+struct XAdjArgs {
+    int X, Z, ld, nshots, n_rx, T, t;
+    long long sstride;
+    const R *u0, *w0;             // u_{t+1}, w_{t+1}
+    const R *a, *b;               // a_t, b_t
+    R *u1, *w1;                   // u_t, w_t
+    R *a1, *b1;                   // a_{t-1}, b_{t-1}
+    const R *Ax, *Az;             // forward A(t-1), per shot
+    R *grad;                      // dJ/dm, per shot
+    const R *cm, *kx, *kz;
+    R cA;
+    const int *rxmap;             // padded plane: receiver index or -1
+    const R *rho;                 // [nshots][n_rx][T] dJ/d(trace)
+};
+template <typename R>
+hipError_t launch_xadj_step(const XAdjArgs<R> &a, hipStream_t s);
+
 // Launchers (prop.hip).  th = tile height (16, 32 or 64); persistent_wgs > 0
 // launches that many persistent work-groups instead of one per tile.
 template <typename R>
@@ -98,6 +118,13 @@ int step_occupancy(bool adj, int prec, int th, bool persistent);
 template <typename R, typename O>
 hipError_t launch_misfit(int kind, const R *sim, const O *obs, R *adj, R *jcum, double *mis,
                          int ntraces, int T, double *scratch, hipStream_t s);
+// Exact derivative of the misfit value w.r.t. the simulated traces (rho):
+// kind 0: 2 (sim - obs); kind 1: derivative of the reference W2 value
+// (needs a monotone CDF of sim + c; flags as launch_misfit,
+// plus 3 = CDF of the simulated trace not monotone).
+template <typename R, typename O>
+hipError_t launch_misfit_exact(int kind, const R *sim, const O *obs, R *rho, double *mis,
+                               int ntraces, int T, double *scratch, int *flags, hipStream_t s);
 // jcum of host-provided adjoint sources.
 template <typename R>
 hipError_t launch_jcum(const R *adj, R *jcum, int ntraces, int T, hipStream_t s);

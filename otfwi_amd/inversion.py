@@ -23,17 +23,19 @@ import numpy as np
 from .cuda_interface_pml import Cuda_interface_PML
 
 
-def array_mask(Z, X, pml, rows_blocked, taper=8):
-    """Update mask for an array on the top row: zero in the PML and on rows
-    < rows_blocked (the array and its near field), then a cosine taper over
-    ``taper`` rows, one elsewhere inside."""
+def array_mask(Z, X, pml, rows_blocked, taper=8, band=8):
+    """Update mask for an array on the top row: zero in the PML, on rows
+    < rows_blocked (the array and its near field) and in a band of ``band``
+    cells along the bottom and side PML (where the boundary ring is restored
+    and the imaging condition carries edge artefacts), a cosine taper over
+    ``taper`` rows below rows_blocked, one elsewhere."""
     m = np.zeros((Z, X), np.float32)
-    m[pml:Z - pml, pml:X - pml] = 1.0
+    m[pml:Z - pml - band, pml + band:X - pml - band] = 1.0
     m[:rows_blocked, :] = 0.0
     for k in range(taper):
         z = rows_blocked + k
-        if z < Z - pml:
-            m[z, pml:X - pml] *= 0.5 - 0.5 * np.cos(np.pi * (k + 1) / (taper + 1))
+        if z < Z - pml - band:
+            m[z, :] *= 0.5 - 0.5 * np.cos(np.pi * (k + 1) / (taper + 1))
     return m
 
 
@@ -111,25 +113,27 @@ class FWI:
     def clip(self, c):
         return np.clip(c, self.vmin, self.vmax)
 
-    def run(self, c0, n_iter, learned=None, tol=None, time_limit=None):
+    def run(self, c0, n_iter, learned=None, tol=None, time_limit=None, patience=1):
         """Iterate from c0.  learned(grad, illum, c) -> proposed velocity update
         (or None); a proposal is kept only if the misfit decreases, otherwise
-        the plain step is taken.  Stops after n_iter iterations, at the
-        velocity-error tolerance tol, at time_limit seconds, or when no step
-        size decreases the misfit."""
+        the plain step is taken; after `patience` rejected proposals no more
+        are asked for.  Stops after n_iter iterations, at the velocity-error
+        tolerance tol, at time_limit seconds, or when no step size decreases
+        the misfit."""
         hist = History()
         t0 = time.perf_counter()
         c = np.asarray(c0, np.float64).copy()
         J, g, ill = self.evaluate(c)
         hist.n_grad += 1
         hist.records.append(Record(time.perf_counter() - t0, 0, J, self.err(c), "init"))
+        rejected = 0
         for it in range(1, n_iter + 1):
             if tol is not None and hist.records[-1].err <= tol:
                 break
             if time_limit is not None and time.perf_counter() - t0 > time_limit:
                 break
             accepted = None
-            if learned is not None:
+            if learned is not None and rejected < patience:
                 dc = learned(g, ill, c)
                 if dc is not None:
                     ct = self.clip(c + dc)
@@ -137,6 +141,8 @@ class FWI:
                     hist.n_grad += 1
                     if Jt < J:
                         accepted = ("learned", ct, Jt, gt, it_ill, 0.0)
+                    else:
+                        rejected += 1
             if accepted is None:
                 d = plain_direction(g, ill, c, self.mask, self.eps)
                 a = self.alpha

@@ -1,0 +1,322 @@
+"""Learned-preconditioned inversion: training pairs generated by the
+propagator, U-Net training on the GPU kernels, and the learned step that the
+inversion driver judges with its misfit-decrease acceptance test.
+
+A training pair is (input, target) on one random-defect model:
+
+* input channel 0: the W2 gradient at the current model, restricted to the
+  update mask, divided by the 99th percentile of its magnitude and clipped
+  to +-5;
+* input channel 1: the source-side illumination of the same run,
+  log10(ill / max ill) mapped from [-ILL_DECADES, 0] to [0, 1];
+* target: (c_true - c_current) / C_SCALE on the update mask.
+
+The current model is the background (half of the pairs) or the background
+plus a random fraction of the blurred true model error, so that the network
+also sees the gradients of later iterations.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import time
+
+import numpy as np
+
+from . import configs, inversion
+from .cuda_interface_pml import Cuda_interface_PML
+from . import adjoint_functions as adj
+
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "unet_fmc256.npz")
+C_SCALE = 0.25        # target = model error / C_SCALE
+ILL_DECADES = 6.0
+G_PCT = 99.0          # gradient scaled by this percentile of |g| on the mask ...
+G_CLIP = 5.0          # ... and clipped to +-G_CLIP
+
+
+def features(grad, illum, mask):
+    """(2, Z, X) FP32 network input from a raw gradient and an illumination map."""
+    g = np.asarray(grad, np.float64) * mask
+    a = np.abs(g[mask > 0])
+    s = np.percentile(a, G_PCT) if a.size else 0.0
+    g = np.clip(g / s, -G_CLIP, G_CLIP) if s > 0 else g
+    ill = np.asarray(illum, np.float64)
+    li = np.log10(np.maximum(ill / max(ill.max(), 1e-300), 10.0 ** -ILL_DECADES))
+    li = 1.0 + li / ILL_DECADES
+    return np.stack([g, li]).astype(np.float32)
+
+
+class Problem:
+    """One geometry (grid, array, T) of learn_config, reused across models:
+    the propagator, its boundary ring and the update mask."""
+
+    def __init__(self, N=256, T=1200, n_elem=32, pitch=6, pml=20, illum_stride=4, **defects):
+        self.kw = dict(N=N, T=T, n_elem=n_elem, pitch=pitch, pml=pml, **defects)
+        p = configs.learn_config(0, **self.kw)
+        self.N, self.T, self.pml, self.ns = N, T, pml, p["n_shots"]
+        self.c_bg = p["c_bg"]
+        self.cp = Cuda_interface_PML(N, N, T, pml, p["c_bg"], self.ns, p["src_mask"], self.ns,
+                                     p["rx_mask"], p["src"], multishot=True, R=p["R"])
+        self.mask = inversion.array_mask(N, N, pml, pml + 10, taper=8)
+        self.vmin = float(configs.courant(1400.0))
+        self.vmax = float(configs.courant(5900.0))
+        self.shots = np.arange(self.ns, dtype=np.int32)
+        self._regression_ready = False
+        self.cp.model.set_illum(illum_stride)
+
+    def model(self, seed):
+        return configs.learn_config(seed, **self.kw)["c_true"]
+
+    def observe(self, c_true):
+        """Observed traces of all shots on the true model; installs them."""
+        self.cp.simulate(c=c_true)
+        obs = self.cp.recording.copy()
+        if not self._regression_ready:
+            self.cp.init_regression(obs, adj.wasserstein, prec_deriv=4)
+            self._regression_ready = True
+        else:
+            self.cp.model.set_observed(obs)
+            self.cp.observed = obs
+        return obs
+
+    def gradient(self, c):
+        m = self.cp.model
+        m.set_cquad((np.asarray(c, np.float64) ** 2).astype(np.float32))
+        g, mis = m.gradient(self.shots, "w2")
+        return g, m.illum(), float(np.mean(mis))
+
+    def fwi(self, c_true=None, **kw):
+        return inversion.FWI(self.cp, self.mask, self.vmin, self.vmax, misfit="w2",
+                             c_true=c_true, **kw)
+
+
+def make_pairs(prob: Problem, seeds, augment=0.5, rng_seed=0, log_every=100, log=print):
+    """Training pairs on the given model seeds.  Returns (inp (n,2,N,N),
+    target (n,N,N)) FP32 and the seconds spent."""
+    from scipy.ndimage import gaussian_filter
+
+    rng = np.random.default_rng(rng_seed)
+    n = len(seeds)
+    N = prob.N
+    X = np.zeros((n, 2, N, N), np.float32)
+    Y = np.zeros((n, N, N), np.float32)
+    t0 = time.perf_counter()
+    for k, s in enumerate(seeds):
+        c_true = prob.model(int(s))
+        prob.observe(c_true)
+        c = prob.c_bg.copy()
+        if rng.random() < augment:
+            beta = rng.uniform(0.2, 0.9)
+            sig = rng.uniform(1.0, 4.0)
+            c = c + beta * gaussian_filter(c_true - prob.c_bg, sig)
+        g, ill, _ = prob.gradient(c)
+        X[k] = features(g, ill, prob.mask)
+        Y[k] = ((c_true - c) * prob.mask / C_SCALE).astype(np.float32)
+        if log and log_every and (k + 1) % log_every == 0:
+            log(f"[pairs] {k + 1}/{n} in {time.perf_counter() - t0:.1f} s")
+    return X, Y, time.perf_counter() - t0
+
+
+def train(net, X, Y, epochs, batch=8, lr=1e-4, seed=0, log=print):
+    """Adam on mean squared error over the pairs, data resident on the GPU.
+    Returns the per-epoch mean losses and the seconds spent."""
+    import torch
+
+    dev = net.dev
+    Xd = torch.tensor(X, device=dev)
+    Yd = torch.tensor(Y, device=dev)
+    n = len(X)
+    rng = np.random.default_rng(seed)
+    hist = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for ep in range(epochs):
+        perm = rng.permutation(n)
+        acc = []
+        for i in range(0, n - batch + 1, batch):
+            idx = torch.tensor(perm[i:i + batch], device=dev)
+            acc.append(net.train_step(Xd.index_select(0, idx).contiguous(),
+                                      Yd.index_select(0, idx).contiguous(), lr=lr))
+        loss = float(torch.stack(acc).sum().item()) / (len(acc) * batch * X.shape[-1] ** 2)
+        hist.append(loss)
+        if log:
+            log(f"[train] epoch {ep + 1}/{epochs} loss {loss:.4e} "
+                f"({time.perf_counter() - t0:.1f} s)")
+    torch.cuda.synchronize()
+    return hist, time.perf_counter() - t0
+
+
+def step_pairs(prob: Problem, net, seeds, log_every=100, log=print):
+    """Second-stage pairs: on each model the network's own first step from
+    the background, c1 = clip(c_bg + step), then (features at c1,
+    (c_true - c1) / C_SCALE)."""
+    n = len(seeds)
+    N = prob.N
+    X = np.zeros((n, 2, N, N), np.float32)
+    Y = np.zeros((n, N, N), np.float32)
+    step = learned_step(net, prob.mask)
+    t0 = time.perf_counter()
+    for k, s in enumerate(seeds):
+        c_true = prob.model(int(s))
+        prob.observe(c_true)
+        g, ill, _ = prob.gradient(prob.c_bg)
+        c1 = np.clip(prob.c_bg + step(g, ill, prob.c_bg), prob.vmin, prob.vmax)
+        g1, ill1, _ = prob.gradient(c1)
+        X[k] = features(g1, ill1, prob.mask)
+        Y[k] = ((c_true - c1) * prob.mask / C_SCALE).astype(np.float32)
+        if log and log_every and (k + 1) % log_every == 0:
+            log(f"[pairs2] {k + 1}/{n} in {time.perf_counter() - t0:.1f} s")
+    return X, Y, time.perf_counter() - t0
+
+
+def save_weights(net, path=WEIGHTS):
+    """Weights as BF16 bit patterns (what the kernels compute with), biases
+    and the head in FP32."""
+    import torch
+
+    p = net.params()
+    from .unet import CONVS, param_layout
+    lay, n = param_layout()
+    keep = np.ones(n, bool)
+    for name, *_ in CONVS:
+        off, nw, _, _ = lay[name]
+        keep[off:off + nw] = False
+    bits = torch.tensor(p).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    idx = np.nonzero(keep)[0].astype(np.int32)
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    np.savez_compressed(path, bf16=bits, idx=idx, fp32=p[idx])
+
+
+def load_weights(path=WEIGHTS):
+    """Flat FP32 parameter vector from save_weights."""
+    d = np.load(path)
+    p = (d["bf16"].astype(np.uint32) << 16).view(np.float32)
+    p[d["idx"]] = d["fp32"]
+    return p
+
+
+def train_recipe(prob: Problem, n_pairs=2000, epochs=30, n_stage2=1000, epochs2=15, lr=1e-4,
+                 batch=8, seed=0, log=print):
+    """Stage 1: pairs on seeds 0..n_pairs-1, trained epochs; stage 2: pairs
+    at the network's own first step on the first n_stage2 seeds, trained on
+    the union for epochs2.  Returns (net, info)."""
+    from .unet import UNet
+
+    X, Y, t_pairs = make_pairs(prob, range(n_pairs), log=log)
+    net = UNet(seed=seed)
+    h1, t_train = train(net, X, Y, epochs, batch=batch, lr=lr, seed=seed, log=log)
+    info = dict(pairs=n_pairs, epochs=epochs, lr=lr, batch=batch, t_pairs=t_pairs,
+                t_train=t_train, loss=h1)
+    if n_stage2:
+        X2, Y2, t_p2 = step_pairs(prob, net, range(n_stage2), log=log)
+        h2, t_train2 = train(net, np.concatenate([X, X2]), np.concatenate([Y, Y2]), epochs2,
+                             batch=batch, lr=lr, seed=seed + 1, log=log)
+        info.update(pairs2=n_stage2, epochs2=epochs2, t_pairs2=t_p2, t_train2=t_train2, loss2=h2)
+    return net, info
+
+
+def learned_step(net, mask):
+    """Step proposal for inversion.FWI.run: the network's model update."""
+    import torch
+
+    def step(grad, illum, c):
+        x = torch.tensor(features(grad, illum, mask)[None], device=net.dev)
+        out = net.predict(x)
+        return (out[0].float().cpu().numpy() * C_SCALE * mask).astype(np.float64)
+
+    return step
+
+
+def compare(prob: Problem, net, seeds, n_iter=30, tol=0.02, log=print):
+    """Plain vs learned inversion on held-out models.  Returns a list of
+    per-model dicts with the velocity-error histories and times to tol;
+    "matched" is the time each method needs to reach the plain method's
+    final error."""
+    out = []
+    for s in seeds:
+        c_true = prob.model(int(s))
+        prob.observe(c_true)
+        res = {"seed": int(s)}
+        for kind in ("plain", "learned"):
+            fwi = prob.fwi(c_true=c_true)
+            _, h = fwi.run(prob.c_bg, n_iter=n_iter, tol=tol,
+                           learned=learned_step(net, prob.mask) if kind == "learned" else None)
+            res[kind] = dict(time_to_tol=h.time_to(tol), best_err=h.best_err,
+                             err0=h.records[0].err, n_grad=h.n_grad,
+                             iters=len(h.records) - 1,
+                             wall=h.records[-1].t,
+                             kinds=[r.kind for r in h.records[1:]],
+                             errs=[round(r.err, 5) for r in h.records],
+                             times=[round(r.t, 4) for r in h.records])
+            res[kind]["_h"] = h
+            if log:
+                log(f"[compare] seed {s} {kind}: err {h.records[0].err:.4f} -> {h.best_err:.4f} "
+                    f"in {len(h.records) - 1} it ({h.records[-1].t:.2f} s), "
+                    f"time to {tol:.0%}: {h.time_to(tol)}")
+        target = res["plain"]["errs"][-1]
+        for kind in ("plain", "learned"):
+            res[kind]["time_to_plain_final"] = res[kind].pop("_h").time_to(target + 1e-12)
+        out.append(res)
+    return out
+
+
+def summarize(res, tol=0.02):
+    """Aggregate of compare(): mean best errors, times to tol (None where a
+    method never reached it) and the matched-error speed-up."""
+    def mean(v):
+        v = [x for x in v if x is not None]
+        return float(np.mean(v)) if v else None
+
+    sp = [r["plain"]["time_to_plain_final"] / r["learned"]["time_to_plain_final"]
+          for r in res if r["learned"]["time_to_plain_final"]]
+    return dict(
+        models=len(res), tol=tol,
+        plain_err0=mean([r["plain"]["err0"] for r in res]),
+        plain_best_err=mean([r["plain"]["best_err"] for r in res]),
+        learned_best_err=mean([r["learned"]["best_err"] for r in res]),
+        plain_reached_tol=sum(r["plain"]["time_to_tol"] is not None for r in res),
+        learned_reached_tol=sum(r["learned"]["time_to_tol"] is not None for r in res),
+        plain_time_to_tol=mean([r["plain"]["time_to_tol"] for r in res]),
+        learned_time_to_tol=mean([r["learned"]["time_to_tol"] for r in res]),
+        plain_wall=mean([r["plain"]["wall"] for r in res]),
+        learned_wall=mean([r["learned"]["wall"] for r in res]),
+        matched_error_speedup=mean(sp),
+        matched_error_speedup_min=min(sp) if sp else None,
+        learned_reached_plain_final=len(sp))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description="train / evaluate the learned preconditioner")
+    ap.add_argument("cmd", choices=["train", "compare"])
+    ap.add_argument("--pairs", type=int, default=2000)
+    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--stage2", type=int, default=1000)
+    ap.add_argument("--epochs2", type=int, default=15)
+    ap.add_argument("--lr", type=float, default=1e-4)
+    ap.add_argument("--weights", default=WEIGHTS)
+    ap.add_argument("--models", type=int, default=4)
+    ap.add_argument("--iters", type=int, default=30)
+    ap.add_argument("--out", default=None, help="JSON summary file")
+    a = ap.parse_args(argv)
+    log = lambda s: print(s, flush=True)  # noqa: E731
+    prob = Problem()
+    summary = {}
+    if a.cmd == "train":
+        net, info = train_recipe(prob, a.pairs, a.epochs, a.stage2, a.epochs2, a.lr, log=log)
+        save_weights(net, a.weights)
+        summary["train"] = info
+    else:
+        from .unet import UNet
+        net = UNet(params=load_weights(a.weights))
+    res = compare(prob, net, [100000 + k for k in range(a.models)], n_iter=a.iters, log=log)
+    summary["compare"] = summarize(res)
+    summary["models"] = res
+    log(json.dumps(summary["compare"]))
+    if a.out:
+        with open(a.out, "w") as f:
+            json.dump(summary, f)
+
+
+if __name__ == "__main__":
+    main()

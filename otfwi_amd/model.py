@@ -155,6 +155,30 @@ class PMLModel:
                                               ctypes.c_void_p(int(grad_dev_ptr))))
         return mis
 
+    def set_cquad64(self, cquad):
+        """c^2 (Z,X) in FP64 (see otfwi_set_cquad64)."""
+        c = np.ascontiguousarray(cquad, dtype=np.float64)
+        if c.size != self.X * self.Z:
+            raise ValueError("cquad must have Z*X entries")
+        check(self._lib.otfwi_set_cquad64(self._h, ptr(c)))
+
+    def gradient_exact(self, shot_src=(-1,), misfit="l2", snap_every=0, fp64=False):
+        """Exact discrete-adjoint gradient dJ/d(c^2) of J = mean over shots of
+        the misfit value, with snapshot checkpointing every snap_every steps
+        (0: sqrt(T-1)).  Returns (grad (Z,X) FP32 or FP64, per-shot misfit)."""
+        kind = {"l2": _lib.MISFIT_L2, "w2": _lib.MISFIT_W2}[misfit]
+        s, n = self._shots(shot_src)
+        mis = np.zeros(n, np.float64)
+        if fp64:
+            g = np.zeros((self.Z, self.X), np.float64)
+            check(self._lib.otfwi_gradient_exact(self._h, n, ptr(s), kind, int(snap_every),
+                                                 ptr(mis), None, ptr(g)))
+        else:
+            g = np.zeros((self.Z, self.X), np.float32)
+            check(self._lib.otfwi_gradient_exact(self._h, n, ptr(s), kind, int(snap_every),
+                                                 ptr(mis), ptr(g), None))
+        return g, mis
+
     def objective(self, shot_src=(-1,), misfit="w2"):
         """Per-shot misfit (FP64) of the current model: forward runs and the
         device misfit, no adjoint."""
@@ -195,7 +219,12 @@ class PMLModel:
         return grad
 
 
-def misfit(kind, sim, obs):
+def misfit_exact(kind, sim, obs):
+    """Per-trace misfit value and its exact derivative w.r.t. sim on the GPU (FP64)."""
+    return misfit(kind, sim, obs, _exact=True)
+
+
+def misfit(kind, sim, obs, _exact=False):
     """Per-trace misfit and adjoint source of (n, T) traces on the GPU (FP64)."""
     k = {"l2": _lib.MISFIT_L2, "w2": _lib.MISFIT_W2}[kind]
     s = np.ascontiguousarray(sim, dtype=np.float64)
@@ -207,5 +236,6 @@ def misfit(kind, sim, obs):
     n, T = s.shape
     mis = np.zeros(n, np.float64)
     adj = np.zeros((n, T), np.float64)
-    check(_lib.load().otfwi_misfit(k, n, T, ptr(s), ptr(o), ptr(mis), ptr(adj)))
+    fn = _lib.load().otfwi_misfit_exact if _exact else _lib.load().otfwi_misfit
+    check(fn(k, n, T, ptr(s), ptr(o), ptr(mis), ptr(adj)))
     return mis, adj

@@ -150,3 +150,55 @@ def adjoint(cq, d_x, d_z, rx_xz, adj, ring_xz, ring, seed):
             a_step(ub, dx, dz, inside, revert=True)
         grad += (lam.Px + lam.Pz) * lap8(ub.Px + ub.Pz, inside)
     return grad
+
+
+def exact_gradient(cq, d_x, d_z, src_xz, src, rx_xz, obs, T, idx=-1, rho_fn=None):
+    """Exact discrete adjoint of forward() (unit spacing): dJ/d(cq) for
+    J = misfit(rec), rho_fn(rec) -> (J, dJ/drec); default L2 sum (rec-obs)^2.
+    Stores every A field (tiny grids only)."""
+    Z, X = cq.shape
+    inside = _interior(Z, X)
+    dx, dz = _damp(np.asarray(d_x, float), np.asarray(d_z, float))
+    m = np.asarray(cq, float)
+    s = State(Z, X)
+    sx, sz = src_xz
+    rx, rz = rx_xz
+    integ = np.zeros(len(sx))
+    rec = np.zeros((len(rx), T))
+    Ax = [np.zeros((Z, X))]
+    Az = [np.zeros((Z, X))]
+    for t in range(1, T):
+        p_step(s, m, dx, dz, inside)
+        for n in range(len(sx)):
+            if idx != -1 and idx != n:
+                continue
+            integ[n] += src[n][t]
+            s.Px[sz[n], sx[n]] += integ[n] / 2
+            s.Pz[sz[n], sx[n]] += integ[n] / 2
+        rec[:, t] = (s.Px + s.Pz)[rz, rx]
+        a_step(s, dx, dz, inside)
+        Ax.append(s.Ax.copy())
+        Az.append(s.Az.copy())
+    if rho_fn is None:
+        J = float(((rec - obs) ** 2).sum())
+        rho = 2.0 * (rec - obs)
+    else:
+        J, rho = rho_fn(rec)
+    u = np.zeros((Z, X))
+    w = np.zeros((Z, X))
+    a = np.zeros((Z, X))
+    b = np.zeros((Z, X))
+    g = np.zeros((Z, X))
+    kx, kz = 1.0 - dx, 1.0 - dz
+    rin = inside[rz, rx]
+    for t in range(T - 1, 0, -1):
+        dA = d_minus(a, 1, inside) + d_minus(b, 0, inside)
+        u = np.where(inside, kx * u - dA, 0.0)
+        w = np.where(inside, kz * w - dA, 0.0)
+        u[rz[rin], rx[rin]] += rho[rin, t]
+        w[rz[rin], rx[rin]] += rho[rin, t]
+        g += np.where(inside, u * d_minus(Ax[t - 1], 1, inside) + w * d_minus(Az[t - 1], 0, inside),
+                      0.0)
+        a = np.where(inside, kx * a - d_plus(m * u, 1, inside), 0.0)
+        b = np.where(inside, kz * b - d_plus(m * w, 0, inside), 0.0)
+    return J, g, rec

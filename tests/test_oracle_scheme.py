@@ -124,3 +124,49 @@ def test_illumination_matches_numpy_fields(stride):
                          stride, precision="f64")
     assert np.abs(got.ravel() - ill).max() <= 1e-12 * np.abs(ill).max()
     assert ill.max() > 0
+
+
+def _exact_fd_case():
+    p = small_problem(X=40, Z=36, T=90, pml=6, n_src=1, n_rx=5, seed=2, f0=0.07)
+    cq0 = p["cq_bg"].astype(np.float64)
+    cqt = p["cq_true"].astype(np.float64)
+    obs = NP.forward(cqt, p["d_x"], p["d_z"], p["src_xz"], p["src"], p["rx_xz"], p["T"])[0]
+    return p, 0.5 * (cq0 + cqt), obs
+
+
+@pytest.mark.parametrize("kind", ["l2", "w2"])
+def test_exact_adjoint_central_fd(kind):
+    """The exact discrete adjoint restated in NumPy (the transpose of the
+    forward recursion) against central finite differences of J in FP64."""
+    p, cq, obs = _exact_fd_case()
+    fn = None if kind == "l2" else (lambda rec: O.wasserstein_exact(rec, obs))
+    args = (p["d_x"], p["d_z"], p["src_xz"], p["src"], p["rx_xz"], obs, p["T"])
+    _, g, _ = NP.exact_gradient(cq, *args, rho_fn=fn)
+    zz, xx = np.mgrid[0:p["Z"], 0:p["X"]]
+    for cz, cx in ((20, 18), (14, 25)):
+        d = np.exp(-((zz - cz) ** 2 + (xx - cx) ** 2) / (2 * 3.0 ** 2))
+        h = 1e-5
+        jp = NP.exact_gradient(cq + h * d, *args, rho_fn=fn)[0]
+        jm = NP.exact_gradient(cq - h * d, *args, rho_fn=fn)[0]
+        fd = (jp - jm) / (2 * h)
+        assert abs(fd - (g * d).sum()) < 1e-6 * abs(fd)
+
+
+def test_w2_exact_trace_derivative():
+    rng = np.random.default_rng(0)
+    T = 300
+    tt = np.arange(T)
+
+    def ricker(t0, f=0.03):
+        a = (np.pi * f * (tt - t0)) ** 2
+        return (1 - 2 * a) * np.exp(-a)
+
+    obs, sim = ricker(120), 1.05 * ricker(126)
+    w, rho = O.wasserstein_exact_trace(sim, obs)
+    assert w == O.wasserstein_trace(sim, obs)[0]
+    for _ in range(3):
+        d = rng.standard_normal(T)
+        h = 1e-6
+        fd = (O.wasserstein_exact_trace(sim + h * d, obs)[0]
+              - O.wasserstein_exact_trace(sim - h * d, obs)[0]) / (2 * h)
+        assert abs(fd - rho @ d) < 1e-7 * abs(fd)
