@@ -175,6 +175,19 @@ int otfwi_gradient(otfwi_model *m, int n_shots, const int *shot_src, int misfit_
 int otfwi_gradient_device(otfwi_model *m, int n_shots, const int *shot_src, int misfit_kind,
                           double *misfit_out, float *grad_out_dev);
 
+/* Gradients of several independent models (uneven grids, T and shot
+ * counts) at once: n_workers host threads (<= 0: 4) take the models in
+ * longest-processing-time order (cost X*Z*T*n_shots) and run each one's
+ * otfwi_gradient on that model's own stream, so kernels of different models
+ * share the GPU.  Per model i: n_shots[i], shot_src[i] (may be NULL = all
+ * fire -1), misfit_kind[i], misfit_out[i] / grad_out[i] (host, may be NULL).
+ * Results equal those of separate otfwi_gradient calls.  A model must not be
+ * used by another thread meanwhile.  No reference counterpart (one model
+ * per process, regression_PML_single.cu:21-31). */
+int otfwi_gradient_multi(int n_models, otfwi_model *const *models, const int *n_shots,
+                         const int *const *shot_src, const int *misfit_kind,
+                         double *const *misfit_out, float *const *grad_out, int n_workers);
+
 /* Two-stage gradient for a host-side adjoint source (an arbitrary adj_fun,
  * cuda_interface_pml.py:238-249): otfwi_forward_for_grad runs the forward
  * with ring save and returns the traces; otfwi_adjoint_from_host takes the

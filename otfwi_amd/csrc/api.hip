@@ -7,11 +7,13 @@
 // The host loop only enqueues work; nothing synchronizes inside it.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "otfwi_internal.h"
@@ -1404,6 +1406,51 @@ int otfwi_gradient_device(otfwi_model *m, int n_shots, const int *shot_src, int 
     RCK(check_model(m));
     return DISPATCH(m, gradient_impl)(m, n_shots, shot_src, misfit_kind, misfit_out, grad_out_dev,
                                       true, nullptr);
+}
+
+int otfwi_gradient_multi(int n_models, otfwi_model *const *models, const int *n_shots,
+                         const int *const *shot_src, const int *misfit_kind,
+                         double *const *misfit_out, float *const *grad_out, int n_workers)
+{
+    if (n_models <= 0 || !models || !n_shots || !misfit_kind)
+        return fail(OTFWI_E_ARG, "gradient_multi: bad arguments");
+    for (int i = 0; i < n_models; i++) {
+        RCK(check_model(models[i]));
+        for (int j = 0; j < i; j++)
+            if (models[j] == models[i])
+                return fail(OTFWI_E_ARG, "gradient_multi: model %d is listed twice", i);
+    }
+    // longest processing time first: cost = cells x steps x shots
+    std::vector<int> order(n_models);
+    for (int i = 0; i < n_models; i++) order[i] = i;
+    auto cost = [&](int i) {
+        const otfwi_model *m = models[i];
+        return (double)m->X * m->Z * m->T * std::max(n_shots[i], 1);
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+    const int nw = std::max(1, std::min(n_workers > 0 ? n_workers : 4, n_models));
+    std::atomic<int> next{0};
+    std::vector<int> rc(n_models, OTFWI_OK);
+    std::vector<std::string> msg(n_models);
+    auto worker = [&]() {
+        for (;;) {
+            const int q = next.fetch_add(1);
+            if (q >= n_models) return;
+            const int i = order[q];
+            otfwi_model *m = models[i];
+            rc[i] = DISPATCH(m, gradient_impl)(m, n_shots[i], shot_src ? shot_src[i] : nullptr,
+                                               misfit_kind[i], misfit_out ? misfit_out[i] : nullptr,
+                                               grad_out ? grad_out[i] : nullptr, false, nullptr);
+            if (rc[i] != OTFWI_OK) msg[i] = g_err;
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int w = 1; w < nw; w++) pool.emplace_back(worker);
+    worker();
+    for (auto &t : pool) t.join();
+    for (int i = 0; i < n_models; i++)
+        if (rc[i] != OTFWI_OK) return fail(rc[i], "model %d: %s", i, msg[i].c_str());
+    return OTFWI_OK;
 }
 
 int otfwi_forward_for_grad(otfwi_model *m, int n_shots, const int *shot_src, float *rec_out)

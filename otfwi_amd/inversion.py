@@ -113,11 +113,13 @@ class FWI:
     def clip(self, c):
         return np.clip(c, self.vmin, self.vmax)
 
-    def run(self, c0, n_iter, learned=None, tol=None, time_limit=None, patience=1):
+    def run(self, c0, n_iter, learned=None, tol=None, time_limit=None, patience=1,
+            learned_scales=(1.0, 0.5)):
         """Iterate from c0.  learned(grad, illum, c) -> proposed velocity update
-        (or None); a proposal is kept only if the misfit decreases, otherwise
-        the plain step is taken; after `patience` rejected proposals no more
-        are asked for.  Stops after n_iter iterations, at the velocity-error
+        (or None); the proposal, scaled by each of learned_scales in turn, is
+        kept at the first scale that decreases the misfit, otherwise the plain
+        step is taken; after `patience` rejected proposals no more are asked
+        for.  Stops after n_iter iterations, at the velocity-error
         tolerance tol, at time_limit seconds, or when no step size decreases
         the misfit."""
         hist = History()
@@ -135,14 +137,15 @@ class FWI:
             accepted = None
             if learned is not None and rejected < patience:
                 dc = learned(g, ill, c)
-                if dc is not None:
-                    ct = self.clip(c + dc)
+                for scale in (learned_scales if dc is not None else ()):
+                    ct = self.clip(c + scale * dc)
                     Jt, gt, it_ill = self.evaluate(ct)
                     hist.n_grad += 1
                     if Jt < J:
-                        accepted = ("learned", ct, Jt, gt, it_ill, 0.0)
-                    else:
-                        rejected += 1
+                        accepted = ("learned", ct, Jt, gt, it_ill, scale)
+                        break
+                if dc is not None and accepted is None:
+                    rejected += 1
             if accepted is None:
                 d = plain_direction(g, ill, c, self.mask, self.eps)
                 a = self.alpha

@@ -254,9 +254,9 @@ def compare(prob: Problem, net, seeds, n_iter=30, tol=0.02, log=print):
                 log(f"[compare] seed {s} {kind}: err {h.records[0].err:.4f} -> {h.best_err:.4f} "
                     f"in {len(h.records) - 1} it ({h.records[-1].t:.2f} s), "
                     f"time to {tol:.0%}: {h.time_to(tol)}")
-        target = res["plain"]["errs"][-1]
+        target = res["plain"]["_h"].records[-1].err
         for kind in ("plain", "learned"):
-            res[kind]["time_to_plain_final"] = res[kind].pop("_h").time_to(target + 1e-12)
+            res[kind]["time_to_plain_final"] = res[kind].pop("_h").time_to(target)
         out.append(res)
     return out
 
@@ -269,7 +269,7 @@ def summarize(res, tol=0.02):
         return float(np.mean(v)) if v else None
 
     sp = [r["plain"]["time_to_plain_final"] / r["learned"]["time_to_plain_final"]
-          for r in res if r["learned"]["time_to_plain_final"]]
+          for r in res if r["learned"]["time_to_plain_final"] and r["plain"]["time_to_plain_final"]]
     return dict(
         models=len(res), tol=tol,
         plain_err0=mean([r["plain"]["err0"] for r in res]),

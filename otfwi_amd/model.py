@@ -239,3 +239,26 @@ def misfit(kind, sim, obs, _exact=False):
     fn = _lib.load().otfwi_misfit_exact if _exact else _lib.load().otfwi_misfit
     check(fn(k, n, T, ptr(s), ptr(o), ptr(mis), ptr(adj)))
     return mis, adj
+
+
+def gradient_multi(models, shots, misfit="w2", workers=4):
+    """Gradients of several independent PMLModels (uneven sizes) at once,
+    scheduled longest-first over `workers` host threads, each model on its
+    own stream.  shots: one shot_src list per model; misfit: one kind or a
+    list.  Returns [(grad (Z,X) FP32, per-shot misfit FP64)] in model order."""
+    n = len(models)
+    kinds = [misfit] * n if isinstance(misfit, str) else list(misfit)
+    code = {"l2": _lib.MISFIT_L2, "w2": _lib.MISFIT_W2}
+    ss = [i32(np.atleast_1d(s)) for s in shots]
+    grads = [np.zeros((m.Z, m.X), np.float32) for m in models]
+    mis = [np.zeros(len(s), np.float64) for s in ss]
+    P = ctypes.c_void_p
+    arr_m = (P * n)(*[m._h.value for m in models])
+    arr_n = (ctypes.c_int * n)(*[len(s) for s in ss])
+    arr_s = (P * n)(*[s.ctypes.data for s in ss])
+    arr_k = (ctypes.c_int * n)(*[code[k] for k in kinds])
+    arr_mis = (P * n)(*[x.ctypes.data for x in mis])
+    arr_g = (P * n)(*[g.ctypes.data for g in grads])
+    check(_lib.load().otfwi_gradient_multi(n, arr_m, arr_n, arr_s, arr_k, arr_mis, arr_g,
+                                           int(workers)))
+    return list(zip(grads, mis))
