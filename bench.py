@@ -26,6 +26,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+MFMA_BF16_PEAK = 2.5e15      # dense BF16 matrix-core peak, FLOP/s
 HBM_PEAK = 8.0e12           # MI355X HBM3E, bytes/s
 FWD_BYTES = 28              # read P, Ax, Az, c^2; write P, Ax, Az (FP32)
 ADJ_BYTES = 60              # lambda 3R+3W, u_b 3R+3W, c^2 R, grad R+W
@@ -117,6 +118,64 @@ def calib_copy(nbytes):
     print(f"[bench] copied {nbytes} bytes device-to-device", file=sys.stderr, flush=True)
 
 
+def unet_timing(N=8, H=1024, reps=5):
+    """Training step and forward of the U-Net at the learned preconditioner's
+    full-size shape (batch 8 of 1024^2, two input channels) on the MFMA
+    kernels: TFLOP/s of the 3x3 convolutions against the dense BF16 peak."""
+    import torch
+
+    from otfwi_amd import unet as U
+
+    net = U.UNet(seed=0)
+    x = torch.randn(N, 2, H, H, device="cuda")
+    t = 0.1 * torch.randn(N, H, H, device="cuda")
+    for _ in range(2):
+        net.train_step(x, t)
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    e0.record()
+    for _ in range(reps):
+        net.train_step(x, t)
+    e1.record()
+    for _ in range(reps):
+        net.predict(x)
+    e2.record()
+    torch.cuda.synchronize()
+    train_s = e0.elapsed_time(e1) / 1e3 / reps
+    fwd_s = e1.elapsed_time(e2) / 1e3 / reps
+    ftrain, ffwd = U.train_flops(N, H, H), sum(U.conv_flops(N, H, H).values())
+    print(f"[bench] U-Net {N}x{H}^2: train step {train_s * 1e3:.2f} ms, forward {fwd_s * 1e3:.2f} ms",
+          file=sys.stderr, flush=True)
+    return {"bound": "mfma", "kernel": "conv3x3 + wgrad (BF16, FP32 accumulate)",
+            "workload": f"U-Net 4 levels / 32 base channels, batch {N} of {H}x{H}",
+            "achieved": ftrain / train_s / 1e12, "peak": MFMA_BF16_PEAK / 1e12,
+            "unit": "TFLOP/s", "frac": ftrain / train_s / MFMA_BF16_PEAK,
+            "train_step_ms": train_s * 1e3, "train_flops": ftrain,
+            "forward_ms": fwd_s * 1e3, "forward_tflops": ffwd / fwd_s / 1e12}
+
+
+def inversion_compare(n_models, n_iter):
+    """Learned vs plain inversion on held-out random-defect models with the
+    shipped network weights (trained by python -m otfwi_amd.learned train)."""
+    from otfwi_amd import learned as L
+    from otfwi_amd.unet import UNet
+
+    if not os.path.exists(L.WEIGHTS):
+        return {"skipped": "no trained weights"}
+    prob = L.Problem()
+    net = UNet(params=L.load_weights())
+    res = L.compare(prob, net, [100000 + k for k in range(n_models)], n_iter=n_iter,
+                    log=lambda s: print(s, file=sys.stderr, flush=True))
+    summ = L.summarize(res)
+    meta = os.path.splitext(L.WEIGHTS)[0] + ".json"
+    if os.path.exists(meta):
+        with open(meta) as f:
+            summ["training"] = json.load(f)
+    summ["iterations"] = n_iter
+    summ["workload"] = ("256x256 full-matrix-capture block (32 shots, T=1200, W2), held-out "
+                        f"seeds 100000..{100000 + n_models - 1}")
+    return summ
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,7 +189,11 @@ def main():
     ap.add_argument("--misfit", default="w2", choices=["w2", "l2"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=300)
+    ap.add_argument("--cpu-steps", type=int, default=2000)
+    ap.add_argument("--inversion-models", type=int, default=3,
+                    help="held-out models for the learned vs plain inversion (0: skip)")
+    ap.add_argument("--inversion-iters", type=int, default=30)
+    ap.add_argument("--no-unet", action="store_true", help="skip the U-Net MFMA timing")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the run, copy 1 GiB device-to-device (scale for HBM byte counters)")
     args = ap.parse_args()
@@ -219,6 +282,11 @@ def main():
     if not args.no_cpu:
         cb = cpu_baseline(N, T, S, S, args.cpu_steps)
         out["cpu_baseline"] = cb
+    del m
+    if not args.no_unet:
+        out["unet_roofline"] = unet_timing()
+    if args.inversion_models > 0:
+        out["inversion"] = inversion_compare(args.inversion_models, args.inversion_iters)
     if args.calib_copy:
         calib_copy(1 << 30)
     print(json.dumps(out))

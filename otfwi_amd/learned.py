@@ -306,6 +306,8 @@ def main(argv=None):
         net, info = train_recipe(prob, a.pairs, a.epochs, a.stage2, a.epochs2, a.lr, log=log)
         save_weights(net, a.weights)
         summary["train"] = info
+        with open(os.path.splitext(a.weights)[0] + ".json", "w") as f:
+            json.dump({k: v for k, v in info.items() if not k.startswith("loss")}, f, indent=1)
     else:
         from .unet import UNet
         net = UNet(params=load_weights(a.weights))

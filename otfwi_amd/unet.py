@@ -75,6 +75,26 @@ def init_params(seed=0):
     return p
 
 
+def conv_flops(N, H, W):
+    """Multiply-add FLOPs (2 per MAC) of the 3x3 convolutions for one
+    forward pass of N images of H x W, per layer name; the input layer counts
+    its 16 stored channels.  The data gradient of a layer costs the same as
+    its forward pass (not needed for the input layer), the weight gradient
+    too."""
+    lvl = {"enc0": 0, "enc1": 1, "enc2": 2, "bot": 3, "dec2": 2, "dec1": 1, "dec0": 0}
+    out = {}
+    for name, ca, cb, co in CONVS:
+        l = lvl[name[:-1]]
+        px = N * (H >> l) * (W >> l)
+        out[name] = 2.0 * px * co * 9 * (ca + cb)
+    return out
+
+
+def train_flops(N, H, W):
+    f = conv_flops(N, H, W)
+    return sum(f.values()) * 3 - f["enc0a"]
+
+
 def _vp(t):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
 
