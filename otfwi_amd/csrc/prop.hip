@@ -38,6 +38,19 @@ template <typename R>
 __device__ __forceinline__ V4<R> ld4(const R *p) { return *reinterpret_cast<const V4<R> *>(p); }
 template <typename R>
 __device__ __forceinline__ void st4(R *p, const V4<R> &v) { *reinterpret_cast<V4<R> *>(p) = v; }
+// Global store of a step's output with the non-temporal hint: the outputs of
+// one launch (GBs) are read back only by the next launch, long after they
+// would have left L2, so they stream past it (+6-7 % on a 4-read/3-write
+// streaming kernel on MI355X).
+template <typename R> struct ExtV4;
+template <> struct ExtV4<float> { typedef float T __attribute__((ext_vector_type(4))); };
+template <> struct ExtV4<double> { typedef double T __attribute__((ext_vector_type(4))); };
+template <typename R>
+__device__ __forceinline__ void stg4(R *p, const V4<R> &v)
+{
+    const typename ExtV4<R>::T e = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(e, reinterpret_cast<typename ExtV4<R>::T *>(p));
+}
 template <typename R>
 __device__ __forceinline__ void unpack(const V4<R> &v, R *a) { a[0] = v.x; a[1] = v.y; a[2] = v.z; a[3] = v.w; }
 template <typename R>
@@ -284,12 +297,12 @@ __device__ __forceinline__ void p_quad(const FwdCtx<R> &c, int x, int z, bool ow
             if (!in_box(x + i, z, c.X, c.Z)) { p[i] = R(0); nx[i] = R(0); }
     }
     if (owned) {
-        st4(c.P1 + gi, pack<R>(p));
+        stg4(c.P1 + gi, pack<R>(p));
         if (c.fl & TF_STRIP) {
             R ps[4];
 #pragma unroll
             for (int i = 0; i < 4; i++) ps[i] = (kx[i] != R(1) || kz != R(1)) ? nx[i] : R(0);
-            st4(c.psi1 + gi, pack<R>(ps));
+            stg4(c.psi1 + gi, pack<R>(ps));
         }
     }
 }
@@ -442,8 +455,8 @@ __device__ __forceinline__ void fwd_compute(const FwdArgs<R> &a, const TileId &T
                     if (!in_box(x + i, z, c.X, c.Z)) { nax[i] = R(0); naz[i] = R(0); }
             }
             const unsigned gi = (unsigned)((z + ZOFF) * ld + x + XOFF);
-            st4(Ax1 + gi, pack<R>(nax));
-            st4(Az1 + gi, pack<R>(naz));
+            stg4(Ax1 + gi, pack<R>(nax));
+            stg4(Az1 + gi, pack<R>(naz));
         }
     }
 }
@@ -588,13 +601,13 @@ __device__ __forceinline__ void lu_quad(const AdjCtx<R> &c, int x, int z, bool o
             if (!in_box(x + i, z, c.X, c.Z)) { lp[i] = R(0); up[i] = R(0); nx[i] = R(0); }
     }
     if (owned) {
-        st4(c.LP1 + gi, pack<R>(lp));
-        st4(c.U1 + gi, pack<R>(up));
+        stg4(c.LP1 + gi, pack<R>(lp));
+        stg4(c.U1 + gi, pack<R>(up));
         if (c.fl & TF_STRIP) {
             R ps[4];
 #pragma unroll
             for (int i = 0; i < 4; i++) ps[i] = (kx[i] != R(1) || kz != R(1)) ? nx[i] : R(0);
-            st4(c.psi1 + gi, pack<R>(ps));
+            stg4(c.psi1 + gi, pack<R>(ps));
         }
     }
 }
@@ -790,11 +803,11 @@ __device__ __forceinline__ void adj_compute(const AdjArgs<R> &a, const TileId &T
                     if (!in_box(x + i, z, c.X, c.Z)) { o1[i] = o2[i] = o3[i] = o4[i] = R(0); }
             }
             const unsigned gi = (unsigned)((z + ZOFF) * ld + x + XOFF);
-            st4(LAx1 + gi, pack<R>(o1));
-            st4(LAz1 + gi, pack<R>(o2));
-            st4(UAx1 + gi, pack<R>(o3));
-            st4(UAz1 + gi, pack<R>(o4));
-            st4(G + gi, pack<R>(gg));
+            stg4(LAx1 + gi, pack<R>(o1));
+            stg4(LAz1 + gi, pack<R>(o2));
+            stg4(UAx1 + gi, pack<R>(o3));
+            stg4(UAz1 + gi, pack<R>(o4));
+            stg4(G + gi, pack<R>(gg));
         }
     }
 }
