@@ -134,6 +134,18 @@ int otfwi_get_tuning(otfwi_model *m, int *th_fwd, int *th_adj, int *max_batch);
 int otfwi_set_launch(otfwi_model *m, int persistent_fwd, int persistent_adj);
 int otfwi_get_launch(otfwi_model *m, int *persistent_fwd, int *persistent_adj);
 
+/* Cache blocking of the step loops: the forward (adjoint) loop runs
+ * steps_fwd (steps_adj) consecutive time steps on a group of shots_fwd
+ * (shots_adj) shots before moving to the next group (0 shots = all at once,
+ * 1 step = plain step-major order), so a group's state can stay in the
+ * Infinity Cache between its steps.  Results are bitwise independent of the
+ * blocking.  otfwi_get_blocking writes {steps_fwd, shots_fwd, steps_adj,
+ * shots_adj}.  No reference counterpart (regression_PML_single.cu:745, 793
+ * step every field of one shot per launch). */
+int otfwi_set_blocking(otfwi_model *m, int steps_fwd, int shots_fwd, int steps_adj,
+                       int shots_adj);
+int otfwi_get_blocking(otfwi_model *m, int *out);
+
 /* Replay the per-step launch loops as captured hipGraphs (default 1); a
  * graph is re-captured only when shots, tiles, launch style or device buffers
  * change.  No reference counterpart (regression_PML_single.cu:661-678

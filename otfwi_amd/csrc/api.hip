@@ -106,6 +106,10 @@ struct otfwi_model {
     bool points_dirty = true;
     int th_fwd = 32, th_adj = 16, max_batch = 0;
     int pers_fwd = 0, pers_adj = 0;   // persistent-work-group launches (0 = one work-group per tile)
+    // cache blocking of the step loops: runs of tb_steps time steps on groups
+    // of tb_shots shots (0 = all), so that a group's state stays in the
+    // Infinity Cache between consecutive steps; [0] forward, [1] adjoint
+    int tb_steps[2] = {1, 1}, tb_shots[2] = {0, 0};
     int n_cu = 0;
     // hipGraph replay of the step loops: slot 0 forward, 1 forward with ring save,
     // 2 adjoint; a graph is reused while its key (shots, tiles, launch style and
@@ -523,23 +527,40 @@ int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum)
     if (!save_ring) a.ring = nullptr;
     const int pw = persistent_wgs(m, false);
     const int ist = (illum && m->ill) ? m->ill_stride : 0;
+    const int K = std::max(1, m->tb_steps[0]);
+    const int G = m->tb_shots[0] > 0 ? std::min(m->tb_shots[0], n) : n;
     auto body = [&]() -> int {
-        for (int t = t0; t < t1; t++) {
-            const int p = (t - 1) & 1, c = t & 1;
-            a.t = t;
-            a.P0 = (const R *)b.P[p]; a.P1 = (R *)b.P[c];
-            a.Ax0 = (const R *)b.Ax[p]; a.Ax1 = (R *)b.Ax[c];
-            a.Az0 = (const R *)b.Az[p]; a.Az1 = (R *)b.Az[c];
-            a.Px0 = (const R *)b.Px[p]; a.Px1 = (R *)b.Px[c];
-            HIPCK(launch_fwd_step<R>(a, m->th_fwd, pw, st));
-            if (ist > 0 && t % ist == 0)
-                HIPCK(launch_illum<R>((const R *)b.P[c], n, m->X, m->Z, m->ld, (long long)plane,
-                                      (float)ist, m->ill, st));
+        for (int tb = t0; tb < t1; tb += K) {
+            const int te = std::min(tb + K, t1);
+            for (int g0 = 0; g0 < n; g0 += G) {
+                const int ng = std::min(G, n - g0);
+                const size_t so = (size_t)g0 * plane;               // plane offset of the group
+                const size_t to = (size_t)g0 * m->n_rx * m->T;      // trace offset
+                FwdArgs<R> ag = a;
+                ag.nshots = ng;
+                ag.shot_src = b.shot_src + g0;
+                ag.rec = a.rec + to;
+                ag.ring = a.ring ? a.ring + (size_t)g0 * m->T * m->n_ring : nullptr;
+                for (int t = tb; t < te; t++) {
+                    const int p = (t - 1) & 1, c = t & 1;
+                    ag.t = t;
+                    ag.P0 = (const R *)b.P[p] + so; ag.P1 = (R *)b.P[c] + so;
+                    ag.Ax0 = (const R *)b.Ax[p] + so; ag.Ax1 = (R *)b.Ax[c] + so;
+                    ag.Az0 = (const R *)b.Az[p] + so; ag.Az1 = (R *)b.Az[c] + so;
+                    ag.Px0 = (const R *)b.Px[p] + so; ag.Px1 = (R *)b.Px[c] + so;
+                    HIPCK(launch_fwd_step<R>(ag, m->th_fwd, pw, st));
+                    if (ist > 0 && t % ist == 0)
+                        HIPCK(launch_illum<R>((const R *)b.P[c] + so, ng, m->X, m->Z, m->ld,
+                                              (long long)plane, (float)ist, m->ill, st));
+                }
+            }
         }
         return OTFWI_OK;
     };
+    const unsigned long long kb = (unsigned long long)K * 4099ULL + (unsigned long long)G;
     return run_loop(m, save_ring ? 1 : 0,
-                    loop_key(m, n, m->th_fwd, pw, t0, t1) * 31ULL + (unsigned long long)ist, body);
+                    (loop_key(m, n, m->th_fwd, pw, t0, t1) * 31ULL + (unsigned long long)ist) *
+                            1000003ULL + kb, body);
 }
 
 template <typename R>
@@ -581,26 +602,42 @@ int adj_loop(otfwi_model *m, int n, int t0, int t1)
     a.ring = (const R *)b.ring; a.n_ring = m->n_ring;
     const int T = m->T;
     const int pw = persistent_wgs(m, true);
+    const int K = std::max(1, m->tb_steps[1]);
+    const int G = m->tb_shots[1] > 0 ? std::min(m->tb_shots[1], n) : n;
     auto body = [&]() -> int {
-    for (int t = t0; t < t1; t++) {
-        const int lp = (t - 1) & 1, lc = t & 1;
-        // reconstructed field: U(t) lives in P slot (T-1+t)%2, A_b(t) in slot (T-2+t)%2,
-        // starting from the forward's P(T-1) and A(T-2) (regression_PML_single.cu:787-790)
-        const int up = (T - 2 + t) & 1, uc = (T - 1 + t) & 1;
-        const int ap = (T - 1 + t) & 1, ac = (T - 2 + t) & 1;
-        a.t = t;
-        a.LP0 = (const R *)b.LP[lp]; a.LP1 = (R *)b.LP[lc];
-        a.LAx0 = (const R *)b.LAx[lp]; a.LAx1 = (R *)b.LAx[lc];
-        a.LAz0 = (const R *)b.LAz[lp]; a.LAz1 = (R *)b.LAz[lc];
-        a.LPx0 = (const R *)b.LPx[lp]; a.LPx1 = (R *)b.LPx[lc];
-        a.U0 = (const R *)b.P[up]; a.U1 = (R *)b.P[uc];
-        a.UAx0 = (const R *)b.Ax[ap]; a.UAx1 = (R *)b.Ax[ac];
-        a.UAz0 = (const R *)b.Az[ap]; a.UAz1 = (R *)b.Az[ac];
-        HIPCK(launch_adj_step<R>(a, m->th_adj, pw, st));
-    }
-    return OTFWI_OK;
+        for (int tb = t0; tb < t1; tb += K) {
+            const int te = std::min(tb + K, t1);
+            for (int g0 = 0; g0 < n; g0 += G) {
+                const int ng = std::min(G, n - g0);
+                const size_t so = (size_t)g0 * plane;
+                AdjArgs<R> ag = a;
+                ag.nshots = ng;
+                ag.grad = a.grad + so;
+                ag.jcum = a.jcum + (size_t)g0 * m->n_rx * T;
+                ag.ring = a.ring + (size_t)g0 * T * m->n_ring;
+                for (int t = tb; t < te; t++) {
+                    const int lp = (t - 1) & 1, lc = t & 1;
+                    // reconstructed field: U(t) lives in P slot (T-1+t)%2, A_b(t) in slot
+                    // (T-2+t)%2, starting from the forward's P(T-1) and A(T-2)
+                    // (regression_PML_single.cu:787-790)
+                    const int up = (T - 2 + t) & 1, uc = (T - 1 + t) & 1;
+                    const int ap = (T - 1 + t) & 1, ac = (T - 2 + t) & 1;
+                    ag.t = t;
+                    ag.LP0 = (const R *)b.LP[lp] + so; ag.LP1 = (R *)b.LP[lc] + so;
+                    ag.LAx0 = (const R *)b.LAx[lp] + so; ag.LAx1 = (R *)b.LAx[lc] + so;
+                    ag.LAz0 = (const R *)b.LAz[lp] + so; ag.LAz1 = (R *)b.LAz[lc] + so;
+                    ag.LPx0 = (const R *)b.LPx[lp] + so; ag.LPx1 = (R *)b.LPx[lc] + so;
+                    ag.U0 = (const R *)b.P[up] + so; ag.U1 = (R *)b.P[uc] + so;
+                    ag.UAx0 = (const R *)b.Ax[ap] + so; ag.UAx1 = (R *)b.Ax[ac] + so;
+                    ag.UAz0 = (const R *)b.Az[ap] + so; ag.UAz1 = (R *)b.Az[ac] + so;
+                    HIPCK(launch_adj_step<R>(ag, m->th_adj, pw, st));
+                }
+            }
+        }
+        return OTFWI_OK;
     };
-    return run_loop(m, 2, loop_key(m, n, m->th_adj, pw, t0, t1), body);
+    const unsigned long long kb = (unsigned long long)K * 4099ULL + (unsigned long long)G;
+    return run_loop(m, 2, loop_key(m, n, m->th_adj, pw, t0, t1) * 1000003ULL + kb, body);
 }
 
 int upload_shot_src(otfwi_model *m, int n, const int *shot_src)
@@ -1340,6 +1377,26 @@ int otfwi_set_launch(otfwi_model *m, int persistent_fwd, int persistent_adj)
     RCK(check_model(m));
     m->pers_fwd = persistent_fwd ? 1 : 0;
     m->pers_adj = persistent_adj ? 1 : 0;
+    return OTFWI_OK;
+}
+
+int otfwi_set_blocking(otfwi_model *m, int steps_fwd, int shots_fwd, int steps_adj,
+                       int shots_adj)
+{
+    RCK(check_model(m));
+    if (steps_fwd < 1 || steps_adj < 1 || shots_fwd < 0 || shots_adj < 0)
+        return fail(OTFWI_E_ARG, "blocking: steps >= 1 and shots >= 0 (0 = all)");
+    m->tb_steps[0] = steps_fwd; m->tb_shots[0] = shots_fwd;
+    m->tb_steps[1] = steps_adj; m->tb_shots[1] = shots_adj;
+    return OTFWI_OK;
+}
+
+int otfwi_get_blocking(otfwi_model *m, int *out)
+{
+    RCK(check_model(m));
+    if (!out) return fail(OTFWI_E_ARG, "output is NULL");
+    out[0] = m->tb_steps[0]; out[1] = m->tb_shots[0];
+    out[2] = m->tb_steps[1]; out[3] = m->tb_shots[1];
     return OTFWI_OK;
 }
 

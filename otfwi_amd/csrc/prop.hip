@@ -40,8 +40,9 @@ template <typename R>
 __device__ __forceinline__ void st4(R *p, const V4<R> &v) { *reinterpret_cast<V4<R> *>(p) = v; }
 // Global store of a step's output with the non-temporal hint: the outputs of
 // one launch (GBs) are read back only by the next launch, long after they
-// would have left L2, so they stream past it (+6-7 % on a 4-read/3-write
-// streaming kernel on MI355X).
+// would have left L2.  Measured on MI355X: +6 % on a plain 4-read/3-write
+// streaming kernel, neutral on the step kernels (same launch times as plain
+// stores).
 template <typename R> struct ExtV4;
 template <> struct ExtV4<float> { typedef float T __attribute__((ext_vector_type(4))); };
 template <> struct ExtV4<double> { typedef double T __attribute__((ext_vector_type(4))); };

@@ -102,6 +102,16 @@ class PMLModel:
         return dict(th_fwd=int(ch[0]), persistent_fwd=bool(ch[1]), th_adj=int(ch[2]),
                     persistent_adj=bool(ch[3]), fwd_ms=float(ms[0]), adj_ms=float(ms[1]))
 
+    def set_blocking(self, steps_fwd=1, shots_fwd=0, steps_adj=1, shots_adj=0):
+        """Cache blocking of the step loops (see otfwi_set_blocking)."""
+        check(self._lib.otfwi_set_blocking(self._h, int(steps_fwd), int(shots_fwd),
+                                           int(steps_adj), int(shots_adj)))
+
+    def get_blocking(self):
+        out = np.zeros(4, np.int32)
+        check(self._lib.otfwi_get_blocking(self._h, ptr(out)))
+        return tuple(int(v) for v in out)
+
     def get_launch(self):
         a, b = ctypes.c_int(), ctypes.c_int()
         check(self._lib.otfwi_get_launch(self._h, ctypes.byref(a), ctypes.byref(b)))

@@ -62,3 +62,18 @@ def test_config1_graph_speedup(gpu):
         out[graph] = (time.perf_counter() - t0) / 3
     print(f"config1 L2 gradient: {out[False] * 1e3:.1f} ms direct, {out[True] * 1e3:.1f} ms graph")
     assert out[True] <= 1.2 * out[False]
+
+
+def test_cache_blocking_bitwise(gpu):
+    p = small_problem(X=200, Z=160, T=300, seed=51, n_src=3, n_rx=8)
+    shots = [0, 1, 2, 1]
+    obs = np.stack([O.forward(p["cq_true"], p["d_x"], p["d_z"], p["src_xz"], p["src"],
+                              p["rx_xz"], p["T"], idx=i)["rec"] for i in shots])
+    m = make_model(p, cq="cq_bg")
+    m.set_observed(obs)
+    g0, mis0 = m.gradient(shots, "l2")
+    for kf, gf, ka, ga in ((3, 2, 5, 1), (7, 3, 2, 2)):
+        m.set_blocking(kf, gf, ka, ga)
+        assert m.get_blocking() == (kf, gf, ka, ga)
+        g, mis = m.gradient(shots, "l2")
+        assert np.array_equal(g, g0) and np.array_equal(mis, mis0)
