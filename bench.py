@@ -118,6 +118,24 @@ def calib_copy(nbytes):
     print(f"[bench] copied {nbytes} bytes device-to-device", file=sys.stderr, flush=True)
 
 
+def profiled_traffic(kernel, N, S):
+    """HBM bytes per launch of `kernel` (e.g. "adj_step_kernel<float,32>") from
+    the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the 64-shot
+    1024^2 workload (scaled by the 1 GiB copy calibration); None for other
+    workloads or when the summary has no line for the kernel."""
+    import re
+
+    if (N, S) != (1024, 64):
+        return None
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "hbm_traffic_fmc64_1024x1024.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        mt = re.search(r"traffic_bytes_per_launch " + re.escape(kernel) + r": (\S+)", f.read())
+    return float(mt.group(1)) if mt else None
+
+
 def unet_timing(N=8, H=1024, reps=5):
     """Training step and forward of the U-Net at the learned preconditioner's
     full-size shape (batch 8 of 1024^2, two input channels) on the MFMA
@@ -266,11 +284,16 @@ def main():
                    "autotune": tune},
         "roofline": {"bound": "hbm", "kernel": "adj_step_kernel",
                      "achieved": adj_ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": adj_ach / HBM_PEAK, "traffic": None,
+                     "frac": adj_ach / HBM_PEAK,
+                     "traffic": profiled_traffic(f"adj_step_kernel<float,{m.get_tuning()[1]}>", N, S),
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1, "
+                                     "profiles/hbm_traffic_fmc64_1024x1024.md)",
                      "bytes_per_cell_step": ADJ_BYTES, "cells_per_launch": cells,
                      "launch_ms": adj_launch_s * 1e3},
         "fwd_kernel": {"kernel": "fwd_step_kernel", "achieved_GBps": fwd_ach / 1e9,
                        "frac": fwd_ach / HBM_PEAK, "launch_ms": fwd_launch_s * 1e3,
+                       "traffic": profiled_traffic(f"fwd_step_kernel<float,{m.get_tuning()[0]}>",
+                                                   N, S),
                        "bytes_per_cell_step": FWD_BYTES},
         "whole_gradient_hbm_frac": alg_total / (ms_step / 1e3) / HBM_PEAK,
         "phase_ms": {"forward": t_ev["fwd_ms"] / args.steps,

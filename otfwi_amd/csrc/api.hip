@@ -87,6 +87,10 @@ struct Batch {
     int *shot_src = nullptr;
     float *obs = nullptr;   // obs of the shots in this batch
     size_t ring_elems = 0;
+    // which observed traces `obs` holds: shots [obs_s0, obs_s0 + obs_nb) of
+    // version obs_ver of the model's observed data (obs_nb = 0: none)
+    int obs_s0 = 0, obs_nb = 0;
+    unsigned long long obs_ver = 0;
 };
 
 struct otfwi_model {
@@ -122,6 +126,7 @@ struct otfwi_model {
     int tf_fwd_th = 0, tf_adj_th = 0;
     std::vector<float> obs_h;
     int obs_shots = 0;
+    unsigned long long obs_ver = 1;   // bumped by every otfwi_set_observed
     Batch b;
     double *gacc = nullptr;          // FP64 sum of per-shot gradients across batches
     // source-side illumination of the last gradient call: sum over its shots of
@@ -161,6 +166,7 @@ void free_batch(Batch &b)
     dfree(b.mis_tr); dfree(b.mis_shot); dfree(b.scratch); dfree(b.shot_src); dfree(b.obs);
     b.cap = 0;
     b.ring_elems = 0;
+    b.obs_nb = 0;
 }
 
 int ensure_batch(otfwi_model *m, int n, bool need_adj, bool need_ring = true)
@@ -698,9 +704,15 @@ int upload_obs_batch(otfwi_model *m, int s0, int nb)
 {
     if (m->obs_shots < s0 + nb)
         return fail(OTFWI_E_STATE, "observed data holds %d shots, need %d", m->obs_shots, s0 + nb);
+    Batch &b = m->b;
+    // the traces stay resident while the same shots of the same data are asked for
+    if (b.obs_nb == nb && b.obs_s0 == s0 && b.obs_ver == m->obs_ver) return OTFWI_OK;
     const size_t per = (size_t)m->n_rx * m->T;
-    HIPCK(hipMemcpyAsync(m->b.obs, m->obs_h.data() + (size_t)s0 * per, nb * per * sizeof(float),
+    HIPCK(hipMemcpyAsync(b.obs, m->obs_h.data() + (size_t)s0 * per, nb * per * sizeof(float),
                          hipMemcpyHostToDevice, m->stream));
+    b.obs_s0 = s0;
+    b.obs_nb = nb;
+    b.obs_ver = m->obs_ver;
     return OTFWI_OK;
 }
 
@@ -1350,6 +1362,7 @@ int otfwi_set_observed(otfwi_model *m, int n_shots, const float *obs)
     const size_t cnt = (size_t)n_shots * m->n_rx * m->T;
     m->obs_h.assign(obs, obs + cnt);
     m->obs_shots = n_shots;
+    m->obs_ver++;
     return OTFWI_OK;
 }
 
