@@ -37,10 +37,12 @@ int otfwi_unet_conv3x3(const otfwi_bf16 *A, int CA, const otfwi_bf16 *B, int CB,
                        otfwi_bf16 *Y, int N, int H, int W, int Cout, int relu, void *stream);
 
 /* Weight gradient of that convolution: dW[co][tap][ci] += sum over pixels of
- * G[p][co] * input[p + tap][ci] (FP32, atomically added: zero dW first).
- * tiles_per_wg = 256-pixel tiles summed per work-group before its atomics. */
+ * G[p][co] * input[p + tap][ci], and (dB may be NULL) the bias gradient
+ * dB[co] += sum over pixels of G[p][co] (FP32, atomically added: zero both
+ * first).  tiles_per_wg = 256-pixel tiles summed per work-group before its
+ * atomics. */
 int otfwi_unet_wgrad(const otfwi_bf16 *A, int CA, const otfwi_bf16 *B, int CB, int b_half,
-                     const otfwi_bf16 *G, float *dW, int N, int H, int W, int Cout,
+                     const otfwi_bf16 *G, float *dW, float *dB, int N, int H, int W, int Cout,
                      int tiles_per_wg, void *stream);
 
 /* Wf[ci][tap][co] = W[co][8 - tap][ci]. */

@@ -201,6 +201,7 @@ struct WgradArgs {
     Src in;
     const bf16 *G;   // [N][H][W][Cout]
     float *dW;       // [Cout][9][Cin]
+    float *dB;       // [Cout] bias gradient (added by the ci-block 0 work-groups) or null
     int N, Cout, tiles_per_wg;
 };
 
@@ -238,6 +239,8 @@ __global__ __launch_bounds__(192) void wgrad_kernel(WgradArgs a)
 #pragma unroll
             for (int e = 0; e < 16; e++) acc[m][d][e] = 0.0f;
 
+    const bool do_bias = a.dB && blockIdx.z == 0 && tid < CO;
+    float bsum = 0.0f;
     const int t_begin = blockIdx.x * a.tiles_per_wg;
     const int t_end = min(t_begin + a.tiles_per_wg, tiles_all);
     for (int t = t_begin; t < t_end; t++) {
@@ -263,6 +266,8 @@ __global__ __launch_bounds__(192) void wgrad_kernel(WgradArgs a)
             for (int k = 0; k < 8; k++) xT[((part * 8 + k) * PH + py) * PW + px] = xs[k];
         }
         __syncthreads();
+        if (do_bias)
+            for (int q = 0; q < 256; q++) bsum += (float)gT[tid * GS + q];
         const int dy = wave;
 #pragma unroll 2
         for (int ks = 0; ks < 16; ks++) {
@@ -286,6 +291,7 @@ __global__ __launch_bounds__(192) void wgrad_kernel(WgradArgs a)
         }
         __syncthreads();
     }
+    if (do_bias) atomicAdd(a.dB + co0 + tid, bsum);
     const int ci = ci0 + r;
     if (ci >= Cin) return;
 #pragma unroll
@@ -560,12 +566,12 @@ int otfwi_unet_conv3x3(const otfwi_bf16 *A, int CA, const otfwi_bf16 *B, int CB,
 }
 
 int otfwi_unet_wgrad(const otfwi_bf16 *A, int CA, const otfwi_bf16 *B, int CB, int b_half,
-                     const otfwi_bf16 *G, float *dW, int N, int H, int W, int Cout,
+                     const otfwi_bf16 *G, float *dW, float *dB, int N, int H, int W, int Cout,
                      int tiles_per_wg, void *stream)
 {
     WgradArgs a{};
     a.in = Src{BF(A), CA, BF(B), CB, b_half, H, W};
-    a.G = BF(G); a.dW = dW; a.N = N; a.Cout = Cout; a.tiles_per_wg = tiles_per_wg;
+    a.G = BF(G); a.dW = dW; a.dB = dB; a.N = N; a.Cout = Cout; a.tiles_per_wg = tiles_per_wg;
     if (int rc = check_src(a.in, N)) return rc;
     if (!G || !dW || !al16(G) || Cout <= 0 || Cout % 32)
         return ufail("bad gradient/output (Cout must be a multiple of 32)");

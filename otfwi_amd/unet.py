@@ -187,9 +187,8 @@ class UNet:
         other = max(1, co // 64) * ((ca + cb + 31) // 32)
         tpw = max(1, -(-tiles * other // 2048))
         _lib.ucheck(self.lib.otfwi_unet_wgrad(_vp(A), ca, _vp(B), cb, 1 if cb else 0, _vp(G),
-                                              self._gw(name), N, H, W, co, tpw, self._st()))
-        _lib.ucheck(self.lib.otfwi_unet_bias_grad(_vp(G), N * H * W, co, self._gb(name),
-                                                  self._st()))
+                                              self._gw(name), self._gb(name), N, H, W, co, tpw,
+                                              self._st()))
 
     def _pool(self, X, Y, N, H, W, C):
         _lib.ucheck(self.lib.otfwi_unet_pool2(_vp(X), N, H, W, C, _vp(Y), self._st()))

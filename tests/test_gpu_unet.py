@@ -93,8 +93,9 @@ def test_wgrad_and_data_grad_vs_autograd(gpu, N, H, W, CA, CB, Cout):
     # device copies stay referenced until the kernels have run
     Ad, Bd, Gc, dwc = A.cuda(), None if B is None else B.cuda(), G.cuda(), w.cuda()
     dW = torch.zeros((Cout, 9, CA + CB), dtype=torch.float32, device="cuda")
-    _lib.ucheck(lib.otfwi_unet_wgrad(vp(Ad), CA, vp(Bd), CB, 1 if CB else 0, vp(Gc), vp(dW), N, H,
-                                     W, Cout, 3, st()))
+    dB = torch.zeros(Cout, dtype=torch.float32, device="cuda")
+    _lib.ucheck(lib.otfwi_unet_wgrad(vp(Ad), CA, vp(Bd), CB, 1 if CB else 0, vp(Gc), vp(dW),
+                                     vp(dB), N, H, W, Cout, 3, st()))
     wf = torch.empty((CA + CB) * 9 * Cout, dtype=torch.bfloat16, device="cuda")
     _lib.ucheck(lib.otfwi_unet_flip_weights(vp(dwc), Cout, CA + CB, vp(wf), st()))
     dX = torch.empty((N, H, W, CA + CB), dtype=torch.bfloat16, device="cuda")
@@ -103,8 +104,9 @@ def test_wgrad_and_data_grad_vs_autograd(gpu, N, H, W, CA, CB, Cout):
                                            N, H, W, CA + CB, 0, st()))
     torch.cuda.synchronize()
     ew = rel(dW.cpu().numpy(), dw_ref)
-    print(f"wgrad {H}x{W} {CA}+{CB}->{Cout}: rel {ew:.2e}")
-    assert ew < 1e-4
+    eb = rel(dB.cpu().numpy(), G.float().sum(dim=(0, 1, 2)).numpy())
+    print(f"wgrad {H}x{W} {CA}+{CB}->{Cout}: rel {ew:.2e}, bias grad rel {eb:.2e}")
+    assert ew < 1e-4 and eb < 1e-5
     if (CA + CB) % 32 == 0:
         # the data gradient at full resolution (before the upsample's 2x2 sum)
         ex = rel(dX.float().cpu().numpy(), dx_ref)
