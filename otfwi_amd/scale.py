@@ -1,0 +1,118 @@
+"""Throughput runs of the uneven multi-model batch and of the checkpointed
+exact gradient on one GPU (python -m otfwi_amd.scale c3|exact).  Each prints
+one JSON summary line; progress goes to stderr.
+
+c3     16 independent models with grid sizes drawn from {512, 768, 1024,
+       1536, 2048}^2, a 32-element array and 32 shots each, T scaled with the
+       grid depth as T = round(2500 + 3500 (N - 512) / 1536) (times --t-scale),
+       one gradient of every model (--misfit) through otfwi_gradient_multi with
+       1 and with --workers host threads.
+exact  one checkpointed exact-adjoint gradient (L2) of the 1024^2 FMC block,
+       beside the reference-mode gradient of the same shots.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import configs
+from .cuda_interface_pml import Cuda_interface_PML
+from .model import gradient_multi
+
+FWD_B, ADJ_B = 28, 60
+
+
+def log(s):
+    print(s, file=sys.stderr, flush=True)
+
+
+def _model(N, T, n_elem, pitch, seed, max_batch):
+    p = configs.config2(model_seed=seed, N=N, T=T, n_elem=n_elem, pitch=pitch)
+    cp = Cuda_interface_PML(N, N, T, p["pml"], p["c_true"], n_elem, p["src_mask"], n_elem,
+                            p["rx_mask"], p["src"], multishot=True, R=p["R"],
+                            max_batch=max_batch)
+    cp.simulate()
+    from . import adjoint_functions as adj
+    cp.init_regression(cp.recording.copy(), adj.wasserstein, prec_deriv=4)
+    cp.model.set_cquad((p["c_bg"] ** 2).astype(np.float32))
+    return cp
+
+
+def run_c3(n_models=16, shots=32, t_scale=1.0, workers=4, max_batch=8, seed=0, misfit="w2"):
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice([512, 768, 1024, 1536, 2048], size=n_models)
+    models, cps, alg = [], [], 0.0
+    for i, N in enumerate(sizes):
+        T = int(round((2500 + 3500 * (N - 512) / 1536) * t_scale))
+        pitch = max(2, (N - 2 * 20 - 40) // (shots - 1) // 2)
+        cp = _model(int(N), T, shots, pitch, seed=1000 + i, max_batch=max_batch)
+        cps.append(cp)
+        models.append(cp.model)
+        alg += (FWD_B + ADJ_B) * float(N) * N * shots * (T - 1)
+        log(f"[c3] model {i}: {N}^2, T={T}, {shots} shots")
+    shot_lists = [np.arange(shots)] * n_models
+    out = {"workload": f"C3: {n_models} models, sizes {sorted(int(s) for s in sizes)}, "
+                       f"{shots} shots each, T scale {t_scale}, {misfit}",
+           "algorithmic_bytes": alg}
+    for w in (1, workers):
+        gradient_multi(models, shot_lists, misfit, workers=w)      # graphs, buffers
+        t0 = time.perf_counter()
+        res = gradient_multi(models, shot_lists, misfit, workers=w)
+        dt = time.perf_counter() - t0
+        assert all(np.isfinite(g).all() for g, _ in res)
+        out[f"workers_{w}"] = {"seconds": dt, "hbm_frac": alg / dt / 8e12,
+                               "model_gradients_per_s": n_models / dt}
+        log(f"[c3] workers {w}: {dt:.2f} s ({alg / dt / 8e12:.1%} of HBM peak, algorithmic)")
+    return out
+
+
+def run_exact(N=1024, T=4000, shots=8, snap=0):
+    cp = _model(N, T, 64, 12, seed=0, max_batch=0)
+    m = cp.model
+    s = np.arange(shots)
+    m.gradient(s, "l2")
+    t0 = time.perf_counter()
+    g_ref, _ = m.gradient(s, "l2")
+    t_ref = time.perf_counter() - t0
+    m.gradient_exact(s, "l2", snap_every=snap)
+    t0 = time.perf_counter()
+    g_ex, mis = m.gradient_exact(s, "l2", snap_every=snap)
+    t_ex = time.perf_counter() - t0
+    k = snap or int(round(np.sqrt(T - 1)))
+    pml = 20
+    inner = np.zeros((N, N), bool)
+    inner[pml + 16:N - pml - 16, pml + 16:N - pml - 16] = True
+    a, b = g_ref[inner].astype(np.float64), g_ex[inner].astype(np.float64)
+    cos = float(a @ b / np.linalg.norm(a) / np.linalg.norm(b))
+    plane = (N + 128) * (N + 16) * 4
+    return {"workload": f"{shots}-shot {N}^2 T={T} L2 gradient", "snap_every": k,
+            "exact_seconds": t_ex, "reference_mode_seconds": t_ref,
+            "exact_over_reference": t_ex / t_ref,
+            "checkpoint_bytes": (int(np.ceil((T - 1) / k)) * 4 + k * 2) * plane * shots,
+            "full_history_bytes": (T - 1) * 2 * plane * shots,
+            "interior_cosine_exact_vs_reference": cos}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("what", choices=["c3", "exact"])
+    ap.add_argument("--t-scale", type=float, default=1.0)
+    ap.add_argument("--shots", type=int, default=32)
+    ap.add_argument("--workers", type=int, default=4)
+    ap.add_argument("--models", type=int, default=16)
+    ap.add_argument("--T", type=int, default=4000)
+    ap.add_argument("--misfit", default="w2", choices=["w2", "l2"])
+    a = ap.parse_args(argv)
+    if a.what == "c3":
+        out = run_c3(a.models, a.shots, a.t_scale, a.workers, misfit=a.misfit)
+    else:
+        out = run_exact(T=a.T, shots=min(a.shots, 64))
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
