@@ -119,20 +119,51 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(ConvArgs a)
         pbase[i] = (q / TX) * PW + (q % TX);
     }
 
-    for (int c0 = 0; c0 < Cin; c0 += CK) {
-        for (int v = tid; v < PH * PW * PV; v += 256) {
-            const int pp = v / PV, part = v - pp * PV;
-            const int py = pp / PW, px = pp - py * PW;
-            *(u32x4 *)(patch + pp * PS + part * 8) =
-                load8(a.in, n, ty0 + py - 1, tx0 + px - 1, c0 + part * 8);
+    // The next chunk's patch and weights are loaded into registers while the
+    // current chunk's MFMAs run, then written to LDS after the barrier.
+    constexpr int NPV = (PH * PW * PV + 255) / 256, NWV = (9 * TN * PV + 255) / 256;
+    u32x4 rp[NPV], rw[NWV];
+    auto fetch = [&](int c0) {
+#pragma unroll
+        for (int k = 0; k < NPV; k++) {
+            const int v = tid + 256 * k;
+            if (v < PH * PW * PV) {
+                const int pp = v / PV, part = v - pp * PV;
+                const int py = pp / PW, px = pp - py * PW;
+                rp[k] = load8(a.in, n, ty0 + py - 1, tx0 + px - 1, c0 + part * 8);
+            }
         }
-        for (int v = tid; v < 9 * TN * PV; v += 256) {
-            const int row = v / PV, part = v - row * PV;   // row = tap*TN + co
-            const int tap = row / TN, co = row - tap * TN;
-            *(u32x4 *)(wl + row * WS + part * 8) =
-                *(const u32x4 *)(a.Wt + ((size_t)(co0 + co) * 9 + tap) * Cin + c0 + part * 8);
+#pragma unroll
+        for (int k = 0; k < NWV; k++) {
+            const int v = tid + 256 * k;
+            if (v < 9 * TN * PV) {
+                const int row = v / PV, part = v - row * PV;   // row = tap*TN + co
+                const int tap = row / TN, co = row - tap * TN;
+                rw[k] = *(const u32x4 *)(a.Wt + ((size_t)(co0 + co) * 9 + tap) * Cin + c0 +
+                                         part * 8);
+            }
+        }
+    };
+    fetch(0);
+    for (int c0 = 0; c0 < Cin; c0 += CK) {
+#pragma unroll
+        for (int k = 0; k < NPV; k++) {
+            const int v = tid + 256 * k;
+            if (v < PH * PW * PV) {
+                const int pp = v / PV, part = v - pp * PV;
+                *(u32x4 *)(patch + pp * PS + part * 8) = rp[k];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NWV; k++) {
+            const int v = tid + 256 * k;
+            if (v < 9 * TN * PV) {
+                const int row = v / PV, part = v - row * PV;
+                *(u32x4 *)(wl + row * WS + part * 8) = rw[k];
+            }
         }
         __syncthreads();
+        if (c0 + CK < Cin) fetch(c0 + CK);
 #pragma unroll 1
         for (int tap = 0; tap < 9; tap++) {
             const int dy = tap / 3, dx = tap - 3 * (tap / 3);
@@ -243,29 +274,60 @@ __global__ __launch_bounds__(192) void wgrad_kernel(WgradArgs a)
     float bsum = 0.0f;
     const int t_begin = blockIdx.x * a.tiles_per_wg;
     const int t_end = min(t_begin + a.tiles_per_wg, tiles_all);
-    for (int t = t_begin; t < t_end; t++) {
+    // the next tile's gradient and input patch are loaded into registers
+    // while the current tile's MFMAs run
+    constexpr int NG = 256 * (CO / 8), NX = PH * (TX + 2) * 4;
+    constexpr int KG = (NG + 191) / 192, KX = (NX + 191) / 192;
+    u32x4 rg[KG], rx[KX];
+    auto fetch = [&](int t) {
         const int n = t / tiles_img, tt = t - n * tiles_img;
         const int tx0 = (tt % tiles_x) * TX, ty0 = (tt / tiles_x) * TY;
-        // gradient tile -> gT[co][q]
-        for (int v = tid; v < 256 * (CO / 8); v += 192) {
-            const int q = v / (CO / 8), part = v - q * (CO / 8);
-            const int y = ty0 + q / TX, x = tx0 + q % TX;
-            const u32x4 g =
-                *(const u32x4 *)(a.G + (((size_t)n * H + y) * W + x) * a.Cout + co0 + part * 8);
-            const bf16 *gv = (const bf16 *)&g;
 #pragma unroll
-            for (int k = 0; k < 8; k++) gT[(part * 8 + k) * GS + q] = gv[k];
+        for (int k = 0; k < KG; k++) {
+            const int v = tid + 192 * k;
+            if (v < NG) {
+                const int q = v / (CO / 8), part = v - q * (CO / 8);
+                const int y = ty0 + q / TX, x = tx0 + q % TX;
+                rg[k] = *(const u32x4 *)(a.G + (((size_t)n * H + y) * W + x) * a.Cout + co0 +
+                                         part * 8);
+            }
         }
-        // input patch -> xT[ci][py][px]
-        for (int v = tid; v < PH * (TX + 2) * 4; v += 192) {
-            const int pp = v >> 2, part = v & 3;
-            const int py = pp / (TX + 2), px = pp - py * (TX + 2);
-            const u32x4 xv = load8(a.in, n, ty0 + py - 1, tx0 + px - 1, ci0 + part * 8);
-            const bf16 *xs = (const bf16 *)&xv;
 #pragma unroll
-            for (int k = 0; k < 8; k++) xT[((part * 8 + k) * PH + py) * PW + px] = xs[k];
+        for (int k = 0; k < KX; k++) {
+            const int v = tid + 192 * k;
+            if (v < NX) {
+                const int pp = v >> 2, part = v & 3;
+                const int py = pp / (TX + 2), px = pp - py * (TX + 2);
+                rx[k] = load8(a.in, n, ty0 + py - 1, tx0 + px - 1, ci0 + part * 8);
+            }
+        }
+    };
+    if (t_begin < t_end) fetch(t_begin);
+    for (int t = t_begin; t < t_end; t++) {
+        // gradient tile -> gT[co][q], input patch -> xT[ci][py][px]
+#pragma unroll
+        for (int k = 0; k < KG; k++) {
+            const int v = tid + 192 * k;
+            if (v < NG) {
+                const int q = v / (CO / 8), part = v - q * (CO / 8);
+                const bf16 *gv = (const bf16 *)&rg[k];
+#pragma unroll
+                for (int e = 0; e < 8; e++) gT[(part * 8 + e) * GS + q] = gv[e];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < KX; k++) {
+            const int v = tid + 192 * k;
+            if (v < NX) {
+                const int pp = v >> 2, part = v & 3;
+                const int py = pp / (TX + 2), px = pp - py * (TX + 2);
+                const bf16 *xs = (const bf16 *)&rx[k];
+#pragma unroll
+                for (int e = 0; e < 8; e++) xT[((part * 8 + e) * PH + py) * PW + px] = xs[e];
+            }
         }
         __syncthreads();
+        if (t + 1 < t_end) fetch(t + 1);
         if (do_bias)
             for (int q = 0; q < 256; q++) bsum += (float)gT[tid * GS + q];
         const int dy = wave;
@@ -277,12 +339,20 @@ __global__ __launch_bounds__(192) void wgrad_kernel(WgradArgs a)
 #pragma unroll
             for (int m = 0; m < MJ; m++)
                 af[m] = *(const bf16x8 *)(gT + (m * 32 + r) * GS + q0);
+            // pixels tx+dx .. tx+dx+7 of channel r for dx = 0, 1, 2: two aligned
+            // 16-byte reads (elements tx .. tx+15) and 2-byte funnel shifts
             const bf16 *xrow = xT + (r * PH + ty + dy) * PW + tx;
+            const u32x4 lo = *(const u32x4 *)xrow, hi = *(const u32x4 *)(xrow + 8);
+            u32x4 bw[3];
+            bw[0] = lo;
+            bw[1] = u32x4{__builtin_amdgcn_alignbyte(lo.y, lo.x, 2),
+                          __builtin_amdgcn_alignbyte(lo.z, lo.y, 2),
+                          __builtin_amdgcn_alignbyte(lo.w, lo.z, 2),
+                          __builtin_amdgcn_alignbyte(hi.x, lo.w, 2)};
+            bw[2] = u32x4{lo.y, lo.z, lo.w, hi.x};
 #pragma unroll
             for (int dx = 0; dx < 3; dx++) {
-                bf16x8 bv;
-#pragma unroll
-                for (int k = 0; k < 8; k++) bv[k] = xrow[dx + k];
+                const bf16x8 bv = __builtin_bit_cast(bf16x8, bw[dx]);
 #pragma unroll
                 for (int m = 0; m < MJ; m++)
                     acc[m][dx] =
