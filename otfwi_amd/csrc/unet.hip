@@ -431,45 +431,65 @@ __global__ void pool2_kernel(const bf16 *X, int N, int H, int W, int C, bf16 *Y)
 }
 
 // out[n][y][x][c] (coarse, C channels) = mask > 0 ? sum over the 2x2 fine
-// pixels of src[.][.][.][off + c] : 0   (backward of nearest upsample + ReLU)
+// pixels of src[.][.][.][off + c] : 0   (backward of nearest upsample + ReLU);
+// 8 channels per thread
 __global__ void upsum_mask_kernel(const bf16 *src, int Cs, int off, const bf16 *mask, int N,
                                   int Ho, int Wo, int C, bf16 *out)
 {
+    const int C8 = C / 8;
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)N * Ho * Wo * C) return;
-    const int c = (int)(i % C);
-    size_t p = i / C;
-    const int x = (int)(p % Wo); p /= Wo;
-    const int y = (int)(p % Ho);
-    const int n = (int)(p / Ho);
-    float v = 0.0f;
-    if ((float)mask[i] > 0.0f) {
-        const int H = 2 * Ho, W = 2 * Wo;
-        for (int dy = 0; dy < 2; dy++)
-            for (int dx = 0; dx < 2; dx++)
-                v += (float)src[(((size_t)n * H + 2 * y + dy) * W + 2 * x + dx) * Cs + off + c];
-    }
-    out[i] = (bf16)v;
+    if (i >= (size_t)N * Ho * Wo * C8) return;
+    const int c = (int)(i % C8) * 8;
+    const size_t p = i / C8;
+    const int x = (int)(p % Wo);
+    const int y = (int)((p / Wo) % Ho);
+    const int n = (int)(p / ((size_t)Wo * Ho));
+    const int H = 2 * Ho, W = 2 * Wo;
+    const u32x4 mv = *(const u32x4 *)(mask + p * C + c);
+    const bf16 *mb = (const bf16 *)&mv;
+    float v[8] = {};
+    for (int dy = 0; dy < 2; dy++)
+        for (int dx = 0; dx < 2; dx++) {
+            const u32x4 sv = *(const u32x4 *)(src + (((size_t)n * H + 2 * y + dy) * W + 2 * x + dx) *
+                                                       Cs + off + c);
+            const bf16 *sb = (const bf16 *)&sv;
+#pragma unroll
+            for (int k = 0; k < 8; k++) v[k] += (float)sb[k];
+        }
+    u32x4 o;
+    bf16 *ob = (bf16 *)&o;
+#pragma unroll
+    for (int k = 0; k < 8; k++) ob[k] = (bf16)((float)mb[k] > 0.0f ? v[k] : 0.0f);
+    *(u32x4 *)(out + p * C + c) = o;
 }
 
 // out[n][y][x][c] = mask > 0 ? src[.][off + c] + below[n][y/2][x/2][c] / 4 : 0
-// (gradient of an encoder output: skip path + average-pool path, then ReLU)
+// (gradient of an encoder output: skip path + average-pool path, then ReLU);
+// 8 channels per thread
 __global__ void skip_combine_kernel(const bf16 *src, int Cs, int off, const bf16 *below,
                                     const bf16 *mask, int N, int H, int W, int C, bf16 *out)
 {
+    const int C8 = C / 8;
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)N * H * W * C) return;
-    const int c = (int)(i % C);
-    size_t p = i / C;
-    const int x = (int)(p % W); p /= W;
-    const int y = (int)(p % H);
-    const int n = (int)(p / H);
-    float v = 0.0f;
-    if ((float)mask[i] > 0.0f) {
-        v = (float)src[(((size_t)n * H + y) * W + x) * Cs + off + c];
-        v += 0.25f * (float)below[(((size_t)n * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c];
+    if (i >= (size_t)N * H * W * C8) return;
+    const int c = (int)(i % C8) * 8;
+    const size_t p = i / C8;
+    const int x = (int)(p % W);
+    const int y = (int)((p / W) % H);
+    const int n = (int)(p / ((size_t)W * H));
+    const u32x4 mv = *(const u32x4 *)(mask + p * C + c);
+    const u32x4 sv = *(const u32x4 *)(src + p * Cs + off + c);
+    const u32x4 bv =
+        *(const u32x4 *)(below + (((size_t)n * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c);
+    const bf16 *mb = (const bf16 *)&mv, *sb = (const bf16 *)&sv, *bb = (const bf16 *)&bv;
+    u32x4 o;
+    bf16 *ob = (bf16 *)&o;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const float v = (float)sb[k] + 0.25f * (float)bb[k];
+        ob[k] = (bf16)((float)mb[k] > 0.0f ? v : 0.0f);
     }
-    out[i] = (bf16)v;
+    *(u32x4 *)(out + p * C + c) = o;
 }
 
 // x0[n][y][x][0..15] = (in[n][0][y][x], in[n][1][y][x], 0, ...) in BF16
@@ -510,7 +530,7 @@ __global__ __launch_bounds__(256) void head_bwd_kernel(const bf16 *X, const floa
                                                        size_t np, float scale, bf16 *gX,
                                                        float *dw, float *db, double *loss)
 {
-    __shared__ float red[256][34];
+    __shared__ float red[4][34];
     const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     float part[34];
     for (int k = 0; k < 34; k++) part[k] = 0.0f;
@@ -532,13 +552,19 @@ __global__ __launch_bounds__(256) void head_bwd_kernel(const bf16 *X, const floa
             *(u32x4 *)(gX + p * 32 + c8) = o;
         }
     }
-    for (int k = 0; k < 34; k++) red[threadIdx.x][k] = part[k];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 34; k++) {
+        float v = part[k];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (lane == 0) red[wave][k] = v;
+    }
     __syncthreads();
     if (threadIdx.x < 34) {
-        float s = 0.0f;
-        for (int t = 0; t < 256; t++) s += red[t][threadIdx.x];
-        if (threadIdx.x < 32) atomicAdd(dw + threadIdx.x, s);
-        else if (threadIdx.x == 32) atomicAdd(db, s);
+        const int k = threadIdx.x;
+        const float s = red[0][k] + red[1][k] + red[2][k] + red[3][k];
+        if (k < 32) atomicAdd(dw + k, s);
+        else if (k == 32) atomicAdd(db, s);
         else atomicAdd(loss, (double)s);
     }
 }
@@ -677,7 +703,9 @@ int otfwi_unet_upsum_mask(const otfwi_bf16 *src, int Cs, int off, const otfwi_bf
                           int Ho, int Wo, int C, otfwi_bf16 *out, void *stream)
 {
     if (!src || !mask || !out || off < 0 || off + C > Cs) return ufail("bad upsum args");
-    const size_t n = (size_t)N * Ho * Wo * C;
+    if (C % 8 || Cs % 8 || off % 8 || !al16(src) || !al16(mask) || !al16(out))
+        return ufail("upsum: channel counts must be multiples of 8, tensors 16-byte aligned");
+    const size_t n = (size_t)N * Ho * Wo * (C / 8);
     hipLaunchKernelGGL(upsum_mask_kernel, dim3(blocks(n)), dim3(256), 0, ST(stream), BF(src), Cs,
                        off, BF(mask), N, Ho, Wo, C, BFW(out));
     return hipck(hipGetLastError());
@@ -689,7 +717,9 @@ int otfwi_unet_skip_combine(const otfwi_bf16 *src, int Cs, int off, const otfwi_
 {
     if (!src || !below || !mask || !out || off < 0 || off + C > Cs || H % 2 || W % 2)
         return ufail("bad skip-combine args");
-    const size_t n = (size_t)N * H * W * C;
+    if (C % 8 || Cs % 8 || off % 8 || !al16(src) || !al16(below) || !al16(mask) || !al16(out))
+        return ufail("skip-combine: channel counts must be multiples of 8, tensors 16-byte aligned");
+    const size_t n = (size_t)N * H * W * (C / 8);
     hipLaunchKernelGGL(skip_combine_kernel, dim3(blocks(n)), dim3(256), 0, ST(stream), BF(src), Cs,
                        off, BF(below), BF(mask), N, H, W, C, BFW(out));
     return hipck(hipGetLastError());
