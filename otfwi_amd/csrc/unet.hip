@@ -15,6 +15,7 @@
 // K = pixels.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <string>
@@ -531,14 +532,15 @@ __global__ __launch_bounds__(256) void head_bwd_kernel(const bf16 *X, const floa
                                                        float *dw, float *db, double *loss)
 {
     __shared__ float red[4][34];
-    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     float part[34];
     for (int k = 0; k < 34; k++) part[k] = 0.0f;
-    if (p < np) {
+    // grid-stride: few blocks, so that few partial sums meet in the atomics
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < np;
+         p += (size_t)gridDim.x * blockDim.x) {
         const float d = out[p] - target[p];
         const float g = scale * d;
-        part[32] = g;
-        part[33] = d * d;
+        part[32] += g;
+        part[33] += d * d;
         for (int c8 = 0; c8 < 32; c8 += 8) {
             const u32x4 v = *(const u32x4 *)(X + p * 32 + c8);
             const bf16 *xb = (const bf16 *)&v;
@@ -546,7 +548,7 @@ __global__ __launch_bounds__(256) void head_bwd_kernel(const bf16 *X, const floa
             bf16 *ob = (bf16 *)&o;
             for (int k = 0; k < 8; k++) {
                 const float xv = (float)xb[k];
-                part[c8 + k] = g * xv;
+                part[c8 + k] += g * xv;
                 ob[k] = (bf16)(xv > 0.0f ? g * w[c8 + k] : 0.0f);
             }
             *(u32x4 *)(gX + p * 32 + c8) = o;
@@ -750,7 +752,8 @@ int otfwi_unet_head_bwd(const otfwi_bf16 *X, const float *w, const float *out,
     if (!X || !w || !out || !target || !gX || !dw || !db || !loss || np <= 0 || !al16(X) ||
         !al16(gX))
         return ufail("bad head-backward args");
-    hipLaunchKernelGGL(head_bwd_kernel, dim3(blocks((size_t)np)), dim3(256), 0, ST(stream), BF(X),
+    const unsigned nb = std::min(blocks((size_t)np), 1024u);
+    hipLaunchKernelGGL(head_bwd_kernel, dim3(nb), dim3(256), 0, ST(stream), BF(X),
                        w, out, target, (size_t)np, scale, BFW(gX), dw, db, loss);
     return hipck(hipGetLastError());
 }
