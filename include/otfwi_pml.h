@@ -146,6 +146,17 @@ int otfwi_set_blocking(otfwi_model *m, int steps_fwd, int shots_fwd, int steps_a
                        int shots_adj);
 int otfwi_get_blocking(otfwi_model *m, int *out);
 
+/* Forward time steps per launch: 1 (default) or 2.  With 2, the forward loop
+ * advances pairs of steps in one launch of a z-marching kernel that keeps the
+ * intermediate state on chip (the first steps of a range and its last step
+ * stay single launches, so the buffers of the adjoint's seeds are where a
+ * step-per-launch loop leaves them).  Results are bitwise identical to 1.  It
+ * applies to forward loops without illumination or cache blocking.  No
+ * reference counterpart (regression_PML_single.cu:745 launches one forward
+ * step at a time). */
+int otfwi_set_fwd_steps(otfwi_model *m, int steps);
+int otfwi_get_fwd_steps(otfwi_model *m, int *steps);
+
 /* Replay the per-step launch loops as captured hipGraphs (default 1); a
  * graph is re-captured only when shots, tiles, launch style or device buffers
  * change.  No reference counterpart (regression_PML_single.cu:661-678

@@ -45,6 +45,8 @@ _SIGS = [
     ("otfwi_set_graph", _i, [_vp, _i]),
     ("otfwi_set_blocking", _i, [_vp, _i, _i, _i, _i]),
     ("otfwi_get_blocking", _i, [_vp, _vp]),
+    ("otfwi_set_fwd_steps", _i, [_vp, _i]),
+    ("otfwi_get_fwd_steps", _i, [_vp, _vp]),
     ("otfwi_autotune", _i, [_vp, _i, _i, _vp, _vp]),
     ("otfwi_get_timing", _i, [_vp, _vp]),
     ("otfwi_forward", _i, [_vp, _i, _vp, _vp]),

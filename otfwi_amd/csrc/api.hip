@@ -107,6 +107,9 @@ struct otfwi_model {
     int n_src = 0, n_rx = 0, n_ring = 0;
     int *evt = nullptr;
     PointEvt *pts = nullptr;
+    unsigned *evmask = nullptr;      // two-step forward: event lane masks per column block and row
+    int *pmap = nullptr;             // two-step forward: per-cell {ring+1, rx+1, src+1, 0}
+    int fwd_steps = 1;               // time steps per forward launch (1 or 2)
     bool points_dirty = true;
     int th_fwd = 32, th_adj = 16, max_batch = 0;
     int pers_fwd = 0, pers_adj = 0;   // persistent-work-group launches (0 = one work-group per tile)
@@ -288,6 +291,30 @@ int build_points(otfwi_model *m)
     for (int i = 0; i < m->n_src; i++) get(m->sx[i], m->sz[i]).src = i;
     for (int i = 0; i < m->n_rx; i++) get(m->rxx[i], m->rxz[i]).rx = i;
     for (int i = 0; i < m->n_ring; i++) get(m->vx[i], m->vz[i]).ring = i;
+    // two-step forward: lane masks of the event cells of each 256-column block
+    // (owned columns plus halo) and the per-cell event map
+    const int nbx = ceil_div(m->X, F2_OWN);
+    std::vector<unsigned> evmask((size_t)nbx * m->Z * 8, 0u);
+    std::vector<int> pmap(map.size() * 4, 0);
+    for (size_t q = 0; q < map.size(); q++) {
+        if (map[q] < 0) continue;
+        const PointEvt &pe = pts[map[q]];
+        pmap[4 * q] = pe.ring + 1;
+        pmap[4 * q + 1] = pe.rx + 1;
+        pmap[4 * q + 2] = pe.src + 1;
+        const int z = (int)(q / m->ld) - ZOFF, x = (int)(q % m->ld) - XOFF;
+        for (int bx = 0; bx < nbx; bx++) {
+            const int c = x - (bx * F2_OWN - F2_HALO);
+            if (c >= 0 && c < 256) evmask[((size_t)bx * m->Z + z) * 8 + c / 32] |= 1u << (c % 32);
+        }
+    }
+    dfree(m->evmask);
+    dfree(m->pmap);
+    RCK(dalloc(&m->evmask, evmask.size()));
+    HIPCK(hipMemcpy(m->evmask, evmask.data(), evmask.size() * sizeof(unsigned),
+                    hipMemcpyHostToDevice));
+    RCK(dalloc(&m->pmap, pmap.size()));
+    HIPCK(hipMemcpy(m->pmap, pmap.data(), pmap.size() * sizeof(int), hipMemcpyHostToDevice));
     dfree(m->evt);
     dfree(m->pts);
     RCK(dalloc(&m->evt, map.size()));
@@ -535,7 +562,45 @@ int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum)
     const int ist = (illum && m->ill) ? m->ill_stride : 0;
     const int K = std::max(1, m->tb_steps[0]);
     const int G = m->tb_shots[0] > 0 ? std::min(m->tb_shots[0], n) : n;
+    // two steps per launch: leading single steps, then an even number of
+    // two-step launches (each writes the slot it does not read, so an even
+    // count leaves state t in slot t&1 again), then single steps -- the last
+    // step is always single, so A(t1-2) stays in slot (t1-2)&1 for the
+    // adjoint's reconstruction
+    const bool two = m->fwd_steps == 2 && ist == 0 && K == 1 && G == n && m->evmask &&
+                     fwd2_supported((long long)plane, (int)sizeof(R));
+    auto body2 = [&]() -> int {
+        const int S = t1 - t0;
+        int pairs = S > 1 ? (S - 1) / 2 : 0;
+        pairs -= pairs & 1;
+        const int lead = (S - 1) - 2 * pairs;
+        int t = t0;
+        auto single = [&](int tt) -> int {
+            const int p = (tt - 1) & 1, c = tt & 1;
+            a.t = tt;
+            a.P0 = (const R *)b.P[p]; a.P1 = (R *)b.P[c];
+            a.Ax0 = (const R *)b.Ax[p]; a.Ax1 = (R *)b.Ax[c];
+            a.Az0 = (const R *)b.Az[p]; a.Az1 = (R *)b.Az[c];
+            a.Px0 = (const R *)b.Px[p]; a.Px1 = (R *)b.Px[c];
+            HIPCK(launch_fwd_step<R>(a, m->th_fwd, pw, st));
+            return OTFWI_OK;
+        };
+        for (int k = 0; k < lead; k++, t++) RCK(single(t));
+        for (int k = 0; k < pairs; k++, t += 2) {
+            // state t-1 in slot ((t-1) ^ k) & 1 ... after k pairs the parity is flipped k times
+            const int in = ((t - 1) & 1) ^ (k & 1), out = in ^ 1;
+            a.t = t;
+            a.P0 = (const R *)b.P[in]; a.P1 = (R *)b.P[out];
+            a.Ax0 = (const R *)b.Ax[in]; a.Ax1 = (R *)b.Ax[out];
+            a.Az0 = (const R *)b.Az[in]; a.Az1 = (R *)b.Az[out];
+            a.Px0 = (const R *)b.Px[in]; a.Px1 = (R *)b.Px[out];
+            HIPCK(launch_fwd2_step<R>(a, m->evmask, m->pmap, st));
+        }
+        for (; t < t1; t++) RCK(single(t));
+        return OTFWI_OK;
+    };
     auto body = [&]() -> int {
+        if (two) return body2();
         for (int tb = t0; tb < t1; tb += K) {
             const int te = std::min(tb + K, t1);
             for (int g0 = 0; g0 < n; g0 += G) {
@@ -563,7 +628,8 @@ int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum)
         }
         return OTFWI_OK;
     };
-    const unsigned long long kb = (unsigned long long)K * 4099ULL + (unsigned long long)G;
+    const unsigned long long kb =
+        ((unsigned long long)K * 4099ULL + (unsigned long long)G) * 3ULL + (two ? 1ULL : 0ULL);
     return run_loop(m, save_ring ? 1 : 0,
                     (loop_key(m, n, m->th_fwd, pw, t0, t1) * 31ULL + (unsigned long long)ist) *
                             1000003ULL + kb, body);
@@ -1251,7 +1317,8 @@ int otfwi_destroy(otfwi_model *m)
     (void)hipStreamSynchronize(m->stream);
     free_batch(m->b);
     dfree(m->cm); dfree(m->kx); dfree(m->kz); dfree(m->icum); dfree(m->cq32);
-    dfree(m->evt); dfree(m->pts); dfree(m->tf_fwd); dfree(m->tf_adj);
+    dfree(m->evt); dfree(m->pts); dfree(m->tf_fwd); dfree(m->tf_adj); dfree(m->evmask);
+    dfree(m->pmap);
     dfree(m->gacc); dfree(m->gout); dfree(m->ill);
     dfree(m->xsnap); dfree(m->xseg); dfree(m->rxmap);
     for (int i = 0; i < 4; i++)
@@ -1401,6 +1468,22 @@ int otfwi_set_blocking(otfwi_model *m, int steps_fwd, int shots_fwd, int steps_a
         return fail(OTFWI_E_ARG, "blocking: steps >= 1 and shots >= 0 (0 = all)");
     m->tb_steps[0] = steps_fwd; m->tb_shots[0] = shots_fwd;
     m->tb_steps[1] = steps_adj; m->tb_shots[1] = shots_adj;
+    return OTFWI_OK;
+}
+
+int otfwi_set_fwd_steps(otfwi_model *m, int steps)
+{
+    RCK(check_model(m));
+    if (steps != 1 && steps != 2) return fail(OTFWI_E_ARG, "steps per forward launch must be 1 or 2");
+    m->fwd_steps = steps;
+    return OTFWI_OK;
+}
+
+int otfwi_get_fwd_steps(otfwi_model *m, int *steps)
+{
+    RCK(check_model(m));
+    if (!steps) return fail(OTFWI_E_ARG, "output is NULL");
+    *steps = m->fwd_steps;
     return OTFWI_OK;
 }
 

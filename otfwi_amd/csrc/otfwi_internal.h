@@ -109,6 +109,18 @@ template <typename R>
 hipError_t launch_fwd_step(const FwdArgs<R> &a, int th, int persistent_wgs, hipStream_t s);
 template <typename R>
 hipError_t launch_adj_step(const AdjArgs<R> &a, int th, int persistent_wgs, hipStream_t s);
+// Two forward time steps t, t+1 per launch (z-marching strips): reads the
+// state at t-1 from P0/Ax0/Az0/Px0, writes the state at t+1 to
+// P1/Ax1/Az1/Px1 (a different slot), samples traces and the ring at t and
+// t+1.  A work-group owns F2_OWN columns starting at bx * F2_OWN and computes
+// F2_HALO more on each side.  evmask[(bx * Z + z) * 8 + w] bit b marks an
+// event cell at column bx * F2_OWN - F2_HALO + 32 w + b of row z; pmap is the
+// padded-plane map of int4 {ring + 1, rx + 1, src + 1, 0} (0: none).
+constexpr int F2_OWN = 224, F2_HALO = 16;
+bool fwd2_supported(long long sstride, int prec);
+template <typename R>
+hipError_t launch_fwd2_step(const FwdArgs<R> &a, const unsigned *evmask, const int *pmap,
+                            hipStream_t s);
 // Resident work-groups per CU of a step kernel (0 if the variant does not exist).
 int step_occupancy(bool adj, int prec, int th, bool persistent);
 

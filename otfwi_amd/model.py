@@ -112,6 +112,15 @@ class PMLModel:
         check(self._lib.otfwi_get_blocking(self._h, ptr(out)))
         return tuple(int(v) for v in out)
 
+    def set_fwd_steps(self, steps):
+        """Forward time steps per launch, 1 or 2 (see otfwi_set_fwd_steps)."""
+        check(self._lib.otfwi_set_fwd_steps(self._h, int(steps)))
+
+    def get_fwd_steps(self):
+        v = ctypes.c_int()
+        check(self._lib.otfwi_get_fwd_steps(self._h, ctypes.byref(v)))
+        return int(v.value)
+
     def get_launch(self):
         a, b = ctypes.c_int(), ctypes.c_int()
         check(self._lib.otfwi_get_launch(self._h, ctypes.byref(a), ctypes.byref(b)))

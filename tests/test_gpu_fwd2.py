@@ -1,0 +1,60 @@
+"""Two forward steps per launch (otfwi_set_fwd_steps(m, 2)): traces, ring,
+reference-mode and exact gradients must be bitwise equal to one step per
+launch, for grids spanning several column blocks (224 owned columns) and row
+chunks (256 rows), with event cells in the blocks' halos, odd and even step
+counts, and both precisions."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.problems import make_model, small_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def _obs(p, shots):
+    return np.stack([O.forward(p["cq_true"], p["d_x"], p["d_z"], p["src_xz"], p["src"],
+                               p["rx_xz"], p["T"], idx=i)["rec"] for i in shots])
+
+
+def _run(p, shots, obs, steps, precision, w2):
+    m = make_model(p, precision=precision, cq="cq_bg")
+    m.set_fwd_steps(steps)
+    assert m.get_fwd_steps() == steps
+    m.set_observed(obs)
+    out = {"rec": m.forward(shots)}
+    out["g_l2"], out["mis_l2"] = m.gradient(shots, "l2")
+    if w2:
+        out["g_w2"], out["mis_w2"] = m.gradient(shots, "w2")
+    out["g_exact"], out["mis_exact"] = m.gradient_exact(shots, "l2", snap_every=7)
+    return out
+
+
+@pytest.mark.parametrize("X,Z,T,precision", [
+    (200, 160, 301, 4),     # one block, one chunk, odd step count
+    (500, 600, 240, 4),     # 3 column blocks x 3 row chunks
+    (470, 300, 6, 4),       # pairs only (5 steps after the first)
+    (96, 80, 5, 4),         # too few steps for an even number of pairs
+    (300, 280, 120, 8),     # FP64
+])
+def test_fwd2_bitwise(gpu, X, Z, T, precision):
+    p = small_problem(X=X, Z=Z, T=T, pml=10, seed=X + Z, n_src=5, n_rx=min(40, X // 8))
+    shots = [0, 3, 4, 1]
+    long = T > 20
+    obs = _obs(p, shots) if long else np.zeros((len(shots), len(p["rx_xz"][0]), T), np.float32)
+    w2 = long and bool((np.abs(obs).max(axis=-1) > 0).all())   # W2 needs nonzero traces
+    a = _run(p, shots, obs, 1, precision, w2=w2)
+    b = _run(p, shots, obs, 2, precision, w2=w2)
+    assert a.keys() == b.keys()
+    for name in a:
+        assert np.array_equal(a[name], b[name]), name
+    if long:
+        assert np.abs(a["rec"]).max() > 0 and np.abs(a["g_l2"]).max() > 0
+
+
+def test_fwd2_rejects_bad_value(gpu):
+    p = small_problem()
+    m = make_model(p)
+    with pytest.raises(Exception):
+        m.set_fwd_steps(3)
+    assert m.get_fwd_steps() == 1
