@@ -157,6 +157,18 @@ int otfwi_get_blocking(otfwi_model *m, int *out);
 int otfwi_set_fwd_steps(otfwi_model *m, int steps);
 int otfwi_get_fwd_steps(otfwi_model *m, int *steps);
 
+/* Adjoint time steps per launch pair: 1 (default) or 2.  With 2, the
+ * adjoint loop advances pairs of steps with two z-marching kernels: one
+ * reconstructs the forward field over both steps (boundary ring written
+ * back) and writes the Laplacian of both reconstructed states, the next
+ * advances the adjoint field over both steps and adds both gradient terms.
+ * Results are bitwise identical to 1; two extra planes per shot hold the
+ * Laplacians.  It applies to adjoint loops without cache blocking.  No
+ * reference counterpart (regression_PML_single.cu:793 launches one adjoint
+ * step at a time). */
+int otfwi_set_adj_steps(otfwi_model *m, int steps);
+int otfwi_get_adj_steps(otfwi_model *m, int *steps);
+
 /* Replay the per-step launch loops as captured hipGraphs (default 1); a
  * graph is re-captured only when shots, tiles, launch style or device buffers
  * change.  No reference counterpart (regression_PML_single.cu:661-678

@@ -83,6 +83,8 @@ struct Batch {
     void *P[2] = {}, *Ax[2] = {}, *Az[2] = {}, *Px[2] = {};
     void *LP[2] = {}, *LAx[2] = {}, *LAz[2] = {}, *LPx[2] = {};
     void *grad = nullptr, *ring = nullptr, *rec = nullptr, *adj = nullptr, *jcum = nullptr;
+    void *lap[2] = {};      // two-step adjoint: lap of the reconstructed field at t, t+1
+    int lap_cap = 0;
     double *mis_tr = nullptr, *mis_shot = nullptr, *scratch = nullptr;
     int *shot_src = nullptr;
     float *obs = nullptr;   // obs of the shots in this batch
@@ -107,9 +109,11 @@ struct otfwi_model {
     int n_src = 0, n_rx = 0, n_ring = 0;
     int *evt = nullptr;
     PointEvt *pts = nullptr;
-    unsigned *evmask = nullptr;      // two-step forward: event lane masks per column block and row
-    int *pmap = nullptr;             // two-step forward: per-cell {ring+1, rx+1, src+1, 0}
-    int fwd_steps = 1;               // time steps per forward launch (1 or 2)
+    // two-step kernels: event lane masks per column block and row, the per-cell
+    // event map {ring+1, rx+1, src+1, 0}, and the ring-cell masks with slots
+    unsigned *evmask = nullptr, *ringtab = nullptr;
+    int *pmap = nullptr;
+    int fwd_steps = 1, adj_steps = 1;   // time steps per forward / adjoint launch (1 or 2)
     bool points_dirty = true;
     int th_fwd = 32, th_adj = 16, max_batch = 0;
     int pers_fwd = 0, pers_adj = 0;   // persistent-work-group launches (0 = one work-group per tile)
@@ -166,6 +170,8 @@ void free_batch(Batch &b)
         dfree(b.LP[i]); dfree(b.LAx[i]); dfree(b.LAz[i]); dfree(b.LPx[i]);
     }
     dfree(b.grad); dfree(b.ring); dfree(b.rec); dfree(b.adj); dfree(b.jcum);
+    dfree(b.lap[0]); dfree(b.lap[1]);
+    b.lap_cap = 0;
     dfree(b.mis_tr); dfree(b.mis_shot); dfree(b.scratch); dfree(b.shot_src); dfree(b.obs);
     b.cap = 0;
     b.ring_elems = 0;
@@ -290,7 +296,14 @@ int build_points(otfwi_model *m)
     };
     for (int i = 0; i < m->n_src; i++) get(m->sx[i], m->sz[i]).src = i;
     for (int i = 0; i < m->n_rx; i++) get(m->rxx[i], m->rxz[i]).rx = i;
-    for (int i = 0; i < m->n_ring; i++) get(m->vx[i], m->vz[i]).ring = i;
+    // ring slots in raster order (z, then x): the two-step reconstruction
+    // kernel derives a cell's slot from lane masks and per-word counts
+    std::vector<int> ord(m->n_ring);
+    for (int i = 0; i < m->n_ring; i++) ord[i] = i;
+    std::sort(ord.begin(), ord.end(), [&](int u, int v) {
+        return m->vz[u] != m->vz[v] ? m->vz[u] < m->vz[v] : m->vx[u] < m->vx[v];
+    });
+    for (int r = 0; r < m->n_ring; r++) get(m->vx[ord[r]], m->vz[ord[r]]).ring = r;
     // two-step forward: lane masks of the event cells of each 256-column block
     // (owned columns plus halo) and the per-cell event map
     const int nbx = ceil_div(m->X, F2_OWN);
@@ -308,8 +321,32 @@ int build_points(otfwi_model *m)
             if (c >= 0 && c < 256) evmask[((size_t)bx * m->Z + z) * 8 + c / 32] |= 1u << (c % 32);
         }
     }
+    // ring cells per column block and row: {lane mask, slot of the word's first lane}
+    std::vector<std::vector<int>> ring_rows(m->Z);
+    for (int r = 0; r < m->n_ring; r++) ring_rows[m->vz[ord[r]]].push_back(m->vx[ord[r]]);
+    std::vector<unsigned> ringtab((size_t)nbx * m->Z * 16, 0u);
+    int before = 0;
+    for (int z = 0; z < m->Z; z++) {
+        const std::vector<int> &row = ring_rows[z];   // ascending x
+        for (int bx = 0; bx < nbx; bx++)
+            for (int w = 0; w < 8; w++) {
+                const int col0 = bx * F2_OWN - F2_HALO + 32 * w;
+                const int lo = (int)(std::lower_bound(row.begin(), row.end(), col0) - row.begin());
+                unsigned mk = 0;
+                for (int j = lo; j < (int)row.size() && row[j] < col0 + 32; j++)
+                    mk |= 1u << (row[j] - col0);
+                unsigned *e = &ringtab[(((size_t)bx * m->Z + z) * 8 + w) * 2];
+                e[0] = mk;
+                e[1] = (unsigned)(before + lo);
+            }
+        before += (int)row.size();
+    }
     dfree(m->evmask);
     dfree(m->pmap);
+    dfree(m->ringtab);
+    RCK(dalloc(&m->ringtab, ringtab.size()));
+    HIPCK(hipMemcpy(m->ringtab, ringtab.data(), ringtab.size() * sizeof(unsigned),
+                    hipMemcpyHostToDevice));
     RCK(dalloc(&m->evmask, evmask.size()));
     HIPCK(hipMemcpy(m->evmask, evmask.data(), evmask.size() * sizeof(unsigned),
                     hipMemcpyHostToDevice));
@@ -568,7 +605,7 @@ int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum)
     // step is always single, so A(t1-2) stays in slot (t1-2)&1 for the
     // adjoint's reconstruction
     const bool two = m->fwd_steps == 2 && ist == 0 && K == 1 && G == n && m->evmask &&
-                     fwd2_supported((long long)plane, (int)sizeof(R));
+                     pair2_supported((long long)plane, (int)sizeof(R));
     auto body2 = [&]() -> int {
         const int S = t1 - t0;
         int pairs = S > 1 ? (S - 1) / 2 : 0;
@@ -586,15 +623,22 @@ int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum)
             return OTFWI_OK;
         };
         for (int k = 0; k < lead; k++, t++) RCK(single(t));
+        Pair2Args<R> q{};
+        q.X = a.X; q.Z = a.Z; q.ld = a.ld; q.nshots = n; q.T = a.T;
+        q.n_rx = a.n_rx; q.n_ring = a.n_ring; q.sstride = a.sstride;
+        q.cm = a.cm; q.kx = a.kx; q.kz = a.kz; q.cA = a.cA;
+        q.shot_src = a.shot_src; q.icum = a.icum; q.rec = a.rec; q.ring = a.ring;
+        q.mask = m->evmask; q.pmap = m->pmap;
         for (int k = 0; k < pairs; k++, t += 2) {
-            // state t-1 in slot ((t-1) ^ k) & 1 ... after k pairs the parity is flipped k times
+            // each pair writes the slot it does not read: after k pairs the
+            // state sits in the other slot than one step per launch leaves it
             const int in = ((t - 1) & 1) ^ (k & 1), out = in ^ 1;
-            a.t = t;
-            a.P0 = (const R *)b.P[in]; a.P1 = (R *)b.P[out];
-            a.Ax0 = (const R *)b.Ax[in]; a.Ax1 = (R *)b.Ax[out];
-            a.Az0 = (const R *)b.Az[in]; a.Az1 = (R *)b.Az[out];
-            a.Px0 = (const R *)b.Px[in]; a.Px1 = (R *)b.Px[out];
-            HIPCK(launch_fwd2_step<R>(a, m->evmask, m->pmap, st));
+            q.t = t;
+            q.P0 = (const R *)b.P[in]; q.P1 = (R *)b.P[out];
+            q.Ax0 = (const R *)b.Ax[in]; q.Ax1 = (R *)b.Ax[out];
+            q.Az0 = (const R *)b.Az[in]; q.Az1 = (R *)b.Az[out];
+            q.Px0 = (const R *)b.Px[in]; q.Px1 = (R *)b.Px[out];
+            HIPCK(launch_pair2<R>(q, 0, st));
         }
         for (; t < t1; t++) RCK(single(t));
         return OTFWI_OK;
@@ -676,7 +720,74 @@ int adj_loop(otfwi_model *m, int n, int t0, int t1)
     const int pw = persistent_wgs(m, true);
     const int K = std::max(1, m->tb_steps[1]);
     const int G = m->tb_shots[1] > 0 ? std::min(m->tb_shots[1], n) : n;
+    // two steps per launch pair: the reconstruction kernel advances U and
+    // writes lap(U) of both steps, then the lambda kernel advances lambda and
+    // adds both gradient terms.  Leading single steps, then an even number of
+    // pairs (each writes the slots it does not read), so the slots end where
+    // one step per launch leaves them.
+    const bool two = m->adj_steps == 2 && K == 1 && G == n && m->evmask && m->ringtab &&
+                     pair2_supported((long long)plane, (int)sizeof(R));
+    if (two && (b.lap_cap < n || !b.lap[0])) {
+        dfree(b.lap[0]); dfree(b.lap[1]);
+        b.lap_cap = 0;
+        for (int i = 0; i < 2; i++) {
+            char *q = nullptr;
+            RCK(dalloc(&q, plane * b.cap * sizeof(R)));
+            b.lap[i] = q;
+        }
+        b.lap_cap = b.cap;
+        m->epoch++;
+    }
+    auto single = [&](int t) -> int {
+        const int lp = (t - 1) & 1, lc = t & 1;
+        const int up = (T - 2 + t) & 1, uc = (T - 1 + t) & 1;
+        const int ap = (T - 1 + t) & 1, ac = (T - 2 + t) & 1;
+        a.t = t;
+        a.LP0 = (const R *)b.LP[lp]; a.LP1 = (R *)b.LP[lc];
+        a.LAx0 = (const R *)b.LAx[lp]; a.LAx1 = (R *)b.LAx[lc];
+        a.LAz0 = (const R *)b.LAz[lp]; a.LAz1 = (R *)b.LAz[lc];
+        a.LPx0 = (const R *)b.LPx[lp]; a.LPx1 = (R *)b.LPx[lc];
+        a.U0 = (const R *)b.P[up]; a.U1 = (R *)b.P[uc];
+        a.UAx0 = (const R *)b.Ax[ap]; a.UAx1 = (R *)b.Ax[ac];
+        a.UAz0 = (const R *)b.Az[ap]; a.UAz1 = (R *)b.Az[ac];
+        HIPCK(launch_adj_step<R>(a, m->th_adj, pw, st));
+        return OTFWI_OK;
+    };
+    auto body2 = [&]() -> int {
+        const int S = t1 - t0;
+        const int pairs = (S / 2) & ~1;
+        const int lead = S - 2 * pairs;
+        int t = t0;
+        for (int k = 0; k < lead; k++, t++) RCK(single(t));
+        Pair2Args<R> u{}, l{};
+        u.X = l.X = m->X; u.Z = l.Z = m->Z; u.ld = l.ld = m->ld;
+        u.nshots = l.nshots = n; u.T = l.T = T;
+        u.n_rx = l.n_rx = m->n_rx; u.n_ring = l.n_ring = m->n_ring;
+        u.sstride = l.sstride = (long long)plane;
+        u.cm = l.cm = (const R *)m->cm;
+        u.kx = l.kx = (const R *)m->kx; u.kz = l.kz = (const R *)m->kz;
+        u.cA = l.cA = a.cA;
+        u.lap0 = l.lap0 = (R *)b.lap[0]; u.lap1 = l.lap1 = (R *)b.lap[1];
+        u.ring_in = (const R *)b.ring; u.mask = m->ringtab;
+        l.jcum = (const R *)b.jcum; l.grad = (R *)b.grad; l.mask = m->evmask; l.pmap = m->pmap;
+        for (int k = 0; k < pairs; k++, t += 2) {
+            const int f = k & 1;
+            const int li = ((t - 1) & 1) ^ f, ui = ((T - 2 + t) & 1) ^ f, ai = ((T - 1 + t) & 1) ^ f;
+            u.t = l.t = t;
+            u.P0 = (const R *)b.P[ui]; u.P1 = (R *)b.P[ui ^ 1];
+            u.Ax0 = (const R *)b.Ax[ai]; u.Ax1 = (R *)b.Ax[ai ^ 1];
+            u.Az0 = (const R *)b.Az[ai]; u.Az1 = (R *)b.Az[ai ^ 1];
+            HIPCK(launch_pair2<R>(u, 2, st));
+            l.P0 = (const R *)b.LP[li]; l.P1 = (R *)b.LP[li ^ 1];
+            l.Ax0 = (const R *)b.LAx[li]; l.Ax1 = (R *)b.LAx[li ^ 1];
+            l.Az0 = (const R *)b.LAz[li]; l.Az1 = (R *)b.LAz[li ^ 1];
+            l.Px0 = (const R *)b.LPx[li]; l.Px1 = (R *)b.LPx[li ^ 1];
+            HIPCK(launch_pair2<R>(l, 1, st));
+        }
+        return OTFWI_OK;
+    };
     auto body = [&]() -> int {
+        if (two) return body2();
         for (int tb = t0; tb < t1; tb += K) {
             const int te = std::min(tb + K, t1);
             for (int g0 = 0; g0 < n; g0 += G) {
@@ -708,7 +819,8 @@ int adj_loop(otfwi_model *m, int n, int t0, int t1)
         }
         return OTFWI_OK;
     };
-    const unsigned long long kb = (unsigned long long)K * 4099ULL + (unsigned long long)G;
+    const unsigned long long kb =
+        ((unsigned long long)K * 4099ULL + (unsigned long long)G) * 3ULL + (two ? 1ULL : 0ULL);
     return run_loop(m, 2, loop_key(m, n, m->th_adj, pw, t0, t1) * 1000003ULL + kb, body);
 }
 
@@ -1318,7 +1430,7 @@ int otfwi_destroy(otfwi_model *m)
     free_batch(m->b);
     dfree(m->cm); dfree(m->kx); dfree(m->kz); dfree(m->icum); dfree(m->cq32);
     dfree(m->evt); dfree(m->pts); dfree(m->tf_fwd); dfree(m->tf_adj); dfree(m->evmask);
-    dfree(m->pmap);
+    dfree(m->pmap); dfree(m->ringtab);
     dfree(m->gacc); dfree(m->gout); dfree(m->ill);
     dfree(m->xsnap); dfree(m->xseg); dfree(m->rxmap);
     for (int i = 0; i < 4; i++)
@@ -1484,6 +1596,22 @@ int otfwi_get_fwd_steps(otfwi_model *m, int *steps)
     RCK(check_model(m));
     if (!steps) return fail(OTFWI_E_ARG, "output is NULL");
     *steps = m->fwd_steps;
+    return OTFWI_OK;
+}
+
+int otfwi_set_adj_steps(otfwi_model *m, int steps)
+{
+    RCK(check_model(m));
+    if (steps != 1 && steps != 2) return fail(OTFWI_E_ARG, "steps per adjoint launch must be 1 or 2");
+    m->adj_steps = steps;
+    return OTFWI_OK;
+}
+
+int otfwi_get_adj_steps(otfwi_model *m, int *steps)
+{
+    RCK(check_model(m));
+    if (!steps) return fail(OTFWI_E_ARG, "output is NULL");
+    *steps = m->adj_steps;
     return OTFWI_OK;
 }
 

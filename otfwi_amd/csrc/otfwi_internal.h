@@ -109,18 +109,45 @@ template <typename R>
 hipError_t launch_fwd_step(const FwdArgs<R> &a, int th, int persistent_wgs, hipStream_t s);
 template <typename R>
 hipError_t launch_adj_step(const AdjArgs<R> &a, int th, int persistent_wgs, hipStream_t s);
-// Two forward time steps t, t+1 per launch (z-marching strips): reads the
-// state at t-1 from P0/Ax0/Az0/Px0, writes the state at t+1 to
-// P1/Ax1/Az1/Px1 (a different slot), samples traces and the ring at t and
-// t+1.  A work-group owns F2_OWN columns starting at bx * F2_OWN and computes
-// F2_HALO more on each side.  evmask[(bx * Z + z) * 8 + w] bit b marks an
-// event cell at column bx * F2_OWN - F2_HALO + 32 w + b of row z; pmap is the
+// Two time steps t, t+1 per launch (z-marching strips, prop.hip): reads the
+// state before the pair from P0/Ax0/Az0/Px0 and writes the state after it to
+// P1/Ax1/Az1/Px1 (a different slot).  A work-group owns F2_OWN columns
+// starting at bx * F2_OWN and computes F2_HALO more on each side.
+//   mode 0 forward field (fwd_step_kernel x2): samples rec / ring at t, t+1;
+//   mode 1 adjoint field lambda (adj_step_kernel's lambda x2): injects jcum,
+//          grad += lambda_P(t) lap0 + lambda_P(t+1) lap1;
+//   mode 2 reconstructed field U (adj_step_kernel's U x2): ring values
+//          written back at ring times T-1-t and T-2-t, writes lap0 / lap1 =
+//          (lap9_x + lap9_z)(U(t)) / 5040 and the same of U(t+1); no psi.
+// mask (modes 0, 1): mask[(bx * Z + z) * 8 + w] bit b marks an event cell at
+// column bx * F2_OWN - F2_HALO + 32 w + b of row z, and pmap is the
 // padded-plane map of int4 {ring + 1, rx + 1, src + 1, 0} (0: none).
+// mask (mode 2): pairs {lane mask of the ring cells, ring slot of the
+// word's first lane} at mask[((bx * Z + z) * 8 + w) * 2]; ring slots are
+// numbered in raster order (z, then x).
 constexpr int F2_OWN = 224, F2_HALO = 16;
-bool fwd2_supported(long long sstride, int prec);
 template <typename R>
-hipError_t launch_fwd2_step(const FwdArgs<R> &a, const unsigned *evmask, const int *pmap,
-                            hipStream_t s);
+struct Pair2Args {
+    int X, Z, ld, nshots, t, T, n_rx, n_ring;
+    long long sstride;
+    const R *P0, *Ax0, *Az0, *Px0;
+    R *P1, *Ax1, *Az1, *Px1;
+    const R *cm, *kx, *kz;
+    R cA;
+    const int *shot_src;          // mode 0
+    const R *icum;                // mode 0
+    R *rec;                       // mode 0 (may be null)
+    R *ring;                      // mode 0: [nshots][T][n_ring] written (may be null)
+    const R *ring_in;             // mode 2: the same, read
+    const R *jcum;                // mode 1
+    R *grad;                      // mode 1
+    R *lap0, *lap1;               // mode 1 reads, mode 2 writes (padded planes per shot)
+    const unsigned *mask;
+    const int *pmap;
+};
+bool pair2_supported(long long sstride, int prec);
+template <typename R>
+hipError_t launch_pair2(const Pair2Args<R> &a, int mode, hipStream_t s);
 // Resident work-groups per CU of a step kernel (0 if the variant does not exist).
 int step_occupancy(bool adj, int prec, int th, bool persistent);
 

@@ -958,32 +958,46 @@ hipError_t launch_adj_step<double>(const AdjArgs<double> &a, int th, int pw, hip
 }
 
 // ---------------------------------------------------------------------------
-// forward, two time steps per launch: z-marching strips
+// two time steps per launch: z-marching strips
 // ---------------------------------------------------------------------------
 // A work-group of 256 threads owns columns [xs, xs + F2_OWN) of one shot
 // (thread tid holds column xs - F2_HALO + tid) and output rows
 // [zc0, zc0 + F2_CH), and walks down the rows.  Iteration j takes row
-// i = zc0 - 14 + j of the state at t-1 and computes, each stage on rows
-// already complete below it,
+// i = zc0 - 15 + j of the state before the pair and computes, each stage on
+// rows already complete below it,
 //   P(t) on row i - 3,  A(t) on row i - 7,  P(t+1) on row i - 10 (written
 //   out),  A(t+1) on row i - 14 (written out).
-// z-stencils read per-thread register rings of the last 8 rows; x-stencils
-// and the delayed A values read LDS rings of 8 rows written in earlier
+// z-stencils read per-thread register rings of the last 8 rows (a stage
+// that needs 9 rows reads the slot about to be overwritten); x-stencils and
+// the delayed A values read LDS rings of 8 rows written in earlier
 // iterations, so one LDS barrier per iteration orders everything.  The
-// 16-column halo shrinks by at most 4 columns per stage and ends 2 columns
+// 16-column halo shrinks by at most 4 columns per stage and ends 1 column
 // outside the owned ones.  Global loads run F2_PF rows ahead in registers.
-// Every cell is updated with the arithmetic of fwd_step_kernel, so results
-// are bitwise equal; state t-1 is read from one slot and state t+1 written to
-// the other.
+// Every cell is updated with the arithmetic of the one-step kernels, so
+// results are bitwise equal; the state before the pair is read from one slot
+// and the state after it written to the other.
 //
+// Three field kinds share the scheme (MODE):
+//   F2_FWD  forward field: damped operator, source injection (icum),
+//           receiver and boundary-ring sampling (fwd_step_kernel);
+//   F2_LAM  adjoint field lambda: damped operator, injection of the running
+//           sum of the adjoint source at the receivers (jcum), and
+//           grad += lambda_P * lap of the reconstructed field (read from
+//           the lap planes written by F2_REC) (adj_step_kernel's lambda);
+//   F2_REC  reconstructed forward field U: undamped reverse operator,
+//           boundary-ring values written back, and lap(U) (x plus z
+//           9-point second differences / 5040) written for both steps
+//           (adj_step_kernel's U).
 // Memory is accessed through buffer resources: an offset at or beyond the
 // buffer's size reads 0 and drops a store without touching memory, so rows
 // and columns outside the plane, undamped psi cells, cells without an event
 // and lanes that do not own a cell need no branch.
+constexpr int F2_FWD = 0, F2_LAM = 1, F2_REC = 2;
 constexpr int F2_CH = 256, F2_LP = 256 + 16, F2_PF = 2;
-// LDS row tables (kz, event lane masks) cover rows [zc0 - F2_RB, zc0 - F2_RB + F2_NR):
-// iterations touch rows zc0 - 28 .. zc0 + F2_CH + 21 + F2_PF
+// LDS row tables (kz, lane masks) cover rows [zc0 - F2_RB, zc0 - F2_RB + F2_NR):
+// iterations touch rows zc0 - 29 .. zc0 + F2_CH + 20 + F2_PF
 constexpr int F2_RB = 32, F2_NR = F2_CH + 64;
+constexpr int F2_LEAD = 15;   // rows loaded above a chunk
 constexpr unsigned F2_OOB = 0xFFFFFFF0u;
 static_assert(F2_HALO + F2_OWN + F2_HALO == 256, "two-step block is 256 columns");
 
@@ -1021,74 +1035,39 @@ __device__ __forceinline__ R stag_lds(const R *row, int c0)
                    row[c0 + 6], row[c0 + 7]);
 }
 
-// z-stencil over a register ring of 8 rows whose newest entry is at PH
-template <typename R, int PH>
-__device__ __forceinline__ R stag_ring(const R (&r)[8])
-{
-    return stag<R>(r[(PH + 1) & 7], r[(PH + 2) & 7], r[(PH + 3) & 7], r[(PH + 4) & 7],
-                   r[(PH + 5) & 7], r[(PH + 6) & 7], r[(PH + 7) & 7], r[PH]);
-}
-
 // Event of one cell, each index + 1 (0: none), as the per-cell event map holds it.
 struct F2Evt {
     int ring1, rx1, src1;
 };
 
-// P update of one cell: p_quad's arithmetic for one lane.  rec_here = this
-// block samples the cell.
-template <typename R>
-__device__ __forceinline__ R f2_p(const FwdArgs<R> &a, const F2Evt &e, int s, int ss, int x, int z,
-                                  int t, bool rec_here, R pp, R psi, R mm, R kx, R kz, R dAx,
-                                  R dAz, R &psi_out)
-{
-    R nx = fma(mm, dAx, mul_rn(psi, kx));
-    R nz = fma(mm, dAz, mul_rn(pp - psi, kz));
-    if ((e.ring1 | e.rx1 | e.src1) != 0) {
-        const int src = e.src1 - 1;
-        if (src >= 0) {
-            if (ss < 0 || ss == src) {
-                const R h = a.icum[(size_t)src * a.T + t] * R(0.5);
-                nx = add_rn(nx, h);
-                nz = add_rn(nz, h);
-            }
-        }
-        if (rec_here) {
-            const R pv = add_rn(nx, nz);
-            if (e.rx1 > 0 && a.rec) a.rec[((size_t)s * a.n_rx + (e.rx1 - 1)) * a.T + t] = pv;
-            if (e.ring1 > 0 && a.ring)
-                a.ring[((size_t)s * a.T + t) * a.n_ring + (e.ring1 - 1)] = pv;
-        }
-    }
-    R p = add_rn(nx, nz);
-    if (!in_box(x, z, a.X, a.Z)) { p = R(0); nx = R(0); }
-    psi_out = (kx != R(1) || kz != R(1)) ? nx : R(0);
-    return p;
-}
-
 // What iteration i needs from global memory, loaded F2_PF iterations ahead.
-template <typename R>
+template <typename R, int MODE>
 struct F2Row {
-    R ax, az, p, ps, m;   // state at t-1 and m on row i
+    R ax, az, p, ps, m;   // state before the pair and m on row i (ps: FWD, LAM)
     R m3;                 // m on row i - 10
-    F2Evt e1, e3;         // events on rows i - 3 and i - 10
+    F2Evt e1, e3;         // FWD, LAM: events on rows i - 3 and i - 10
+    R g, l0, l1;          // LAM: grad and lap(U(t)) on row i - 7, lap(U(t+1)) on row i - 14
+    R r1, r3;             // REC: ring values on rows i - 3 (step t) and i - 10 (step t+1)
+    bool h1, h3;          // REC: the cell is a ring cell
 };
 
-template <typename R>
-struct Fwd2Regs {
-    R az0[8];                 // Az(t-1) by row, newest = row i
-    R p0[4], s0[4], m0[4];    // P(t-1), psi(t-1), m: rows i-3 .. i
-    R p1[8], s1[8];           // P(t), psi(t): rows i-10 .. i-3
+template <typename R, int MODE>
+struct Pair2Regs {
+    R az0[8];                 // Az before the pair by row, newest = row i
+    R p0[4], s0[4], m0[4];    // P, psi, m before the pair: rows i-3 .. i
+    R p1[8], s1[8];           // P(t), psi(t): rows i-11 .. i-4 (before this iteration's write)
     R az1[8];                 // Az(t): rows i-14 .. i-7
-    R p2[8];                  // P(t+1): rows i-17 .. i-10
-    F2Row<R> pf[F2_PF];
+    R p2[8];                  // P(t+1): rows i-18 .. i-11 (before this iteration's write)
+    R gq[8];                  // LAM: grad after step t, rows i-14 .. i-7
+    F2Row<R, MODE> pf[F2_PF];
 };
 
 struct F2Ctx {
-    int tid, x, s, ss, zc0, zc1, lc, X, Z, ld;   // ss: active source of the shot (-1: all)
+    int tid, x, s, ss, zc0, zc1, lc, X, Z, ld, t, T, nring;   // ss: active source (-1: all)
     unsigned lo;   // byte offset of this lane's column in a padded row (element size R)
     unsigned lo4;  // the same for 4-byte elements
-    bool colv, own;
-    Rsrc rAx0, rAz0, rP0, rPx0, rcm, rpm, rP1, rPx1, rAx1, rAz1;
+    bool colv, own, xin;   // xin: 4 <= x < X - 4
+    Rsrc rAx0, rAz0, rP0, rPx0, rcm, rpm, rP1, rPx1, rAx1, rAz1, rg, rl0, rl1, rring;
 };
 
 template <typename R>
@@ -1097,6 +1076,9 @@ __device__ __forceinline__ unsigned f2_off(const F2Ctx &c, int z)
     return (unsigned)((z + ZOFF) * c.ld) * (unsigned)sizeof(R) + c.lo;
 }
 
+__device__ __forceinline__ bool f2_zin(const F2Ctx &c, int z) { return z >= BOX && z < c.Z - BOX; }
+
+// event of this lane's cell on row z (FWD, LAM): smask holds the lane masks
 __device__ __forceinline__ F2Evt f2_evt(const F2Ctx &c, const unsigned *smask, int z)
 {
     const unsigned w = smask[(z - c.zc0 + F2_RB) * 8 + (c.tid >> 5)];
@@ -1106,9 +1088,25 @@ __device__ __forceinline__ F2Evt f2_evt(const F2Ctx &c, const unsigned *smask, i
     return F2Evt{(int)v[0], (int)v[1], (int)v[2]};
 }
 
+// ring value of this lane's cell on row z at ring time trev (REC): smask
+// holds {lane mask, ring slot of the word's first lane} pairs; ring slots
+// are numbered in raster order
 template <typename R>
+__device__ __forceinline__ R f2_ring(const F2Ctx &c, const unsigned *smask, int z, int trev,
+                                     bool &hit)
+{
+    const int w = c.tid >> 5, b = c.tid & 31;
+    const unsigned *e = smask + ((z - c.zc0 + F2_RB) * 8 + w) * 2;
+    const unsigned mk = e[0];
+    hit = (mk >> b) & 1u;
+    const int k = (int)e[1] + __builtin_popcount(mk & ((1u << b) - 1u));
+    const unsigned off = hit ? (unsigned)(trev * c.nring + k) * (unsigned)sizeof(R) : F2_OOB;
+    return bload<R>(c.rring, off);
+}
+
+template <typename R, int MODE>
 __device__ __forceinline__ void f2_fetch(const F2Ctx &c, const unsigned *smask, const R *skz, R kx,
-                                         int i, F2Row<R> &f)
+                                         int i, F2Row<R, MODE> &f)
 {
     const bool rv = c.colv && i >= -ZOFF && i < c.Z + 4;
     const unsigned o = rv ? f2_off<R>(c, i) : F2_OOB;
@@ -1116,174 +1114,325 @@ __device__ __forceinline__ void f2_fetch(const F2Ctx &c, const unsigned *smask, 
     f.az = bload<R>(c.rAz0, o);
     f.p = bload<R>(c.rP0, o);
     f.m = bload<R>(c.rcm, o);
-    const bool damped = kx != R(1) || skz[i - c.zc0 + F2_RB] != R(1);
-    f.ps = bload<R>(c.rPx0, damped ? o : F2_OOB);
     const int i3 = i - 10;
     const bool rv3 = c.colv && i3 >= -ZOFF && i3 < c.Z + 4;
     f.m3 = bload<R>(c.rcm, rv3 ? f2_off<R>(c, i3) : F2_OOB);
-    f.e1 = f2_evt(c, smask, i - 3);
-    f.e3 = f2_evt(c, smask, i3);
+    f.ps = f.g = f.l0 = f.l1 = f.r1 = f.r3 = R(0);
+    f.h1 = f.h3 = false;
+    f.e1 = f.e3 = F2Evt{0, 0, 0};
+    if constexpr (MODE != F2_REC) {
+        const bool damped = kx != R(1) || skz[i - c.zc0 + F2_RB] != R(1);
+        f.ps = bload<R>(c.rPx0, damped ? o : F2_OOB);
+        f.e1 = f2_evt(c, smask, i - 3);
+        f.e3 = f2_evt(c, smask, i3);
+    }
+    if constexpr (MODE == F2_LAM) {
+        const int z2 = i - 7, z4 = i - 14;
+        const bool o2 = c.own && z2 >= c.zc0 && z2 < c.zc1;
+        const bool o4 = c.own && z4 >= c.zc0 && z4 < c.zc1;
+        f.g = bload<R>(c.rg, o2 ? f2_off<R>(c, z2) : F2_OOB);
+        f.l0 = bload<R>(c.rl0, o2 ? f2_off<R>(c, z2) : F2_OOB);
+        f.l1 = bload<R>(c.rl1, o4 ? f2_off<R>(c, z4) : F2_OOB);
+    }
+    if constexpr (MODE == F2_REC) {
+        f.r1 = f2_ring<R>(c, smask, i - 3, c.T - 1 - c.t, f.h1);
+        f.r3 = f2_ring<R>(c, smask, i3, c.T - 2 - c.t, f.h3);
+    }
 }
 
-template <typename R, int PH>
-__device__ __forceinline__ void f2_iter(const FwdArgs<R> &a, const F2Ctx &c, Fwd2Regs<R> &g,
-                                        R (*sAx0)[F2_LP], R (*sP1)[F2_LP], R (*sAx1)[F2_LP],
-                                        R (*sP2)[F2_LP], const unsigned *smask, const R *skz,
-                                        int i, R kx)
+// P-kind update of one cell at step tt (p_quad / lu_quad arithmetic for one
+// lane).  rec_here = this block samples the cell (FWD).
+template <typename R, int MODE>
+__device__ __forceinline__ R f2_p(const Pair2Args<R> &a, const F2Ctx &c, const F2Evt &e, R rv,
+                                  bool rh, bool inb, int tt, bool rec_here, R pp, R psi, R mm, R kx,
+                                  R kz, R dAx, R dAz, R &psi_out)
+{
+    if constexpr (MODE == F2_REC) {
+        R up = fma(-mm, dAx + dAz, pp);
+        if (rh) up = rv;
+        psi_out = R(0);
+        return inb ? up : R(0);
+    } else {
+        R nx = fma(mm, dAx, mul_rn(psi, kx));
+        R nz = fma(mm, dAz, mul_rn(pp - psi, kz));
+        if ((e.ring1 | e.rx1 | e.src1) != 0) {
+            if constexpr (MODE == F2_FWD) {
+                const int src = e.src1 - 1;
+                if (src >= 0 && (c.ss < 0 || c.ss == src)) {
+                    const R h = a.icum[(size_t)src * a.T + tt] * R(0.5);
+                    nx = add_rn(nx, h);
+                    nz = add_rn(nz, h);
+                }
+                if (rec_here) {
+                    const R pv = add_rn(nx, nz);
+                    if (e.rx1 > 0 && a.rec)
+                        a.rec[((size_t)c.s * a.n_rx + (e.rx1 - 1)) * a.T + tt] = pv;
+                    if (e.ring1 > 0 && a.ring)
+                        a.ring[((size_t)c.s * a.T + tt) * a.n_ring + (e.ring1 - 1)] = pv;
+                }
+            } else {
+                if (e.rx1 > 0) {
+                    const R h = a.jcum[((size_t)c.s * a.n_rx + (e.rx1 - 1)) * a.T + tt] * R(0.5);
+                    nx = add_rn(nx, h);
+                    nz = add_rn(nz, h);
+                }
+            }
+        }
+        R p = add_rn(nx, nz);
+        if (!inb) { p = R(0); nx = R(0); }
+        psi_out = (kx != R(1) || kz != R(1)) ? nx : R(0);
+        return p;
+    }
+}
+
+// A-kind update of one cell
+template <typename R, int MODE>
+__device__ __forceinline__ void f2_a(R cA, R dPx, R dPz, R ax, R az, R kx, R kz, bool inb, R &nax,
+                                     R &naz)
+{
+    if constexpr (MODE == F2_REC) {
+        nax = fma(-cA, dPx, ax);
+        naz = fma(-cA, dPz, az);
+    } else {
+        nax = fma(cA, dPx, mul_rn(ax, kx));
+        naz = fma(cA, dPz, mul_rn(az, kz));
+    }
+    if (!inb) { nax = R(0); naz = R(0); }
+}
+
+// (lap9 along x + lap9 along z) / 5040 as adj_step_kernel forms it: row =
+// the LDS row (9 taps around lc), w = the 9 rows (z-4 .. z+4) of this column
+template <typename R>
+__device__ __forceinline__ R f2_lap(const R *row, int lc, const R (&w)[9])
+{
+    constexpr R inv5040 = R(1) / R(5040);
+    const R lx = lap9<R>(row[lc - 4], row[lc - 3], row[lc - 2], row[lc - 1], row[lc], row[lc + 1],
+                         row[lc + 2], row[lc + 3], row[lc + 4]);
+    const R lz = lap9<R>(w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8]);
+    return (lx + lz) * inv5040;
+}
+
+template <typename R, int MODE, int PH>
+__device__ __forceinline__ void f2_iter(const Pair2Args<R> &a, const F2Ctx &c,
+                                        Pair2Regs<R, MODE> &g, R (*sAx0)[F2_LP],
+                                        R (*sP1)[F2_LP], R (*sAx1)[F2_LP], R (*sP2)[F2_LP],
+                                        const unsigned *smask, const R *skz, int i, R kx)
 {
     constexpr int Q1 = (PH + 1) & 7, Q4 = (PH + 4) & 7, Q5 = (PH + 5) & 7;
     constexpr int F = PH % F2_PF;
     constexpr int H = PH & 3, H1 = (PH + 1) & 3;   // 4-row rings: newest, 3 rows back
     const int lc = c.lc;
-    auto kz_at = [&](int z) -> R { return skz[z - c.zc0 + F2_RB]; };
-    F2Row<R> &f = g.pf[F];
-    // ---- 1. take row i of the state at t-1 ----
-    g.az0[PH] = f.az; g.p0[H] = f.p; g.s0[H] = f.ps; g.m0[H] = f.m;
+    auto kz_at = [&](int z) -> R {
+        if constexpr (MODE == F2_REC) return R(1);
+        else return skz[z - c.zc0 + F2_RB];
+    };
+    F2Row<R, MODE> &f = g.pf[F];
+    // ---- 1. take row i of the state before the pair ----
+    g.az0[PH] = f.az; g.p0[H] = f.p; g.m0[H] = f.m;
+    if constexpr (MODE != F2_REC) g.s0[H] = f.ps;
     sAx0[PH][lc] = f.ax;
     // ---- 2. P(t) on row z1 = i - 3 ----
+    const int z1 = i - 3;
+    R pn1, sn1;
     {
-        const int z1 = i - 3;
         const R dAx = stag_lds<R>(sAx0[Q5], lc - 4);
-        const R dAz = stag_ring<R, PH>(g.az0);
-        R ps;
+        const R dAz = stag<R>(g.az0[(PH + 1) & 7], g.az0[(PH + 2) & 7], g.az0[(PH + 3) & 7],
+                              g.az0[(PH + 4) & 7], g.az0[(PH + 5) & 7], g.az0[(PH + 6) & 7],
+                              g.az0[(PH + 7) & 7], g.az0[PH]);
         const bool rec = c.own && z1 >= c.zc0 && z1 < c.zc1;
-        const R p = f2_p<R>(a, f.e1, c.s, c.ss, c.x, z1, a.t, rec, g.p0[H1], g.s0[H1], g.m0[H1], kx,
-                            kz_at(z1), dAx, dAz, ps);
-        g.p1[PH] = p; g.s1[PH] = ps;
-        sP1[PH][lc] = p;
+        pn1 = f2_p<R, MODE>(a, c, f.e1, f.r1, f.h1, c.xin && f2_zin(c, z1), c.t, rec, g.p0[H1],
+                            g.s0[H1], g.m0[H1], kx, kz_at(z1), dAx, dAz, sn1);
+        sP1[PH][lc] = pn1;
     }
     // ---- 3. A(t) on row z2 = i - 7 ----
     {
         const int z2 = i - 7;
+        // P(t) rows z2-4 .. z2+4: p1[PH] .. p1[PH+7] hold rows i-11 .. i-4
         const R dPx = stag_lds<R>(sP1[Q4], lc - 3);
-        const R dPz = stag_ring<R, PH>(g.p1);
-        R nax = fma(a.cA, dPx, mul_rn(sAx0[Q1][lc], kx));
-        R naz = fma(a.cA, dPz, mul_rn(g.az0[Q1], kz_at(z2)));
-        if (!in_box(c.x, z2, c.X, c.Z)) { nax = R(0); naz = R(0); }
+        const R dPz = stag<R>(g.p1[(PH + 1) & 7], g.p1[(PH + 2) & 7], g.p1[(PH + 3) & 7],
+                              g.p1[(PH + 4) & 7], g.p1[(PH + 5) & 7], g.p1[(PH + 6) & 7],
+                              g.p1[(PH + 7) & 7], pn1);
+        R nax, naz;
+        f2_a<R, MODE>(a.cA, dPx, dPz, sAx0[Q1][lc], g.az0[Q1], kx, kz_at(z2),
+                      c.xin && f2_zin(c, z2), nax, naz);
         g.az1[PH] = naz;
         sAx1[PH][lc] = nax;
+        if constexpr (MODE == F2_LAM) g.gq[PH] = fma(g.p1[Q4], f.l0, f.g);
+        if constexpr (MODE == F2_REC) {
+            const R w[9] = {g.p1[PH], g.p1[(PH + 1) & 7], g.p1[(PH + 2) & 7], g.p1[(PH + 3) & 7],
+                            g.p1[(PH + 4) & 7], g.p1[(PH + 5) & 7], g.p1[(PH + 6) & 7],
+                            g.p1[(PH + 7) & 7], pn1};
+            const R lp = f2_lap<R>(sP1[Q4], lc, w);
+            const bool o2 = c.own && z2 >= c.zc0 && z2 < c.zc1;
+            bstore<R>(c.rl0, o2 ? f2_off<R>(c, z2) : F2_OOB, lp);
+        }
+        g.p1[PH] = pn1;
+        if constexpr (MODE != F2_REC) g.s1[PH] = sn1;
     }
     // ---- 4. P(t+1) on row z3 = i - 10, written out ----
+    const int z3 = i - 10;
+    R pn2;
     {
-        const int z3 = i - 10;
         const R dAx = stag_lds<R>(sAx1[Q5], lc - 4);
-        const R dAz = stag_ring<R, PH>(g.az1);
+        const R dAz = stag<R>(g.az1[(PH + 1) & 7], g.az1[(PH + 2) & 7], g.az1[(PH + 3) & 7],
+                              g.az1[(PH + 4) & 7], g.az1[(PH + 5) & 7], g.az1[(PH + 6) & 7],
+                              g.az1[(PH + 7) & 7], g.az1[PH]);
         const R kz3 = kz_at(z3);
-        R ps;
+        R sn2;
         const bool mine = c.own && z3 >= c.zc0 && z3 < c.zc1;
-        const R p = f2_p<R>(a, f.e3, c.s, c.ss, c.x, z3, a.t + 1, mine, g.p1[Q1], g.s1[Q1], f.m3, kx,
-                            kz3, dAx, dAz, ps);
-        g.p2[PH] = p;
-        sP2[PH][lc] = p;
+        pn2 = f2_p<R, MODE>(a, c, f.e3, f.r3, f.h3, c.xin && f2_zin(c, z3), c.t + 1, mine,
+                            g.p1[Q1], g.s1[Q1], f.m3, kx, kz3, dAx, dAz, sn2);
+        sP2[PH][lc] = pn2;
         const unsigned o = mine ? f2_off<R>(c, z3) : F2_OOB;
-        bstore<R>(c.rP1, o, p);
-        bstore<R>(c.rPx1, (kx != R(1) || kz3 != R(1)) ? o : F2_OOB, ps);
+        bstore<R>(c.rP1, o, pn2);
+        if constexpr (MODE != F2_REC)
+            bstore<R>(c.rPx1, (kx != R(1) || kz3 != R(1)) ? o : F2_OOB, sn2);
     }
     // ---- 5. A(t+1) on row z4 = i - 14, written out ----
     {
         const int z4 = i - 14;
+        // P(t+1) rows z4-4 .. z4+4: p2[PH] .. p2[PH+7] hold rows i-18 .. i-11
         const R dPx = stag_lds<R>(sP2[Q4], lc - 3);
-        const R dPz = stag_ring<R, PH>(g.p2);
-        R nax = fma(a.cA, dPx, mul_rn(sAx1[Q1][lc], kx));
-        R naz = fma(a.cA, dPz, mul_rn(g.az1[Q1], kz_at(z4)));
-        if (!in_box(c.x, z4, c.X, c.Z)) { nax = R(0); naz = R(0); }
-        const unsigned o = (c.own && z4 >= c.zc0 && z4 < c.zc1) ? f2_off<R>(c, z4) : F2_OOB;
+        const R dPz = stag<R>(g.p2[(PH + 1) & 7], g.p2[(PH + 2) & 7], g.p2[(PH + 3) & 7],
+                              g.p2[(PH + 4) & 7], g.p2[(PH + 5) & 7], g.p2[(PH + 6) & 7],
+                              g.p2[(PH + 7) & 7], pn2);
+        R nax, naz;
+        f2_a<R, MODE>(a.cA, dPx, dPz, sAx1[Q1][lc], g.az1[Q1], kx, kz_at(z4),
+                      c.xin && f2_zin(c, z4), nax, naz);
+        const bool o4 = c.own && z4 >= c.zc0 && z4 < c.zc1;
+        const unsigned o = o4 ? f2_off<R>(c, z4) : F2_OOB;
         bstore<R>(c.rAx1, o, nax);
         bstore<R>(c.rAz1, o, naz);
+        if constexpr (MODE == F2_LAM) bstore<R>(c.rg, o, fma(g.p2[Q4], f.l1, g.gq[Q1]));
+        if constexpr (MODE == F2_REC) {
+            const R w[9] = {g.p2[PH], g.p2[(PH + 1) & 7], g.p2[(PH + 2) & 7], g.p2[(PH + 3) & 7],
+                            g.p2[(PH + 4) & 7], g.p2[(PH + 5) & 7], g.p2[(PH + 6) & 7],
+                            g.p2[(PH + 7) & 7], pn2};
+            bstore<R>(c.rl1, o, f2_lap<R>(sP2[Q4], lc, w));
+        }
+        g.p2[PH] = pn2;
     }
     // ---- 6. loads for row i + F2_PF into the slot just consumed ----
-    f2_fetch<R>(c, smask, skz, kx, i + F2_PF, f);
+    f2_fetch<R, MODE>(c, smask, skz, kx, i + F2_PF, f);
     lds_barrier();
 }
 
-// evmask: [ceil(X / F2_OWN)][Z][8] lane masks of the event cells of each
-// block's 256 columns; pmap: per-cell event map of the padded plane, int4
-// {ring + 1, rx + 1, src + 1, 0} (0: none).
-template <typename R>
-__global__ void __launch_bounds__(256) fwd2_kernel(const FwdArgs<R> a, const unsigned *evmask,
-                                                   const int *pmap)
+// mask: FWD, LAM: [ceil(X / F2_OWN)][Z][8] lane masks of the event cells of
+// each block's 256 columns, with pmap the per-cell event map of the padded
+// plane, int4 {ring + 1, rx + 1, src + 1, 0} (0: none); REC: [..][Z][8]
+// {lane mask of the ring cells, ring slot of the word's first lane}.
+// Waves per SIMD the register allocation targets: 3 work-groups of 4 waves
+// per CU (the LDS of FWD and LAM allows 3; REC's lane tables allow 2).
+template <int MODE> constexpr int f2_waves() { return MODE == F2_REC ? 2 : 3; }
+
+template <typename R, int MODE>
+__global__ void __launch_bounds__(256, f2_waves<MODE>()) pair2_kernel(const Pair2Args<R> a)
 {
+    constexpr int MW = MODE == F2_REC ? 16 : 8;   // mask words per row
     __shared__ R sAx0[8][F2_LP], sP1[8][F2_LP], sAx1[8][F2_LP], sP2[8][F2_LP];
-    __shared__ R skz[F2_NR];
-    __shared__ unsigned smask[F2_NR * 8];
+    __shared__ R skz[MODE == F2_REC ? 1 : F2_NR];
+    __shared__ unsigned smask[F2_NR * MW];
     F2Ctx c;
     c.tid = threadIdx.x;
-    c.X = a.X; c.Z = a.Z; c.ld = a.ld;
+    c.X = a.X; c.Z = a.Z; c.ld = a.ld; c.t = a.t; c.T = a.T; c.nring = a.n_ring;
     const int xs = blockIdx.x * F2_OWN;
     c.x = xs - F2_HALO + c.tid;
     c.zc0 = blockIdx.y * F2_CH;
     c.zc1 = min(c.zc0 + F2_CH, a.Z);
     c.s = blockIdx.z;
-    c.ss = a.shot_src[c.s];
+    c.ss = MODE == F2_FWD ? a.shot_src[c.s] : -1;
     c.lc = c.tid + 8;
     // columns beyond X + 12 never reach an owned cell: not loaded
     c.colv = c.x < a.X + 12;
     c.own = c.tid >= F2_HALO && c.tid < F2_HALO + F2_OWN && c.x < a.X;
+    c.xin = c.x >= BOX && c.x < a.X - BOX;
     c.lo = (unsigned)(min(c.x, a.X + 11) + XOFF) * (unsigned)sizeof(R);
     c.lo4 = (unsigned)(min(c.x, a.X + 11) + XOFF) * 4u;
     const size_t sb = (size_t)c.s * (size_t)a.sstride;
     const unsigned pb = (unsigned)((size_t)a.sstride * sizeof(R));
     c.rAx0 = f2_rsrc(a.Ax0 + sb, pb); c.rAz0 = f2_rsrc(a.Az0 + sb, pb);
-    c.rP0 = f2_rsrc(a.P0 + sb, pb);   c.rPx0 = f2_rsrc(a.Px0 + sb, pb);
-    c.rP1 = f2_rsrc(a.P1 + sb, pb);   c.rPx1 = f2_rsrc(a.Px1 + sb, pb);
+    c.rP0 = f2_rsrc(a.P0 + sb, pb);
+    c.rP1 = f2_rsrc(a.P1 + sb, pb);
     c.rAx1 = f2_rsrc(a.Ax1 + sb, pb); c.rAz1 = f2_rsrc(a.Az1 + sb, pb);
     c.rcm = f2_rsrc(a.cm, pb);
-    c.rpm = f2_rsrc(pmap, (unsigned)((size_t)a.sstride * 16));
-    const R kx = c.colv ? a.kx[XOFF + c.x] : R(1);
-    for (int r = c.tid; r < F2_NR; r += 256) {
-        const int z = c.zc0 - F2_RB + r;
-        skz[r] = (z >= -ZOFF && z < a.Z + ZOFF) ? a.kz[ZOFF + z] : R(1);
+    if constexpr (MODE != F2_REC) {
+        c.rPx0 = f2_rsrc(a.Px0 + sb, pb);
+        c.rPx1 = f2_rsrc(a.Px1 + sb, pb);
+        c.rpm = f2_rsrc(a.pmap, (unsigned)((size_t)a.sstride * 16));
     }
-    for (int k = c.tid; k < F2_NR * 8; k += 256) {
-        const int z = c.zc0 - F2_RB + (k >> 3);
-        smask[k] = (z >= 0 && z < a.Z) ? evmask[((size_t)blockIdx.x * a.Z + z) * 8 + (k & 7)] : 0u;
+    if constexpr (MODE == F2_LAM) {
+        c.rg = f2_rsrc(a.grad + sb, pb);
+        c.rl0 = f2_rsrc(a.lap0 + sb, pb);
+        c.rl1 = f2_rsrc(a.lap1 + sb, pb);
+    }
+    if constexpr (MODE == F2_REC) {
+        c.rl0 = f2_rsrc(a.lap0 + sb, pb);
+        c.rl1 = f2_rsrc(a.lap1 + sb, pb);
+        // ring samples of this shot, time-major
+        c.rring = f2_rsrc(a.ring_in + (size_t)c.s * a.T * a.n_ring,
+                          (unsigned)((size_t)a.T * a.n_ring * sizeof(R)));
+    }
+    const R kx = (MODE != F2_REC && c.colv) ? a.kx[XOFF + c.x] : R(1);
+    if constexpr (MODE != F2_REC)
+        for (int r = c.tid; r < F2_NR; r += 256) {
+            const int z = c.zc0 - F2_RB + r;
+            skz[r] = (z >= -ZOFF && z < a.Z + ZOFF) ? a.kz[ZOFF + z] : R(1);
+        }
+    for (int k = c.tid; k < F2_NR * MW; k += 256) {
+        const int z = c.zc0 - F2_RB + k / MW;
+        smask[k] = (z >= 0 && z < a.Z) ? a.mask[((size_t)blockIdx.x * a.Z + z) * MW + k % MW] : 0u;
     }
     for (int r = 0; r < 8; r++)
         for (int k = c.tid; k < F2_LP; k += 256) {
             sAx0[r][k] = R(0); sP1[r][k] = R(0); sAx1[r][k] = R(0); sP2[r][k] = R(0);
         }
-    Fwd2Regs<R> g;
+    Pair2Regs<R, MODE> g;
 #pragma unroll
-    for (int k = 0; k < 8; k++) g.az0[k] = g.p1[k] = g.s1[k] = g.az1[k] = g.p2[k] = R(0);
+    for (int k = 0; k < 8; k++)
+        g.az0[k] = g.p1[k] = g.s1[k] = g.az1[k] = g.p2[k] = g.gq[k] = R(0);
 #pragma unroll
     for (int k = 0; k < 4; k++) g.p0[k] = g.s0[k] = g.m0[k] = R(0);
     __syncthreads();
-    const int i0 = c.zc0 - 14;
+    const int i0 = c.zc0 - F2_LEAD;
 #pragma unroll
-    for (int k = 0; k < F2_PF; k++) f2_fetch<R>(c, smask, skz, kx, i0 + k, g.pf[k]);
-    const int nit = c.zc1 - c.zc0 + 28;
+    for (int k = 0; k < F2_PF; k++) f2_fetch<R, MODE>(c, smask, skz, kx, i0 + k, g.pf[k]);
+    const int nit = c.zc1 - c.zc0 + F2_LEAD + 14;
     for (int j0 = 0; j0 < nit; j0 += 8) {
         const int i = i0 + j0;
-        f2_iter<R, 0>(a, c, g, sAx0, sP1, sAx1, sP2, smask, skz, i + 0, kx);
-        f2_iter<R, 1>(a, c, g, sAx0, sP1, sAx1, sP2, smask, skz, i + 1, kx);
-        f2_iter<R, 2>(a, c, g, sAx0, sP1, sAx1, sP2, smask, skz, i + 2, kx);
-        f2_iter<R, 3>(a, c, g, sAx0, sP1, sAx1, sP2, smask, skz, i + 3, kx);
-        f2_iter<R, 4>(a, c, g, sAx0, sP1, sAx1, sP2, smask, skz, i + 4, kx);
-        f2_iter<R, 5>(a, c, g, sAx0, sP1, sAx1, sP2, smask, skz, i + 5, kx);
-        f2_iter<R, 6>(a, c, g, sAx0, sP1, sAx1, sP2, smask, skz, i + 6, kx);
-        f2_iter<R, 7>(a, c, g, sAx0, sP1, sAx1, sP2, smask, skz, i + 7, kx);
+        f2_iter<R, MODE, 0>(a, c, g, sAx0, sP1, sAx1, sP2, smask, skz, i + 0, kx);
+        f2_iter<R, MODE, 1>(a, c, g, sAx0, sP1, sAx1, sP2, smask, skz, i + 1, kx);
+        f2_iter<R, MODE, 2>(a, c, g, sAx0, sP1, sAx1, sP2, smask, skz, i + 2, kx);
+        f2_iter<R, MODE, 3>(a, c, g, sAx0, sP1, sAx1, sP2, smask, skz, i + 3, kx);
+        f2_iter<R, MODE, 4>(a, c, g, sAx0, sP1, sAx1, sP2, smask, skz, i + 4, kx);
+        f2_iter<R, MODE, 5>(a, c, g, sAx0, sP1, sAx1, sP2, smask, skz, i + 5, kx);
+        f2_iter<R, MODE, 6>(a, c, g, sAx0, sP1, sAx1, sP2, smask, skz, i + 6, kx);
+        f2_iter<R, MODE, 7>(a, c, g, sAx0, sP1, sAx1, sP2, smask, skz, i + 7, kx);
     }
 }
 
-bool fwd2_supported(long long sstride, int prec)
+bool pair2_supported(long long sstride, int prec)
 {
     // 32-bit buffer offsets: the event map (16 bytes per cell) must fit
     return sstride > 0 && sstride * 16 < (long long)F2_OOB && sstride * prec < (long long)F2_OOB;
 }
 
 template <typename R>
-hipError_t launch_fwd2_step(const FwdArgs<R> &a, const unsigned *evmask, const int *pmap,
-                            hipStream_t st)
+hipError_t launch_pair2(const Pair2Args<R> &a, int mode, hipStream_t st)
 {
-    if (!evmask || !pmap || !fwd2_supported(a.sstride, (int)sizeof(R))) return hipErrorInvalidValue;
+    if (!a.mask || (mode != F2_REC && !a.pmap) || !pair2_supported(a.sstride, (int)sizeof(R)))
+        return hipErrorInvalidValue;
+    if (mode == F2_REC && (size_t)a.T * a.n_ring * sizeof(R) >= F2_OOB) return hipErrorInvalidValue;
     dim3 grid((a.X + F2_OWN - 1) / F2_OWN, (a.Z + F2_CH - 1) / F2_CH, a.nshots);
-    hipLaunchKernelGGL((fwd2_kernel<R>), grid, dim3(256), 0, st, a, evmask, pmap);
+    switch (mode) {
+    case F2_FWD: hipLaunchKernelGGL((pair2_kernel<R, F2_FWD>), grid, dim3(256), 0, st, a); break;
+    case F2_LAM: hipLaunchKernelGGL((pair2_kernel<R, F2_LAM>), grid, dim3(256), 0, st, a); break;
+    case F2_REC: hipLaunchKernelGGL((pair2_kernel<R, F2_REC>), grid, dim3(256), 0, st, a); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
-template hipError_t launch_fwd2_step<float>(const FwdArgs<float> &, const unsigned *, const int *,
-                                            hipStream_t);
-template hipError_t launch_fwd2_step<double>(const FwdArgs<double> &, const unsigned *, const int *,
-                                             hipStream_t);
+template hipError_t launch_pair2<float>(const Pair2Args<float> &, int, hipStream_t);
+template hipError_t launch_pair2<double>(const Pair2Args<double> &, int, hipStream_t);
 
 int step_occupancy(bool adj, int prec, int th, bool persistent)
 {

@@ -121,6 +121,20 @@ class PMLModel:
         check(self._lib.otfwi_get_fwd_steps(self._h, ctypes.byref(v)))
         return int(v.value)
 
+    def set_adj_steps(self, steps):
+        """Adjoint time steps per launch pair, 1 or 2 (see otfwi_set_adj_steps)."""
+        check(self._lib.otfwi_set_adj_steps(self._h, int(steps)))
+
+    def get_adj_steps(self):
+        v = ctypes.c_int()
+        check(self._lib.otfwi_get_adj_steps(self._h, ctypes.byref(v)))
+        return int(v.value)
+
+    def set_steps_per_launch(self, fwd=2, adj=2):
+        """Both loops at once (see set_fwd_steps, set_adj_steps)."""
+        self.set_fwd_steps(fwd)
+        self.set_adj_steps(adj)
+
     def get_launch(self):
         a, b = ctypes.c_int(), ctypes.c_int()
         check(self._lib.otfwi_get_launch(self._h, ctypes.byref(a), ctypes.byref(b)))

@@ -1,8 +1,8 @@
-"""Two forward steps per launch (otfwi_set_fwd_steps(m, 2)): traces, ring,
-reference-mode and exact gradients must be bitwise equal to one step per
-launch, for grids spanning several column blocks (224 owned columns) and row
-chunks (256 rows), with event cells in the blocks' halos, odd and even step
-counts, and both precisions."""
+"""Two time steps per launch (otfwi_set_fwd_steps / otfwi_set_adj_steps):
+traces, ring, reference-mode and exact gradients must be bitwise equal to
+one step per launch, for grids spanning several column blocks (224 owned
+columns) and row chunks (256 rows), with event cells in the blocks' halos,
+odd and even step counts, and both precisions."""
 import numpy as np
 import pytest
 
@@ -17,10 +17,11 @@ def _obs(p, shots):
                                p["rx_xz"], p["T"], idx=i)["rec"] for i in shots])
 
 
-def _run(p, shots, obs, steps, precision, w2):
+def _run(p, shots, obs, steps, precision, w2, adj_steps=1):
     m = make_model(p, precision=precision, cq="cq_bg")
     m.set_fwd_steps(steps)
-    assert m.get_fwd_steps() == steps
+    m.set_adj_steps(adj_steps)
+    assert m.get_fwd_steps() == steps and m.get_adj_steps() == adj_steps
     m.set_observed(obs)
     out = {"rec": m.forward(shots)}
     out["g_l2"], out["mis_l2"] = m.gradient(shots, "l2")
@@ -44,10 +45,11 @@ def test_fwd2_bitwise(gpu, X, Z, T, precision):
     obs = _obs(p, shots) if long else np.zeros((len(shots), len(p["rx_xz"][0]), T), np.float32)
     w2 = long and bool((np.abs(obs).max(axis=-1) > 0).all())   # W2 needs nonzero traces
     a = _run(p, shots, obs, 1, precision, w2=w2)
-    b = _run(p, shots, obs, 2, precision, w2=w2)
-    assert a.keys() == b.keys()
-    for name in a:
-        assert np.array_equal(a[name], b[name]), name
+    for fs, ads in ((2, 1), (1, 2), (2, 2)):
+        b = _run(p, shots, obs, fs, precision, w2=w2, adj_steps=ads)
+        assert a.keys() == b.keys()
+        for name in a:
+            assert np.array_equal(a[name], b[name]), (fs, ads, name)
     if long:
         assert np.abs(a["rec"]).max() > 0 and np.abs(a["g_l2"]).max() > 0
 
@@ -57,4 +59,6 @@ def test_fwd2_rejects_bad_value(gpu):
     m = make_model(p)
     with pytest.raises(Exception):
         m.set_fwd_steps(3)
-    assert m.get_fwd_steps() == 1
+    with pytest.raises(Exception):
+        m.set_adj_steps(0)
+    assert m.get_fwd_steps() == 1 and m.get_adj_steps() == 1
