@@ -118,6 +118,12 @@ def calib_copy(nbytes):
     print(f"[bench] copied {nbytes} bytes device-to-device", file=sys.stderr, flush=True)
 
 
+def sum_traffic(kernels, N, S):
+    """Summed HBM bytes per launch of the listed kernels (None if any is missing)."""
+    vals = [profiled_traffic(k, N, S) for k in kernels]
+    return None if any(v is None for v in vals) else float(sum(vals))
+
+
 def profiled_traffic(kernel, N, S):
     """HBM bytes per launch of `kernel` (e.g. "adj_step_kernel<float,32>") from
     the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the 64-shot
@@ -207,6 +213,8 @@ def main():
     ap.add_argument("--misfit", default="w2", choices=["w2", "l2"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--steps-per-launch", type=int, default=0, choices=[0, 1, 2],
+                    help="time steps per launch of both loops (0: the autotuner's choice)")
     ap.add_argument("--cpu-steps", type=int, default=2000)
     ap.add_argument("--inversion-models", type=int, default=3,
                     help="held-out models for the learned vs plain inversion (0: skip)")
@@ -238,11 +246,16 @@ def main():
         m.set_tuning(args.th_fwd, args.th_adj, 0)
     elif not args.no_autotune:
         tune = m.autotune(S, steps=20)
+    if args.steps_per_launch:
+        m.set_steps_per_launch(args.steps_per_launch, args.steps_per_launch)
+    ktimes = m.kernel_times(S, reps=10)
     obs = m.forward(shots)
     m.set_observed(obs)
     m.set_cquad((cfg["c_bg"] ** 2).astype(np.float32))
+    fs, ads = m.get_fwd_steps(), m.get_adj_steps()
     print(f"[bench] device {_lib.device_info()}; {S} shots {N}x{N} T={T}; tiles {m.get_tuning()}"
-          f" persistent {m.get_launch()} autotune {tune}", file=sys.stderr, flush=True)
+          f" persistent {m.get_launch()} autotune {tune}; steps per launch fwd {fs} adj {ads};"
+          f" kernel ms {ktimes}", file=sys.stderr, flush=True)
 
     for i in range(args.warmup):
         m.gradient(shots, misfit=args.misfit)
@@ -260,10 +273,44 @@ def main():
     ms_step = 1e3 * wall / args.steps
     value = args.steps / wall
     cells = S * N * N
-    adj_launch_s = t_ev["adj_ms"] / 1e3 / t_ev["n_adj"]
-    fwd_launch_s = t_ev["fwd_ms"] / 1e3 / t_ev["n_fwd"]
-    adj_ach = ADJ_BYTES * cells / adj_launch_s
-    fwd_ach = FWD_BYTES * cells / fwd_launch_s
+    # Launch schedule of one gradient (the loops' own rule): the forward runs
+    # T-1 steps as lead singles, an even number of pairs and a last single;
+    # the adjoint runs T-1 steps as lead singles and an even number of pairs.
+    fp = ((T - 2) // 2) & ~1 if fs == 2 else 0
+    f1 = (T - 1) - 2 * fp
+    ap_ = ((T - 1) // 2) & ~1 if ads == 2 else 0
+    a1 = (T - 1) - 2 * ap_
+    fwd_phase_s = t_ev["fwd_ms"] / 1e3 / args.steps
+    adj_phase_s = t_ev["adj_ms"] / 1e3 / args.steps
+    # Dominant unit: the adjoint step pair (reconstruction kernel + lambda
+    # kernel, 2 x 60 B per cell) or the one-step adjoint kernel (60 B per
+    # cell).  Its average duration comes from the timed region's adjoint-phase
+    # HIP events, less the lead single steps at their own measured duration.
+    if ap_:
+        unit_s = (adj_phase_s - a1 * ktimes["adj"] / 1e3) / ap_
+        unit_bytes = 2 * ADJ_BYTES * cells
+        unit_name = (f"adjoint step pair: pair2_kernel<float, 2> (reconstruction) + "
+                     f"pair2_kernel<float, 1> (lambda, gradient)")
+        traffic = sum_traffic(["pair2_kernel<float,2>", "pair2_kernel<float,1>"], N, S)
+        split = {"rec_pair_ms": ktimes["rec_pair"], "lam_pair_ms": ktimes["lam_pair"]}
+    else:
+        unit_s = adj_phase_s / (T - 1)
+        unit_bytes = ADJ_BYTES * cells
+        unit_name = f"adj_step_kernel<float,{m.get_tuning()[1]}>"
+        traffic = sum_traffic([unit_name], N, S)
+        split = {}
+    if fp:
+        fwd_unit_s = (fwd_phase_s - f1 * ktimes["fwd"] / 1e3) / fp
+        fwd_bytes = 2 * FWD_BYTES * cells
+        fwd_name = "pair2_kernel<float, 0>"
+        fwd_traffic = sum_traffic(["pair2_kernel<float,0>"], N, S)
+    else:
+        fwd_unit_s = fwd_phase_s / (T - 1)
+        fwd_bytes = FWD_BYTES * cells
+        fwd_name = f"fwd_step_kernel<float,{m.get_tuning()[0]}>"
+        fwd_traffic = sum_traffic([fwd_name], N, S)
+    unit_ach = unit_bytes / unit_s
+    fwd_ach = fwd_bytes / fwd_unit_s
     alg_total = (FWD_BYTES + ADJ_BYTES) * cells * (T - 1)
     out = {
         "metric": "64-shot gradient evals/s @1024² (% HBM peak); learned vs plain time-to-tol",
@@ -281,20 +328,21 @@ def main():
         "config": {"workload": f"fmc{S}_{N}x{N}_T{T}_{args.misfit}_gradient", "grid": [N, N],
                    "T": T, "shots": S, "receivers": S, "misfit": args.misfit, "pml": 20,
                    "tiles": list(m.get_tuning()[:2]), "persistent": list(m.get_launch()),
-                   "autotune": tune},
-        "roofline": {"bound": "hbm", "kernel": "adj_step_kernel",
-                     "achieved": adj_ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": adj_ach / HBM_PEAK,
-                     "traffic": profiled_traffic(f"adj_step_kernel<float,{m.get_tuning()[1]}>", N, S),
-                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1, "
+                   "steps_per_launch": [fs, ads], "autotune": tune},
+        "roofline": {"bound": "hbm", "kernel": unit_name,
+                     "achieved": unit_ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": unit_ach / HBM_PEAK,
+                     "traffic": traffic,
+                     "traffic_unit": "bytes per unit (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1, "
                                      "profiles/hbm_traffic_fmc64_1024x1024.md)",
                      "bytes_per_cell_step": ADJ_BYTES, "cells_per_launch": cells,
-                     "launch_ms": adj_launch_s * 1e3},
-        "fwd_kernel": {"kernel": "fwd_step_kernel", "achieved_GBps": fwd_ach / 1e9,
-                       "frac": fwd_ach / HBM_PEAK, "launch_ms": fwd_launch_s * 1e3,
-                       "traffic": profiled_traffic(f"fwd_step_kernel<float,{m.get_tuning()[0]}>",
-                                                   N, S),
-                       "bytes_per_cell_step": FWD_BYTES},
+                     "steps_per_unit": 2 if ap_ else 1,
+                     "unit_ms": unit_s * 1e3, **split},
+        "fwd_kernel": {"kernel": fwd_name, "achieved_GBps": fwd_ach / 1e9,
+                       "frac": fwd_ach / HBM_PEAK, "unit_ms": fwd_unit_s * 1e3,
+                       "steps_per_unit": 2 if fp else 1,
+                       "traffic": fwd_traffic, "bytes_per_cell_step": FWD_BYTES},
+        "kernel_ms": ktimes,
         "whole_gradient_hbm_frac": alg_total / (ms_step / 1e3) / HBM_PEAK,
         "phase_ms": {"forward": t_ev["fwd_ms"] / args.steps,
                      "misfit": t_ev["misfit_ms"] / args.steps,

@@ -177,11 +177,21 @@ int otfwi_set_graph(otfwi_model *m, int use_graph);
 
 /* Autotuner: times `steps` launches of every tile height and launch style of
  * the forward and adjoint step kernels on a batch of n_shots shots of this
- * model and keeps the fastest.  choice[0..3] (may be NULL) = th_fwd,
- * persistent_fwd, th_adj, persistent_adj; ms[0..1] (may be NULL) = best
- * average launch time of the forward / adjoint step in milliseconds.  No
- * reference counterpart (fixed 32x32 blocks). */
+ * model and keeps the fastest; then sets the forward and adjoint steps per
+ * launch (otfwi_set_fwd_steps / otfwi_set_adj_steps) to 2 where the
+ * two-step kernels take less time per step.  choice[0..3] (may be NULL) =
+ * th_fwd, persistent_fwd, th_adj, persistent_adj; ms[0..1] (may be NULL) =
+ * best average launch time of the one-step forward / adjoint kernels in
+ * milliseconds.  No reference counterpart (fixed 32x32 blocks). */
 int otfwi_autotune(otfwi_model *m, int n_shots, int steps, int *choice, float *ms);
+
+/* Device time per launch (HIP events on the model's stream, mean of `reps`
+ * launches after a warm-up launch) of the step kernels on a batch of n_shots
+ * shots: ms[0] one forward step, ms[1] one adjoint step, ms[2] a forward
+ * pair, ms[3] a reconstruction pair, ms[4] a lambda pair (-1 where a kernel
+ * does not apply).  Leaves the batch's field buffers in an unspecified state.
+ * No reference counterpart. */
+int otfwi_kernel_times(otfwi_model *m, int n_shots, int reps, float *ms);
 
 /* Device time of the last otfwi_gradient / otfwi_gradient_device call,
  * from HIP events on the model's stream: out[0] forward time loops (ms),

@@ -113,7 +113,7 @@ struct otfwi_model {
     // event map {ring+1, rx+1, src+1, 0}, and the ring-cell masks with slots
     unsigned *evmask = nullptr, *ringtab = nullptr;
     int *pmap = nullptr;
-    int fwd_steps = 1, adj_steps = 1;   // time steps per forward / adjoint launch (1 or 2)
+    int fwd_steps = 2, adj_steps = 2;   // time steps per forward / adjoint launch (1 or 2)
     bool points_dirty = true;
     int th_fwd = 32, th_adj = 16, max_batch = 0;
     int pers_fwd = 0, pers_adj = 0;   // persistent-work-group launches (0 = one work-group per tile)
@@ -587,6 +587,68 @@ FwdArgs<R> fwd_args(otfwi_model *m, int n)
     return a;
 }
 
+// Step-invariant arguments of the two-step kernels on the current batch of n shots.
+template <typename R>
+Pair2Args<R> pair_args(otfwi_model *m, int n)
+{
+    Pair2Args<R> q{};
+    q.X = m->X; q.Z = m->Z; q.ld = m->ld; q.nshots = n; q.T = m->T;
+    q.n_rx = m->n_rx; q.n_ring = m->n_ring;
+    q.sstride = (long long)m->zp * m->ld;
+    q.cm = (const R *)m->cm; q.kx = (const R *)m->kx; q.kz = (const R *)m->kz;
+    q.cA = (R)m->dt / (R)m->dx;
+    return q;
+}
+
+// mode 0 (forward field) from the one-step forward arguments
+template <typename R>
+Pair2Args<R> pair_fwd_args(otfwi_model *m, int n, const FwdArgs<R> &a)
+{
+    Pair2Args<R> q = pair_args<R>(m, n);
+    q.shot_src = a.shot_src; q.icum = a.icum; q.rec = a.rec; q.ring = a.ring;
+    q.mask = m->evmask; q.pmap = m->pmap;
+    return q;
+}
+
+// mode 2 (reconstruction) and mode 1 (lambda); slot pointers are set per launch
+template <typename R>
+Pair2Args<R> pair_rec_args(otfwi_model *m, int n)
+{
+    Pair2Args<R> u = pair_args<R>(m, n);
+    Batch &b = m->b;
+    u.lap0 = (R *)b.lap[0]; u.lap1 = (R *)b.lap[1];
+    u.ring_in = (const R *)b.ring; u.mask = m->ringtab;
+    return u;
+}
+template <typename R>
+Pair2Args<R> pair_lam_args(otfwi_model *m, int n)
+{
+    Pair2Args<R> l = pair_args<R>(m, n);
+    Batch &b = m->b;
+    l.lap0 = (R *)b.lap[0]; l.lap1 = (R *)b.lap[1];
+    l.jcum = (const R *)b.jcum; l.grad = (R *)b.grad; l.mask = m->evmask; l.pmap = m->pmap;
+    return l;
+}
+
+// the two-step adjoint's lap planes (two per shot of the batch)
+template <typename R>
+int ensure_lap(otfwi_model *m)
+{
+    Batch &b = m->b;
+    if (b.lap[0] && b.lap_cap >= b.cap) return OTFWI_OK;
+    dfree(b.lap[0]); dfree(b.lap[1]);
+    b.lap_cap = 0;
+    const size_t plane = (size_t)m->zp * m->ld;
+    for (int i = 0; i < 2; i++) {
+        char *q = nullptr;
+        RCK(dalloc(&q, plane * b.cap * sizeof(R)));
+        b.lap[i] = q;
+    }
+    b.lap_cap = b.cap;
+    m->epoch++;
+    return OTFWI_OK;
+}
+
 template <typename R>
 int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum)
 {
@@ -623,12 +685,7 @@ int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum)
             return OTFWI_OK;
         };
         for (int k = 0; k < lead; k++, t++) RCK(single(t));
-        Pair2Args<R> q{};
-        q.X = a.X; q.Z = a.Z; q.ld = a.ld; q.nshots = n; q.T = a.T;
-        q.n_rx = a.n_rx; q.n_ring = a.n_ring; q.sstride = a.sstride;
-        q.cm = a.cm; q.kx = a.kx; q.kz = a.kz; q.cA = a.cA;
-        q.shot_src = a.shot_src; q.icum = a.icum; q.rec = a.rec; q.ring = a.ring;
-        q.mask = m->evmask; q.pmap = m->pmap;
+        Pair2Args<R> q = pair_fwd_args<R>(m, n, a);
         for (int k = 0; k < pairs; k++, t += 2) {
             // each pair writes the slot it does not read: after k pairs the
             // state sits in the other slot than one step per launch leaves it
@@ -697,17 +754,16 @@ int run_adjoint(otfwi_model *m, int n)
 }
 
 // Launches of adjoint steps t0 .. t1-1 on the current batch (no memsets).
+// Step-invariant arguments of the one-step adjoint on the current batch of n shots.
 template <typename R>
-int adj_loop(otfwi_model *m, int n, int t0, int t1)
+AdjArgs<R> adj_args(otfwi_model *m, int n)
 {
     Batch &b = m->b;
-    const size_t plane = (size_t)m->zp * m->ld;
-    hipStream_t st = m->stream;
     AdjArgs<R> a{};
     a.X = m->X; a.Z = m->Z; a.ld = m->ld;
     a.ntx = ceil_div(m->X, TW); a.ntz = ceil_div(m->Z, m->th_adj);
     a.nshots = n; a.T = m->T;
-    a.sstride = (long long)plane;
+    a.sstride = (long long)m->zp * m->ld;
     a.grad = (R *)b.grad;
     a.cm = (const R *)m->cm;
     a.cA = (R)m->dt / (R)m->dx;
@@ -716,6 +772,16 @@ int adj_loop(otfwi_model *m, int n, int t0, int t1)
     a.evt = m->evt; a.pts = m->pts;
     a.jcum = (const R *)b.jcum; a.n_rx = m->n_rx;
     a.ring = (const R *)b.ring; a.n_ring = m->n_ring;
+    return a;
+}
+
+template <typename R>
+int adj_loop(otfwi_model *m, int n, int t0, int t1)
+{
+    Batch &b = m->b;
+    const size_t plane = (size_t)m->zp * m->ld;
+    hipStream_t st = m->stream;
+    AdjArgs<R> a = adj_args<R>(m, n);
     const int T = m->T;
     const int pw = persistent_wgs(m, true);
     const int K = std::max(1, m->tb_steps[1]);
@@ -727,17 +793,7 @@ int adj_loop(otfwi_model *m, int n, int t0, int t1)
     // one step per launch leaves them.
     const bool two = m->adj_steps == 2 && K == 1 && G == n && m->evmask && m->ringtab &&
                      pair2_supported((long long)plane, (int)sizeof(R));
-    if (two && (b.lap_cap < n || !b.lap[0])) {
-        dfree(b.lap[0]); dfree(b.lap[1]);
-        b.lap_cap = 0;
-        for (int i = 0; i < 2; i++) {
-            char *q = nullptr;
-            RCK(dalloc(&q, plane * b.cap * sizeof(R)));
-            b.lap[i] = q;
-        }
-        b.lap_cap = b.cap;
-        m->epoch++;
-    }
+    if (two) RCK(ensure_lap<R>(m));
     auto single = [&](int t) -> int {
         const int lp = (t - 1) & 1, lc = t & 1;
         const int up = (T - 2 + t) & 1, uc = (T - 1 + t) & 1;
@@ -759,17 +815,7 @@ int adj_loop(otfwi_model *m, int n, int t0, int t1)
         const int lead = S - 2 * pairs;
         int t = t0;
         for (int k = 0; k < lead; k++, t++) RCK(single(t));
-        Pair2Args<R> u{}, l{};
-        u.X = l.X = m->X; u.Z = l.Z = m->Z; u.ld = l.ld = m->ld;
-        u.nshots = l.nshots = n; u.T = l.T = T;
-        u.n_rx = l.n_rx = m->n_rx; u.n_ring = l.n_ring = m->n_ring;
-        u.sstride = l.sstride = (long long)plane;
-        u.cm = l.cm = (const R *)m->cm;
-        u.kx = l.kx = (const R *)m->kx; u.kz = l.kz = (const R *)m->kz;
-        u.cA = l.cA = a.cA;
-        u.lap0 = l.lap0 = (R *)b.lap[0]; u.lap1 = l.lap1 = (R *)b.lap[1];
-        u.ring_in = (const R *)b.ring; u.mask = m->ringtab;
-        l.jcum = (const R *)b.jcum; l.grad = (R *)b.grad; l.mask = m->evmask; l.pmap = m->pmap;
+        Pair2Args<R> u = pair_rec_args<R>(m, n), l = pair_lam_args<R>(m, n);
         for (int k = 0; k < pairs; k++, t += 2) {
             const int f = k & 1;
             const int li = ((t - 1) & 1) ^ f, ui = ((T - 2 + t) & 1) ^ f, ai = ((T - 1 + t) & 1) ^ f;
@@ -1289,10 +1335,84 @@ int exact_gradient_impl(otfwi_model *m, int n, const int *shot_src, int kind, in
     return OTFWI_OK;
 }
 
+// Per-launch device time (HIP events on the model's stream, mean of reps
+// launches after one warm-up launch) of the step kernels on the current
+// batch of n shots: ms[0] one forward step, ms[1] one adjoint step, ms[2] a
+// forward pair, ms[3] a reconstruction pair, ms[4] a lambda pair.  The
+// fields hold whatever the batch buffers hold (zeros after a memset); the
+// kernels' cost does not depend on the values.
+template <typename R>
+int kernel_times_impl(otfwi_model *m, int n, int reps, float *ms)
+{
+    if (n <= 0 || reps <= 0) return fail(OTFWI_E_ARG, "kernel times: n_shots and reps must be > 0");
+    if (m->T < 4) return fail(OTFWI_E_ARG, "kernel times: need T >= 4");
+    RCK(ready(m, true));
+    RCK(ensure_batch(m, n, true));
+    RCK(ensure_lap<R>(m));
+    RCK(upload_shot_src(m, n, nullptr));
+    Batch &b = m->b;
+    hipStream_t st = m->stream;
+    const bool have_pairs = m->evmask && m->ringtab &&
+                            pair2_supported((long long)m->zp * m->ld, (int)sizeof(R));
+    FwdArgs<R> fa = fwd_args<R>(m, n);
+    fa.t = 1;
+    fa.P0 = (const R *)b.P[0]; fa.P1 = (R *)b.P[1];
+    fa.Ax0 = (const R *)b.Ax[0]; fa.Ax1 = (R *)b.Ax[1];
+    fa.Az0 = (const R *)b.Az[0]; fa.Az1 = (R *)b.Az[1];
+    fa.Px0 = (const R *)b.Px[0]; fa.Px1 = (R *)b.Px[1];
+    AdjArgs<R> aa = adj_args<R>(m, n);
+    aa.t = 1;
+    aa.LP0 = (const R *)b.LP[0]; aa.LP1 = (R *)b.LP[1];
+    aa.LAx0 = (const R *)b.LAx[0]; aa.LAx1 = (R *)b.LAx[1];
+    aa.LAz0 = (const R *)b.LAz[0]; aa.LAz1 = (R *)b.LAz[1];
+    aa.LPx0 = (const R *)b.LPx[0]; aa.LPx1 = (R *)b.LPx[1];
+    aa.U0 = (const R *)b.P[0]; aa.U1 = (R *)b.P[1];
+    aa.UAx0 = (const R *)b.Ax[0]; aa.UAx1 = (R *)b.Ax[1];
+    aa.UAz0 = (const R *)b.Az[0]; aa.UAz1 = (R *)b.Az[1];
+    Pair2Args<R> pf = pair_fwd_args<R>(m, n, fa), pu = pair_rec_args<R>(m, n),
+                 pl = pair_lam_args<R>(m, n);
+    for (Pair2Args<R> *q : {&pf, &pu}) {
+        q->t = 1;
+        q->P0 = (const R *)b.P[0]; q->P1 = (R *)b.P[1];
+        q->Ax0 = (const R *)b.Ax[0]; q->Ax1 = (R *)b.Ax[1];
+        q->Az0 = (const R *)b.Az[0]; q->Az1 = (R *)b.Az[1];
+        q->Px0 = (const R *)b.Px[0]; q->Px1 = (R *)b.Px[1];
+    }
+    pl.t = 1;
+    pl.P0 = (const R *)b.LP[0]; pl.P1 = (R *)b.LP[1];
+    pl.Ax0 = (const R *)b.LAx[0]; pl.Ax1 = (R *)b.LAx[1];
+    pl.Az0 = (const R *)b.LAz[0]; pl.Az1 = (R *)b.LAz[1];
+    pl.Px0 = (const R *)b.LPx[0]; pl.Px1 = (R *)b.LPx[1];
+    const int pwf = persistent_wgs(m, false), pwa = persistent_wgs(m, true);
+    for (int k = 0; k < 5; k++) {
+        ms[k] = -1.0f;
+        if (k >= 2 && !have_pairs) continue;
+        auto launch = [&]() -> hipError_t {
+            switch (k) {
+            case 0: return launch_fwd_step<R>(fa, m->th_fwd, pwf, st);
+            case 1: return launch_adj_step<R>(aa, m->th_adj, pwa, st);
+            case 2: return launch_pair2<R>(pf, 0, st);
+            case 3: return launch_pair2<R>(pu, 2, st);
+            default: return launch_pair2<R>(pl, 1, st);
+            }
+        };
+        HIPCK(launch());
+        HIPCK(hipEventRecord(m->ev[0], st));
+        for (int r = 0; r < reps; r++) HIPCK(launch());
+        HIPCK(hipEventRecord(m->ev[1], st));
+        HIPCK(hipEventSynchronize(m->ev[1]));
+        float t = 0;
+        HIPCK(hipEventElapsedTime(&t, m->ev[0], m->ev[1]));
+        ms[k] = t / reps;
+    }
+    return OTFWI_OK;
+}
+
 // Times `steps` launches of each tile height / launch style of the forward
 // and adjoint step kernels on the live batch of n shots and keeps the
-// fastest.  out[0..3] = th_fwd, persistent_fwd, th_adj, persistent_adj;
-// ms[0], ms[1] = best average launch time (ms) of the forward / adjoint.
+// fastest, then picks the steps per launch (1 or 2) of both loops by time
+// per step.  out[0..3] = th_fwd, persistent_fwd, th_adj, persistent_adj;
+// ms[0], ms[1] = best average launch time (ms) of the one-step kernels.
 template <typename R>
 int autotune_impl(otfwi_model *m, int n, int steps, int *out, float *ms)
 {
@@ -1311,6 +1431,8 @@ int autotune_impl(otfwi_model *m, int n, int steps, int *out, float *ms)
     HIPCK(hipMemsetAsync(b.ring, 0, (size_t)n * m->T * m->n_ring * es, m->stream));
     const bool graph = m->use_graph;
     m->use_graph = false;
+    const int steps_saved[2] = {m->fwd_steps, m->adj_steps};
+    m->fwd_steps = m->adj_steps = 1;
     const int ths[3] = {16, 32, 64};
     int best[4] = {m->th_fwd, m->pers_fwd, m->th_adj, m->pers_adj};
     float bms[2] = {1e30f, 1e30f};
@@ -1342,11 +1464,22 @@ int autotune_impl(otfwi_model *m, int n, int steps, int *out, float *ms)
                     best[2 * adj + 1] = pers;
                 }
             }
-    m->use_graph = graph;
     m->th_fwd = best[0]; m->pers_fwd = best[1];
     m->th_adj = best[2]; m->pers_adj = best[3];
+    m->fwd_steps = steps_saved[0];
+    m->adj_steps = steps_saved[1];
+    if (rc == OTFWI_OK) rc = ready(m, true);
+    // steps per launch: two-step launches when their time per step is lower
+    if (rc == OTFWI_OK) {
+        float kt[5] = {};
+        rc = kernel_times_impl<R>(m, n, 5, kt);
+        if (rc == OTFWI_OK) {
+            m->fwd_steps = kt[2] / 2 < kt[0] ? 2 : 1;
+            m->adj_steps = (kt[3] + kt[4]) / 2 < kt[1] ? 2 : 1;
+        }
+    }
+    m->use_graph = graph;
     RCK(rc);
-    RCK(ready(m, true));
     if (out) for (int i = 0; i < 4; i++) out[i] = best[i];
     if (ms) { ms[0] = bms[0]; ms[1] = bms[1]; }
     return OTFWI_OK;
@@ -1643,6 +1776,13 @@ int otfwi_autotune(otfwi_model *m, int n_shots, int steps, int *choice, float *m
 {
     RCK(check_model(m));
     return DISPATCH(m, autotune_impl)(m, n_shots, steps, choice, ms);
+}
+
+int otfwi_kernel_times(otfwi_model *m, int n_shots, int reps, float *ms)
+{
+    RCK(check_model(m));
+    if (!ms) return fail(OTFWI_E_ARG, "ms is NULL");
+    return DISPATCH(m, kernel_times_impl)(m, n_shots, reps, ms);
 }
 
 int otfwi_get_tuning(otfwi_model *m, int *th_fwd, int *th_adj, int *max_batch)

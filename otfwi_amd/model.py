@@ -95,12 +95,14 @@ class PMLModel:
 
     def autotune(self, n_shots, steps=20):
         """Time every tile height / launch style on a batch of n_shots and keep
-        the fastest.  Returns the choice and the best per-launch times (ms)."""
+        the fastest, then choose the steps per launch of both loops.  Returns
+        the choice and the best one-step per-launch times (ms)."""
         ch = np.zeros(4, np.int32)
         ms = np.zeros(2, np.float32)
         check(self._lib.otfwi_autotune(self._h, int(n_shots), int(steps), ptr(ch), ptr(ms)))
         return dict(th_fwd=int(ch[0]), persistent_fwd=bool(ch[1]), th_adj=int(ch[2]),
-                    persistent_adj=bool(ch[3]), fwd_ms=float(ms[0]), adj_ms=float(ms[1]))
+                    persistent_adj=bool(ch[3]), fwd_ms=float(ms[0]), adj_ms=float(ms[1]),
+                    fwd_steps=self.get_fwd_steps(), adj_steps=self.get_adj_steps())
 
     def set_blocking(self, steps_fwd=1, shots_fwd=0, steps_adj=1, shots_adj=0):
         """Cache blocking of the step loops (see otfwi_set_blocking)."""
@@ -120,6 +122,15 @@ class PMLModel:
         v = ctypes.c_int()
         check(self._lib.otfwi_get_fwd_steps(self._h, ctypes.byref(v)))
         return int(v.value)
+
+    def kernel_times(self, n_shots, reps=10):
+        """Device ms per launch of the step kernels on a batch of n_shots
+        (see otfwi_kernel_times): dict fwd, adj, fwd_pair, rec_pair,
+        lam_pair (None where a kernel does not apply)."""
+        ms = np.zeros(5, np.float32)
+        check(self._lib.otfwi_kernel_times(self._h, int(n_shots), int(reps), ptr(ms)))
+        keys = ("fwd", "adj", "fwd_pair", "rec_pair", "lam_pair")
+        return {k: (float(v) if v >= 0 else None) for k, v in zip(keys, ms)}
 
     def set_adj_steps(self, steps):
         """Adjoint time steps per launch pair, 1 or 2 (see otfwi_set_adj_steps)."""

@@ -57,8 +57,9 @@ def test_fwd2_bitwise(gpu, X, Z, T, precision):
 def test_fwd2_rejects_bad_value(gpu):
     p = small_problem()
     m = make_model(p)
+    assert m.get_fwd_steps() == 2 and m.get_adj_steps() == 2     # default: two-step launches
     with pytest.raises(Exception):
         m.set_fwd_steps(3)
     with pytest.raises(Exception):
         m.set_adj_steps(0)
-    assert m.get_fwd_steps() == 1 and m.get_adj_steps() == 1
+    assert m.get_fwd_steps() == 2 and m.get_adj_steps() == 2
