@@ -324,21 +324,25 @@ int build_points(otfwi_model *m)
     // ring cells per column block and row: {lane mask, slot of the word's first lane}
     std::vector<std::vector<int>> ring_rows(m->Z);
     for (int r = 0; r < m->n_ring; r++) ring_rows[m->vz[ord[r]]].push_back(m->vx[ord[r]]);
-    std::vector<unsigned> ringtab((size_t)nbx * m->Z * 16, 0u);
+    std::vector<unsigned> ringtab((size_t)nbx * m->Z * 11, 0u);
     int before = 0;
     for (int z = 0; z < m->Z; z++) {
         const std::vector<int> &row = ring_rows[z];   // ascending x
-        for (int bx = 0; bx < nbx; bx++)
+        for (int bx = 0; bx < nbx; bx++) {
+            unsigned *e = &ringtab[((size_t)bx * m->Z + z) * 11];
+            const int c0 = bx * F2_OWN - F2_HALO;
+            const int first = (int)(std::lower_bound(row.begin(), row.end(), c0) - row.begin());
+            e[0] = (unsigned)(before + first);
             for (int w = 0; w < 8; w++) {
-                const int col0 = bx * F2_OWN - F2_HALO + 32 * w;
+                const int col0 = c0 + 32 * w;
                 const int lo = (int)(std::lower_bound(row.begin(), row.end(), col0) - row.begin());
                 unsigned mk = 0;
                 for (int j = lo; j < (int)row.size() && row[j] < col0 + 32; j++)
                     mk |= 1u << (row[j] - col0);
-                unsigned *e = &ringtab[(((size_t)bx * m->Z + z) * 8 + w) * 2];
-                e[0] = mk;
-                e[1] = (unsigned)(before + lo);
+                e[1 + w / 4] |= (unsigned)(lo - first) << (8 * (w % 4));   // <= 224
+                e[3 + w] = mk;
             }
+        }
         before += (int)row.size();
     }
     dfree(m->evmask);

@@ -122,9 +122,11 @@ hipError_t launch_adj_step(const AdjArgs<R> &a, int th, int persistent_wgs, hipS
 // mask (modes 0, 1): mask[(bx * Z + z) * 8 + w] bit b marks an event cell at
 // column bx * F2_OWN - F2_HALO + 32 w + b of row z, and pmap is the
 // padded-plane map of int4 {ring + 1, rx + 1, src + 1, 0} (0: none).
-// mask (mode 2): pairs {lane mask of the ring cells, ring slot of the
-// word's first lane} at mask[((bx * Z + z) * 8 + w) * 2]; ring slots are
-// numbered in raster order (z, then x).
+// mask (mode 2): 11 words per (bx, z) at mask[(bx * Z + z) * 11]: the ring
+// slot of the first ring cell in the block's columns, the number of ring
+// cells before each 32-column word (one byte each, words 1-2), and the 8
+// lane masks of the ring cells (words 3-10); ring slots are numbered in
+// raster order (z, then x).
 constexpr int F2_OWN = 224, F2_HALO = 16;
 template <typename R>
 struct Pair2Args {

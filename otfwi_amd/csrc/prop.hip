@@ -1088,18 +1088,21 @@ __device__ __forceinline__ F2Evt f2_evt(const F2Ctx &c, const unsigned *smask, i
     return F2Evt{(int)v[0], (int)v[1], (int)v[2]};
 }
 
-// ring value of this lane's cell on row z at ring time trev (REC): smask
-// holds {lane mask, ring slot of the word's first lane} pairs; ring slots
-// are numbered in raster order
+// ring value of this lane's cell on row z at ring time trev (REC).  Per row
+// smask holds F2_RW words: the ring slot of the block's first ring cell, the
+// ring cells before each 32-lane word (8 bytes), and the 8 lane masks; ring
+// slots are numbered in raster order.
+constexpr int F2_RW = 11;
 template <typename R>
 __device__ __forceinline__ R f2_ring(const F2Ctx &c, const unsigned *smask, int z, int trev,
                                      bool &hit)
 {
     const int w = c.tid >> 5, b = c.tid & 31;
-    const unsigned *e = smask + ((z - c.zc0 + F2_RB) * 8 + w) * 2;
-    const unsigned mk = e[0];
+    const unsigned *e = smask + (z - c.zc0 + F2_RB) * F2_RW;
+    const unsigned mk = e[3 + w];
     hit = (mk >> b) & 1u;
-    const int k = (int)e[1] + __builtin_popcount(mk & ((1u << b) - 1u));
+    const unsigned pre = (e[1 + (w >> 2)] >> (8 * (w & 3))) & 0xFFu;
+    const int k = (int)(e[0] + pre) + __builtin_popcount(mk & ((1u << b) - 1u));
     const unsigned off = hit ? (unsigned)(trev * c.nring + k) * (unsigned)sizeof(R) : F2_OOB;
     return bload<R>(c.rring, off);
 }
@@ -1319,16 +1322,16 @@ __device__ __forceinline__ void f2_iter(const Pair2Args<R> &a, const F2Ctx &c,
 
 // mask: FWD, LAM: [ceil(X / F2_OWN)][Z][8] lane masks of the event cells of
 // each block's 256 columns, with pmap the per-cell event map of the padded
-// plane, int4 {ring + 1, rx + 1, src + 1, 0} (0: none); REC: [..][Z][8]
-// {lane mask of the ring cells, ring slot of the word's first lane}.
+// plane, int4 {ring + 1, rx + 1, src + 1, 0} (0: none); REC: [..][Z][F2_RW]
+// ring-cell tables as f2_ring reads them.
 // Waves per SIMD the register allocation targets: 3 work-groups of 4 waves
-// per CU (the LDS of FWD and LAM allows 3; REC's lane tables allow 2).
-template <int MODE> constexpr int f2_waves() { return MODE == F2_REC ? 2 : 3; }
+// per CU, as many as the LDS allows.
+template <int MODE> constexpr int f2_waves() { return 3; }
 
 template <typename R, int MODE>
 __global__ void __launch_bounds__(256, f2_waves<MODE>()) pair2_kernel(const Pair2Args<R> a)
 {
-    constexpr int MW = MODE == F2_REC ? 16 : 8;   // mask words per row
+    constexpr int MW = MODE == F2_REC ? F2_RW : 8;   // mask words per row
     __shared__ R sAx0[8][F2_LP], sP1[8][F2_LP], sAx1[8][F2_LP], sP2[8][F2_LP];
     __shared__ R skz[MODE == F2_REC ? 1 : F2_NR];
     __shared__ unsigned smask[F2_NR * MW];
