@@ -109,9 +109,10 @@ struct otfwi_model {
     int n_src = 0, n_rx = 0, n_ring = 0;
     int *evt = nullptr;
     PointEvt *pts = nullptr;
-    // two-step kernels: event lane masks per column block and row, the per-cell
-    // event map {ring+1, rx+1, src+1, 0}, and the ring-cell masks with slots
-    unsigned *evmask = nullptr, *ringtab = nullptr;
+    // two-step kernels: rows holding a source or receiver per column block,
+    // the per-cell event map {ring+1, rx+1, src+1, 0}, the ring-cell tables
+    uint8_t *evrow = nullptr;
+    unsigned *ringtab = nullptr;
     int *pmap = nullptr;
     int fwd_steps = 2, adj_steps = 2;   // time steps per forward / adjoint launch (1 or 2)
     bool points_dirty = true;
@@ -304,10 +305,10 @@ int build_points(otfwi_model *m)
         return m->vz[u] != m->vz[v] ? m->vz[u] < m->vz[v] : m->vx[u] < m->vx[v];
     });
     for (int r = 0; r < m->n_ring; r++) get(m->vx[ord[r]], m->vz[ord[r]]).ring = r;
-    // two-step forward: lane masks of the event cells of each 256-column block
-    // (owned columns plus halo) and the per-cell event map
+    // two-step kernels: rows of each 256-column block (owned columns plus
+    // halo) holding a source or receiver, and the per-cell event map
     const int nbx = ceil_div(m->X, F2_OWN);
-    std::vector<unsigned> evmask((size_t)nbx * m->Z * 8, 0u);
+    std::vector<uint8_t> evrow((size_t)nbx * m->Z, 0);
     std::vector<int> pmap(map.size() * 4, 0);
     for (size_t q = 0; q < map.size(); q++) {
         if (map[q] < 0) continue;
@@ -315,10 +316,11 @@ int build_points(otfwi_model *m)
         pmap[4 * q] = pe.ring + 1;
         pmap[4 * q + 1] = pe.rx + 1;
         pmap[4 * q + 2] = pe.src + 1;
+        if (pe.rx < 0 && pe.src < 0) continue;
         const int z = (int)(q / m->ld) - ZOFF, x = (int)(q % m->ld) - XOFF;
         for (int bx = 0; bx < nbx; bx++) {
             const int c = x - (bx * F2_OWN - F2_HALO);
-            if (c >= 0 && c < 256) evmask[((size_t)bx * m->Z + z) * 8 + c / 32] |= 1u << (c % 32);
+            if (c >= 0 && c < 256) evrow[(size_t)bx * m->Z + z] = 1;
         }
     }
     // ring cells per column block and row: {lane mask, slot of the word's first lane}
@@ -345,15 +347,14 @@ int build_points(otfwi_model *m)
         }
         before += (int)row.size();
     }
-    dfree(m->evmask);
+    dfree(m->evrow);
     dfree(m->pmap);
     dfree(m->ringtab);
     RCK(dalloc(&m->ringtab, ringtab.size()));
     HIPCK(hipMemcpy(m->ringtab, ringtab.data(), ringtab.size() * sizeof(unsigned),
                     hipMemcpyHostToDevice));
-    RCK(dalloc(&m->evmask, evmask.size()));
-    HIPCK(hipMemcpy(m->evmask, evmask.data(), evmask.size() * sizeof(unsigned),
-                    hipMemcpyHostToDevice));
+    RCK(dalloc(&m->evrow, evrow.size()));
+    HIPCK(hipMemcpy(m->evrow, evrow.data(), evrow.size(), hipMemcpyHostToDevice));
     RCK(dalloc(&m->pmap, pmap.size()));
     HIPCK(hipMemcpy(m->pmap, pmap.data(), pmap.size() * sizeof(int), hipMemcpyHostToDevice));
     dfree(m->evt);
@@ -610,7 +611,7 @@ Pair2Args<R> pair_fwd_args(otfwi_model *m, int n, const FwdArgs<R> &a)
 {
     Pair2Args<R> q = pair_args<R>(m, n);
     q.shot_src = a.shot_src; q.icum = a.icum; q.rec = a.rec; q.ring = a.ring;
-    q.mask = m->evmask; q.pmap = m->pmap;
+    q.mask = m->ringtab; q.evrow = m->evrow; q.pmap = m->pmap;
     return q;
 }
 
@@ -630,7 +631,7 @@ Pair2Args<R> pair_lam_args(otfwi_model *m, int n)
     Pair2Args<R> l = pair_args<R>(m, n);
     Batch &b = m->b;
     l.lap0 = (R *)b.lap[0]; l.lap1 = (R *)b.lap[1];
-    l.jcum = (const R *)b.jcum; l.grad = (R *)b.grad; l.mask = m->evmask; l.pmap = m->pmap;
+    l.jcum = (const R *)b.jcum; l.grad = (R *)b.grad; l.evrow = m->evrow; l.pmap = m->pmap;
     return l;
 }
 
@@ -670,7 +671,7 @@ int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum)
     // count leaves state t in slot t&1 again), then single steps -- the last
     // step is always single, so A(t1-2) stays in slot (t1-2)&1 for the
     // adjoint's reconstruction
-    const bool two = m->fwd_steps == 2 && ist == 0 && K == 1 && G == n && m->evmask &&
+    const bool two = m->fwd_steps == 2 && ist == 0 && K == 1 && G == n && m->evrow &&
                      pair2_supported((long long)plane, (int)sizeof(R));
     auto body2 = [&]() -> int {
         const int S = t1 - t0;
@@ -795,7 +796,7 @@ int adj_loop(otfwi_model *m, int n, int t0, int t1)
     // adds both gradient terms.  Leading single steps, then an even number of
     // pairs (each writes the slots it does not read), so the slots end where
     // one step per launch leaves them.
-    const bool two = m->adj_steps == 2 && K == 1 && G == n && m->evmask && m->ringtab &&
+    const bool two = m->adj_steps == 2 && K == 1 && G == n && m->evrow && m->ringtab &&
                      pair2_supported((long long)plane, (int)sizeof(R));
     if (two) RCK(ensure_lap<R>(m));
     auto single = [&](int t) -> int {
@@ -1356,7 +1357,7 @@ int kernel_times_impl(otfwi_model *m, int n, int reps, float *ms)
     RCK(upload_shot_src(m, n, nullptr));
     Batch &b = m->b;
     hipStream_t st = m->stream;
-    const bool have_pairs = m->evmask && m->ringtab &&
+    const bool have_pairs = m->evrow && m->ringtab &&
                             pair2_supported((long long)m->zp * m->ld, (int)sizeof(R));
     FwdArgs<R> fa = fwd_args<R>(m, n);
     fa.t = 1;
@@ -1566,7 +1567,7 @@ int otfwi_destroy(otfwi_model *m)
     (void)hipStreamSynchronize(m->stream);
     free_batch(m->b);
     dfree(m->cm); dfree(m->kx); dfree(m->kz); dfree(m->icum); dfree(m->cq32);
-    dfree(m->evt); dfree(m->pts); dfree(m->tf_fwd); dfree(m->tf_adj); dfree(m->evmask);
+    dfree(m->evt); dfree(m->pts); dfree(m->tf_fwd); dfree(m->tf_adj); dfree(m->evrow);
     dfree(m->pmap); dfree(m->ringtab);
     dfree(m->gacc); dfree(m->gout); dfree(m->ill);
     dfree(m->xsnap); dfree(m->xseg); dfree(m->rxmap);

@@ -119,10 +119,11 @@ hipError_t launch_adj_step(const AdjArgs<R> &a, int th, int persistent_wgs, hipS
 //   mode 2 reconstructed field U (adj_step_kernel's U x2): ring values
 //          written back at ring times T-1-t and T-2-t, writes lap0 / lap1 =
 //          (lap9_x + lap9_z)(U(t)) / 5040 and the same of U(t+1); no psi.
-// mask (modes 0, 1): mask[(bx * Z + z) * 8 + w] bit b marks an event cell at
-// column bx * F2_OWN - F2_HALO + 32 w + b of row z, and pmap is the
-// padded-plane map of int4 {ring + 1, rx + 1, src + 1, 0} (0: none).
-// mask (mode 2): 11 words per (bx, z) at mask[(bx * Z + z) * 11]: the ring
+// evrow (modes 0, 1): evrow[bx * Z + z] != 0 where row z holds a source or
+// receiver in columns [bx * F2_OWN - F2_HALO, bx * F2_OWN - F2_HALO + 256), and
+// pmap is the padded-plane map of int4 {ring + 1, rx + 1, src + 1, 0}
+// (0: none).
+// mask (modes 0, 2): 11 words per (bx, z) at mask[(bx * Z + z) * 11]: the ring
 // slot of the first ring cell in the block's columns, the number of ring
 // cells before each 32-column word (one byte each, words 1-2), and the 8
 // lane masks of the ring cells (words 3-10); ring slots are numbered in
@@ -145,6 +146,7 @@ struct Pair2Args {
     R *grad;                      // mode 1
     R *lap0, *lap1;               // mode 1 reads, mode 2 writes (padded planes per shot)
     const unsigned *mask;
+    const uint8_t *evrow;
     const int *pmap;
 };
 bool pair2_supported(long long sstride, int prec);

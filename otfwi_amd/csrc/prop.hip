@@ -1047,8 +1047,8 @@ struct F2Row {
     R m3;                 // m on row i - 10
     F2Evt e1, e3;         // FWD, LAM: events on rows i - 3 and i - 10
     R g, l0, l1;          // LAM: grad and lap(U(t)) on row i - 7, lap(U(t+1)) on row i - 14
-    R r1, r3;             // REC: ring values on rows i - 3 (step t) and i - 10 (step t+1)
-    bool h1, h3;          // REC: the cell is a ring cell
+    int k1, k3;           // FWD, REC: ring slots on rows i - 3 and i - 10 (-1: none)
+    R r1, r3;             // REC: ring values there at steps t and t+1
 };
 
 template <typename R, int MODE>
@@ -1078,38 +1078,35 @@ __device__ __forceinline__ unsigned f2_off(const F2Ctx &c, int z)
 
 __device__ __forceinline__ bool f2_zin(const F2Ctx &c, int z) { return z >= BOX && z < c.Z - BOX; }
 
-// event of this lane's cell on row z (FWD, LAM): smask holds the lane masks
-__device__ __forceinline__ F2Evt f2_evt(const F2Ctx &c, const unsigned *smask, int z)
+// source / receiver event of this lane's cell on row z (FWD, LAM): sev
+// flags the rows holding one in the block's columns; on those rows every
+// lane reads its cell of the event map (a few rows per block)
+__device__ __forceinline__ F2Evt f2_evt(const F2Ctx &c, const uint8_t *sev, int z)
 {
-    const unsigned w = smask[(z - c.zc0 + F2_RB) * 8 + (c.tid >> 5)];
-    const bool hit = (w >> (c.tid & 31)) & 1u;
-    const unsigned off = hit ? (unsigned)((z + ZOFF) * c.ld) * 16u + c.lo4 * 4u : F2_OOB;
+    const bool row = sev[z - c.zc0 + F2_RB] != 0;
+    const unsigned off = row && c.colv ? (unsigned)((z + ZOFF) * c.ld) * 16u + c.lo4 * 4u : F2_OOB;
     const auto v = __builtin_amdgcn_raw_buffer_load_b96(c.rpm, off, 0, 0);
     return F2Evt{(int)v[0], (int)v[1], (int)v[2]};
 }
 
-// ring value of this lane's cell on row z at ring time trev (REC).  Per row
-// smask holds F2_RW words: the ring slot of the block's first ring cell, the
-// ring cells before each 32-lane word (8 bytes), and the 8 lane masks; ring
-// slots are numbered in raster order.
+// Ring slot of this lane's cell on row z, or -1 (FWD, REC).  Per row smask
+// holds F2_RW words: the ring slot of the block's first ring cell, the ring
+// cells before each 32-lane word (8 bytes), and the 8 lane masks; ring slots
+// are numbered in raster order.
 constexpr int F2_RW = 11;
-template <typename R>
-__device__ __forceinline__ R f2_ring(const F2Ctx &c, const unsigned *smask, int z, int trev,
-                                     bool &hit)
+__device__ __forceinline__ int f2_rslot(const F2Ctx &c, const unsigned *smask, int z)
 {
     const int w = c.tid >> 5, b = c.tid & 31;
     const unsigned *e = smask + (z - c.zc0 + F2_RB) * F2_RW;
     const unsigned mk = e[3 + w];
-    hit = (mk >> b) & 1u;
     const unsigned pre = (e[1 + (w >> 2)] >> (8 * (w & 3))) & 0xFFu;
     const int k = (int)(e[0] + pre) + __builtin_popcount(mk & ((1u << b) - 1u));
-    const unsigned off = hit ? (unsigned)(trev * c.nring + k) * (unsigned)sizeof(R) : F2_OOB;
-    return bload<R>(c.rring, off);
+    return ((mk >> b) & 1u) ? k : -1;
 }
 
 template <typename R, int MODE>
-__device__ __forceinline__ void f2_fetch(const F2Ctx &c, const unsigned *smask, const R *skz, R kx,
-                                         int i, F2Row<R, MODE> &f)
+__device__ __forceinline__ void f2_fetch(const F2Ctx &c, const unsigned *smask, const uint8_t *sev,
+                                         const R *skz, R kx, int i, F2Row<R, MODE> &f)
 {
     const bool rv = c.colv && i >= -ZOFF && i < c.Z + 4;
     const unsigned o = rv ? f2_off<R>(c, i) : F2_OOB;
@@ -1121,13 +1118,17 @@ __device__ __forceinline__ void f2_fetch(const F2Ctx &c, const unsigned *smask, 
     const bool rv3 = c.colv && i3 >= -ZOFF && i3 < c.Z + 4;
     f.m3 = bload<R>(c.rcm, rv3 ? f2_off<R>(c, i3) : F2_OOB);
     f.ps = f.g = f.l0 = f.l1 = f.r1 = f.r3 = R(0);
-    f.h1 = f.h3 = false;
+    f.k1 = f.k3 = -1;
     f.e1 = f.e3 = F2Evt{0, 0, 0};
     if constexpr (MODE != F2_REC) {
         const bool damped = kx != R(1) || skz[i - c.zc0 + F2_RB] != R(1);
         f.ps = bload<R>(c.rPx0, damped ? o : F2_OOB);
-        f.e1 = f2_evt(c, smask, i - 3);
-        f.e3 = f2_evt(c, smask, i3);
+        f.e1 = f2_evt(c, sev, i - 3);
+        f.e3 = f2_evt(c, sev, i3);
+    }
+    if constexpr (MODE != F2_LAM) {
+        f.k1 = f2_rslot(c, smask, i - 3);
+        f.k3 = f2_rslot(c, smask, i3);
     }
     if constexpr (MODE == F2_LAM) {
         const int z2 = i - 7, z4 = i - 14;
@@ -1138,27 +1139,31 @@ __device__ __forceinline__ void f2_fetch(const F2Ctx &c, const unsigned *smask, 
         f.l1 = bload<R>(c.rl1, o4 ? f2_off<R>(c, z4) : F2_OOB);
     }
     if constexpr (MODE == F2_REC) {
-        f.r1 = f2_ring<R>(c, smask, i - 3, c.T - 1 - c.t, f.h1);
-        f.r3 = f2_ring<R>(c, smask, i3, c.T - 2 - c.t, f.h3);
+        // ring samples of the shot, time-major: ring times T-1-t and T-2-t
+        const unsigned es = (unsigned)sizeof(R);
+        f.r1 = bload<R>(c.rring, f.k1 >= 0 ? (unsigned)((c.T - 1 - c.t) * c.nring + f.k1) * es : F2_OOB);
+        f.r3 = bload<R>(c.rring, f.k3 >= 0 ? (unsigned)((c.T - 2 - c.t) * c.nring + f.k3) * es : F2_OOB);
     }
 }
 
 // P-kind update of one cell at step tt (p_quad / lu_quad arithmetic for one
 // lane).  rec_here = this block samples the cell (FWD).
+// ks: ring slot of the cell (-1: none): REC writes the ring value rv back,
+// FWD samples P there (both branch-free through buffer offsets).
 template <typename R, int MODE>
 __device__ __forceinline__ R f2_p(const Pair2Args<R> &a, const F2Ctx &c, const F2Evt &e, R rv,
-                                  bool rh, bool inb, int tt, bool rec_here, R pp, R psi, R mm, R kx,
+                                  int ks, bool inb, int tt, bool rec_here, R pp, R psi, R mm, R kx,
                                   R kz, R dAx, R dAz, R &psi_out)
 {
     if constexpr (MODE == F2_REC) {
         R up = fma(-mm, dAx + dAz, pp);
-        if (rh) up = rv;
+        if (ks >= 0) up = rv;
         psi_out = R(0);
         return inb ? up : R(0);
     } else {
         R nx = fma(mm, dAx, mul_rn(psi, kx));
         R nz = fma(mm, dAz, mul_rn(pp - psi, kz));
-        if ((e.ring1 | e.rx1 | e.src1) != 0) {
+        if ((e.rx1 | e.src1) != 0) {
             if constexpr (MODE == F2_FWD) {
                 const int src = e.src1 - 1;
                 if (src >= 0 && (c.ss < 0 || c.ss == src)) {
@@ -1166,13 +1171,8 @@ __device__ __forceinline__ R f2_p(const Pair2Args<R> &a, const F2Ctx &c, const F
                     nx = add_rn(nx, h);
                     nz = add_rn(nz, h);
                 }
-                if (rec_here) {
-                    const R pv = add_rn(nx, nz);
-                    if (e.rx1 > 0 && a.rec)
-                        a.rec[((size_t)c.s * a.n_rx + (e.rx1 - 1)) * a.T + tt] = pv;
-                    if (e.ring1 > 0 && a.ring)
-                        a.ring[((size_t)c.s * a.T + tt) * a.n_ring + (e.ring1 - 1)] = pv;
-                }
+                if (rec_here && e.rx1 > 0 && a.rec)
+                    a.rec[((size_t)c.s * a.n_rx + (e.rx1 - 1)) * a.T + tt] = add_rn(nx, nz);
             } else {
                 if (e.rx1 > 0) {
                     const R h = a.jcum[((size_t)c.s * a.n_rx + (e.rx1 - 1)) * a.T + tt] * R(0.5);
@@ -1182,6 +1182,9 @@ __device__ __forceinline__ R f2_p(const Pair2Args<R> &a, const F2Ctx &c, const F
             }
         }
         R p = add_rn(nx, nz);
+        if constexpr (MODE == F2_FWD)   // ring cells lie inside the box: p is the sample
+            bstore<R>(c.rring, rec_here && ks >= 0
+                                   ? (unsigned)(tt * c.nring + ks) * (unsigned)sizeof(R) : F2_OOB, p);
         if (!inb) { p = R(0); nx = R(0); }
         psi_out = (kx != R(1) || kz != R(1)) ? nx : R(0);
         return p;
@@ -1219,7 +1222,8 @@ template <typename R, int MODE, int PH>
 __device__ __forceinline__ void f2_iter(const Pair2Args<R> &a, const F2Ctx &c,
                                         Pair2Regs<R, MODE> &g, R (*sAx0)[F2_LP],
                                         R (*sP1)[F2_LP], R (*sAx1)[F2_LP], R (*sP2)[F2_LP],
-                                        const unsigned *smask, const R *skz, int i, R kx)
+                                        const unsigned *smask, const uint8_t *sev, const R *skz,
+                                        int i, R kx)
 {
     constexpr int Q1 = (PH + 1) & 7, Q4 = (PH + 4) & 7, Q5 = (PH + 5) & 7;
     constexpr int F = PH % F2_PF;
@@ -1243,7 +1247,7 @@ __device__ __forceinline__ void f2_iter(const Pair2Args<R> &a, const F2Ctx &c,
                               g.az0[(PH + 4) & 7], g.az0[(PH + 5) & 7], g.az0[(PH + 6) & 7],
                               g.az0[(PH + 7) & 7], g.az0[PH]);
         const bool rec = c.own && z1 >= c.zc0 && z1 < c.zc1;
-        pn1 = f2_p<R, MODE>(a, c, f.e1, f.r1, f.h1, c.xin && f2_zin(c, z1), c.t, rec, g.p0[H1],
+        pn1 = f2_p<R, MODE>(a, c, f.e1, f.r1, f.k1, c.xin && f2_zin(c, z1), c.t, rec, g.p0[H1],
                             g.s0[H1], g.m0[H1], kx, kz_at(z1), dAx, dAz, sn1);
         sP1[PH][lc] = pn1;
     }
@@ -1283,7 +1287,7 @@ __device__ __forceinline__ void f2_iter(const Pair2Args<R> &a, const F2Ctx &c,
         const R kz3 = kz_at(z3);
         R sn2;
         const bool mine = c.own && z3 >= c.zc0 && z3 < c.zc1;
-        pn2 = f2_p<R, MODE>(a, c, f.e3, f.r3, f.h3, c.xin && f2_zin(c, z3), c.t + 1, mine,
+        pn2 = f2_p<R, MODE>(a, c, f.e3, f.r3, f.k3, c.xin && f2_zin(c, z3), c.t + 1, mine,
                             g.p1[Q1], g.s1[Q1], f.m3, kx, kz3, dAx, dAz, sn2);
         sP2[PH][lc] = pn2;
         const unsigned o = mine ? f2_off<R>(c, z3) : F2_OOB;
@@ -1316,25 +1320,27 @@ __device__ __forceinline__ void f2_iter(const Pair2Args<R> &a, const F2Ctx &c,
         g.p2[PH] = pn2;
     }
     // ---- 6. loads for row i + F2_PF into the slot just consumed ----
-    f2_fetch<R, MODE>(c, smask, skz, kx, i + F2_PF, f);
+    f2_fetch<R, MODE>(c, smask, sev, skz, kx, i + F2_PF, f);
     lds_barrier();
 }
 
-// mask: FWD, LAM: [ceil(X / F2_OWN)][Z][8] lane masks of the event cells of
-// each block's 256 columns, with pmap the per-cell event map of the padded
-// plane, int4 {ring + 1, rx + 1, src + 1, 0} (0: none); REC: [..][Z][F2_RW]
-// ring-cell tables as f2_ring reads them.
-// Waves per SIMD the register allocation targets: 3 work-groups of 4 waves
-// per CU, as many as the LDS allows.
+// evrow (FWD, LAM): [ceil(X / F2_OWN)][Z] flags of the rows holding a source
+// or receiver in each block's 256 columns, with pmap the per-cell event map
+// of the padded plane, int4 {ring + 1, rx + 1, src + 1, 0} (0: none);
+// mask (FWD, REC): [..][Z][F2_RW] ring-cell tables as f2_rslot reads them.
+// Waves per SIMD the register allocation targets: work-groups of 4 waves,
+// three per CU (FWD 50 KB and REC 49 KB of LDS allow 3; LAM at 4 was slower).
 template <int MODE> constexpr int f2_waves() { return 3; }
 
 template <typename R, int MODE>
 __global__ void __launch_bounds__(256, f2_waves<MODE>()) pair2_kernel(const Pair2Args<R> a)
 {
-    constexpr int MW = MODE == F2_REC ? F2_RW : 8;   // mask words per row
+    constexpr int MW = MODE == F2_LAM ? 1 : F2_RW;   // ring-table words per row
+    constexpr int EV = MODE == F2_REC ? 4 : F2_NR;    // event-row flags
     __shared__ R sAx0[8][F2_LP], sP1[8][F2_LP], sAx1[8][F2_LP], sP2[8][F2_LP];
     __shared__ R skz[MODE == F2_REC ? 1 : F2_NR];
     __shared__ unsigned smask[F2_NR * MW];
+    __shared__ uint8_t sev[EV];
     F2Ctx c;
     c.tid = threadIdx.x;
     c.X = a.X; c.Z = a.Z; c.ld = a.ld; c.t = a.t; c.T = a.T; c.nring = a.n_ring;
@@ -1363,6 +1369,9 @@ __global__ void __launch_bounds__(256, f2_waves<MODE>()) pair2_kernel(const Pair
         c.rPx1 = f2_rsrc(a.Px1 + sb, pb);
         c.rpm = f2_rsrc(a.pmap, (unsigned)((size_t)a.sstride * 16));
     }
+    if constexpr (MODE == F2_FWD)   // ring samples of this shot, time-major (none: no bytes)
+        c.rring = f2_rsrc(a.ring ? a.ring + (size_t)c.s * a.T * a.n_ring : a.cm,
+                          a.ring ? (unsigned)((size_t)a.T * a.n_ring * sizeof(R)) : 0u);
     if constexpr (MODE == F2_LAM) {
         c.rg = f2_rsrc(a.grad + sb, pb);
         c.rl0 = f2_rsrc(a.lap0 + sb, pb);
@@ -1381,10 +1390,16 @@ __global__ void __launch_bounds__(256, f2_waves<MODE>()) pair2_kernel(const Pair
             const int z = c.zc0 - F2_RB + r;
             skz[r] = (z >= -ZOFF && z < a.Z + ZOFF) ? a.kz[ZOFF + z] : R(1);
         }
-    for (int k = c.tid; k < F2_NR * MW; k += 256) {
-        const int z = c.zc0 - F2_RB + k / MW;
-        smask[k] = (z >= 0 && z < a.Z) ? a.mask[((size_t)blockIdx.x * a.Z + z) * MW + k % MW] : 0u;
-    }
+    if constexpr (MODE != F2_LAM)
+        for (int k = c.tid; k < F2_NR * MW; k += 256) {
+            const int z = c.zc0 - F2_RB + k / MW;
+            smask[k] = (z >= 0 && z < a.Z) ? a.mask[((size_t)blockIdx.x * a.Z + z) * MW + k % MW] : 0u;
+        }
+    if constexpr (MODE != F2_REC)
+        for (int r = c.tid; r < F2_NR; r += 256) {
+            const int z = c.zc0 - F2_RB + r;
+            sev[r] = (z >= 0 && z < a.Z) ? a.evrow[(size_t)blockIdx.x * a.Z + z] : (uint8_t)0;
+        }
     for (int r = 0; r < 8; r++)
         for (int k = c.tid; k < F2_LP; k += 256) {
             sAx0[r][k] = R(0); sP1[r][k] = R(0); sAx1[r][k] = R(0); sP2[r][k] = R(0);
@@ -1398,18 +1413,18 @@ __global__ void __launch_bounds__(256, f2_waves<MODE>()) pair2_kernel(const Pair
     __syncthreads();
     const int i0 = c.zc0 - F2_LEAD;
 #pragma unroll
-    for (int k = 0; k < F2_PF; k++) f2_fetch<R, MODE>(c, smask, skz, kx, i0 + k, g.pf[k]);
+    for (int k = 0; k < F2_PF; k++) f2_fetch<R, MODE>(c, smask, sev, skz, kx, i0 + k, g.pf[k]);
     const int nit = c.zc1 - c.zc0 + F2_LEAD + 14;
     for (int j0 = 0; j0 < nit; j0 += 8) {
         const int i = i0 + j0;
-        f2_iter<R, MODE, 0>(a, c, g, sAx0, sP1, sAx1, sP2, smask, skz, i + 0, kx);
-        f2_iter<R, MODE, 1>(a, c, g, sAx0, sP1, sAx1, sP2, smask, skz, i + 1, kx);
-        f2_iter<R, MODE, 2>(a, c, g, sAx0, sP1, sAx1, sP2, smask, skz, i + 2, kx);
-        f2_iter<R, MODE, 3>(a, c, g, sAx0, sP1, sAx1, sP2, smask, skz, i + 3, kx);
-        f2_iter<R, MODE, 4>(a, c, g, sAx0, sP1, sAx1, sP2, smask, skz, i + 4, kx);
-        f2_iter<R, MODE, 5>(a, c, g, sAx0, sP1, sAx1, sP2, smask, skz, i + 5, kx);
-        f2_iter<R, MODE, 6>(a, c, g, sAx0, sP1, sAx1, sP2, smask, skz, i + 6, kx);
-        f2_iter<R, MODE, 7>(a, c, g, sAx0, sP1, sAx1, sP2, smask, skz, i + 7, kx);
+        f2_iter<R, MODE, 0>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 0, kx);
+        f2_iter<R, MODE, 1>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 1, kx);
+        f2_iter<R, MODE, 2>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 2, kx);
+        f2_iter<R, MODE, 3>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 3, kx);
+        f2_iter<R, MODE, 4>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 4, kx);
+        f2_iter<R, MODE, 5>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 5, kx);
+        f2_iter<R, MODE, 6>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 6, kx);
+        f2_iter<R, MODE, 7>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 7, kx);
     }
 }
 
@@ -1422,9 +1437,10 @@ bool pair2_supported(long long sstride, int prec)
 template <typename R>
 hipError_t launch_pair2(const Pair2Args<R> &a, int mode, hipStream_t st)
 {
-    if (!a.mask || (mode != F2_REC && !a.pmap) || !pair2_supported(a.sstride, (int)sizeof(R)))
+    if ((mode != F2_LAM && !a.mask) || (mode != F2_REC && (!a.pmap || !a.evrow)) ||
+        !pair2_supported(a.sstride, (int)sizeof(R)))
         return hipErrorInvalidValue;
-    if (mode == F2_REC && (size_t)a.T * a.n_ring * sizeof(R) >= F2_OOB) return hipErrorInvalidValue;
+    if (mode != F2_LAM && (size_t)a.T * a.n_ring * sizeof(R) >= F2_OOB) return hipErrorInvalidValue;
     dim3 grid((a.X + F2_OWN - 1) / F2_OWN, (a.Z + F2_CH - 1) / F2_CH, a.nshots);
     switch (mode) {
     case F2_FWD: hipLaunchKernelGGL((pair2_kernel<R, F2_FWD>), grid, dim3(256), 0, st, a); break;
