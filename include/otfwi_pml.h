@@ -169,6 +169,11 @@ int otfwi_get_fwd_steps(otfwi_model *m, int *steps);
 int otfwi_set_adj_steps(otfwi_model *m, int steps);
 int otfwi_get_adj_steps(otfwi_model *m, int *steps);
 
+/* Work-group width of the two-step kernels: 256 or 384 threads (owning 224
+ * or 352 columns each), or 0 (default, 256).  Results do not depend on it;
+ * 384 was slower on MI355X at 1024^2.  No reference counterpart. */
+int otfwi_set_pair_width(otfwi_model *m, int nw);
+
 /* Replay the per-step launch loops as captured hipGraphs (default 1); a
  * graph is re-captured only when shots, tiles, launch style or device buffers
  * change.  No reference counterpart (regression_PML_single.cu:661-678

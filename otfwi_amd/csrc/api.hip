@@ -115,6 +115,8 @@ struct otfwi_model {
     unsigned *ringtab = nullptr;
     int *pmap = nullptr;
     int fwd_steps = 2, adj_steps = 2;   // time steps per forward / adjoint launch (1 or 2)
+    int pair_nw = 0;                 // two-step block width requested (0: fewest threads)
+    int pair_nw_built = 256;         // the width the two-step tables were built for
     bool points_dirty = true;
     int th_fwd = 32, th_adj = 16, max_batch = 0;
     int pers_fwd = 0, pers_adj = 0;   // persistent-work-group launches (0 = one work-group per tile)
@@ -166,12 +168,14 @@ int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
 void free_batch(Batch &b)
 {
-    for (int i = 0; i < 2; i++) {
-        dfree(b.P[i]); dfree(b.Ax[i]); dfree(b.Az[i]); dfree(b.Px[i]);
-        dfree(b.LP[i]); dfree(b.LAx[i]); dfree(b.LAz[i]); dfree(b.LPx[i]);
+    // both slots of a field are one allocation (slot 1 follows slot 0)
+    for (void **f : {b.P, b.Ax, b.Az, b.Px, b.LP, b.LAx, b.LAz, b.LPx}) {
+        dfree(f[0]);
+        f[1] = nullptr;
     }
     dfree(b.grad); dfree(b.ring); dfree(b.rec); dfree(b.adj); dfree(b.jcum);
-    dfree(b.lap[0]); dfree(b.lap[1]);
+    dfree(b.lap[0]);
+    b.lap[1] = nullptr;
     b.lap_cap = 0;
     dfree(b.mis_tr); dfree(b.mis_shot); dfree(b.scratch); dfree(b.shot_src); dfree(b.obs);
     b.cap = 0;
@@ -196,18 +200,16 @@ int ensure_batch(otfwi_model *m, int n, bool need_adj, bool need_ring = true)
         *p = q;
         return OTFWI_OK;
     };
-    for (int i = 0; i < 2; i++) {
-        RCK(fa(&b.P[i], plane * cap));
-        RCK(fa(&b.Ax[i], plane * cap));
-        RCK(fa(&b.Az[i], plane * cap));
-        RCK(fa(&b.Px[i], plane * cap));
-        if (need_adj) {
-            RCK(fa(&b.LP[i], plane * cap));
-            RCK(fa(&b.LAx[i], plane * cap));
-            RCK(fa(&b.LAz[i], plane * cap));
-            RCK(fa(&b.LPx[i], plane * cap));
-        }
-    }
+    // the two slots of a field in one allocation, slot 1 right after slot 0
+    // (the two-step kernels address both through one buffer resource)
+    auto fa2 = [&](void **f, size_t elems) -> int {
+        RCK(fa(&f[0], 2 * elems));
+        f[1] = (char *)f[0] + elems * es;
+        return OTFWI_OK;
+    };
+    for (void **f : {b.P, b.Ax, b.Az, b.Px}) RCK(fa2(f, plane * cap));
+    if (need_adj)
+        for (void **f : {b.LP, b.LAx, b.LAz, b.LPx}) RCK(fa2(f, plane * cap));
     const size_t ntr = (size_t)cap * std::max(m->n_rx, 1);
     RCK(fa(&b.rec, ntr * m->T));
     if (need_adj) {
@@ -305,9 +307,11 @@ int build_points(otfwi_model *m)
         return m->vz[u] != m->vz[v] ? m->vz[u] < m->vz[v] : m->vx[u] < m->vx[v];
     });
     for (int r = 0; r < m->n_ring; r++) get(m->vx[ord[r]], m->vz[ord[r]]).ring = r;
-    // two-step kernels: rows of each 256-column block (owned columns plus
+    // two-step kernels: rows of each nw-column block (owned columns plus
     // halo) holding a source or receiver, and the per-cell event map
-    const int nbx = ceil_div(m->X, F2_OWN);
+    const int nw = m->pair_nw ? m->pair_nw : f2_width(m->X);
+    const int own = f2_own(nw), nwords = nw / 32, rw = f2_ring_words(nw);
+    const int nbx = ceil_div(m->X, own);
     std::vector<uint8_t> evrow((size_t)nbx * m->Z, 0);
     std::vector<int> pmap(map.size() * 4, 0);
     for (size_t q = 0; q < map.size(); q++) {
@@ -319,34 +323,36 @@ int build_points(otfwi_model *m)
         if (pe.rx < 0 && pe.src < 0) continue;
         const int z = (int)(q / m->ld) - ZOFF, x = (int)(q % m->ld) - XOFF;
         for (int bx = 0; bx < nbx; bx++) {
-            const int c = x - (bx * F2_OWN - F2_HALO);
-            if (c >= 0 && c < 256) evrow[(size_t)bx * m->Z + z] = 1;
+            const int c = x - (bx * own - F2_HALO);
+            if (c >= 0 && c < nw) evrow[(size_t)bx * m->Z + z] = 1;
         }
     }
-    // ring cells per column block and row: {lane mask, slot of the word's first lane}
+    // ring cells per column block and row: {slot of the block's first ring
+    // cell, 16-bit count before each 32-lane word, lane masks}
     std::vector<std::vector<int>> ring_rows(m->Z);
     for (int r = 0; r < m->n_ring; r++) ring_rows[m->vz[ord[r]]].push_back(m->vx[ord[r]]);
-    std::vector<unsigned> ringtab((size_t)nbx * m->Z * 11, 0u);
+    std::vector<unsigned> ringtab((size_t)nbx * m->Z * rw, 0u);
     int before = 0;
     for (int z = 0; z < m->Z; z++) {
         const std::vector<int> &row = ring_rows[z];   // ascending x
         for (int bx = 0; bx < nbx; bx++) {
-            unsigned *e = &ringtab[((size_t)bx * m->Z + z) * 11];
-            const int c0 = bx * F2_OWN - F2_HALO;
+            unsigned *e = &ringtab[((size_t)bx * m->Z + z) * rw];
+            const int c0 = bx * own - F2_HALO;
             const int first = (int)(std::lower_bound(row.begin(), row.end(), c0) - row.begin());
             e[0] = (unsigned)(before + first);
-            for (int w = 0; w < 8; w++) {
+            for (int w = 0; w < nwords; w++) {
                 const int col0 = c0 + 32 * w;
                 const int lo = (int)(std::lower_bound(row.begin(), row.end(), col0) - row.begin());
                 unsigned mk = 0;
                 for (int j = lo; j < (int)row.size() && row[j] < col0 + 32; j++)
                     mk |= 1u << (row[j] - col0);
-                e[1 + w / 4] |= (unsigned)(lo - first) << (8 * (w % 4));   // <= 224
-                e[3 + w] = mk;
+                e[1 + w / 2] |= (unsigned)(lo - first) << (16 * (w % 2));   // < nw
+                e[1 + nw / 64 + w] = mk;
             }
         }
         before += (int)row.size();
     }
+    m->pair_nw_built = nw;
     dfree(m->evrow);
     dfree(m->pmap);
     dfree(m->ringtab);
@@ -602,6 +608,8 @@ Pair2Args<R> pair_args(otfwi_model *m, int n)
     q.sstride = (long long)m->zp * m->ld;
     q.cm = (const R *)m->cm; q.kx = (const R *)m->kx; q.kz = (const R *)m->kz;
     q.cA = (R)m->dt / (R)m->dx;
+    q.slot_elems = (long long)m->b.cap * q.sstride;
+    q.nw = m->pair_nw_built;
     return q;
 }
 
@@ -641,14 +649,14 @@ int ensure_lap(otfwi_model *m)
 {
     Batch &b = m->b;
     if (b.lap[0] && b.lap_cap >= b.cap) return OTFWI_OK;
-    dfree(b.lap[0]); dfree(b.lap[1]);
+    dfree(b.lap[0]);
+    b.lap[1] = nullptr;
     b.lap_cap = 0;
     const size_t plane = (size_t)m->zp * m->ld;
-    for (int i = 0; i < 2; i++) {
-        char *q = nullptr;
-        RCK(dalloc(&q, plane * b.cap * sizeof(R)));
-        b.lap[i] = q;
-    }
+    char *q = nullptr;   // both planes in one allocation, like the field slots
+    RCK(dalloc(&q, 2 * plane * b.cap * sizeof(R)));
+    b.lap[0] = q;
+    b.lap[1] = q + plane * b.cap * sizeof(R);
     b.lap_cap = b.cap;
     m->epoch++;
     return OTFWI_OK;
@@ -672,7 +680,7 @@ int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum)
     // step is always single, so A(t1-2) stays in slot (t1-2)&1 for the
     // adjoint's reconstruction
     const bool two = m->fwd_steps == 2 && ist == 0 && K == 1 && G == n && m->evrow &&
-                     pair2_supported((long long)plane, (int)sizeof(R));
+                     pair2_supported((long long)plane, (long long)b.cap * (long long)plane, (int)sizeof(R));
     auto body2 = [&]() -> int {
         const int S = t1 - t0;
         int pairs = S > 1 ? (S - 1) / 2 : 0;
@@ -797,7 +805,7 @@ int adj_loop(otfwi_model *m, int n, int t0, int t1)
     // pairs (each writes the slots it does not read), so the slots end where
     // one step per launch leaves them.
     const bool two = m->adj_steps == 2 && K == 1 && G == n && m->evrow && m->ringtab &&
-                     pair2_supported((long long)plane, (int)sizeof(R));
+                     pair2_supported((long long)plane, (long long)b.cap * (long long)plane, (int)sizeof(R));
     if (two) RCK(ensure_lap<R>(m));
     auto single = [&](int t) -> int {
         const int lp = (t - 1) & 1, lc = t & 1;
@@ -1358,7 +1366,7 @@ int kernel_times_impl(otfwi_model *m, int n, int reps, float *ms)
     Batch &b = m->b;
     hipStream_t st = m->stream;
     const bool have_pairs = m->evrow && m->ringtab &&
-                            pair2_supported((long long)m->zp * m->ld, (int)sizeof(R));
+                            pair2_supported((long long)m->zp * m->ld, (long long)m->b.cap * m->zp * m->ld, (int)sizeof(R));
     FwdArgs<R> fa = fwd_args<R>(m, n);
     fa.t = 1;
     fa.P0 = (const R *)b.P[0]; fa.P1 = (R *)b.P[1];
@@ -1734,6 +1742,18 @@ int otfwi_get_fwd_steps(otfwi_model *m, int *steps)
     RCK(check_model(m));
     if (!steps) return fail(OTFWI_E_ARG, "output is NULL");
     *steps = m->fwd_steps;
+    return OTFWI_OK;
+}
+
+int otfwi_set_pair_width(otfwi_model *m, int nw)
+{
+    RCK(check_model(m));
+    if (nw != 0 && nw != 256 && nw != 384)
+        return fail(OTFWI_E_ARG, "two-step block width must be 0 (automatic), 256 or 384");
+    if (nw != m->pair_nw) {
+        m->pair_nw = nw;
+        m->points_dirty = true;   // tables are rebuilt for the new width
+    }
     return OTFWI_OK;
 }
 

@@ -111,8 +111,9 @@ template <typename R>
 hipError_t launch_adj_step(const AdjArgs<R> &a, int th, int persistent_wgs, hipStream_t s);
 // Two time steps t, t+1 per launch (z-marching strips, prop.hip): reads the
 // state before the pair from P0/Ax0/Az0/Px0 and writes the state after it to
-// P1/Ax1/Az1/Px1 (a different slot).  A work-group owns F2_OWN columns
-// starting at bx * F2_OWN and computes F2_HALO more on each side.
+// P1/Ax1/Az1/Px1 (a different slot).  A work-group of nw = 256 or 384 threads
+// owns nw - 32 columns starting at bx * (nw - 32) and computes F2_HALO more
+// on each side.
 //   mode 0 forward field (fwd_step_kernel x2): samples rec / ring at t, t+1;
 //   mode 1 adjoint field lambda (adj_step_kernel's lambda x2): injects jcum,
 //          grad += lambda_P(t) lap0 + lambda_P(t+1) lap1;
@@ -120,19 +121,29 @@ hipError_t launch_adj_step(const AdjArgs<R> &a, int th, int persistent_wgs, hipS
 //          written back at ring times T-1-t and T-2-t, writes lap0 / lap1 =
 //          (lap9_x + lap9_z)(U(t)) / 5040 and the same of U(t+1); no psi.
 // evrow (modes 0, 1): evrow[bx * Z + z] != 0 where row z holds a source or
-// receiver in columns [bx * F2_OWN - F2_HALO, bx * F2_OWN - F2_HALO + 256), and
+// receiver in columns [bx * own - F2_HALO, bx * own - F2_HALO + nw), and
 // pmap is the padded-plane map of int4 {ring + 1, rx + 1, src + 1, 0}
 // (0: none).
-// mask (modes 0, 2): 11 words per (bx, z) at mask[(bx * Z + z) * 11]: the ring
-// slot of the first ring cell in the block's columns, the number of ring
-// cells before each 32-column word (one byte each, words 1-2), and the 8
-// lane masks of the ring cells (words 3-10); ring slots are numbered in
-// raster order (z, then x).
-constexpr int F2_OWN = 224, F2_HALO = 16;
+// mask (modes 0, 2): f2_ring_words(nw) words per (bx, z) at
+// mask[(bx * Z + z) * f2_ring_words(nw)]: the ring slot of the first ring
+// cell in the block's columns, the number of ring cells before each
+// 32-column word (16 bits each, nw / 64 words), and the nw / 32 lane masks of
+// the ring cells; ring slots are numbered in raster order (z, then x).
+constexpr int F2_HALO = 16;
+inline int f2_own(int nw) { return nw == 256 || nw == 384 ? nw - 2 * F2_HALO : 0; }
+inline int f2_ring_words(int nw) { return 1 + nw / 64 + nw / 32; }
+// Automatic block width: 256.  384-thread blocks waste fewer threads on
+// partial column blocks (1152 vs 1280 threads over 1024 columns) but ran
+// 1.3-1.45x slower per launch on MI355X at 64 x 1024^2: two 6-wave work-groups
+// per CU hide latency worse than three 4-wave ones.
+inline int f2_width(int X) { (void)X; return 256; }
+// The two slots of each field are one allocation: P1 - P0 = +-slot_elems
+// (likewise Ax, Az, Px), so one buffer resource per field covers both.
 template <typename R>
 struct Pair2Args {
     int X, Z, ld, nshots, t, T, n_rx, n_ring;
-    long long sstride;
+    int nw;                       // threads per work-group (256 or 384); the tables match it
+    long long sstride, slot_elems;
     const R *P0, *Ax0, *Az0, *Px0;
     R *P1, *Ax1, *Az1, *Px1;
     const R *cm, *kx, *kz;
@@ -144,12 +155,12 @@ struct Pair2Args {
     const R *ring_in;             // mode 2: the same, read
     const R *jcum;                // mode 1
     R *grad;                      // mode 1
-    R *lap0, *lap1;               // mode 1 reads, mode 2 writes (padded planes per shot)
+    R *lap0, *lap1;               // mode 1 reads, mode 2 writes; lap1 = lap0 + slot_elems
     const unsigned *mask;
     const uint8_t *evrow;
     const int *pmap;
 };
-bool pair2_supported(long long sstride, int prec);
+bool pair2_supported(long long sstride, long long slot_elems, int prec);
 template <typename R>
 hipError_t launch_pair2(const Pair2Args<R> &a, int mode, hipStream_t s);
 // Resident work-groups per CU of a step kernel (0 if the variant does not exist).

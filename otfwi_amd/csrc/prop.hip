@@ -960,8 +960,9 @@ hipError_t launch_adj_step<double>(const AdjArgs<double> &a, int th, int pw, hip
 // ---------------------------------------------------------------------------
 // two time steps per launch: z-marching strips
 // ---------------------------------------------------------------------------
-// A work-group of 256 threads owns columns [xs, xs + F2_OWN) of one shot
-// (thread tid holds column xs - F2_HALO + tid) and output rows
+// A work-group of NW = 256 or 384 threads owns NW - 32 columns
+// [xs, xs + NW - 32) of one shot (thread tid holds column xs - F2_HALO + tid)
+// and output rows
 // [zc0, zc0 + F2_CH), and walks down the rows.  Iteration j takes row
 // i = zc0 - 15 + j of the state before the pair and computes, each stage on
 // rows already complete below it,
@@ -993,13 +994,24 @@ hipError_t launch_adj_step<double>(const AdjArgs<double> &a, int th, int pw, hip
 // and columns outside the plane, undamped psi cells, cells without an event
 // and lanes that do not own a cell need no branch.
 constexpr int F2_FWD = 0, F2_LAM = 1, F2_REC = 2;
-constexpr int F2_CH = 256, F2_LP = 256 + 16, F2_PF = 2;
+#ifndef F2_PREFETCH
+#define F2_PREFETCH 2   // rows of global loads in flight ahead of the row being computed
+#endif
+constexpr int F2_CH = 256, F2_PF = F2_PREFETCH;
+// Block geometry for NW threads: owned columns, LDS row width (8 padding
+// cells each side), ring-table words per row (slot base, one 16-bit prefix
+// per 32-lane word, one lane mask per word).
+template <int NW>
+struct F2Geo {
+    static_assert(NW == 256 || NW == 384, "two-step blocks are 256 or 384 threads");
+    static constexpr int OWN = NW - 2 * F2_HALO, LP = NW + 16;
+    static constexpr int WORDS = NW / 32, RW = 1 + NW / 64 + NW / 32;
+};
 // LDS row tables (kz, lane masks) cover rows [zc0 - F2_RB, zc0 - F2_RB + F2_NR):
 // iterations touch rows zc0 - 29 .. zc0 + F2_CH + 20 + F2_PF
 constexpr int F2_RB = 32, F2_NR = F2_CH + 64;
 constexpr int F2_LEAD = 15;   // rows loaded above a chunk
 constexpr unsigned F2_OOB = 0xFFFFFFF0u;
-static_assert(F2_HALO + F2_OWN + F2_HALO == 256, "two-step block is 256 columns");
 
 typedef __amdgpu_buffer_rsrc_t Rsrc;
 
@@ -1028,11 +1040,27 @@ template <> __device__ __forceinline__ void bstore<double>(Rsrc r, unsigned off,
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, off, 0, 0);
 }
 
-template <typename R>
-__device__ __forceinline__ R stag_lds(const R *row, int c0)
+// LDS row segment ring[q][col ..]: the byte offset passes through an empty
+// asm, so the compiler forms one address per row and pairs the reads with
+// small immediate offsets (folding the row constant into each access would
+// need an add per pair of reads).
+template <typename R, int N, int LP>
+__device__ __forceinline__ void lds_seg(R (*ring)[LP], int q, int col, R (&v)[N])
 {
-    return stag<R>(row[c0], row[c0 + 1], row[c0 + 2], row[c0 + 3], row[c0 + 4], row[c0 + 5],
-                   row[c0 + 6], row[c0 + 7]);
+    typedef const R __attribute__((address_space(3))) *LdsPtr;
+    unsigned o = (unsigned)(uintptr_t)(LdsPtr)&ring[q][col];
+    asm("" : "+v"(o));
+    const LdsPtr row = (LdsPtr)(uintptr_t)o;
+#pragma unroll
+    for (int k = 0; k < N; k++) v[k] = row[k];
+}
+
+template <typename R, int LP>
+__device__ __forceinline__ R stag_lds(R (*ring)[LP], int q, int c0)
+{
+    R v[8];
+    lds_seg<R, 8, LP>(ring, q, c0, v);
+    return stag<R>(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]);
 }
 
 // Event of one cell, each index + 1 (0: none), as the per-cell event map holds it.
@@ -1067,7 +1095,10 @@ struct F2Ctx {
     unsigned lo;   // byte offset of this lane's column in a padded row (element size R)
     unsigned lo4;  // the same for 4-byte elements
     bool colv, own, xin;   // xin: 4 <= x < X - 4
-    Rsrc rAx0, rAz0, rP0, rPx0, rcm, rpm, rP1, rPx1, rAx1, rAz1, rg, rl0, rl1, rring;
+    // one resource per field covers both slots of the shot; the slot read and
+    // the slot written are byte offsets (P kind: P, psi; A kind: Ax, Az)
+    Rsrc rP, rAx, rAz, rPx, rcm, rpm, rg, rl, rring;
+    unsigned dPi, dPo, dAi, dAo, dL1;   // dL1: lap1 relative to lap0 (one resource)
 };
 
 template <typename R>
@@ -1090,29 +1121,31 @@ __device__ __forceinline__ F2Evt f2_evt(const F2Ctx &c, const uint8_t *sev, int 
 }
 
 // Ring slot of this lane's cell on row z, or -1 (FWD, REC).  Per row smask
-// holds F2_RW words: the ring slot of the block's first ring cell, the ring
-// cells before each 32-lane word (8 bytes), and the 8 lane masks; ring slots
-// are numbered in raster order.
-constexpr int F2_RW = 11;
+// holds F2Geo<NW>::RW words: the ring slot of the block's first ring cell,
+// the ring cells before each 32-lane word (16 bits each), and the lane masks;
+// ring slots are numbered in raster order.
+template <int NW>
 __device__ __forceinline__ int f2_rslot(const F2Ctx &c, const unsigned *smask, int z)
 {
     const int w = c.tid >> 5, b = c.tid & 31;
-    const unsigned *e = smask + (z - c.zc0 + F2_RB) * F2_RW;
-    const unsigned mk = e[3 + w];
-    const unsigned pre = (e[1 + (w >> 2)] >> (8 * (w & 3))) & 0xFFu;
+    const unsigned *e = smask + (z - c.zc0 + F2_RB) * F2Geo<NW>::RW;
+    const unsigned mk = e[1 + NW / 64 + w];
+    const unsigned pre = (e[1 + (w >> 1)] >> (16 * (w & 1))) & 0xFFFFu;
     const int k = (int)(e[0] + pre) + __builtin_popcount(mk & ((1u << b) - 1u));
     return ((mk >> b) & 1u) ? k : -1;
 }
 
-template <typename R, int MODE>
+template <typename R, int MODE, int NW>
 __device__ __forceinline__ void f2_fetch(const F2Ctx &c, const unsigned *smask, const uint8_t *sev,
                                          const R *skz, R kx, int i, F2Row<R, MODE> &f)
 {
     const bool rv = c.colv && i >= -ZOFF && i < c.Z + 4;
-    const unsigned o = rv ? f2_off<R>(c, i) : F2_OOB;
-    f.ax = bload<R>(c.rAx0, o);
-    f.az = bload<R>(c.rAz0, o);
-    f.p = bload<R>(c.rP0, o);
+    const unsigned ro = f2_off<R>(c, i);
+    const unsigned o = rv ? ro : F2_OOB, oP = rv ? ro + c.dPi : F2_OOB,
+                   oA = rv ? ro + c.dAi : F2_OOB;
+    f.ax = bload<R>(c.rAx, oA);
+    f.az = bload<R>(c.rAz, oA);
+    f.p = bload<R>(c.rP, oP);
     f.m = bload<R>(c.rcm, o);
     const int i3 = i - 10;
     const bool rv3 = c.colv && i3 >= -ZOFF && i3 < c.Z + 4;
@@ -1122,21 +1155,21 @@ __device__ __forceinline__ void f2_fetch(const F2Ctx &c, const unsigned *smask, 
     f.e1 = f.e3 = F2Evt{0, 0, 0};
     if constexpr (MODE != F2_REC) {
         const bool damped = kx != R(1) || skz[i - c.zc0 + F2_RB] != R(1);
-        f.ps = bload<R>(c.rPx0, damped ? o : F2_OOB);
+        f.ps = bload<R>(c.rPx, damped ? oP : F2_OOB);
         f.e1 = f2_evt(c, sev, i - 3);
         f.e3 = f2_evt(c, sev, i3);
     }
     if constexpr (MODE != F2_LAM) {
-        f.k1 = f2_rslot(c, smask, i - 3);
-        f.k3 = f2_rslot(c, smask, i3);
+        f.k1 = f2_rslot<NW>(c, smask, i - 3);
+        f.k3 = f2_rslot<NW>(c, smask, i3);
     }
     if constexpr (MODE == F2_LAM) {
         const int z2 = i - 7, z4 = i - 14;
         const bool o2 = c.own && z2 >= c.zc0 && z2 < c.zc1;
         const bool o4 = c.own && z4 >= c.zc0 && z4 < c.zc1;
         f.g = bload<R>(c.rg, o2 ? f2_off<R>(c, z2) : F2_OOB);
-        f.l0 = bload<R>(c.rl0, o2 ? f2_off<R>(c, z2) : F2_OOB);
-        f.l1 = bload<R>(c.rl1, o4 ? f2_off<R>(c, z4) : F2_OOB);
+        f.l0 = bload<R>(c.rl, o2 ? f2_off<R>(c, z2) : F2_OOB);
+        f.l1 = bload<R>(c.rl, o4 ? f2_off<R>(c, z4) + c.dL1 : F2_OOB);
     }
     if constexpr (MODE == F2_REC) {
         // ring samples of the shot, time-major: ring times T-1-t and T-2-t
@@ -1208,20 +1241,22 @@ __device__ __forceinline__ void f2_a(R cA, R dPx, R dPz, R ax, R az, R kx, R kz,
 
 // (lap9 along x + lap9 along z) / 5040 as adj_step_kernel forms it: row =
 // the LDS row (9 taps around lc), w = the 9 rows (z-4 .. z+4) of this column
-template <typename R>
-__device__ __forceinline__ R f2_lap(const R *row, int lc, const R (&w)[9])
+template <typename R, int LP>
+__device__ __forceinline__ R f2_lap(R (*ring)[LP], int q, int lc, const R (&w)[9])
 {
     constexpr R inv5040 = R(1) / R(5040);
-    const R lx = lap9<R>(row[lc - 4], row[lc - 3], row[lc - 2], row[lc - 1], row[lc], row[lc + 1],
-                         row[lc + 2], row[lc + 3], row[lc + 4]);
+    R row[9];
+    lds_seg<R, 9, LP>(ring, q, lc - 4, row);
+    const R lx = lap9<R>(row[0], row[1], row[2], row[3], row[4], row[5], row[6], row[7], row[8]);
     const R lz = lap9<R>(w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8]);
     return (lx + lz) * inv5040;
 }
 
-template <typename R, int MODE, int PH>
+template <typename R, int MODE, int NW, int PH>
 __device__ __forceinline__ void f2_iter(const Pair2Args<R> &a, const F2Ctx &c,
-                                        Pair2Regs<R, MODE> &g, R (*sAx0)[F2_LP],
-                                        R (*sP1)[F2_LP], R (*sAx1)[F2_LP], R (*sP2)[F2_LP],
+                                        Pair2Regs<R, MODE> &g, R (*sAx0)[F2Geo<NW>::LP],
+                                        R (*sP1)[F2Geo<NW>::LP], R (*sAx1)[F2Geo<NW>::LP],
+                                        R (*sP2)[F2Geo<NW>::LP],
                                         const unsigned *smask, const uint8_t *sev, const R *skz,
                                         int i, R kx)
 {
@@ -1242,7 +1277,7 @@ __device__ __forceinline__ void f2_iter(const Pair2Args<R> &a, const F2Ctx &c,
     const int z1 = i - 3;
     R pn1, sn1;
     {
-        const R dAx = stag_lds<R>(sAx0[Q5], lc - 4);
+        const R dAx = stag_lds<R>(sAx0, Q5, lc - 4);
         const R dAz = stag<R>(g.az0[(PH + 1) & 7], g.az0[(PH + 2) & 7], g.az0[(PH + 3) & 7],
                               g.az0[(PH + 4) & 7], g.az0[(PH + 5) & 7], g.az0[(PH + 6) & 7],
                               g.az0[(PH + 7) & 7], g.az0[PH]);
@@ -1255,7 +1290,7 @@ __device__ __forceinline__ void f2_iter(const Pair2Args<R> &a, const F2Ctx &c,
     {
         const int z2 = i - 7;
         // P(t) rows z2-4 .. z2+4: p1[PH] .. p1[PH+7] hold rows i-11 .. i-4
-        const R dPx = stag_lds<R>(sP1[Q4], lc - 3);
+        const R dPx = stag_lds<R>(sP1, Q4, lc - 3);
         const R dPz = stag<R>(g.p1[(PH + 1) & 7], g.p1[(PH + 2) & 7], g.p1[(PH + 3) & 7],
                               g.p1[(PH + 4) & 7], g.p1[(PH + 5) & 7], g.p1[(PH + 6) & 7],
                               g.p1[(PH + 7) & 7], pn1);
@@ -1269,9 +1304,9 @@ __device__ __forceinline__ void f2_iter(const Pair2Args<R> &a, const F2Ctx &c,
             const R w[9] = {g.p1[PH], g.p1[(PH + 1) & 7], g.p1[(PH + 2) & 7], g.p1[(PH + 3) & 7],
                             g.p1[(PH + 4) & 7], g.p1[(PH + 5) & 7], g.p1[(PH + 6) & 7],
                             g.p1[(PH + 7) & 7], pn1};
-            const R lp = f2_lap<R>(sP1[Q4], lc, w);
+            const R lp = f2_lap<R>(sP1, Q4, lc, w);
             const bool o2 = c.own && z2 >= c.zc0 && z2 < c.zc1;
-            bstore<R>(c.rl0, o2 ? f2_off<R>(c, z2) : F2_OOB, lp);
+            bstore<R>(c.rl, o2 ? f2_off<R>(c, z2) : F2_OOB, lp);
         }
         g.p1[PH] = pn1;
         if constexpr (MODE != F2_REC) g.s1[PH] = sn1;
@@ -1280,7 +1315,7 @@ __device__ __forceinline__ void f2_iter(const Pair2Args<R> &a, const F2Ctx &c,
     const int z3 = i - 10;
     R pn2;
     {
-        const R dAx = stag_lds<R>(sAx1[Q5], lc - 4);
+        const R dAx = stag_lds<R>(sAx1, Q5, lc - 4);
         const R dAz = stag<R>(g.az1[(PH + 1) & 7], g.az1[(PH + 2) & 7], g.az1[(PH + 3) & 7],
                               g.az1[(PH + 4) & 7], g.az1[(PH + 5) & 7], g.az1[(PH + 6) & 7],
                               g.az1[(PH + 7) & 7], g.az1[PH]);
@@ -1290,16 +1325,16 @@ __device__ __forceinline__ void f2_iter(const Pair2Args<R> &a, const F2Ctx &c,
         pn2 = f2_p<R, MODE>(a, c, f.e3, f.r3, f.k3, c.xin && f2_zin(c, z3), c.t + 1, mine,
                             g.p1[Q1], g.s1[Q1], f.m3, kx, kz3, dAx, dAz, sn2);
         sP2[PH][lc] = pn2;
-        const unsigned o = mine ? f2_off<R>(c, z3) : F2_OOB;
-        bstore<R>(c.rP1, o, pn2);
+        const unsigned o = mine ? f2_off<R>(c, z3) + c.dPo : F2_OOB;
+        bstore<R>(c.rP, o, pn2);
         if constexpr (MODE != F2_REC)
-            bstore<R>(c.rPx1, (kx != R(1) || kz3 != R(1)) ? o : F2_OOB, sn2);
+            bstore<R>(c.rPx, (kx != R(1) || kz3 != R(1)) ? o : F2_OOB, sn2);
     }
     // ---- 5. A(t+1) on row z4 = i - 14, written out ----
     {
         const int z4 = i - 14;
         // P(t+1) rows z4-4 .. z4+4: p2[PH] .. p2[PH+7] hold rows i-18 .. i-11
-        const R dPx = stag_lds<R>(sP2[Q4], lc - 3);
+        const R dPx = stag_lds<R>(sP2, Q4, lc - 3);
         const R dPz = stag<R>(g.p2[(PH + 1) & 7], g.p2[(PH + 2) & 7], g.p2[(PH + 3) & 7],
                               g.p2[(PH + 4) & 7], g.p2[(PH + 5) & 7], g.p2[(PH + 6) & 7],
                               g.p2[(PH + 7) & 7], pn2);
@@ -1308,43 +1343,48 @@ __device__ __forceinline__ void f2_iter(const Pair2Args<R> &a, const F2Ctx &c,
                       c.xin && f2_zin(c, z4), nax, naz);
         const bool o4 = c.own && z4 >= c.zc0 && z4 < c.zc1;
         const unsigned o = o4 ? f2_off<R>(c, z4) : F2_OOB;
-        bstore<R>(c.rAx1, o, nax);
-        bstore<R>(c.rAz1, o, naz);
+        const unsigned oA = o4 ? f2_off<R>(c, z4) + c.dAo : F2_OOB;
+        bstore<R>(c.rAx, oA, nax);
+        bstore<R>(c.rAz, oA, naz);
         if constexpr (MODE == F2_LAM) bstore<R>(c.rg, o, fma(g.p2[Q4], f.l1, g.gq[Q1]));
         if constexpr (MODE == F2_REC) {
             const R w[9] = {g.p2[PH], g.p2[(PH + 1) & 7], g.p2[(PH + 2) & 7], g.p2[(PH + 3) & 7],
                             g.p2[(PH + 4) & 7], g.p2[(PH + 5) & 7], g.p2[(PH + 6) & 7],
                             g.p2[(PH + 7) & 7], pn2};
-            bstore<R>(c.rl1, o, f2_lap<R>(sP2[Q4], lc, w));
+            bstore<R>(c.rl, o4 ? f2_off<R>(c, z4) + c.dL1 : F2_OOB, f2_lap<R>(sP2, Q4, lc, w));
         }
         g.p2[PH] = pn2;
     }
     // ---- 6. loads for row i + F2_PF into the slot just consumed ----
-    f2_fetch<R, MODE>(c, smask, sev, skz, kx, i + F2_PF, f);
+    f2_fetch<R, MODE, NW>(c, smask, sev, skz, kx, i + F2_PF, f);
     lds_barrier();
 }
 
-// evrow (FWD, LAM): [ceil(X / F2_OWN)][Z] flags of the rows holding a source
-// or receiver in each block's 256 columns, with pmap the per-cell event map
-// of the padded plane, int4 {ring + 1, rx + 1, src + 1, 0} (0: none);
-// mask (FWD, REC): [..][Z][F2_RW] ring-cell tables as f2_rslot reads them.
+// evrow (FWD, LAM): [ceil(X / (NW - 32))][Z] flags of the rows holding a
+// source or receiver in each block's NW columns, with pmap the per-cell event
+// map of the padded plane, int4 {ring + 1, rx + 1, src + 1, 0} (0: none);
+// mask (FWD, REC): [..][Z][F2Geo<NW>::RW] ring-cell tables as f2_rslot reads.
 // Waves per SIMD the register allocation targets: work-groups of 4 waves,
-// three per CU (FWD 50 KB and REC 49 KB of LDS allow 3; LAM at 4 was slower).
-template <int MODE> constexpr int f2_waves() { return 3; }
+// three per CU in FP32 (FWD 50 KB and REC 49 KB of LDS allow 3; LAM at 4 was
+// slower); FP64 (validation) leaves the allocation unconstrained.
+template <typename R, int MODE, int NW>
+constexpr int f2_waves() { return sizeof(R) == 4 ? 3 : 1; }
 
-template <typename R, int MODE>
-__global__ void __launch_bounds__(256, f2_waves<MODE>()) pair2_kernel(const Pair2Args<R> a)
+template <typename R, int MODE, int NW>
+__global__ void __launch_bounds__(NW, (f2_waves<R, MODE, NW>())) pair2_kernel(const Pair2Args<R> a)
 {
-    constexpr int MW = MODE == F2_LAM ? 1 : F2_RW;   // ring-table words per row
+    using G = F2Geo<NW>;
+    constexpr int LP = G::LP;
+    constexpr int MW = MODE == F2_LAM ? 1 : G::RW;    // ring-table words per row
     constexpr int EV = MODE == F2_REC ? 4 : F2_NR;    // event-row flags
-    __shared__ R sAx0[8][F2_LP], sP1[8][F2_LP], sAx1[8][F2_LP], sP2[8][F2_LP];
+    __shared__ R sAx0[8][LP], sP1[8][LP], sAx1[8][LP], sP2[8][LP];
     __shared__ R skz[MODE == F2_REC ? 1 : F2_NR];
     __shared__ unsigned smask[F2_NR * MW];
     __shared__ uint8_t sev[EV];
     F2Ctx c;
     c.tid = threadIdx.x;
     c.X = a.X; c.Z = a.Z; c.ld = a.ld; c.t = a.t; c.T = a.T; c.nring = a.n_ring;
-    const int xs = blockIdx.x * F2_OWN;
+    const int xs = blockIdx.x * G::OWN;
     c.x = xs - F2_HALO + c.tid;
     c.zc0 = blockIdx.y * F2_CH;
     c.zc1 = min(c.zc0 + F2_CH, a.Z);
@@ -1353,55 +1393,59 @@ __global__ void __launch_bounds__(256, f2_waves<MODE>()) pair2_kernel(const Pair
     c.lc = c.tid + 8;
     // columns beyond X + 12 never reach an owned cell: not loaded
     c.colv = c.x < a.X + 12;
-    c.own = c.tid >= F2_HALO && c.tid < F2_HALO + F2_OWN && c.x < a.X;
+    c.own = c.tid >= F2_HALO && c.tid < F2_HALO + G::OWN && c.x < a.X;
     c.xin = c.x >= BOX && c.x < a.X - BOX;
     c.lo = (unsigned)(min(c.x, a.X + 11) + XOFF) * (unsigned)sizeof(R);
     c.lo4 = (unsigned)(min(c.x, a.X + 11) + XOFF) * 4u;
     const size_t sb = (size_t)c.s * (size_t)a.sstride;
     const unsigned pb = (unsigned)((size_t)a.sstride * sizeof(R));
-    c.rAx0 = f2_rsrc(a.Ax0 + sb, pb); c.rAz0 = f2_rsrc(a.Az0 + sb, pb);
-    c.rP0 = f2_rsrc(a.P0 + sb, pb);
-    c.rP1 = f2_rsrc(a.P1 + sb, pb);
-    c.rAx1 = f2_rsrc(a.Ax1 + sb, pb); c.rAz1 = f2_rsrc(a.Az1 + sb, pb);
+    // both slots of a field: slot 0 + slot_elems (the lower pointer is slot 0)
+    const unsigned fb = (unsigned)((size_t)(a.slot_elems + a.sstride) * sizeof(R));
+    const R *bP = a.P0 < a.P1 ? a.P0 : a.P1, *bA = a.Ax0 < a.Ax1 ? a.Ax0 : a.Ax1;
+    const R *bAz = a.Az0 < a.Az1 ? a.Az0 : a.Az1;
+    c.dPi = (unsigned)((size_t)(a.P0 - bP) * sizeof(R));
+    c.dPo = (unsigned)((size_t)(a.P1 - bP) * sizeof(R));
+    c.dAi = (unsigned)((size_t)(a.Ax0 - bA) * sizeof(R));
+    c.dAo = (unsigned)((size_t)(a.Ax1 - bA) * sizeof(R));
+    c.rP = f2_rsrc(bP + sb, fb);
+    c.rAx = f2_rsrc(bA + sb, fb);
+    c.rAz = f2_rsrc(bAz + sb, fb);
     c.rcm = f2_rsrc(a.cm, pb);
     if constexpr (MODE != F2_REC) {
-        c.rPx0 = f2_rsrc(a.Px0 + sb, pb);
-        c.rPx1 = f2_rsrc(a.Px1 + sb, pb);
+        c.rPx = f2_rsrc((a.Px0 < a.Px1 ? a.Px0 : a.Px1) + sb, fb);
         c.rpm = f2_rsrc(a.pmap, (unsigned)((size_t)a.sstride * 16));
     }
     if constexpr (MODE == F2_FWD)   // ring samples of this shot, time-major (none: no bytes)
         c.rring = f2_rsrc(a.ring ? a.ring + (size_t)c.s * a.T * a.n_ring : a.cm,
                           a.ring ? (unsigned)((size_t)a.T * a.n_ring * sizeof(R)) : 0u);
-    if constexpr (MODE == F2_LAM) {
-        c.rg = f2_rsrc(a.grad + sb, pb);
-        c.rl0 = f2_rsrc(a.lap0 + sb, pb);
-        c.rl1 = f2_rsrc(a.lap1 + sb, pb);
+    if constexpr (MODE != F2_FWD) {   // lap1 = lap0 + slot_elems
+        c.rl = f2_rsrc(a.lap0 + sb, fb);
+        c.dL1 = (unsigned)((size_t)a.slot_elems * sizeof(R));
     }
+    if constexpr (MODE == F2_LAM) c.rg = f2_rsrc(a.grad + sb, pb);
     if constexpr (MODE == F2_REC) {
-        c.rl0 = f2_rsrc(a.lap0 + sb, pb);
-        c.rl1 = f2_rsrc(a.lap1 + sb, pb);
         // ring samples of this shot, time-major
         c.rring = f2_rsrc(a.ring_in + (size_t)c.s * a.T * a.n_ring,
                           (unsigned)((size_t)a.T * a.n_ring * sizeof(R)));
     }
     const R kx = (MODE != F2_REC && c.colv) ? a.kx[XOFF + c.x] : R(1);
     if constexpr (MODE != F2_REC)
-        for (int r = c.tid; r < F2_NR; r += 256) {
+        for (int r = c.tid; r < F2_NR; r += NW) {
             const int z = c.zc0 - F2_RB + r;
             skz[r] = (z >= -ZOFF && z < a.Z + ZOFF) ? a.kz[ZOFF + z] : R(1);
         }
     if constexpr (MODE != F2_LAM)
-        for (int k = c.tid; k < F2_NR * MW; k += 256) {
+        for (int k = c.tid; k < F2_NR * MW; k += NW) {
             const int z = c.zc0 - F2_RB + k / MW;
             smask[k] = (z >= 0 && z < a.Z) ? a.mask[((size_t)blockIdx.x * a.Z + z) * MW + k % MW] : 0u;
         }
     if constexpr (MODE != F2_REC)
-        for (int r = c.tid; r < F2_NR; r += 256) {
+        for (int r = c.tid; r < F2_NR; r += NW) {
             const int z = c.zc0 - F2_RB + r;
             sev[r] = (z >= 0 && z < a.Z) ? a.evrow[(size_t)blockIdx.x * a.Z + z] : (uint8_t)0;
         }
     for (int r = 0; r < 8; r++)
-        for (int k = c.tid; k < F2_LP; k += 256) {
+        for (int k = c.tid; k < LP; k += NW) {
             sAx0[r][k] = R(0); sP1[r][k] = R(0); sAx1[r][k] = R(0); sP2[r][k] = R(0);
         }
     Pair2Regs<R, MODE> g;
@@ -1413,39 +1457,54 @@ __global__ void __launch_bounds__(256, f2_waves<MODE>()) pair2_kernel(const Pair
     __syncthreads();
     const int i0 = c.zc0 - F2_LEAD;
 #pragma unroll
-    for (int k = 0; k < F2_PF; k++) f2_fetch<R, MODE>(c, smask, sev, skz, kx, i0 + k, g.pf[k]);
+    for (int k = 0; k < F2_PF; k++) f2_fetch<R, MODE, NW>(c, smask, sev, skz, kx, i0 + k, g.pf[k]);
     const int nit = c.zc1 - c.zc0 + F2_LEAD + 14;
     for (int j0 = 0; j0 < nit; j0 += 8) {
         const int i = i0 + j0;
-        f2_iter<R, MODE, 0>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 0, kx);
-        f2_iter<R, MODE, 1>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 1, kx);
-        f2_iter<R, MODE, 2>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 2, kx);
-        f2_iter<R, MODE, 3>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 3, kx);
-        f2_iter<R, MODE, 4>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 4, kx);
-        f2_iter<R, MODE, 5>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 5, kx);
-        f2_iter<R, MODE, 6>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 6, kx);
-        f2_iter<R, MODE, 7>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 7, kx);
+        f2_iter<R, MODE, NW, 0>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 0, kx);
+        f2_iter<R, MODE, NW, 1>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 1, kx);
+        f2_iter<R, MODE, NW, 2>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 2, kx);
+        f2_iter<R, MODE, NW, 3>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 3, kx);
+        f2_iter<R, MODE, NW, 4>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 4, kx);
+        f2_iter<R, MODE, NW, 5>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 5, kx);
+        f2_iter<R, MODE, NW, 6>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 6, kx);
+        f2_iter<R, MODE, NW, 7>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 7, kx);
     }
 }
 
-bool pair2_supported(long long sstride, int prec)
+bool pair2_supported(long long sstride, long long slot_elems, int prec)
 {
-    // 32-bit buffer offsets: the event map (16 bytes per cell) must fit
-    return sstride > 0 && sstride * 16 < (long long)F2_OOB && sstride * prec < (long long)F2_OOB;
+    // 32-bit buffer offsets: the event map (16 bytes per cell) and both slots
+    // of a field seen from one shot must fit
+    return sstride > 0 && sstride * 16 < (long long)F2_OOB &&
+           (slot_elems + sstride) * prec < (long long)F2_OOB;
 }
 
 template <typename R>
 hipError_t launch_pair2(const Pair2Args<R> &a, int mode, hipStream_t st)
 {
     if ((mode != F2_LAM && !a.mask) || (mode != F2_REC && (!a.pmap || !a.evrow)) ||
-        !pair2_supported(a.sstride, (int)sizeof(R)))
+        !pair2_supported(a.sstride, a.slot_elems, (int)sizeof(R)))
+        return hipErrorInvalidValue;
+    // both slots of each field one allocation: the written slot is slot_elems away
+    auto gap = [&](const R *u, const R *v) { return u > v ? u - v : v - u; };
+    if (gap(a.P0, a.P1) != a.slot_elems || gap(a.Ax0, a.Ax1) != a.slot_elems ||
+        gap(a.Az0, a.Az1) != a.slot_elems || (a.Ax0 < a.Ax1) != (a.Az0 < a.Az1) ||
+        (mode != F2_FWD && a.lap1 - a.lap0 != a.slot_elems) ||
+        (mode != F2_REC && (gap(a.Px0, a.Px1) != a.slot_elems || (a.P0 < a.P1) != (a.Px0 < a.Px1))))
         return hipErrorInvalidValue;
     if (mode != F2_LAM && (size_t)a.T * a.n_ring * sizeof(R) >= F2_OOB) return hipErrorInvalidValue;
-    dim3 grid((a.X + F2_OWN - 1) / F2_OWN, (a.Z + F2_CH - 1) / F2_CH, a.nshots);
-    switch (mode) {
-    case F2_FWD: hipLaunchKernelGGL((pair2_kernel<R, F2_FWD>), grid, dim3(256), 0, st, a); break;
-    case F2_LAM: hipLaunchKernelGGL((pair2_kernel<R, F2_LAM>), grid, dim3(256), 0, st, a); break;
-    case F2_REC: hipLaunchKernelGGL((pair2_kernel<R, F2_REC>), grid, dim3(256), 0, st, a); break;
+    const int own = f2_own(a.nw);
+    if (own <= 0) return hipErrorInvalidValue;
+    dim3 grid((a.X + own - 1) / own, (a.Z + F2_CH - 1) / F2_CH, a.nshots);
+    const dim3 blk(a.nw);
+    switch (mode * 2 + (a.nw == 384 ? 1 : 0)) {
+    case 0: hipLaunchKernelGGL((pair2_kernel<R, F2_FWD, 256>), grid, blk, 0, st, a); break;
+    case 1: hipLaunchKernelGGL((pair2_kernel<R, F2_FWD, 384>), grid, blk, 0, st, a); break;
+    case 2: hipLaunchKernelGGL((pair2_kernel<R, F2_LAM, 256>), grid, blk, 0, st, a); break;
+    case 3: hipLaunchKernelGGL((pair2_kernel<R, F2_LAM, 384>), grid, blk, 0, st, a); break;
+    case 4: hipLaunchKernelGGL((pair2_kernel<R, F2_REC, 256>), grid, blk, 0, st, a); break;
+    case 5: hipLaunchKernelGGL((pair2_kernel<R, F2_REC, 384>), grid, blk, 0, st, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

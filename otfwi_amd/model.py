@@ -141,6 +141,10 @@ class PMLModel:
         check(self._lib.otfwi_get_adj_steps(self._h, ctypes.byref(v)))
         return int(v.value)
 
+    def set_pair_width(self, nw=0):
+        """Work-group width of the two-step kernels: 256, 384 or 0 (automatic)."""
+        check(self._lib.otfwi_set_pair_width(self._h, int(nw)))
+
     def set_steps_per_launch(self, fwd=2, adj=2):
         """Both loops at once (see set_fwd_steps, set_adj_steps)."""
         self.set_fwd_steps(fwd)
