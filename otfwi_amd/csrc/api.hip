@@ -1748,8 +1748,8 @@ int otfwi_get_fwd_steps(otfwi_model *m, int *steps)
 int otfwi_set_pair_width(otfwi_model *m, int nw)
 {
     RCK(check_model(m));
-    if (nw != 0 && nw != 256 && nw != 384)
-        return fail(OTFWI_E_ARG, "two-step block width must be 0 (automatic), 256 or 384");
+    if (nw != 0 && f2_own(nw) <= 0)
+        return fail(OTFWI_E_ARG, "two-step block width must be 0 (automatic), 128, 192, 256 or 384");
     if (nw != m->pair_nw) {
         m->pair_nw = nw;
         m->points_dirty = true;   // tables are rebuilt for the new width

@@ -130,13 +130,19 @@ hipError_t launch_adj_step(const AdjArgs<R> &a, int th, int persistent_wgs, hipS
 // 32-column word (16 bits each, nw / 64 words), and the nw / 32 lane masks of
 // the ring cells; ring slots are numbered in raster order (z, then x).
 constexpr int F2_HALO = 16;
-inline int f2_own(int nw) { return nw == 256 || nw == 384 ? nw - 2 * F2_HALO : 0; }
+inline int f2_own(int nw)
+{
+    return nw == 128 || nw == 192 || nw == 256 || nw == 384 ? nw - 2 * F2_HALO : 0;
+}
 inline int f2_ring_words(int nw) { return 1 + nw / 64 + nw / 32; }
 // Automatic block width: 256.  384-thread blocks waste fewer threads on
 // partial column blocks (1152 vs 1280 threads over 1024 columns) but ran
 // 1.3-1.45x slower per launch on MI355X at 64 x 1024^2: two 6-wave work-groups
 // per CU hide latency worse than three 4-wave ones.
-inline int f2_width(int X) { (void)X; return 256; }
+#ifndef F2_DEFAULT_NW
+#define F2_DEFAULT_NW 256
+#endif
+inline int f2_width(int X) { (void)X; return F2_DEFAULT_NW; }
 // The two slots of each field are one allocation: P1 - P0 = +-slot_elems
 // (likewise Ax, Az, Px), so one buffer resource per field covers both.
 template <typename R>

@@ -973,7 +973,7 @@ hipError_t launch_adj_step<double>(const AdjArgs<double> &a, int th, int pw, hip
 // the delayed A values read LDS rings of 8 rows written in earlier
 // iterations, so one LDS barrier per iteration orders everything.  The
 // 16-column halo shrinks by at most 4 columns per stage and ends 1 column
-// outside the owned ones.  Global loads run F2_PF rows ahead in registers.
+// outside the owned ones.  Global loads run f2_pf rows ahead in registers.
 // Every cell is updated with the arithmetic of the one-step kernels, so
 // results are bitwise equal; the state before the pair is read from one slot
 // and the state after it written to the other.
@@ -994,21 +994,33 @@ hipError_t launch_adj_step<double>(const AdjArgs<double> &a, int th, int pw, hip
 // and columns outside the plane, undamped psi cells, cells without an event
 // and lanes that do not own a cell need no branch.
 constexpr int F2_FWD = 0, F2_LAM = 1, F2_REC = 2;
-#ifndef F2_PREFETCH
-#define F2_PREFETCH 2   // rows of global loads in flight ahead of the row being computed
+// Rows of global loads in flight ahead of the row being computed, per mode
+// (measured at 64 x 1024^2: the reconstruction kernel gains 8 % at 4, the
+// forward kernel loses 90 % there and the lambda kernel nothing).
+#ifndef F2_PREFETCH_REC
+#define F2_PREFETCH_REC 4
 #endif
-constexpr int F2_CH = 256, F2_PF = F2_PREFETCH;
+#ifndef F2_PREFETCH_LAM
+#define F2_PREFETCH_LAM 2
+#endif
+#ifndef F2_PREFETCH
+#define F2_PREFETCH 2
+#endif
+constexpr int F2_CH = 256;
+template <int MODE>
+constexpr int f2_pf() { return MODE == 2 ? F2_PREFETCH_REC : MODE == 1 ? F2_PREFETCH_LAM : F2_PREFETCH; }
 // Block geometry for NW threads: owned columns, LDS row width (8 padding
 // cells each side), ring-table words per row (slot base, one 16-bit prefix
 // per 32-lane word, one lane mask per word).
 template <int NW>
 struct F2Geo {
-    static_assert(NW == 256 || NW == 384, "two-step blocks are 256 or 384 threads");
+    static_assert(NW == 128 || NW == 192 || NW == 256 || NW == 384,
+                  "two-step blocks are 128, 192, 256 or 384 threads");
     static constexpr int OWN = NW - 2 * F2_HALO, LP = NW + 16;
     static constexpr int WORDS = NW / 32, RW = 1 + NW / 64 + NW / 32;
 };
 // LDS row tables (kz, lane masks) cover rows [zc0 - F2_RB, zc0 - F2_RB + F2_NR):
-// iterations touch rows zc0 - 29 .. zc0 + F2_CH + 20 + F2_PF
+// iterations touch rows zc0 - 29 .. zc0 + F2_CH + 20 + prefetch (<= 4)
 constexpr int F2_RB = 32, F2_NR = F2_CH + 64;
 constexpr int F2_LEAD = 15;   // rows loaded above a chunk
 constexpr unsigned F2_OOB = 0xFFFFFFF0u;
@@ -1040,19 +1052,15 @@ template <> __device__ __forceinline__ void bstore<double>(Rsrc r, unsigned off,
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, off, 0, 0);
 }
 
-// LDS row segment ring[q][col ..]: the byte offset passes through an empty
-// asm, so the compiler forms one address per row and pairs the reads with
-// small immediate offsets (folding the row constant into each access would
-// need an add per pair of reads).
+// LDS row segment ring[q][col .. col + N).  (Hiding the row offset from the
+// compiler, so that it forms one address per row instead of one per pair of
+// reads, cut 15 vector instructions per iteration but made all three kernels
+// 2-5 % slower on MI355X; the plain form is kept.)
 template <typename R, int N, int LP>
 __device__ __forceinline__ void lds_seg(R (*ring)[LP], int q, int col, R (&v)[N])
 {
-    typedef const R __attribute__((address_space(3))) *LdsPtr;
-    unsigned o = (unsigned)(uintptr_t)(LdsPtr)&ring[q][col];
-    asm("" : "+v"(o));
-    const LdsPtr row = (LdsPtr)(uintptr_t)o;
 #pragma unroll
-    for (int k = 0; k < N; k++) v[k] = row[k];
+    for (int k = 0; k < N; k++) v[k] = ring[q][col + k];
 }
 
 template <typename R, int LP>
@@ -1068,7 +1076,7 @@ struct F2Evt {
     int ring1, rx1, src1;
 };
 
-// What iteration i needs from global memory, loaded F2_PF iterations ahead.
+// What iteration i needs from global memory, loaded f2_pf iterations ahead.
 template <typename R, int MODE>
 struct F2Row {
     R ax, az, p, ps, m;   // state before the pair and m on row i (ps: FWD, LAM)
@@ -1087,7 +1095,7 @@ struct Pair2Regs {
     R az1[8];                 // Az(t): rows i-14 .. i-7
     R p2[8];                  // P(t+1): rows i-18 .. i-11 (before this iteration's write)
     R gq[8];                  // LAM: grad after step t, rows i-14 .. i-7
-    F2Row<R, MODE> pf[F2_PF];
+    F2Row<R, MODE> pf[f2_pf<MODE>()];
 };
 
 struct F2Ctx {
@@ -1261,7 +1269,7 @@ __device__ __forceinline__ void f2_iter(const Pair2Args<R> &a, const F2Ctx &c,
                                         int i, R kx)
 {
     constexpr int Q1 = (PH + 1) & 7, Q4 = (PH + 4) & 7, Q5 = (PH + 5) & 7;
-    constexpr int F = PH % F2_PF;
+    constexpr int F = PH % f2_pf<MODE>();
     constexpr int H = PH & 3, H1 = (PH + 1) & 3;   // 4-row rings: newest, 3 rows back
     const int lc = c.lc;
     auto kz_at = [&](int z) -> R {
@@ -1355,8 +1363,8 @@ __device__ __forceinline__ void f2_iter(const Pair2Args<R> &a, const F2Ctx &c,
         }
         g.p2[PH] = pn2;
     }
-    // ---- 6. loads for row i + F2_PF into the slot just consumed ----
-    f2_fetch<R, MODE, NW>(c, smask, sev, skz, kx, i + F2_PF, f);
+    // ---- 6. loads for the row f2_pf ahead into the slot just consumed ----
+    f2_fetch<R, MODE, NW>(c, smask, sev, skz, kx, i + f2_pf<MODE>(), f);
     lds_barrier();
 }
 
@@ -1457,7 +1465,8 @@ __global__ void __launch_bounds__(NW, (f2_waves<R, MODE, NW>())) pair2_kernel(co
     __syncthreads();
     const int i0 = c.zc0 - F2_LEAD;
 #pragma unroll
-    for (int k = 0; k < F2_PF; k++) f2_fetch<R, MODE, NW>(c, smask, sev, skz, kx, i0 + k, g.pf[k]);
+    for (int k = 0; k < f2_pf<MODE>(); k++)
+        f2_fetch<R, MODE, NW>(c, smask, sev, skz, kx, i0 + k, g.pf[k]);
     const int nit = c.zc1 - c.zc0 + F2_LEAD + 14;
     for (int j0 = 0; j0 < nit; j0 += 8) {
         const int i = i0 + j0;
@@ -1498,15 +1507,23 @@ hipError_t launch_pair2(const Pair2Args<R> &a, int mode, hipStream_t st)
     if (own <= 0) return hipErrorInvalidValue;
     dim3 grid((a.X + own - 1) / own, (a.Z + F2_CH - 1) / F2_CH, a.nshots);
     const dim3 blk(a.nw);
-    switch (mode * 2 + (a.nw == 384 ? 1 : 0)) {
-    case 0: hipLaunchKernelGGL((pair2_kernel<R, F2_FWD, 256>), grid, blk, 0, st, a); break;
-    case 1: hipLaunchKernelGGL((pair2_kernel<R, F2_FWD, 384>), grid, blk, 0, st, a); break;
-    case 2: hipLaunchKernelGGL((pair2_kernel<R, F2_LAM, 256>), grid, blk, 0, st, a); break;
-    case 3: hipLaunchKernelGGL((pair2_kernel<R, F2_LAM, 384>), grid, blk, 0, st, a); break;
-    case 4: hipLaunchKernelGGL((pair2_kernel<R, F2_REC, 256>), grid, blk, 0, st, a); break;
-    case 5: hipLaunchKernelGGL((pair2_kernel<R, F2_REC, 384>), grid, blk, 0, st, a); break;
+#define F2_LAUNCH(M, W) hipLaunchKernelGGL((pair2_kernel<R, M, W>), grid, blk, 0, st, a)
+#define F2_WIDTHS(M)                                      \
+    switch (a.nw) {                                       \
+    case 128: F2_LAUNCH(M, 128); break;                   \
+    case 192: F2_LAUNCH(M, 192); break;                   \
+    case 256: F2_LAUNCH(M, 256); break;                   \
+    case 384: F2_LAUNCH(M, 384); break;                   \
+    default: return hipErrorInvalidValue;                 \
+    }
+    switch (mode) {
+    case F2_FWD: F2_WIDTHS(F2_FWD); break;
+    case F2_LAM: F2_WIDTHS(F2_LAM); break;
+    case F2_REC: F2_WIDTHS(F2_REC); break;
     default: return hipErrorInvalidValue;
     }
+#undef F2_WIDTHS
+#undef F2_LAUNCH
     return hipGetLastError();
 }
 template hipError_t launch_pair2<float>(const Pair2Args<float> &, int, hipStream_t);

@@ -67,12 +67,12 @@ def test_fwd2_rejects_bad_value(gpu):
 
 @pytest.mark.parametrize("X,Z,T", [(700, 300, 121), (1100, 270, 60)])
 def test_pair_width_bitwise(gpu, X, Z, T):
-    """256- and 384-thread two-step blocks give the one-step results."""
+    """Every two-step block width gives the one-step results."""
     p = small_problem(X=X, Z=Z, T=T, pml=10, seed=X + T, n_src=5, n_rx=40)
     shots = [4, 0, 2]
     obs = _obs(p, shots)
     a = _run(p, shots, obs, 1, 4, w2=False)
-    for nw in (256, 384, 0):
+    for nw in (128, 192, 256, 384, 0):
         m = make_model(p, cq="cq_bg")
         m.set_pair_width(nw)
         m.set_steps_per_launch(2, 2)
@@ -81,4 +81,4 @@ def test_pair_width_bitwise(gpu, X, Z, T):
         g, mis = m.gradient(shots, "l2")
         assert np.array_equal(g, a["g_l2"]) and np.array_equal(mis, a["mis_l2"]), nw
     with pytest.raises(Exception):
-        m.set_pair_width(128)
+        m.set_pair_width(320)
