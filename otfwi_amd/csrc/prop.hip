@@ -1006,6 +1006,10 @@ constexpr int F2_FWD = 0, F2_LAM = 1, F2_REC = 2;
 #ifndef F2_PREFETCH
 #define F2_PREFETCH 2
 #endif
+#ifndef F2_BARRIER
+#define F2_BARRIER 2
+#endif
+static_assert(F2_BARRIER == 1 || F2_BARRIER == 2, "LDS ring hazards allow a barrier every 1 or 2 rows");
 constexpr int F2_CH = 256;
 template <int MODE>
 constexpr int f2_pf() { return MODE == 2 ? F2_PREFETCH_REC : MODE == 1 ? F2_PREFETCH_LAM : F2_PREFETCH; }
@@ -1365,7 +1369,11 @@ __device__ __forceinline__ void f2_iter(const Pair2Args<R> &a, const F2Ctx &c,
     }
     // ---- 6. loads for the row f2_pf ahead into the slot just consumed ----
     f2_fetch<R, MODE, NW>(c, smask, sev, skz, kx, i + f2_pf<MODE>(), f);
-    lds_barrier();
+    // Cross-lane LDS reads are of rows written 3 or 4 iterations earlier, and
+    // a row slot is rewritten 8 iterations after it was written (4 after its
+    // last cross-lane read); reads of a lane's own column stay in its wave.
+    // A barrier every F2_BARRIER iterations (1 or 2) orders all of them.
+    if constexpr ((PH + 1) % F2_BARRIER == 0) lds_barrier();
 }
 
 // evrow (FWD, LAM): [ceil(X / (NW - 32))][Z] flags of the rows holding a
