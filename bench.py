@@ -289,9 +289,9 @@ def main():
     if ap_:
         unit_s = (adj_phase_s - a1 * ktimes["adj"] / 1e3) / ap_
         unit_bytes = 2 * ADJ_BYTES * cells
-        unit_name = (f"adjoint step pair: pair2_kernel<float, 2> (reconstruction) + "
-                     f"pair2_kernel<float, 1> (lambda, gradient)")
-        traffic = sum_traffic(["pair2_kernel<float,2>", "pair2_kernel<float,1>"], N, S)
+        unit_name = (f"adjoint step pair: pair2_kernel<float, 2, 256> (reconstruction) + "
+                     f"pair2_kernel<float, 1, 256> (lambda, gradient)")
+        traffic = sum_traffic(["pair2_kernel<float,2,256>", "pair2_kernel<float,1,256>"], N, S)
         split = {"rec_pair_ms": ktimes["rec_pair"], "lam_pair_ms": ktimes["lam_pair"]}
     else:
         unit_s = adj_phase_s / (T - 1)
@@ -302,8 +302,8 @@ def main():
     if fp:
         fwd_unit_s = (fwd_phase_s - f1 * ktimes["fwd"] / 1e3) / fp
         fwd_bytes = 2 * FWD_BYTES * cells
-        fwd_name = "pair2_kernel<float, 0>"
-        fwd_traffic = sum_traffic(["pair2_kernel<float,0>"], N, S)
+        fwd_name = "pair2_kernel<float, 0, 256>"
+        fwd_traffic = sum_traffic(["pair2_kernel<float,0,256>"], N, S)
     else:
         fwd_unit_s = fwd_phase_s / (T - 1)
         fwd_bytes = FWD_BYTES * cells

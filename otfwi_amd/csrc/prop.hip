@@ -1010,7 +1010,10 @@ constexpr int F2_FWD = 0, F2_LAM = 1, F2_REC = 2;
 #define F2_BARRIER 2
 #endif
 static_assert(F2_BARRIER == 1 || F2_BARRIER == 2, "LDS ring hazards allow a barrier every 1 or 2 rows");
-constexpr int F2_CH = 256;
+#ifndef F2_CHUNK
+#define F2_CHUNK 256
+#endif
+constexpr int F2_CH = F2_CHUNK;   // output rows per work-group
 template <int MODE>
 constexpr int f2_pf() { return MODE == 2 ? F2_PREFETCH_REC : MODE == 1 ? F2_PREFETCH_LAM : F2_PREFETCH; }
 // Block geometry for NW threads: owned columns, LDS row width (8 padding
