@@ -252,7 +252,7 @@ def main():
     obs = m.forward(shots)
     m.set_observed(obs)
     m.set_cquad((cfg["c_bg"] ** 2).astype(np.float32))
-    fs, ads = m.get_fwd_steps(), m.get_adj_steps()
+    fs, ads = m.steps_per_launch(S)
     print(f"[bench] device {_lib.device_info()}; {S} shots {N}x{N} T={T}; tiles {m.get_tuning()}"
           f" persistent {m.get_launch()} autotune {tune}; steps per launch fwd {fs} adj {ads};"
           f" kernel ms {ktimes}", file=sys.stderr, flush=True)

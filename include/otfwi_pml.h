@@ -146,24 +146,26 @@ int otfwi_set_blocking(otfwi_model *m, int steps_fwd, int shots_fwd, int steps_a
                        int shots_adj);
 int otfwi_get_blocking(otfwi_model *m, int *out);
 
-/* Forward time steps per launch: 1 (default) or 2.  With 2, the forward loop
- * advances pairs of steps in one launch of a z-marching kernel that keeps the
- * intermediate state on chip (the first steps of a range and its last step
- * stay single launches, so the buffers of the adjoint's seeds are where a
- * step-per-launch loop leaves them).  Results are bitwise identical to 1.  It
- * applies to forward loops without illumination or cache blocking.  No
- * reference counterpart (regression_PML_single.cu:745 launches one forward
- * step at a time). */
+/* Forward time steps per launch: 1, 2, or 0 (default: 2 when a two-step
+ * launch has at least 2 work-groups per CU, 1 otherwise).  With 2, the
+ * forward loop advances pairs of steps in one launch of a z-marching kernel
+ * that keeps the intermediate state on chip (the first steps of a range and
+ * its last step stay single launches, so the buffers of the adjoint's seeds
+ * are where a step-per-launch loop leaves them).  Results are bitwise
+ * identical for every setting.  Pairs apply to forward loops without
+ * illumination or cache blocking.  No reference counterpart
+ * (regression_PML_single.cu:745 launches one forward step at a time). */
 int otfwi_set_fwd_steps(otfwi_model *m, int steps);
 int otfwi_get_fwd_steps(otfwi_model *m, int *steps);
 
-/* Adjoint time steps per launch pair: 1 (default) or 2.  With 2, the
+/* Adjoint time steps per launch pair: 1, 2, or 0 (default: 2 when a two-step
+ * launch has at least 1.5 work-groups per CU, 1 otherwise).  With 2, the
  * adjoint loop advances pairs of steps with two z-marching kernels: one
  * reconstructs the forward field over both steps (boundary ring written
  * back) and writes the Laplacian of both reconstructed states, the next
  * advances the adjoint field over both steps and adds both gradient terms.
- * Results are bitwise identical to 1; two extra planes per shot hold the
- * Laplacians.  It applies to adjoint loops without cache blocking.  No
+ * Results are bitwise identical for every setting; two extra planes per shot
+ * hold the Laplacians.  It applies to adjoint loops without cache blocking.  No
  * reference counterpart (regression_PML_single.cu:793 launches one adjoint
  * step at a time). */
 int otfwi_set_adj_steps(otfwi_model *m, int steps);
@@ -173,6 +175,12 @@ int otfwi_get_adj_steps(otfwi_model *m, int *steps);
  * or 352 columns each), or 0 (default, 256).  Results do not depend on it;
  * 384 was slower on MI355X at 1024^2.  No reference counterpart. */
 int otfwi_set_pair_width(otfwi_model *m, int nw);
+
+/* The steps per launch (1 or 2) the forward and adjoint loops use for a
+ * batch of n_shots shots under the current settings (without illumination
+ * or cache blocking, which keep the forward at one step).  No reference
+ * counterpart. */
+int otfwi_steps_per_launch(otfwi_model *m, int n_shots, int *fwd, int *adj);
 
 /* Replay the per-step launch loops as captured hipGraphs (default 1); a
  * graph is re-captured only when shots, tiles, launch style or device buffers

@@ -114,7 +114,9 @@ struct otfwi_model {
     uint8_t *evrow = nullptr;
     unsigned *ringtab = nullptr;
     int *pmap = nullptr;
-    int fwd_steps = 2, adj_steps = 2;   // time steps per forward / adjoint launch (1 or 2)
+    // time steps per forward / adjoint launch: 1, 2, or 0 = by launch size
+    // (two-step launches from pair_groups_fwd / _adj work-groups per CU)
+    int fwd_steps = 0, adj_steps = 0;
     int pair_nw = 0;                 // two-step block width requested (0: fewest threads)
     int pair_nw_built = 256;         // the width the two-step tables were built for
     bool points_dirty = true;
@@ -500,6 +502,21 @@ int ready(otfwi_model *m, bool need_ring)
 
 // Work-groups of a persistent launch: every CU filled to the kernel's
 // occupancy, rounded down to a multiple of the 8 XCDs (0 = per-tile launch).
+// Two steps per launch on this batch?  Explicit 1 / 2, or by size: a
+// two-step work-group marches 285 rows in sequence, so a launch needs enough
+// of them to fill the CUs (measured on MI355X, 64-step runs: at 64-320
+// work-groups one step per launch is 1.05-8x faster; at 640 and more two
+// steps are 1.2-1.6x faster; the adjoint's crossover is lower than the
+// forward's).
+bool use_pairs(const otfwi_model *m, int n, bool adj)
+{
+    const int want = adj ? m->adj_steps : m->fwd_steps;
+    if (want != 0) return want == 2;
+    const long long wg = pair2_groups(m->X, m->Z, m->pair_nw_built, n);
+    const long long cu = std::max(m->n_cu, 1);
+    return adj ? 2 * wg >= 3 * cu : wg >= 2 * cu;
+}
+
 int persistent_wgs(const otfwi_model *m, bool adj)
 {
     if (!(adj ? m->pers_adj : m->pers_fwd) || m->prec == 8) return 0;
@@ -679,7 +696,7 @@ int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum)
     // count leaves state t in slot t&1 again), then single steps -- the last
     // step is always single, so A(t1-2) stays in slot (t1-2)&1 for the
     // adjoint's reconstruction
-    const bool two = m->fwd_steps == 2 && ist == 0 && K == 1 && G == n && m->evrow &&
+    const bool two = use_pairs(m, n, false) && ist == 0 && K == 1 && G == n && m->evrow &&
                      pair2_supported((long long)plane, (long long)b.cap * (long long)plane, (int)sizeof(R));
     auto body2 = [&]() -> int {
         const int S = t1 - t0;
@@ -804,7 +821,7 @@ int adj_loop(otfwi_model *m, int n, int t0, int t1)
     // adds both gradient terms.  Leading single steps, then an even number of
     // pairs (each writes the slots it does not read), so the slots end where
     // one step per launch leaves them.
-    const bool two = m->adj_steps == 2 && K == 1 && G == n && m->evrow && m->ringtab &&
+    const bool two = use_pairs(m, n, true) && K == 1 && G == n && m->evrow && m->ringtab &&
                      pair2_supported((long long)plane, (long long)b.cap * (long long)plane, (int)sizeof(R));
     if (two) RCK(ensure_lap<R>(m));
     auto single = [&](int t) -> int {
@@ -1732,7 +1749,8 @@ int otfwi_set_blocking(otfwi_model *m, int steps_fwd, int shots_fwd, int steps_a
 int otfwi_set_fwd_steps(otfwi_model *m, int steps)
 {
     RCK(check_model(m));
-    if (steps != 1 && steps != 2) return fail(OTFWI_E_ARG, "steps per forward launch must be 1 or 2");
+    if (steps < 0 || steps > 2)
+        return fail(OTFWI_E_ARG, "steps per forward launch must be 0 (by size), 1 or 2");
     m->fwd_steps = steps;
     return OTFWI_OK;
 }
@@ -1742,6 +1760,19 @@ int otfwi_get_fwd_steps(otfwi_model *m, int *steps)
     RCK(check_model(m));
     if (!steps) return fail(OTFWI_E_ARG, "output is NULL");
     *steps = m->fwd_steps;
+    return OTFWI_OK;
+}
+
+int otfwi_steps_per_launch(otfwi_model *m, int n_shots, int *fwd, int *adj)
+{
+    RCK(check_model(m));
+    if (n_shots <= 0 || !fwd || !adj) return fail(OTFWI_E_ARG, "steps_per_launch: bad arguments");
+    RCK(ready(m, false));
+    const long long plane = (long long)m->zp * m->ld;
+    const bool ok = pair2_supported(plane, (long long)std::max(m->b.cap, n_shots) * plane,
+                                    m->prec);
+    *fwd = ok && use_pairs(m, n_shots, false) ? 2 : 1;
+    *adj = ok && use_pairs(m, n_shots, true) ? 2 : 1;
     return OTFWI_OK;
 }
 
@@ -1760,7 +1791,8 @@ int otfwi_set_pair_width(otfwi_model *m, int nw)
 int otfwi_set_adj_steps(otfwi_model *m, int steps)
 {
     RCK(check_model(m));
-    if (steps != 1 && steps != 2) return fail(OTFWI_E_ARG, "steps per adjoint launch must be 1 or 2");
+    if (steps < 0 || steps > 2)
+        return fail(OTFWI_E_ARG, "steps per adjoint launch must be 0 (by size), 1 or 2");
     m->adj_steps = steps;
     return OTFWI_OK;
 }

@@ -167,6 +167,8 @@ struct Pair2Args {
     const int *pmap;
 };
 bool pair2_supported(long long sstride, long long slot_elems, int prec);
+// Work-groups of one two-step launch over nshots shots.
+long long pair2_groups(int X, int Z, int nw, int nshots);
 template <typename R>
 hipError_t launch_pair2(const Pair2Args<R> &a, int mode, hipStream_t s);
 // Resident work-groups per CU of a step kernel (0 if the variant does not exist).

@@ -1492,6 +1492,12 @@ __global__ void __launch_bounds__(NW, (f2_waves<R, MODE, NW>())) pair2_kernel(co
     }
 }
 
+long long pair2_groups(int X, int Z, int nw, int nshots)
+{
+    const int own = f2_own(nw);
+    return own > 0 ? (long long)((X + own - 1) / own) * ((Z + F2_CH - 1) / F2_CH) * nshots : 0;
+}
+
 bool pair2_supported(long long sstride, long long slot_elems, int prec)
 {
     // 32-bit buffer offsets: the event map (16 bytes per cell) and both slots

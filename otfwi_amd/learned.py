@@ -170,6 +170,53 @@ def step_pairs(prob: Problem, net, seeds, log_every=100, log=print):
     return X, Y, time.perf_counter() - t0
 
 
+def chain_pairs(prob: Problem, net, seeds, steps=3, scales=(1.0, 0.5), log_every=100, log=print):
+    """Pairs along the network's own step chain: from the background, take up
+    to `steps` network steps, each kept (at the first of `scales` that
+    lowers the W2 misfit, as inversion.FWI.run keeps learned steps) or ending
+    the chain; every state reached gives (features there, (c_true - c) /
+    C_SCALE)."""
+    X, Y = [], []
+    step = learned_step(net, prob.mask)
+    t0 = time.perf_counter()
+    for k, s in enumerate(seeds):
+        c_true = prob.model(int(s))
+        prob.observe(c_true)
+        c = prob.c_bg.copy()
+        g, ill, J = prob.gradient(c)
+        for _ in range(steps):
+            dc = step(g, ill, c)
+            for sc in scales:
+                ct = np.clip(c + sc * dc, prob.vmin, prob.vmax)
+                gt, illt, Jt = prob.gradient(ct)
+                if Jt < J:
+                    break
+            else:
+                break
+            c, g, ill, J = ct, gt, illt, Jt
+            X.append(features(g, ill, prob.mask))
+            Y.append(((c_true - c) * prob.mask / C_SCALE).astype(np.float32))
+        if log and log_every and (k + 1) % log_every == 0:
+            log(f"[chain] {k + 1}/{len(seeds)}: {len(X)} pairs in {time.perf_counter() - t0:.1f} s")
+    N = prob.N
+    X = np.stack(X) if X else np.zeros((0, 2, N, N), np.float32)
+    Y = np.stack(Y) if Y else np.zeros((0, N, N), np.float32)
+    return X, Y, time.perf_counter() - t0
+
+
+def refine_recipe(prob: Problem, net, n_first=1000, n_chain=1000, steps=3, epochs=15, lr=5e-5,
+                  batch=8, seed=2, log=print):
+    """Fine-tune a trained network on first-step pairs (seeds 0..n_first-1)
+    plus pairs along its own step chains (seeds n_first..n_first+n_chain-1).
+    Returns (net, info)."""
+    X1, Y1, t1 = make_pairs(prob, range(n_first), rng_seed=seed, log=log)
+    X2, Y2, t2 = chain_pairs(prob, net, range(n_first, n_first + n_chain), steps=steps, log=log)
+    h, tt = train(net, np.concatenate([X1, X2]), np.concatenate([Y1, Y2]), epochs, batch=batch,
+                  lr=lr, seed=seed, log=log)
+    return net, dict(first_pairs=n_first, chain_seeds=n_chain, chain_pairs=len(X2),
+                     chain_steps=steps, epochs=epochs, lr=lr, t_pairs=t1 + t2, t_train=tt, loss=h)
+
+
 def save_weights(net, path=WEIGHTS):
     """Weights as BF16 bit patterns (what the kernels compute with), biases
     and the head in FP32."""
@@ -288,7 +335,7 @@ def summarize(res, tol=0.02):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description="train / evaluate the learned preconditioner")
-    ap.add_argument("cmd", choices=["train", "compare"])
+    ap.add_argument("cmd", choices=["train", "refine", "compare"])
     ap.add_argument("--pairs", type=int, default=2000)
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--stage2", type=int, default=1000)
@@ -298,6 +345,10 @@ def main(argv=None):
     ap.add_argument("--models", type=int, default=4)
     ap.add_argument("--iters", type=int, default=30)
     ap.add_argument("--out", default=None, help="JSON summary file")
+    ap.add_argument("--chain", type=int, default=1000, help="refine: seeds of step-chain pairs")
+    ap.add_argument("--chain-steps", type=int, default=3)
+    ap.add_argument("--first", type=int, default=1000, help="refine: first-step pairs")
+    ap.add_argument("--save", default=None, help="refine: write the refined weights here")
     a = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)  # noqa: E731
     prob = Problem()
@@ -307,6 +358,16 @@ def main(argv=None):
         save_weights(net, a.weights)
         summary["train"] = info
         with open(os.path.splitext(a.weights)[0] + ".json", "w") as f:
+            json.dump({k: v for k, v in info.items() if not k.startswith("loss")}, f, indent=1)
+    elif a.cmd == "refine":
+        from .unet import UNet
+        net = UNet(params=load_weights(a.weights))
+        net, info = refine_recipe(prob, net, a.first, a.chain, a.chain_steps, a.epochs2,
+                                  a.lr, log=log)
+        out_w = a.save or a.weights
+        save_weights(net, out_w)
+        summary["refine"] = info
+        with open(os.path.splitext(out_w)[0] + ".refine.json", "w") as f:
             json.dump({k: v for k, v in info.items() if not k.startswith("loss")}, f, indent=1)
     else:
         from .unet import UNet

@@ -115,7 +115,8 @@ class PMLModel:
         return tuple(int(v) for v in out)
 
     def set_fwd_steps(self, steps):
-        """Forward time steps per launch, 1 or 2 (see otfwi_set_fwd_steps)."""
+        """Forward time steps per launch: 1, 2 or 0 = by launch size (see
+        otfwi_set_fwd_steps)."""
         check(self._lib.otfwi_set_fwd_steps(self._h, int(steps)))
 
     def get_fwd_steps(self):
@@ -133,13 +134,21 @@ class PMLModel:
         return {k: (float(v) if v >= 0 else None) for k, v in zip(keys, ms)}
 
     def set_adj_steps(self, steps):
-        """Adjoint time steps per launch pair, 1 or 2 (see otfwi_set_adj_steps)."""
+        """Adjoint time steps per launch pair: 1, 2 or 0 = by launch size (see
+        otfwi_set_adj_steps)."""
         check(self._lib.otfwi_set_adj_steps(self._h, int(steps)))
 
     def get_adj_steps(self):
         v = ctypes.c_int()
         check(self._lib.otfwi_get_adj_steps(self._h, ctypes.byref(v)))
         return int(v.value)
+
+    def steps_per_launch(self, n_shots):
+        """(forward, adjoint) steps per launch used for a batch of n_shots."""
+        f, a = ctypes.c_int(), ctypes.c_int()
+        check(self._lib.otfwi_steps_per_launch(self._h, int(n_shots), ctypes.byref(f),
+                                               ctypes.byref(a)))
+        return int(f.value), int(a.value)
 
     def set_pair_width(self, nw=0):
         """Work-group width of the two-step kernels: 256, 384 or 0 (automatic)."""

@@ -57,12 +57,12 @@ def test_fwd2_bitwise(gpu, X, Z, T, precision):
 def test_fwd2_rejects_bad_value(gpu):
     p = small_problem()
     m = make_model(p)
-    assert m.get_fwd_steps() == 2 and m.get_adj_steps() == 2     # default: two-step launches
+    assert m.get_fwd_steps() == 0 and m.get_adj_steps() == 0     # default: by launch size
     with pytest.raises(Exception):
         m.set_fwd_steps(3)
     with pytest.raises(Exception):
-        m.set_adj_steps(0)
-    assert m.get_fwd_steps() == 2 and m.get_adj_steps() == 2
+        m.set_adj_steps(-1)
+    assert m.get_fwd_steps() == 0 and m.get_adj_steps() == 0
 
 
 @pytest.mark.parametrize("X,Z,T", [(700, 300, 121), (1100, 270, 60)])
