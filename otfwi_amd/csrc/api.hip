@@ -512,6 +512,7 @@ bool use_pairs(const otfwi_model *m, int n, bool adj)
 {
     const int want = adj ? m->adj_steps : m->fwd_steps;
     if (want != 0) return want == 2;
+    if (m->prec != 4) return false;   // the FP64 two-step kernels run at 1 wave per SIMD
     const long long wg = pair2_groups(m->X, m->Z, m->pair_nw_built, n);
     const long long cu = std::max(m->n_cu, 1);
     return adj ? 2 * wg >= 3 * cu : wg >= 2 * cu;

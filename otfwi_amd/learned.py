@@ -275,11 +275,11 @@ def learned_step(net, mask):
     return step
 
 
-def compare(prob: Problem, net, seeds, n_iter=30, tol=0.02, log=print):
+def compare(prob: Problem, net, seeds, n_iter=30, tol=0.02, log=print, **run_kw):
     """Plain vs learned inversion on held-out models.  Returns a list of
     per-model dicts with the velocity-error histories and times to tol;
     "matched" is the time each method needs to reach the plain method's
-    final error."""
+    final error.  run_kw (patience, learned_scales) go to the learned runs."""
     out = []
     for s in seeds:
         c_true = prob.model(int(s))
@@ -288,7 +288,8 @@ def compare(prob: Problem, net, seeds, n_iter=30, tol=0.02, log=print):
         for kind in ("plain", "learned"):
             fwi = prob.fwi(c_true=c_true)
             _, h = fwi.run(prob.c_bg, n_iter=n_iter, tol=tol,
-                           learned=learned_step(net, prob.mask) if kind == "learned" else None)
+                           learned=learned_step(net, prob.mask) if kind == "learned" else None,
+                           **(run_kw if kind == "learned" else {}))
             res[kind] = dict(time_to_tol=h.time_to(tol), best_err=h.best_err,
                              err0=h.records[0].err, n_grad=h.n_grad,
                              iters=len(h.records) - 1,
