@@ -82,3 +82,26 @@ def test_pair_width_bitwise(gpu, X, Z, T):
         assert np.array_equal(g, a["g_l2"]) and np.array_equal(mis, a["mis_l2"]), nw
     with pytest.raises(Exception):
         m.set_pair_width(320)
+
+
+def test_pairs_batches_and_all_sources(gpu):
+    """Two-step launches with the shots split over batches (the last one
+    smaller than the buffers' capacity) and with every source firing at
+    once: bitwise equal to one step per launch."""
+    p = small_problem(X=480, Z=300, T=130, pml=10, seed=7, n_src=5, n_rx=30)
+    shots = [0, 1, 2, 3, 4]
+    obs = _obs(p, shots)
+    out = []
+    for steps in (1, 2):
+        m = make_model(p, cq="cq_bg", max_batch=2)
+        m.set_steps_per_launch(steps, steps)
+        m.set_observed(obs)
+        r = m.forward(shots)
+        g, mis = m.gradient(shots, "l2")
+        m2 = make_model(p, cq="cq_bg")
+        m2.set_steps_per_launch(steps, steps)
+        r_all = m2.forward([-1, 2, -1])
+        out.append((r, g, mis, r_all))
+    for u, v in zip(*out):
+        assert np.array_equal(u, v)
+    assert np.abs(out[0][3]).max() > 0
