@@ -25,6 +25,12 @@
 //   3. owned phase: A(t) on the tile.
 // The per-step HBM traffic is the algorithmic minimum (read P, Ax, Az, m;
 // write P, Ax, Az) plus halo re-reads that hit in L2.
+//
+// The two-step kernels (pair2_kernel, at the end of this file) advance two
+// time steps per launch with z-marching work-groups that keep the
+// intermediate state on chip, for the forward field, the adjoint field and
+// the reconstructed forward field; they halve the HBM traffic per step and
+// are used for launches large enough to fill the GPU.
 #include "otfwi_internal.h"
 
 namespace otfwi {
