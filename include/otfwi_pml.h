@@ -250,7 +250,9 @@ int otfwi_gradient_multi(int n_models, otfwi_model *const *models, const int *n_
  * cuda_interface_pml.py:238-249): otfwi_forward_for_grad runs the forward
  * with ring save and returns the traces; otfwi_adjoint_from_host takes the
  * (n_shots, n_rx, T) FP32 adjoint source and returns the mean gradient.
- * At most max_batch shots (otfwi_set_tuning) per pair of calls. */
+ * At most max_batch shots (otfwi_set_tuning) per pair of calls; any other
+ * run on the model in between invalidates the first stage (the second then
+ * fails with OTFWI_E_STATE). */
 int otfwi_forward_for_grad(otfwi_model *m, int n_shots, const int *shot_src, float *rec_out);
 int otfwi_adjoint_from_host(otfwi_model *m, const float *adj_src, float *grad_out);
 

@@ -187,6 +187,9 @@ void free_batch(Batch &b)
 
 int ensure_batch(otfwi_model *m, int n, bool need_adj, bool need_ring = true)
 {
+    // every run reuses the batch buffers: a pending two-stage gradient
+    // (otfwi_forward_for_grad) is no longer valid after another run
+    m->pend_n = 0;
     Batch &b = m->b;
     const size_t es = esize(m);
     const size_t plane = (size_t)m->zp * m->ld;

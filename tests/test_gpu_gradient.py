@@ -93,6 +93,11 @@ def test_host_adjoint_matches_device(gpu):
     adj = sim.astype(np.float64) - obs
     g_host = m.adjoint_from_host(adj)
     assert rel(g_host, g_dev) < 1e-6
+    # another run in between reuses the buffers: the second stage refuses
+    m.forward_for_grad(shots)
+    m.forward(shots)
+    with pytest.raises(Exception):
+        m.adjoint_from_host(adj)
 
 
 def test_batched_gradient_equals_chunked(gpu):
