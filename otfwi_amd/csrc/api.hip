@@ -158,6 +158,11 @@ struct otfwi_model {
     // HIP events around the time loops of the last call (forward, misfit,
     // adjoint), accumulated over its batches
     hipEvent_t ev[4] = {};
+    // two-step loops can advance two halves of the shots on two streams (the
+    // second half's launches fill the tails of the first's); fork / join events
+    hipStream_t stream2 = nullptr;
+    hipEvent_t evfj[2] = {};
+    int pair_streams = 0;            // 1, 2, or 0 = by launch size
     double t_fwd_ms = 0, t_mis_ms = 0, t_adj_ms = 0;
     long long n_fwd_launch = 0, n_adj_launch = 0;
 };
@@ -521,6 +526,67 @@ bool use_pairs(const otfwi_model *m, int n, bool adj)
     return adj ? 2 * wg >= 3 * cu : wg >= 2 * cu;
 }
 
+// Streams the pair phase of a loop over n shots is split over (1 or 2):
+// two when each half still launches 2 work-groups per CU (measured at
+// 64 x 1024^2: two concurrent 32-shot halves take 0.92x the time of one
+// 64-shot launch sequence, their launches filling each other's tails).
+int pair_split(const otfwi_model *m, int n)
+{
+    if (n < 2 || !m->stream2) return 1;
+    if (m->pair_streams != 0) return m->pair_streams;
+    if (m->prec != 4) return 1;
+    const long long wg = pair2_groups(m->X, m->Z, m->pair_nw_built, n / 2);
+    return wg >= 2LL * std::max(m->n_cu, 1) ? 2 : 1;
+}
+
+// Pair arguments of shots [g0, g0 + ng) of the batch (planes, traces, ring,
+// adjoint sources, gradient and Laplacian planes offset to the group).
+template <typename R>
+Pair2Args<R> pair_group(const otfwi_model *m, Pair2Args<R> q, int g0, int ng)
+{
+    const size_t so = (size_t)g0 * (size_t)q.sstride;
+    const size_t to = (size_t)g0 * m->n_rx * m->T, ro = (size_t)g0 * m->T * m->n_ring;
+    q.nshots = ng;
+    q.P0 += so; q.Ax0 += so; q.Az0 += so;
+    q.P1 += so; q.Ax1 += so; q.Az1 += so;
+    if (q.Px0) { q.Px0 += so; q.Px1 += so; }
+    if (q.shot_src) q.shot_src += g0;
+    if (q.rec) q.rec += to;
+    if (q.ring) q.ring += ro;
+    if (q.ring_in) q.ring_in += ro;
+    if (q.jcum) q.jcum += to;
+    if (q.grad) q.grad += so;
+    if (q.lap0) { q.lap0 += so; q.lap1 += so; }
+    return q;
+}
+
+// Launches of one pair on the batch's shots, split over the model's two
+// streams when ns == 2 (the caller forks and joins).
+template <typename R>
+hipError_t launch_pair_split(const otfwi_model *m, const Pair2Args<R> &q, int n, int ns, int mode)
+{
+    if (ns != 2) return launch_pair2<R>(q, mode, m->stream);
+    const int na = n / 2;
+    hipError_t e = launch_pair2<R>(pair_group<R>(m, q, 0, na), mode, m->stream);
+    if (e != hipSuccess) return e;
+    return launch_pair2<R>(pair_group<R>(m, q, na, n - na), mode, m->stream2);
+}
+
+// Fork: work on stream2 after what is on the model's stream; join: the
+// model's stream after what is on stream2 (both recorded into a captured graph).
+int fork_streams(otfwi_model *m)
+{
+    HIPCK(hipEventRecord(m->evfj[0], m->stream));
+    HIPCK(hipStreamWaitEvent(m->stream2, m->evfj[0], 0));
+    return OTFWI_OK;
+}
+int join_streams(otfwi_model *m)
+{
+    HIPCK(hipEventRecord(m->evfj[1], m->stream2));
+    HIPCK(hipStreamWaitEvent(m->stream, m->evfj[1], 0));
+    return OTFWI_OK;
+}
+
 int persistent_wgs(const otfwi_model *m, bool adj)
 {
     if (!(adj ? m->pers_adj : m->pers_fwd) || m->prec == 8) return 0;
@@ -720,6 +786,8 @@ int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum)
         };
         for (int k = 0; k < lead; k++, t++) RCK(single(t));
         Pair2Args<R> q = pair_fwd_args<R>(m, n, a);
+        const int ns = pair_split(m, n);
+        if (ns == 2 && pairs > 0) RCK(fork_streams(m));
         for (int k = 0; k < pairs; k++, t += 2) {
             // each pair writes the slot it does not read: after k pairs the
             // state sits in the other slot than one step per launch leaves it
@@ -729,8 +797,9 @@ int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum)
             q.Ax0 = (const R *)b.Ax[in]; q.Ax1 = (R *)b.Ax[out];
             q.Az0 = (const R *)b.Az[in]; q.Az1 = (R *)b.Az[out];
             q.Px0 = (const R *)b.Px[in]; q.Px1 = (R *)b.Px[out];
-            HIPCK(launch_pair2<R>(q, 0, st));
+            HIPCK(launch_pair_split<R>(m, q, n, ns, 0));
         }
+        if (ns == 2 && pairs > 0) RCK(join_streams(m));
         for (; t < t1; t++) RCK(single(t));
         return OTFWI_OK;
     };
@@ -764,7 +833,8 @@ int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum)
         return OTFWI_OK;
     };
     const unsigned long long kb =
-        ((unsigned long long)K * 4099ULL + (unsigned long long)G) * 3ULL + (two ? 1ULL : 0ULL);
+        (((unsigned long long)K * 4099ULL + (unsigned long long)G) * 3ULL + (two ? 1ULL : 0ULL)) *
+            3ULL + (unsigned long long)pair_split(m, n);
     return run_loop(m, save_ring ? 1 : 0,
                     (loop_key(m, n, m->th_fwd, pw, t0, t1) * 31ULL + (unsigned long long)ist) *
                             1000003ULL + kb, body);
@@ -850,6 +920,8 @@ int adj_loop(otfwi_model *m, int n, int t0, int t1)
         int t = t0;
         for (int k = 0; k < lead; k++, t++) RCK(single(t));
         Pair2Args<R> u = pair_rec_args<R>(m, n), l = pair_lam_args<R>(m, n);
+        const int ns = pair_split(m, n);
+        if (ns == 2 && pairs > 0) RCK(fork_streams(m));
         for (int k = 0; k < pairs; k++, t += 2) {
             const int f = k & 1;
             const int li = ((t - 1) & 1) ^ f, ui = ((T - 2 + t) & 1) ^ f, ai = ((T - 1 + t) & 1) ^ f;
@@ -857,13 +929,14 @@ int adj_loop(otfwi_model *m, int n, int t0, int t1)
             u.P0 = (const R *)b.P[ui]; u.P1 = (R *)b.P[ui ^ 1];
             u.Ax0 = (const R *)b.Ax[ai]; u.Ax1 = (R *)b.Ax[ai ^ 1];
             u.Az0 = (const R *)b.Az[ai]; u.Az1 = (R *)b.Az[ai ^ 1];
-            HIPCK(launch_pair2<R>(u, 2, st));
+            HIPCK(launch_pair_split<R>(m, u, n, ns, 2));
             l.P0 = (const R *)b.LP[li]; l.P1 = (R *)b.LP[li ^ 1];
             l.Ax0 = (const R *)b.LAx[li]; l.Ax1 = (R *)b.LAx[li ^ 1];
             l.Az0 = (const R *)b.LAz[li]; l.Az1 = (R *)b.LAz[li ^ 1];
             l.Px0 = (const R *)b.LPx[li]; l.Px1 = (R *)b.LPx[li ^ 1];
-            HIPCK(launch_pair2<R>(l, 1, st));
+            HIPCK(launch_pair_split<R>(m, l, n, ns, 1));
         }
+        if (ns == 2 && pairs > 0) RCK(join_streams(m));
         return OTFWI_OK;
     };
     auto body = [&]() -> int {
@@ -900,7 +973,8 @@ int adj_loop(otfwi_model *m, int n, int t0, int t1)
         return OTFWI_OK;
     };
     const unsigned long long kb =
-        ((unsigned long long)K * 4099ULL + (unsigned long long)G) * 3ULL + (two ? 1ULL : 0ULL);
+        (((unsigned long long)K * 4099ULL + (unsigned long long)G) * 3ULL + (two ? 1ULL : 0ULL)) *
+            3ULL + (unsigned long long)pair_split(m, n);
     return run_loop(m, 2, loop_key(m, n, m->th_adj, pw, t0, t1) * 1000003ULL + kb, body);
 }
 
@@ -1581,7 +1655,10 @@ int otfwi_create(otfwi_model **out, int X, int Z, int T, int precision, const fl
     m->kx_h.assign(X, 1.0f);
     m->kz_h.assign(Z, 1.0f);
     hipError_t e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->stream2, hipStreamNonBlocking);
     for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipEventCreate(&m->ev[i]);
+    for (int i = 0; i < 2 && e == hipSuccess; i++)
+        e = hipEventCreateWithFlags(&m->evfj[i], hipEventDisableTiming);
     if (e != hipSuccess) {
         delete m;
         return fail(OTFWI_E_HIP, "stream/event creation failed: %s", hipGetErrorString(e));
@@ -1602,8 +1679,12 @@ int otfwi_destroy(otfwi_model *m)
     dfree(m->xsnap); dfree(m->xseg); dfree(m->rxmap);
     for (int i = 0; i < 4; i++)
         if (m->ev[i]) (void)hipEventDestroy(m->ev[i]);
+    for (int i = 0; i < 2; i++)
+        if (m->evfj[i]) (void)hipEventDestroy(m->evfj[i]);
     for (int i = 0; i < 3; i++)
         if (m->gexec[i]) (void)hipGraphExecDestroy(m->gexec[i]);
+    if (m->stream2) (void)hipStreamSynchronize(m->stream2);
+    if (m->stream2) (void)hipStreamDestroy(m->stream2);
     (void)hipStreamDestroy(m->stream);
     delete m;
     return OTFWI_OK;
@@ -1777,6 +1858,14 @@ int otfwi_steps_per_launch(otfwi_model *m, int n_shots, int *fwd, int *adj)
                                     m->prec);
     *fwd = ok && use_pairs(m, n_shots, false) ? 2 : 1;
     *adj = ok && use_pairs(m, n_shots, true) ? 2 : 1;
+    return OTFWI_OK;
+}
+
+int otfwi_set_pair_streams(otfwi_model *m, int k)
+{
+    RCK(check_model(m));
+    if (k < 0 || k > 2) return fail(OTFWI_E_ARG, "pair streams must be 0 (by size), 1 or 2");
+    m->pair_streams = k;
     return OTFWI_OK;
 }
 

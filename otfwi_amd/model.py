@@ -150,6 +150,10 @@ class PMLModel:
                                                ctypes.byref(a)))
         return int(f.value), int(a.value)
 
+    def set_pair_streams(self, k=0):
+        """Streams of the two-step loops: 1, 2 or 0 (by size)."""
+        check(self._lib.otfwi_set_pair_streams(self._h, int(k)))
+
     def set_pair_width(self, nw=0):
         """Work-group width of the two-step kernels: 256, 384 or 0 (automatic)."""
         check(self._lib.otfwi_set_pair_width(self._h, int(nw)))

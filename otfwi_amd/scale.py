@@ -106,9 +106,12 @@ def main(argv=None):
     ap.add_argument("--models", type=int, default=16)
     ap.add_argument("--T", type=int, default=4000)
     ap.add_argument("--misfit", default="w2", choices=["w2", "l2"])
+    ap.add_argument("--max-batch", type=int, default=8,
+                    help="c3: shots resident per model (0: all of its shots)")
     a = ap.parse_args(argv)
     if a.what == "c3":
-        out = run_c3(a.models, a.shots, a.t_scale, a.workers, misfit=a.misfit)
+        out = run_c3(a.models, a.shots, a.t_scale, a.workers, max_batch=a.max_batch,
+                     misfit=a.misfit)
     else:
         out = run_exact(T=a.T, shots=min(a.shots, 64))
     print(json.dumps(out))
