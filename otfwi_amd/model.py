@@ -155,7 +155,7 @@ class PMLModel:
         check(self._lib.otfwi_set_pair_streams(self._h, int(k)))
 
     def set_pair_width(self, nw=0):
-        """Work-group width of the two-step kernels: 256, 384 or 0 (automatic)."""
+        """Work-group width of the two-step kernels: 128, 192, 256, 384 or 0 (256)."""
         check(self._lib.otfwi_set_pair_width(self._h, int(nw)))
 
     def set_steps_per_launch(self, fwd=2, adj=2):
