@@ -998,7 +998,10 @@ hipError_t launch_adj_step<double>(const AdjArgs<double> &a, int th, int pw, hip
 // Memory is accessed through buffer resources: an offset at or beyond the
 // buffer's size reads 0 and drops a store without touching memory, so rows
 // and columns outside the plane, undamped psi cells, cells without an event
-// and lanes that do not own a cell need no branch.
+// and lanes that do not own a cell need no branch.  An offset is a lane part
+// (column) plus a row part, either of which can rule the access out (F2_SENT).
+// In the forward kernel the iterations whose rows all lie inside the chunk
+// and away from the plane's edges run a copy of the body without row checks.
 constexpr int F2_FWD = 0, F2_LAM = 1, F2_REC = 2;
 // Rows of global loads in flight ahead of the row being computed, per mode
 // (measured at 64 x 1024^2: the reconstruction kernel gains 8 % at 4, the
@@ -1022,6 +1025,26 @@ static_assert(F2_BARRIER == 1 || F2_BARRIER == 2, "LDS ring hazards allow a barr
 constexpr int F2_CH = F2_CHUNK;   // output rows per work-group
 template <int MODE>
 constexpr int f2_pf() { return MODE == 2 ? F2_PREFETCH_REC : MODE == 1 ? F2_PREFETCH_LAM : F2_PREFETCH; }
+// Per-mode code shape, bit MODE of each mask (measured at 64 x 1024^2, ms per
+// launch of the forward / reconstruction / lambda kernel):
+//  F2_STEADY_MODES: iterations in the middle of a chunk run a second copy of
+//    the 8-row body without row checks (254 instead of 357 instructions per
+//    row in the forward kernel: 0.645 -> 0.570 ms; the reconstruction and
+//    lambda kernels ran 6-8 % slower with it, at more registers).
+//  F2_OPAQUE_MODES: the row part of an offset is pinned in a scalar register
+//    (see f2_row): the reconstruction kernel needs 127 vector registers with
+//    it and 168 without (0.604 vs 0.644 ms); the forward kernel ran 3 %
+//    slower with it.
+#ifndef F2_STEADY_MODES
+#define F2_STEADY_MODES 1
+#endif
+#ifndef F2_OPAQUE_MODES
+#define F2_OPAQUE_MODES 6
+#endif
+template <int MODE>
+constexpr bool f2_steady() { return ((F2_STEADY_MODES) >> MODE) & 1; }
+template <int MODE>
+constexpr bool f2_opaque() { return ((F2_OPAQUE_MODES) >> MODE) & 1; }
 // Block geometry for NW threads: owned columns, LDS row width (8 padding
 // cells each side), ring-table words per row (slot base, one 16-bit prefix
 // per 32-lane word, one lane mask per word).
@@ -1036,7 +1059,13 @@ struct F2Geo {
 // iterations touch rows zc0 - 29 .. zc0 + F2_CH + 20 + prefetch (<= 4)
 constexpr int F2_RB = 32, F2_NR = F2_CH + 64;
 constexpr int F2_LEAD = 15;   // rows loaded above a chunk
-constexpr unsigned F2_OOB = 0xFFFFFFF0u;
+// Buffer offsets are formed as a lane part (the column; loop-invariant, in a
+// vector register) plus a row part (row, slot and time offsets; the same for
+// every lane, in a scalar register), each F2_SENT where that part rules the
+// access out.  Every resource spans at most F2_SENT bytes, so a sum holding
+// F2_SENT is out of range, and 2 * F2_SENT does not wrap: no per-access
+// select is needed.
+constexpr unsigned F2_SENT = 0x7FFFFFF0u;
 
 typedef __amdgpu_buffer_rsrc_t Rsrc;
 
@@ -1096,7 +1125,8 @@ struct F2Row {
     R m3;                 // m on row i - 10
     F2Evt e1, e3;         // FWD, LAM: events on rows i - 3 and i - 10
     R g, l0, l1;          // LAM: grad and lap(U(t)) on row i - 7, lap(U(t+1)) on row i - 14
-    int k1, k3;           // FWD, REC: ring slots on rows i - 3 and i - 10 (-1: none)
+    unsigned k1, k3;      // FWD, REC: lane parts of the ring slots on rows i - 3 and
+                          // i - 10 (slot * sizeof(R); F2_SENT: none)
     R r1, r3;             // REC: ring values there at steps t and t+1
 };
 
@@ -1113,105 +1143,127 @@ struct Pair2Regs {
 
 struct F2Ctx {
     int tid, x, s, ss, zc0, zc1, lc, X, Z, ld, t, T, nring;   // ss: active source (-1: all)
-    unsigned lo;   // byte offset of this lane's column in a padded row (element size R)
-    unsigned lo4;  // the same for 4-byte elements
-    bool colv, own, xin;   // xin: 4 <= x < X - 4
+    // lane parts of offsets: the byte offset of this lane's column in a
+    // padded row (element size R) where the lane may access it, else F2_SENT
+    unsigned loc;    // loads: columns of the block that reach an owned cell
+    unsigned loo;    // stores: owned columns
+    unsigned lod;    // psi loads on rows without z damping: loc where x is damped
+    unsigned lodo;   // psi stores on such rows: loo where x is damped
+    unsigned lo16;   // event map (16-byte cells): loads
+    bool own, xin;   // xin: 4 <= x < X - 4
     // one resource per field covers both slots of the shot; the slot read and
     // the slot written are byte offsets (P kind: P, psi; A kind: Ax, Az)
     Rsrc rP, rAx, rAz, rPx, rcm, rpm, rg, rl, rring;
     unsigned dPi, dPo, dAi, dAo, dL1;   // dL1: lap1 relative to lap0 (one resource)
 };
 
-template <typename R>
-__device__ __forceinline__ unsigned f2_off(const F2Ctx &c, int z)
+// Row part of a plane offset: row z of a padded plane plus d bytes, or
+// F2_SENT when ok is false.
+template <typename R, int MODE>
+__device__ __forceinline__ unsigned f2_row(const F2Ctx &c, int z, unsigned d, bool ok)
 {
-    return (unsigned)((z + ZOFF) * c.ld) * (unsigned)sizeof(R) + c.lo;
+    unsigned r = ok ? (unsigned)((z + ZOFF) * c.ld) * (unsigned)sizeof(R) + d : F2_SENT;
+    // f2_opaque: the row part stays a scalar formed where it is used, which
+    // keeps the compiler from re-associating lane + row and holding lane +
+    // (row constant) of every unrolled row in vector registers.  Emits no
+    // instruction.
+    if constexpr (f2_opaque<MODE>()) asm("" : "+s"(r));
+    return r;
 }
 
 __device__ __forceinline__ bool f2_zin(const F2Ctx &c, int z) { return z >= BOX && z < c.Z - BOX; }
+__device__ __forceinline__ bool f2_chunk(const F2Ctx &c, int z) { return z >= c.zc0 && z < c.zc1; }
 
 // source / receiver event of this lane's cell on row z (FWD, LAM): sev
 // flags the rows holding one in the block's columns; on those rows every
-// lane reads its cell of the event map (a few rows per block)
+// lane reads its cell of the event map (a few rows per block).  The flag is
+// the same for every lane, so it is read into a scalar.
 __device__ __forceinline__ F2Evt f2_evt(const F2Ctx &c, const uint8_t *sev, int z)
 {
-    const bool row = sev[z - c.zc0 + F2_RB] != 0;
-    const unsigned off = row && c.colv ? (unsigned)((z + ZOFF) * c.ld) * 16u + c.lo4 * 4u : F2_OOB;
+    const int row = __builtin_amdgcn_readfirstlane((int)sev[z - c.zc0 + F2_RB]);
+    const unsigned off = c.lo16 + (row != 0 ? (unsigned)((z + ZOFF) * c.ld) * 16u : F2_SENT);
     const auto v = __builtin_amdgcn_raw_buffer_load_b96(c.rpm, off, 0, 0);
     return F2Evt{(int)v[0], (int)v[1], (int)v[2]};
 }
 
-// Ring slot of this lane's cell on row z, or -1 (FWD, REC).  Per row smask
-// holds F2Geo<NW>::RW words: the ring slot of the block's first ring cell,
-// the ring cells before each 32-lane word (16 bits each), and the lane masks;
-// ring slots are numbered in raster order.
+// Lane part of the ring slot of this lane's cell on row z (FWD, REC): slot *
+// es, or F2_SENT when the cell is not a ring cell or allow is false.  Per
+// row smask holds F2Geo<NW>::RW words: the ring slot of the block's first
+// ring cell, the ring cells before each 32-lane word (16 bits each), and the
+// lane masks; ring slots are numbered in raster order.
 template <int NW>
-__device__ __forceinline__ int f2_rslot(const F2Ctx &c, const unsigned *smask, int z)
+__device__ __forceinline__ unsigned f2_rslot(const F2Ctx &c, const unsigned *smask, int z,
+                                             unsigned es, bool allow)
 {
     const int w = c.tid >> 5, b = c.tid & 31;
     const unsigned *e = smask + (z - c.zc0 + F2_RB) * F2Geo<NW>::RW;
     const unsigned mk = e[1 + NW / 64 + w];
     const unsigned pre = (e[1 + (w >> 1)] >> (16 * (w & 1))) & 0xFFFFu;
-    const int k = (int)(e[0] + pre) + __builtin_popcount(mk & ((1u << b) - 1u));
-    return ((mk >> b) & 1u) ? k : -1;
+    const unsigned k = e[0] + pre + (unsigned)__builtin_popcount(mk & ((1u << b) - 1u));
+    return allow && ((mk >> b) & 1u) ? k * es : F2_SENT;
 }
 
-template <typename R, int MODE, int NW>
+// ST (steady): the caller guarantees that every row this iteration touches
+// lies in the plane, in the work-group's chunk and 4 or more rows from the
+// plane's top and bottom edges, so no row check is needed.
+template <typename R, int MODE, int NW, bool ST>
 __device__ __forceinline__ void f2_fetch(const F2Ctx &c, const unsigned *smask, const uint8_t *sev,
-                                         const R *skz, R kx, int i, F2Row<R, MODE> &f)
+                                         const R *skz, int i, F2Row<R, MODE> &f)
 {
-    const bool rv = c.colv && i >= -ZOFF && i < c.Z + 4;
-    const unsigned ro = f2_off<R>(c, i);
-    const unsigned o = rv ? ro : F2_OOB, oP = rv ? ro + c.dPi : F2_OOB,
-                   oA = rv ? ro + c.dAi : F2_OOB;
-    f.ax = bload<R>(c.rAx, oA);
-    f.az = bload<R>(c.rAz, oA);
-    f.p = bload<R>(c.rP, oP);
-    f.m = bload<R>(c.rcm, o);
+    const bool rv = ST || (i >= -ZOFF && i < c.Z + 4);
+    const unsigned sA = f2_row<R, MODE>(c, i, c.dAi, rv), sP = f2_row<R, MODE>(c, i, c.dPi, rv);
+    f.ax = bload<R>(c.rAx, c.loc + sA);
+    f.az = bload<R>(c.rAz, c.loc + sA);
+    f.p = bload<R>(c.rP, c.loc + sP);
+    f.m = bload<R>(c.rcm, c.loc + f2_row<R, MODE>(c, i, 0u, rv));
     const int i3 = i - 10;
-    const bool rv3 = c.colv && i3 >= -ZOFF && i3 < c.Z + 4;
-    f.m3 = bload<R>(c.rcm, rv3 ? f2_off<R>(c, i3) : F2_OOB);
+    f.m3 = bload<R>(c.rcm, c.loc + f2_row<R, MODE>(c, i3, 0u, ST || (i3 >= -ZOFF && i3 < c.Z + 4)));
     f.ps = f.g = f.l0 = f.l1 = f.r1 = f.r3 = R(0);
-    f.k1 = f.k3 = -1;
+    f.k1 = f.k3 = F2_SENT;
     f.e1 = f.e3 = F2Evt{0, 0, 0};
     if constexpr (MODE != F2_REC) {
-        const bool damped = kx != R(1) || skz[i - c.zc0 + F2_RB] != R(1);
-        f.ps = bload<R>(c.rPx, damped ? oP : F2_OOB);
+        // psi is stored on damped cells only: x-damped columns, or every
+        // column of a z-damped row
+        const bool zd = skz[i - c.zc0 + F2_RB] != R(1);
+        f.ps = bload<R>(c.rPx, (zd ? c.loc : c.lod) + sP);
         f.e1 = f2_evt(c, sev, i - 3);
         f.e3 = f2_evt(c, sev, i3);
     }
     if constexpr (MODE != F2_LAM) {
-        f.k1 = f2_rslot<NW>(c, smask, i - 3);
-        f.k3 = f2_rslot<NW>(c, smask, i3);
+        const unsigned es = (unsigned)sizeof(R);
+        const bool allow = MODE == F2_REC || c.own;   // the forward samples owned cells
+        f.k1 = f2_rslot<NW>(c, smask, i - 3, es, allow);
+        f.k3 = f2_rslot<NW>(c, smask, i3, es, allow);
     }
     if constexpr (MODE == F2_LAM) {
         const int z2 = i - 7, z4 = i - 14;
-        const bool o2 = c.own && z2 >= c.zc0 && z2 < c.zc1;
-        const bool o4 = c.own && z4 >= c.zc0 && z4 < c.zc1;
-        f.g = bload<R>(c.rg, o2 ? f2_off<R>(c, z2) : F2_OOB);
-        f.l0 = bload<R>(c.rl, o2 ? f2_off<R>(c, z2) : F2_OOB);
-        f.l1 = bload<R>(c.rl, o4 ? f2_off<R>(c, z4) + c.dL1 : F2_OOB);
+        const unsigned s2 = f2_row<R, MODE>(c, z2, 0u, ST || f2_chunk(c, z2));
+        f.g = bload<R>(c.rg, c.loo + s2);
+        f.l0 = bload<R>(c.rl, c.loo + s2);
+        f.l1 = bload<R>(c.rl, c.loo + f2_row<R, MODE>(c, z4, c.dL1, ST || f2_chunk(c, z4)));
     }
     if constexpr (MODE == F2_REC) {
         // ring samples of the shot, time-major: ring times T-1-t and T-2-t
         const unsigned es = (unsigned)sizeof(R);
-        f.r1 = bload<R>(c.rring, f.k1 >= 0 ? (unsigned)((c.T - 1 - c.t) * c.nring + f.k1) * es : F2_OOB);
-        f.r3 = bload<R>(c.rring, f.k3 >= 0 ? (unsigned)((c.T - 2 - c.t) * c.nring + f.k3) * es : F2_OOB);
+        const int r1 = c.T - 1 - c.t, r3 = c.T - 2 - c.t;
+        f.r1 = bload<R>(c.rring, f.k1 + (r1 >= 0 ? (unsigned)(r1 * c.nring) * es : F2_SENT));
+        f.r3 = bload<R>(c.rring, f.k3 + (r3 >= 0 ? (unsigned)(r3 * c.nring) * es : F2_SENT));
     }
 }
 
 // P-kind update of one cell at step tt (p_quad / lu_quad arithmetic for one
 // lane).  rec_here = this block samples the cell (FWD).
-// ks: ring slot of the cell (-1: none): REC writes the ring value rv back,
-// FWD samples P there (both branch-free through buffer offsets).
+// ks: lane part of the cell's ring slot (F2_SENT: none): REC writes the ring
+// value rv back, FWD samples P there at ks + sring (sring: the row part,
+// F2_SENT outside the chunk), both branch-free.
 template <typename R, int MODE>
 __device__ __forceinline__ R f2_p(const Pair2Args<R> &a, const F2Ctx &c, const F2Evt &e, R rv,
-                                  int ks, bool inb, int tt, bool rec_here, R pp, R psi, R mm, R kx,
-                                  R kz, R dAx, R dAz, R &psi_out)
+                                  unsigned ks, unsigned sring, bool inb, int tt, bool rec_here, R pp,
+                                  R psi, R mm, R kx, R kz, R dAx, R dAz, R &psi_out)
 {
     if constexpr (MODE == F2_REC) {
         R up = fma(-mm, dAx + dAz, pp);
-        if (ks >= 0) up = rv;
+        if (ks != F2_SENT) up = rv;
         psi_out = R(0);
         return inb ? up : R(0);
     } else {
@@ -1237,8 +1289,7 @@ __device__ __forceinline__ R f2_p(const Pair2Args<R> &a, const F2Ctx &c, const F
         }
         R p = add_rn(nx, nz);
         if constexpr (MODE == F2_FWD)   // ring cells lie inside the box: p is the sample
-            bstore<R>(c.rring, rec_here && ks >= 0
-                                   ? (unsigned)(tt * c.nring + ks) * (unsigned)sizeof(R) : F2_OOB, p);
+            bstore<R>(c.rring, ks + sring, p);
         if (!inb) { p = R(0); nx = R(0); }
         psi_out = (kx != R(1) || kz != R(1)) ? nx : R(0);
         return p;
@@ -1273,7 +1324,7 @@ __device__ __forceinline__ R f2_lap(R (*ring)[LP], int q, int lc, const R (&w)[9
     return (lx + lz) * inv5040;
 }
 
-template <typename R, int MODE, int NW, int PH>
+template <typename R, int MODE, int NW, int PH, bool ST>
 __device__ __forceinline__ void f2_iter(const Pair2Args<R> &a, const F2Ctx &c,
                                         Pair2Regs<R, MODE> &g, R (*sAx0)[F2Geo<NW>::LP],
                                         R (*sP1)[F2Geo<NW>::LP], R (*sAx1)[F2Geo<NW>::LP],
@@ -1281,6 +1332,7 @@ __device__ __forceinline__ void f2_iter(const Pair2Args<R> &a, const F2Ctx &c,
                                         const unsigned *smask, const uint8_t *sev, const R *skz,
                                         int i, R kx)
 {
+    const unsigned es = (unsigned)sizeof(R);
     constexpr int Q1 = (PH + 1) & 7, Q4 = (PH + 4) & 7, Q5 = (PH + 5) & 7;
     constexpr int F = PH % f2_pf<MODE>();
     constexpr int H = PH & 3, H1 = (PH + 1) & 3;   // 4-row rings: newest, 3 rows back
@@ -1302,9 +1354,11 @@ __device__ __forceinline__ void f2_iter(const Pair2Args<R> &a, const F2Ctx &c,
         const R dAz = stag<R>(g.az0[(PH + 1) & 7], g.az0[(PH + 2) & 7], g.az0[(PH + 3) & 7],
                               g.az0[(PH + 4) & 7], g.az0[(PH + 5) & 7], g.az0[(PH + 6) & 7],
                               g.az0[(PH + 7) & 7], g.az0[PH]);
-        const bool rec = c.own && z1 >= c.zc0 && z1 < c.zc1;
-        pn1 = f2_p<R, MODE>(a, c, f.e1, f.r1, f.k1, c.xin && f2_zin(c, z1), c.t, rec, g.p0[H1],
-                            g.s0[H1], g.m0[H1], kx, kz_at(z1), dAx, dAz, sn1);
+        const bool in1 = ST || f2_chunk(c, z1);
+        const unsigned sring = in1 ? (unsigned)(c.t * c.nring) * es : F2_SENT;
+        pn1 = f2_p<R, MODE>(a, c, f.e1, f.r1, f.k1, sring, c.xin && (ST || f2_zin(c, z1)), c.t,
+                            c.own && in1, g.p0[H1], g.s0[H1], g.m0[H1], kx, kz_at(z1), dAx, dAz,
+                            sn1);
         sP1[PH][lc] = pn1;
     }
     // ---- 3. A(t) on row z2 = i - 7 ----
@@ -1317,7 +1371,7 @@ __device__ __forceinline__ void f2_iter(const Pair2Args<R> &a, const F2Ctx &c,
                               g.p1[(PH + 7) & 7], pn1);
         R nax, naz;
         f2_a<R, MODE>(a.cA, dPx, dPz, sAx0[Q1][lc], g.az0[Q1], kx, kz_at(z2),
-                      c.xin && f2_zin(c, z2), nax, naz);
+                      c.xin && (ST || f2_zin(c, z2)), nax, naz);
         g.az1[PH] = naz;
         sAx1[PH][lc] = nax;
         if constexpr (MODE == F2_LAM) g.gq[PH] = fma(g.p1[Q4], f.l0, f.g);
@@ -1326,8 +1380,7 @@ __device__ __forceinline__ void f2_iter(const Pair2Args<R> &a, const F2Ctx &c,
                             g.p1[(PH + 4) & 7], g.p1[(PH + 5) & 7], g.p1[(PH + 6) & 7],
                             g.p1[(PH + 7) & 7], pn1};
             const R lp = f2_lap<R>(sP1, Q4, lc, w);
-            const bool o2 = c.own && z2 >= c.zc0 && z2 < c.zc1;
-            bstore<R>(c.rl, o2 ? f2_off<R>(c, z2) : F2_OOB, lp);
+            bstore<R>(c.rl, (ST || f2_chunk(c, z2)) ? c.loo + f2_row<R, MODE>(c, z2, 0u, true) : F2_SENT, lp);
         }
         g.p1[PH] = pn1;
         if constexpr (MODE != F2_REC) g.s1[PH] = sn1;
@@ -1342,14 +1395,16 @@ __device__ __forceinline__ void f2_iter(const Pair2Args<R> &a, const F2Ctx &c,
                               g.az1[(PH + 7) & 7], g.az1[PH]);
         const R kz3 = kz_at(z3);
         R sn2;
-        const bool mine = c.own && z3 >= c.zc0 && z3 < c.zc1;
-        pn2 = f2_p<R, MODE>(a, c, f.e3, f.r3, f.k3, c.xin && f2_zin(c, z3), c.t + 1, mine,
-                            g.p1[Q1], g.s1[Q1], f.m3, kx, kz3, dAx, dAz, sn2);
+        const bool in3 = ST || f2_chunk(c, z3);
+        const unsigned sring = in3 ? (unsigned)((c.t + 1) * c.nring) * es : F2_SENT;
+        pn2 = f2_p<R, MODE>(a, c, f.e3, f.r3, f.k3, sring, c.xin && (ST || f2_zin(c, z3)),
+                            c.t + 1, c.own && in3, g.p1[Q1], g.s1[Q1], f.m3, kx, kz3, dAx, dAz,
+                            sn2);
         sP2[PH][lc] = pn2;
-        const unsigned o = mine ? f2_off<R>(c, z3) + c.dPo : F2_OOB;
-        bstore<R>(c.rP, o, pn2);
-        if constexpr (MODE != F2_REC)
-            bstore<R>(c.rPx, (kx != R(1) || kz3 != R(1)) ? o : F2_OOB, sn2);
+        const unsigned so = f2_row<R, MODE>(c, z3, c.dPo, true);
+        bstore<R>(c.rP, in3 ? c.loo + so : F2_SENT, pn2);
+        if constexpr (MODE != F2_REC)   // psi on damped cells
+            bstore<R>(c.rPx, in3 ? (kz3 != R(1) ? c.loo : c.lodo) + so : F2_SENT, sn2);
     }
     // ---- 5. A(t+1) on row z4 = i - 14, written out ----
     {
@@ -1361,23 +1416,24 @@ __device__ __forceinline__ void f2_iter(const Pair2Args<R> &a, const F2Ctx &c,
                               g.p2[(PH + 7) & 7], pn2);
         R nax, naz;
         f2_a<R, MODE>(a.cA, dPx, dPz, sAx1[Q1][lc], g.az1[Q1], kx, kz_at(z4),
-                      c.xin && f2_zin(c, z4), nax, naz);
-        const bool o4 = c.own && z4 >= c.zc0 && z4 < c.zc1;
-        const unsigned o = o4 ? f2_off<R>(c, z4) : F2_OOB;
-        const unsigned oA = o4 ? f2_off<R>(c, z4) + c.dAo : F2_OOB;
-        bstore<R>(c.rAx, oA, nax);
-        bstore<R>(c.rAz, oA, naz);
-        if constexpr (MODE == F2_LAM) bstore<R>(c.rg, o, fma(g.p2[Q4], f.l1, g.gq[Q1]));
+                      c.xin && (ST || f2_zin(c, z4)), nax, naz);
+        const bool in4 = ST || f2_chunk(c, z4);
+        const unsigned sA = f2_row<R, MODE>(c, z4, c.dAo, true);
+        bstore<R>(c.rAx, in4 ? c.loo + sA : F2_SENT, nax);
+        bstore<R>(c.rAz, in4 ? c.loo + sA : F2_SENT, naz);
+        if constexpr (MODE == F2_LAM)
+            bstore<R>(c.rg, in4 ? c.loo + f2_row<R, MODE>(c, z4, 0u, true) : F2_SENT,
+                      fma(g.p2[Q4], f.l1, g.gq[Q1]));
         if constexpr (MODE == F2_REC) {
             const R w[9] = {g.p2[PH], g.p2[(PH + 1) & 7], g.p2[(PH + 2) & 7], g.p2[(PH + 3) & 7],
                             g.p2[(PH + 4) & 7], g.p2[(PH + 5) & 7], g.p2[(PH + 6) & 7],
                             g.p2[(PH + 7) & 7], pn2};
-            bstore<R>(c.rl, o4 ? f2_off<R>(c, z4) + c.dL1 : F2_OOB, f2_lap<R>(sP2, Q4, lc, w));
+            bstore<R>(c.rl, in4 ? c.loo + f2_row<R, MODE>(c, z4, c.dL1, true) : F2_SENT, f2_lap<R>(sP2, Q4, lc, w));
         }
         g.p2[PH] = pn2;
     }
     // ---- 6. loads for the row f2_pf ahead into the slot just consumed ----
-    f2_fetch<R, MODE, NW>(c, smask, sev, skz, kx, i + f2_pf<MODE>(), f);
+    f2_fetch<R, MODE, NW, ST>(c, smask, sev, skz, i + f2_pf<MODE>(), f);
     // Cross-lane LDS reads are of rows written 3 or 4 iterations earlier, and
     // a row slot is rewritten 8 iterations after it was written (4 after its
     // last cross-lane read); reads of a lane's own column stay in its wave.
@@ -1417,11 +1473,16 @@ __global__ void __launch_bounds__(NW, (f2_waves<R, MODE, NW>())) pair2_kernel(co
     c.ss = MODE == F2_FWD ? a.shot_src[c.s] : -1;
     c.lc = c.tid + 8;
     // columns beyond X + 12 never reach an owned cell: not loaded
-    c.colv = c.x < a.X + 12;
+    const bool colv = c.x < a.X + 12;
     c.own = c.tid >= F2_HALO && c.tid < F2_HALO + G::OWN && c.x < a.X;
     c.xin = c.x >= BOX && c.x < a.X - BOX;
-    c.lo = (unsigned)(min(c.x, a.X + 11) + XOFF) * (unsigned)sizeof(R);
-    c.lo4 = (unsigned)(min(c.x, a.X + 11) + XOFF) * 4u;
+    const unsigned col = (unsigned)(min(c.x, a.X + 11) + XOFF);
+    const R kx = (MODE != F2_REC && colv) ? a.kx[XOFF + c.x] : R(1);
+    c.loc = colv ? col * (unsigned)sizeof(R) : F2_SENT;
+    c.loo = c.own ? col * (unsigned)sizeof(R) : F2_SENT;
+    c.lod = kx != R(1) ? c.loc : F2_SENT;
+    c.lodo = kx != R(1) ? c.loo : F2_SENT;
+    c.lo16 = colv ? col * 16u : F2_SENT;
     const size_t sb = (size_t)c.s * (size_t)a.sstride;
     const unsigned pb = (unsigned)((size_t)a.sstride * sizeof(R));
     // both slots of a field: slot 0 + slot_elems (the lower pointer is slot 0)
@@ -1453,7 +1514,6 @@ __global__ void __launch_bounds__(NW, (f2_waves<R, MODE, NW>())) pair2_kernel(co
         c.rring = f2_rsrc(a.ring_in + (size_t)c.s * a.T * a.n_ring,
                           (unsigned)((size_t)a.T * a.n_ring * sizeof(R)));
     }
-    const R kx = (MODE != F2_REC && c.colv) ? a.kx[XOFF + c.x] : R(1);
     if constexpr (MODE != F2_REC)
         for (int r = c.tid; r < F2_NR; r += NW) {
             const int z = c.zc0 - F2_RB + r;
@@ -1483,18 +1543,42 @@ __global__ void __launch_bounds__(NW, (f2_waves<R, MODE, NW>())) pair2_kernel(co
     const int i0 = c.zc0 - F2_LEAD;
 #pragma unroll
     for (int k = 0; k < f2_pf<MODE>(); k++)
-        f2_fetch<R, MODE, NW>(c, smask, sev, skz, kx, i0 + k, g.pf[k]);
+        f2_fetch<R, MODE, NW, false>(c, smask, sev, skz, i0 + k, g.pf[k]);
     const int nit = c.zc1 - c.zc0 + F2_LEAD + 14;
-    for (int j0 = 0; j0 < nit; j0 += 8) {
+    // Steady blocks of 8 iterations [js, je): iteration j (row i = i0 + j)
+    // touches rows i - 14 .. i + f2_pf <= i + 4; all of them lie in the
+    // chunk, 4 or more rows from the plane's edges and in the plane when
+    // i - 14 >= max(zc0, BOX) and i - 3 < min(zc1, Z - BOX) (then also
+    // i + 4 < Z + 4), i.e. j >= max(zc0, BOX) - zc0 + 29 and
+    // j <= min(zc1 + 2, Z - 2) - zc0 + 15.
+    const int js = (max(c.zc0, BOX) - c.zc0 + 29 + 7) & ~7;
+    const int jl = min(c.zc1 + 2, a.Z - 2) - c.zc0 + 15 - 7;   // last start of a steady block
+    const int je = jl >= js ? js + ((jl - js) / 8 + 1) * 8 : js;
+    for (int j0 = 0; j0 < nit;) {
+        if (f2_steady<MODE>() && j0 == js) {
+            for (; j0 < je; j0 += 8) {
+                const int i = i0 + j0;
+                f2_iter<R, MODE, NW, 0, true>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 0, kx);
+                f2_iter<R, MODE, NW, 1, true>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 1, kx);
+                f2_iter<R, MODE, NW, 2, true>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 2, kx);
+                f2_iter<R, MODE, NW, 3, true>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 3, kx);
+                f2_iter<R, MODE, NW, 4, true>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 4, kx);
+                f2_iter<R, MODE, NW, 5, true>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 5, kx);
+                f2_iter<R, MODE, NW, 6, true>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 6, kx);
+                f2_iter<R, MODE, NW, 7, true>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 7, kx);
+            }
+            if (j0 >= nit) break;
+        }
         const int i = i0 + j0;
-        f2_iter<R, MODE, NW, 0>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 0, kx);
-        f2_iter<R, MODE, NW, 1>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 1, kx);
-        f2_iter<R, MODE, NW, 2>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 2, kx);
-        f2_iter<R, MODE, NW, 3>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 3, kx);
-        f2_iter<R, MODE, NW, 4>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 4, kx);
-        f2_iter<R, MODE, NW, 5>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 5, kx);
-        f2_iter<R, MODE, NW, 6>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 6, kx);
-        f2_iter<R, MODE, NW, 7>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 7, kx);
+        f2_iter<R, MODE, NW, 0, false>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 0, kx);
+        f2_iter<R, MODE, NW, 1, false>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 1, kx);
+        f2_iter<R, MODE, NW, 2, false>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 2, kx);
+        f2_iter<R, MODE, NW, 3, false>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 3, kx);
+        f2_iter<R, MODE, NW, 4, false>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 4, kx);
+        f2_iter<R, MODE, NW, 5, false>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 5, kx);
+        f2_iter<R, MODE, NW, 6, false>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 6, kx);
+        f2_iter<R, MODE, NW, 7, false>(a, c, g, sAx0, sP1, sAx1, sP2, smask, sev, skz, i + 7, kx);
+        j0 += 8;
     }
 }
 
@@ -1506,10 +1590,10 @@ long long pair2_groups(int X, int Z, int nw, int nshots)
 
 bool pair2_supported(long long sstride, long long slot_elems, int prec)
 {
-    // 32-bit buffer offsets: the event map (16 bytes per cell) and both slots
-    // of a field seen from one shot must fit
-    return sstride > 0 && sstride * 16 < (long long)F2_OOB &&
-           (slot_elems + sstride) * prec < (long long)F2_OOB;
+    // buffer offsets below F2_SENT: the event map (16 bytes per cell) and
+    // both slots of a field seen from one shot must fit
+    return sstride > 0 && sstride * 16 <= (long long)F2_SENT &&
+           (slot_elems + sstride) * prec <= (long long)F2_SENT;
 }
 
 template <typename R>
@@ -1525,7 +1609,7 @@ hipError_t launch_pair2(const Pair2Args<R> &a, int mode, hipStream_t st)
         (mode != F2_FWD && a.lap1 - a.lap0 != a.slot_elems) ||
         (mode != F2_REC && (gap(a.Px0, a.Px1) != a.slot_elems || (a.P0 < a.P1) != (a.Px0 < a.Px1))))
         return hipErrorInvalidValue;
-    if (mode != F2_LAM && (size_t)a.T * a.n_ring * sizeof(R) >= F2_OOB) return hipErrorInvalidValue;
+    if (mode != F2_LAM && (size_t)a.T * a.n_ring * sizeof(R) > F2_SENT) return hipErrorInvalidValue;
     const int own = f2_own(a.nw);
     if (own <= 0) return hipErrorInvalidValue;
     dim3 grid((a.X + own - 1) / own, (a.Z + F2_CH - 1) / F2_CH, a.nshots);
