@@ -42,7 +42,7 @@ def _model(N, T, n_elem, pitch, seed, max_batch):
     return cp
 
 
-def run_c3(n_models=16, shots=32, t_scale=1.0, workers=4, max_batch=8, seed=0, misfit="w2"):
+def run_c3(n_models=16, shots=32, t_scale=1.0, workers=4, max_batch=0, seed=0, misfit="w2"):
     rng = np.random.default_rng(seed)
     sizes = rng.choice([512, 768, 1024, 1536, 2048], size=n_models)
     models, cps, alg = [], [], 0.0
@@ -106,8 +106,8 @@ def main(argv=None):
     ap.add_argument("--models", type=int, default=16)
     ap.add_argument("--T", type=int, default=4000)
     ap.add_argument("--misfit", default="w2", choices=["w2", "l2"])
-    ap.add_argument("--max-batch", type=int, default=8,
-                    help="c3: shots resident per model (0: all of its shots)")
+    ap.add_argument("--max-batch", type=int, default=0,
+                    help="c3: shots resident per model (0, the default: all of its shots)")
     a = ap.parse_args(argv)
     if a.what == "c3":
         out = run_c3(a.models, a.shots, a.t_scale, a.workers, max_batch=a.max_batch,
