@@ -118,6 +118,26 @@ def calib_copy(nbytes):
     print(f"[bench] copied {nbytes} bytes device-to-device", file=sys.stderr, flush=True)
 
 
+def runtime_copy(nbytes=2 << 30, reps=10):
+    """Measured copy bandwidth of this GPU through the HIP runtime's own copy
+    (a device-to-device copy of nbytes on torch's stream, timed with HIP
+    events); bytes read + written per second (GB/s)."""
+    import torch
+
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) / 1e3 / reps
+    del a, b
+    return 2 * nbytes / sec / 1e9
+
+
 def sum_traffic(kernels, N, S):
     """Summed HBM bytes per launch of the listed kernels (None if any is missing)."""
     vals = [profiled_traffic(k, N, S) for k in kernels]
@@ -354,6 +374,12 @@ def main():
         cb = cpu_baseline(N, T, S, S, args.cpu_steps)
         out["cpu_baseline"] = cb
     del m
+    cp = runtime_copy()
+    out["runtime_copy_GBps"] = cp
+    out["roofline"]["frac_of_runtime_copy"] = out["roofline"]["achieved"] / cp
+    out["fwd_kernel"]["frac_of_runtime_copy"] = out["fwd_kernel"]["achieved_GBps"] / cp
+    out["whole_gradient_frac_of_runtime_copy"] = out["whole_gradient_hbm_frac"] * HBM_PEAK / 1e9 / cp
+    print(f"[bench] device-to-device copy {cp:.0f} GB/s (read + write)", file=sys.stderr, flush=True)
     if not args.no_unet:
         out["unet_roofline"] = unet_timing()
     if args.inversion_models > 0:
