@@ -132,3 +132,16 @@ def test_pair_streams_bitwise(gpu, graph):
             assert np.array_equal(u, v), (steps, k)
     with pytest.raises(Exception):
         m.set_pair_streams(3)
+
+
+def test_pairs_batch_bound(gpu):
+    """Two-step launches address both slots of a field from one shot's base
+    with 32-bit buffer offsets below 0x7FFFFFF0 bytes; a larger batch runs one
+    step per launch (2048^2: padded plane 2176 x 2064, at most 118 shots)."""
+    p = small_problem(X=2048, Z=2048, T=10, pml=20, seed=3, n_src=4, n_rx=8)
+    m = make_model(p, cq="cq_bg")
+    plane = 2176 * 2064
+    cap = 0x7FFFFFF0 // (4 * plane) - 1
+    assert cap == 118
+    assert m.steps_per_launch(cap) == (2, 2)
+    assert m.steps_per_launch(cap + 1) == (1, 1)
