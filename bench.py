@@ -233,6 +233,8 @@ def main():
     ap.add_argument("--misfit", default="w2", choices=["w2", "l2"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--pair-streams", type=int, default=0, choices=[0, 1, 2, 4],
+                    help="streams of the two-step phases (0: by launch size)")
     ap.add_argument("--steps-per-launch", type=int, default=0, choices=[0, 1, 2],
                     help="time steps per launch of both loops (0: the autotuner's choice)")
     ap.add_argument("--cpu-steps", type=int, default=2000)
@@ -268,6 +270,8 @@ def main():
         tune = m.autotune(S, steps=20)
     if args.steps_per_launch:
         m.set_steps_per_launch(args.steps_per_launch, args.steps_per_launch)
+    if args.pair_streams:
+        m.set_pair_streams(args.pair_streams)
     ktimes = m.kernel_times(S, reps=10)
     obs = m.forward(shots)
     m.set_observed(obs)

@@ -176,11 +176,12 @@ int otfwi_get_adj_steps(otfwi_model *m, int *steps);
  * 384 was slower on MI355X at 1024^2.  No reference counterpart. */
 int otfwi_set_pair_width(otfwi_model *m, int nw);
 
-/* Streams the two-step phase of a loop runs on: 1, 2, or 0 (default: 2 when
- * each half of the batch's shots still launches 2 work-groups per CU).  With
- * 2 the two halves of the shots advance on the model's stream and a second
- * one, so each half's launches fill the other's launch tails; results are
- * bitwise identical.  No reference counterpart. */
+/* Streams the two-step phase of a loop runs on: 1, 2, 4, or 0 (default: 2
+ * when each half of the batch's shots still launches 2 work-groups per CU,
+ * else 1).  With k > 1 the shots are split into k groups of consecutive
+ * shots, each advancing on the model's stream or one of the library's extra
+ * streams, so each group's launches fill the others' launch tails; results
+ * are bitwise identical.  No reference counterpart. */
 int otfwi_set_pair_streams(otfwi_model *m, int k);
 
 /* The steps per launch (1 or 2) the forward and adjoint loops use for a

@@ -160,9 +160,9 @@ struct otfwi_model {
     hipEvent_t ev[4] = {};
     // two-step loops can advance two halves of the shots on two streams (the
     // second half's launches fill the tails of the first's); fork / join events
-    hipStream_t stream2 = nullptr;
-    hipEvent_t evfj[2] = {};
-    int pair_streams = 0;            // 1, 2, or 0 = by launch size
+    hipStream_t xs[3] = {};          // extra streams of a split two-step phase
+    hipEvent_t evfj[4] = {};         // fork, and the join of each extra stream
+    int pair_streams = 0;            // 1, 2, 4, or 0 = by launch size
     double t_fwd_ms = 0, t_mis_ms = 0, t_adj_ms = 0;
     long long n_fwd_launch = 0, n_adj_launch = 0;
 };
@@ -532,7 +532,7 @@ bool use_pairs(const otfwi_model *m, int n, bool adj)
 // 64-shot launch sequence, their launches filling each other's tails).
 int pair_split(const otfwi_model *m, int n)
 {
-    if (n < 2 || !m->stream2) return 1;
+    if (n < 2 || !m->xs[0]) return 1;
     if (m->pair_streams != 0) return m->pair_streams;
     if (m->prec != 4) return 1;
     const long long wg = pair2_groups(m->X, m->Z, m->pair_nw_built, n / 2);
@@ -560,30 +560,38 @@ Pair2Args<R> pair_group(const otfwi_model *m, Pair2Args<R> q, int g0, int ng)
     return q;
 }
 
-// Launches of one pair on the batch's shots, split over the model's two
-// streams when ns == 2 (the caller forks and joins).
+// Launches of one pair on the batch's shots, split into ns groups of
+// consecutive shots, group k on the model's stream (k = 0) or extra stream
+// k - 1 (the caller forks and joins).
 template <typename R>
 hipError_t launch_pair_split(const otfwi_model *m, const Pair2Args<R> &q, int n, int ns, int mode)
 {
-    if (ns != 2) return launch_pair2<R>(q, mode, m->stream);
-    const int na = n / 2;
-    hipError_t e = launch_pair2<R>(pair_group<R>(m, q, 0, na), mode, m->stream);
-    if (e != hipSuccess) return e;
-    return launch_pair2<R>(pair_group<R>(m, q, na, n - na), mode, m->stream2);
+    if (ns <= 1) return launch_pair2<R>(q, mode, m->stream);
+    for (int k = 0; k < ns; k++) {
+        const int g0 = (int)((long long)n * k / ns), g1 = (int)((long long)n * (k + 1) / ns);
+        if (g1 <= g0) continue;
+        const hipError_t e =
+            launch_pair2<R>(pair_group<R>(m, q, g0, g1 - g0), mode, k ? m->xs[k - 1] : m->stream);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
-// Fork: work on stream2 after what is on the model's stream; join: the
-// model's stream after what is on stream2 (both recorded into a captured graph).
-int fork_streams(otfwi_model *m)
+// Fork: the ns - 1 extra streams start after what is on the model's stream;
+// join: the model's stream continues after what is on each of them (all
+// recorded into a captured graph).
+int fork_streams(otfwi_model *m, int ns)
 {
     HIPCK(hipEventRecord(m->evfj[0], m->stream));
-    HIPCK(hipStreamWaitEvent(m->stream2, m->evfj[0], 0));
+    for (int k = 1; k < ns; k++) HIPCK(hipStreamWaitEvent(m->xs[k - 1], m->evfj[0], 0));
     return OTFWI_OK;
 }
-int join_streams(otfwi_model *m)
+int join_streams(otfwi_model *m, int ns)
 {
-    HIPCK(hipEventRecord(m->evfj[1], m->stream2));
-    HIPCK(hipStreamWaitEvent(m->stream, m->evfj[1], 0));
+    for (int k = 1; k < ns; k++) {
+        HIPCK(hipEventRecord(m->evfj[k], m->xs[k - 1]));
+        HIPCK(hipStreamWaitEvent(m->stream, m->evfj[k], 0));
+    }
     return OTFWI_OK;
 }
 
@@ -787,7 +795,7 @@ int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum)
         for (int k = 0; k < lead; k++, t++) RCK(single(t));
         Pair2Args<R> q = pair_fwd_args<R>(m, n, a);
         const int ns = pair_split(m, n);
-        if (ns == 2 && pairs > 0) RCK(fork_streams(m));
+        if (ns > 1 && pairs > 0) RCK(fork_streams(m, ns));
         for (int k = 0; k < pairs; k++, t += 2) {
             // each pair writes the slot it does not read: after k pairs the
             // state sits in the other slot than one step per launch leaves it
@@ -799,7 +807,7 @@ int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum)
             q.Px0 = (const R *)b.Px[in]; q.Px1 = (R *)b.Px[out];
             HIPCK(launch_pair_split<R>(m, q, n, ns, 0));
         }
-        if (ns == 2 && pairs > 0) RCK(join_streams(m));
+        if (ns > 1 && pairs > 0) RCK(join_streams(m, ns));
         for (; t < t1; t++) RCK(single(t));
         return OTFWI_OK;
     };
@@ -921,7 +929,7 @@ int adj_loop(otfwi_model *m, int n, int t0, int t1)
         for (int k = 0; k < lead; k++, t++) RCK(single(t));
         Pair2Args<R> u = pair_rec_args<R>(m, n), l = pair_lam_args<R>(m, n);
         const int ns = pair_split(m, n);
-        if (ns == 2 && pairs > 0) RCK(fork_streams(m));
+        if (ns > 1 && pairs > 0) RCK(fork_streams(m, ns));
         for (int k = 0; k < pairs; k++, t += 2) {
             const int f = k & 1;
             const int li = ((t - 1) & 1) ^ f, ui = ((T - 2 + t) & 1) ^ f, ai = ((T - 1 + t) & 1) ^ f;
@@ -936,7 +944,7 @@ int adj_loop(otfwi_model *m, int n, int t0, int t1)
             l.Px0 = (const R *)b.LPx[li]; l.Px1 = (R *)b.LPx[li ^ 1];
             HIPCK(launch_pair_split<R>(m, l, n, ns, 1));
         }
-        if (ns == 2 && pairs > 0) RCK(join_streams(m));
+        if (ns > 1 && pairs > 0) RCK(join_streams(m, ns));
         return OTFWI_OK;
     };
     auto body = [&]() -> int {
@@ -1655,9 +1663,10 @@ int otfwi_create(otfwi_model **out, int X, int Z, int T, int precision, const fl
     m->kx_h.assign(X, 1.0f);
     m->kz_h.assign(Z, 1.0f);
     hipError_t e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->stream2, hipStreamNonBlocking);
+    for (int i = 0; i < 3 && e == hipSuccess; i++)
+        e = hipStreamCreateWithFlags(&m->xs[i], hipStreamNonBlocking);
     for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipEventCreate(&m->ev[i]);
-    for (int i = 0; i < 2 && e == hipSuccess; i++)
+    for (int i = 0; i < 4 && e == hipSuccess; i++)
         e = hipEventCreateWithFlags(&m->evfj[i], hipEventDisableTiming);
     if (e != hipSuccess) {
         delete m;
@@ -1679,12 +1688,15 @@ int otfwi_destroy(otfwi_model *m)
     dfree(m->xsnap); dfree(m->xseg); dfree(m->rxmap);
     for (int i = 0; i < 4; i++)
         if (m->ev[i]) (void)hipEventDestroy(m->ev[i]);
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < 4; i++)
         if (m->evfj[i]) (void)hipEventDestroy(m->evfj[i]);
     for (int i = 0; i < 3; i++)
         if (m->gexec[i]) (void)hipGraphExecDestroy(m->gexec[i]);
-    if (m->stream2) (void)hipStreamSynchronize(m->stream2);
-    if (m->stream2) (void)hipStreamDestroy(m->stream2);
+    for (int i = 0; i < 3; i++)
+        if (m->xs[i]) {
+            (void)hipStreamSynchronize(m->xs[i]);
+            (void)hipStreamDestroy(m->xs[i]);
+        }
     (void)hipStreamDestroy(m->stream);
     delete m;
     return OTFWI_OK;
@@ -1864,7 +1876,8 @@ int otfwi_steps_per_launch(otfwi_model *m, int n_shots, int *fwd, int *adj)
 int otfwi_set_pair_streams(otfwi_model *m, int k)
 {
     RCK(check_model(m));
-    if (k < 0 || k > 2) return fail(OTFWI_E_ARG, "pair streams must be 0 (by size), 1 or 2");
+    if (k != 0 && k != 1 && k != 2 && k != 4)
+        return fail(OTFWI_E_ARG, "pair streams must be 0 (by size), 1, 2 or 4");
     m->pair_streams = k;
     return OTFWI_OK;
 }

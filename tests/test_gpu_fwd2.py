@@ -109,14 +109,14 @@ def test_pairs_batches_and_all_sources(gpu):
 
 @pytest.mark.parametrize("graph", [True, False])
 def test_pair_streams_bitwise(gpu, graph):
-    """The two-step phase split over two streams (halves of the shots) gives
-    the one-step results, replayed from a captured graph or launched
-    directly, for an odd shot count."""
+    """The two-step phase split over two or four streams (groups of
+    consecutive shots) gives the one-step results, replayed from a captured
+    graph or launched directly, for an odd shot count."""
     p = small_problem(X=420, Z=300, T=140, pml=10, seed=11, n_src=5, n_rx=30)
     shots = [3, 0, 4, 1, 2]
     obs = _obs(p, shots)
     ref = None
-    for steps, k in ((1, 1), (2, 1), (2, 2)):
+    for steps, k in ((1, 1), (2, 1), (2, 2), (2, 4)):
         m = make_model(p, cq="cq_bg")
         m.set_graph(graph)
         m.set_steps_per_launch(steps, steps)
