@@ -377,7 +377,7 @@ __device__ __forceinline__ void fwd_compute(const FwdArgs<R> &a, const TileId &T
                                             R (*sAz)[EW], R (*sP)[EW])
 {
     using M = EMap<TH>;
-    constexpr int EH = M::EH, AZH = TH + 16, RO = TH / 16;
+    constexpr int AZH = TH + 16, RO = TH / 16;
     const int ld = a.ld, q = tid & 15, g = tid >> 4;
     const int x0 = T.x0, z0 = T.z0;
     FwdCtx<R> c{a, T.s, a.X, a.Z, ld, a.tflags[T.tl], a.Px0 + T.sb, a.Px1 + T.sb, a.P1 + T.sb,
