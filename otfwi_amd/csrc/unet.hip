@@ -84,7 +84,10 @@ struct ConvShape {
     static constexpr int PW = TX + 2, PH = TY + 2;
     static constexpr int PS = CK + 8;   // LDS stride of one patch pixel (bf16)
     static constexpr int WS = CK + 8;   // LDS stride of one weight row
-    static constexpr size_t lds_bytes = ((size_t)PH * PW * PS + (size_t)9 * TN * WS) * 2;
+    static constexpr int OS = TN + 8;   // LDS stride of one output pixel (epilogue)
+    static constexpr size_t in_bytes = ((size_t)PH * PW * PS + (size_t)9 * TN * WS) * 2;
+    static constexpr size_t out_bytes = (size_t)256 * OS * 2;
+    static constexpr size_t lds_bytes = in_bytes > out_bytes ? in_bytes : out_bytes;
 };
 
 template <int TX, int TN, int CK>
@@ -189,24 +192,44 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(ConvArgs a)
     }
 
     // C layout of the 32x32 MFMA: lane holds column (lane % 32) and rows
-    // 8*(e/4) + 4*(lane/32) + e%4 for e = 0..15.
+    // 8*(e/4) + 4*(lane/32) + e%4 for e = 0..15.  The tile goes to LDS as
+    // BF16 ([pixel][channel], the input staging is no longer read: the loop
+    // ended with a barrier), then out in 16-byte pieces of 8 channels, with
+    // the mask read the same way.
+    bf16 *ot = lds;
 #pragma unroll
     for (int j = 0; j < NJ; j++) {
-        const int co = co0 + j * 32 + r;
-        const float b = a.bias ? a.bias[co] : 0.0f;
+        const int cl = j * 32 + r;
+        const float b = a.bias ? a.bias[co0 + cl] : 0.0f;
 #pragma unroll
         for (int i = 0; i < 2; i++)
 #pragma unroll
             for (int e = 0; e < 16; e++) {
                 const int row = 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
                 const int q = wave * 64 + i * 32 + row;
-                const int y = ty0 + q / TX, x = tx0 + q % TX;
-                const size_t o = (((size_t)n * H + y) * W + x) * a.Cout + co;
                 float v = acc[i][j][e] + b;
                 if (a.relu) v = fmaxf(v, 0.0f);
-                if (a.mask && !((float)a.mask[o] > 0.0f)) v = 0.0f;
-                a.Y[o] = (bf16)v;
+                ot[q * S::OS + cl] = (bf16)v;
             }
+    }
+    __syncthreads();
+    constexpr int PV8 = TN / 8;
+    for (int v = tid; v < 256 * PV8; v += 256) {
+        const int q = v / PV8, part = v - q * PV8;
+        const int y = ty0 + q / TX, x = tx0 + q % TX;
+        const size_t o = (((size_t)n * H + y) * W + x) * a.Cout + co0 + part * 8;
+        u32x4 val = *(const u32x4 *)(ot + q * S::OS + part * 8);
+        if (a.mask) {
+            // zero where the mask is not > 0 (zero, negative or NaN)
+            const bf16x8 mk = __builtin_bit_cast(bf16x8, *(const u32x4 *)(a.mask + o));
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const unsigned lo = (float)mk[2 * k] > 0.0f ? 0x0000FFFFu : 0u;
+                const unsigned hi = (float)mk[2 * k + 1] > 0.0f ? 0xFFFF0000u : 0u;
+                val[k] &= lo | hi;
+            }
+        }
+        *(u32x4 *)(a.Y + o) = val;
     }
 }
 
