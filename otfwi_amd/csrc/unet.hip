@@ -260,23 +260,31 @@ struct WgradArgs {
     int N, Cout, tiles_per_wg;
 };
 
+// The transposed tiles' channel rows are padded by 8 (gradient) or 16
+// (patch) elements per group of 8 rows: the parts of one pixel that
+// neighbouring lanes write (channels 8 apart) then fall in different LDS
+// banks, and every 8-element fragment read stays 16-byte aligned.
 template <int TX, int MJ>
 struct WgShape {
     static constexpr int TY = 256 / TX;
     static constexpr int PW = TX + 8, PH = TY + 2;   // transposed patch row (px contiguous)
     static constexpr int GS = 256 + 8;               // transposed gradient row
-    static constexpr size_t lds_bytes = ((size_t)32 * MJ * GS + (size_t)32 * PH * PW) * 2;
+    static constexpr int CO = 32 * MJ;
+    static constexpr int GBYTES = (CO * GS + CO) * 2;
+    static constexpr size_t lds_bytes = (size_t)GBYTES + ((size_t)32 * PH * PW + 64) * 2;
+    static __device__ int gofs(int c) { return c * GS + (c >> 3) * 8; }
+    static __device__ int xofs(int c) { return c * PH * PW + (c >> 3) * 16; }
 };
 
 template <int TX, int MJ>
 __global__ __launch_bounds__(192) void wgrad_kernel(WgradArgs a)
 {
     using S = WgShape<TX, MJ>;
-    constexpr int TY = S::TY, PW = S::PW, PH = S::PH, GS = S::GS;
+    constexpr int TY = S::TY, PW = S::PW, PH = S::PH;
     constexpr int CO = 32 * MJ;
     extern __shared__ __attribute__((aligned(16))) bf16 lds[];
-    bf16 *gT = lds;                  // [CO][GS]: gradient tile, pixels contiguous
-    bf16 *xT = lds + CO * GS;        // [32][PH][PW]: input patch, pixels contiguous
+    bf16 *gT = lds;                       // [CO][GS]: gradient tile, pixels contiguous
+    bf16 *xT = lds + S::GBYTES / 2;       // [32][PH][PW]: input patch, pixels contiguous
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int H = a.in.H, W = a.in.W;
@@ -336,7 +344,7 @@ __global__ __launch_bounds__(192) void wgrad_kernel(WgradArgs a)
                 const int q = v / (CO / 8), part = v - q * (CO / 8);
                 const bf16 *gv = (const bf16 *)&rg[k];
 #pragma unroll
-                for (int e = 0; e < 8; e++) gT[(part * 8 + e) * GS + q] = gv[e];
+                for (int e = 0; e < 8; e++) gT[S::gofs(part * 8 + e) + q] = gv[e];
             }
         }
 #pragma unroll
@@ -347,13 +355,13 @@ __global__ __launch_bounds__(192) void wgrad_kernel(WgradArgs a)
                 const int py = pp / (TX + 2), px = pp - py * (TX + 2);
                 const bf16 *xs = (const bf16 *)&rx[k];
 #pragma unroll
-                for (int e = 0; e < 8; e++) xT[((part * 8 + e) * PH + py) * PW + px] = xs[e];
+                for (int e = 0; e < 8; e++) xT[S::xofs(part * 8 + e) + py * PW + px] = xs[e];
             }
         }
         __syncthreads();
         if (t + 1 < t_end) fetch(t + 1);
         if (do_bias)
-            for (int q = 0; q < 256; q++) bsum += (float)gT[tid * GS + q];
+            for (int q = 0; q < 256; q++) bsum += (float)gT[S::gofs(tid) + q];
         const int dy = wave;
 #pragma unroll 2
         for (int ks = 0; ks < 16; ks++) {
@@ -362,10 +370,10 @@ __global__ __launch_bounds__(192) void wgrad_kernel(WgradArgs a)
             bf16x8 af[MJ];
 #pragma unroll
             for (int m = 0; m < MJ; m++)
-                af[m] = *(const bf16x8 *)(gT + (m * 32 + r) * GS + q0);
+                af[m] = *(const bf16x8 *)(gT + S::gofs(m * 32 + r) + q0);
             // pixels tx+dx .. tx+dx+7 of channel r for dx = 0, 1, 2: two aligned
             // 16-byte reads (elements tx .. tx+15) and 2-byte funnel shifts
-            const bf16 *xrow = xT + (r * PH + ty + dy) * PW + tx;
+            const bf16 *xrow = xT + S::xofs(r) + (ty + dy) * PW + tx;
             const u32x4 lo = *(const u32x4 *)xrow, hi = *(const u32x4 *)(xrow + 8);
             u32x4 bw[3];
             bw[0] = lo;
