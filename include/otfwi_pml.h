@@ -81,7 +81,13 @@ enum {
     OTFWI_E_NUMERIC = -5    /* non-finite or unsupported numeric input */
 };
 
-enum { OTFWI_MISFIT_L2 = 0, OTFWI_MISFIT_W2 = 1 };
+/* Misfit kinds of the gradient, objective and misfit calls: L2 and W2
+ * (adjoint_functions.L2, .wasserstein, adjoint_functions.py:8-11, 25-67) and
+ * the squared-envelope misfit (adjoint_functions.quad_envelope,
+ * adjoint_functions.py:14-22: E = |hilbert(sim)|^2 - |hilbert(obs)|^2,
+ * misfit sum E^2, adjoint source sim E - Im(hilbert(E Im(hilbert(sim)))),
+ * hilbert as scipy.signal.hilbert forms it; FP64 on the device). */
+enum { OTFWI_MISFIT_L2 = 0, OTFWI_MISFIT_W2 = 1, OTFWI_MISFIT_ENV = 2 };
 
 typedef struct otfwi_model otfwi_model;
 
@@ -303,8 +309,8 @@ int otfwi_get_illum(otfwi_model *m, float *out, int out_on_device);
 
 /* Stand-alone misfit on host arrays (n_traces of T samples, FP64 in and
  * out, as adj_wrapper hands them over): per-trace misfit and adjoint source.
- * Replaces adjoint_functions.L2 / .wasserstein (adjoint_functions.py:8-11,
- * 25-67). */
+ * Replaces adjoint_functions.L2 / .wasserstein / .quad_envelope
+ * (adjoint_functions.py:8-11, 25-67, 14-22). */
 int otfwi_misfit(int misfit_kind, int n_traces, int T, const double *sim, const double *obs,
                  double *misfit_per_trace, double *adj_out);
 
@@ -312,7 +318,8 @@ int otfwi_misfit(int misfit_kind, int n_traces, int T, const double *sim, const 
  * on host arrays (FP64): L2 2 (sim - obs); W2 the derivative of the
  * reference W2 value (adjoint_functions.py:25-67) through its interpolated
  * inverse CDF, trapezoid CDF and normalization (needs sim + c >= 0).  The
- * trace derivative used by otfwi_gradient_exact; no reference counterpart. */
+ * trace derivative used by otfwi_gradient_exact (L2 and W2 only); no
+ * reference counterpart. */
 int otfwi_misfit_exact(int misfit_kind, int n_traces, int T, const double *sim,
                        const double *obs, double *misfit_per_trace, double *rho_out);
 

@@ -15,6 +15,8 @@ Contents, each a restatement of one reference function:
 * :func:`adjoint`        -- phase 2 of ``cuda_grad_ext`` (regression_PML_single.cu:783-836)
 * :func:`l2`             -- ``adjoint_functions.L2`` (adjoint_functions.py:8-11)
 * :func:`wasserstein`    -- ``adjoint_functions.wasserstein`` (adjoint_functions.py:25-67)
+* :func:`hilbert`        -- ``scipy.signal.hilbert`` as adjoint_functions.py:15-16 calls it
+* :func:`quad_envelope`  -- ``adjoint_functions.quad_envelope`` (adjoint_functions.py:14-22)
 * :func:`gradient`       -- ``Cuda_interface_PML.calc_grad`` (cuda_interface_pml.py:208-249)
 """
 from __future__ import annotations
@@ -287,6 +289,38 @@ def wasserstein(sim, obs):
         w, out[s] = wasserstein_trace(sim[s], obs[s])
         W2 += w
     return W2, out
+
+
+def hilbert(x):
+    """Analytic signal along the last axis, as scipy.signal.hilbert forms it
+    (scipy 1.15, signal/_signaltools.py): ifft(fft(x) * h) with h = 1 at k = 0
+    (and k = N/2 for even N), 2 for 0 < k < N/2, 0 above, h in the spectrum's
+    dtype.  numpy's FFT, like scipy.fft, keeps float32 input in single
+    precision (complex64)."""
+    x = np.asarray(x)
+    N = x.shape[-1]
+    Xf = np.fft.fft(x, axis=-1)
+    h = np.zeros(N, dtype=Xf.dtype)
+    if N % 2 == 0:
+        h[0] = h[N // 2] = 1
+        h[1:N // 2] = 2
+    else:
+        h[0] = 1
+        h[1:(N + 1) // 2] = 2
+    return np.fft.ifft(Xf * h, axis=-1)
+
+
+def quad_envelope(sim, obs):
+    """adjoint_functions.quad_envelope (:14-22): E = |hilbert(sim)|^2 -
+    |hilbert(obs)|^2 per sample, misfit sum E^2 over the whole (n_rx, T)
+    array, adjoint source sim E - Im(hilbert(E Im(hilbert(sim)))).  The
+    operands keep the dtypes the reference gives them (sim float64 from the
+    callback, obs as stored)."""
+    hilb_obs = hilbert(obs)
+    hilb_sim = hilbert(sim)
+    temp = np.abs(hilb_sim) ** 2 - np.abs(hilb_obs) ** 2
+    adj = sim * temp - np.imag(hilbert(temp * np.imag(hilb_sim)))
+    return np.sum(temp ** 2), adj
 
 
 def illumination(cquad, d_x, d_z, src_xz, src, T, shots, stride, precision="ref"):

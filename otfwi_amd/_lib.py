@@ -14,7 +14,7 @@ import numpy as np
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libotfwi_pml.so")
 
 OK, E_ARG, E_HIP, E_MEM, E_STATE, E_NUMERIC = 0, -1, -2, -3, -4, -5
-MISFIT_L2, MISFIT_W2 = 0, 1
+MISFIT_L2, MISFIT_W2, MISFIT_ENV = 0, 1, 2
 
 # (name, restype, argtypes) of every exported symbol declared in include/otfwi_pml.h
 _vp = ctypes.c_void_p

@@ -9,8 +9,12 @@ float64)``.
   affinely shifted, normalized traces, with the unit-spacing transport-map
   integral as adjoint source; one wavefront per trace with FP64 CDF scans and
   inverse-CDF searches.
+* :func:`quad_envelope` -- adjoint_functions.py:14-22: squared-envelope
+  difference E = |hilbert(sim)|^2 - |hilbert(obs)|^2, misfit sum E^2, adjoint
+  source sim E - Im(hilbert(E Im(hilbert(sim)))); the Hilbert transforms are
+  FP64 circular convolutions with scipy.signal.hilbert's kernel.
 
-:class:`otfwi_amd.cuda_interface_pml.Cuda_interface_PML` recognizes these two
+:class:`otfwi_amd.cuda_interface_pml.Cuda_interface_PML` recognizes these
 functions and keeps the whole gradient on the device; any other callable is
 called on the host once per shot, as the reference's callback does.
 """
@@ -21,7 +25,7 @@ import numpy as np
 from . import _lib
 from ._lib import check, ptr
 
-__all__ = ["L2", "wasserstein"]
+__all__ = ["L2", "wasserstein", "quad_envelope"]
 
 
 def _run(kind, sim, obs):
@@ -53,6 +57,12 @@ def wasserstein(sim, obs):
     return _run(_lib.MISFIT_W2, sim, obs)
 
 
+def quad_envelope(sim, obs):
+    """Squared-envelope misfit and adjoint source -- adjoint_functions.py:14-22."""
+    return _run(_lib.MISFIT_ENV, sim, obs)
+
+
 # markers the gradient driver uses to keep these misfits on the device
 L2._otfwi_misfit = "l2"
 wasserstein._otfwi_misfit = "w2"
+quad_envelope._otfwi_misfit = "env"

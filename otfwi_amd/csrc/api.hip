@@ -112,6 +112,8 @@ struct otfwi_model {
     // two-step kernels: rows holding a source or receiver per column block,
     // the per-cell event map {ring+1, rx+1, src+1, 0}, the ring-cell tables
     uint8_t *evrow = nullptr;
+    double *hilb = nullptr;          // hilbert_table(T) on the device (envelope misfit)
+    int hilb_n = 0;
     unsigned *ringtab = nullptr;
     int *pmap = nullptr;
     // time steps per forward / adjoint launch: 1, 2, or 0 = by launch size
@@ -1070,6 +1072,26 @@ int check_w2_flags(const otfwi_model *m, const std::vector<int> &fl, int s0)
     return OTFWI_OK;
 }
 
+bool misfit_kind_ok(int k)
+{
+    return k == OTFWI_MISFIT_L2 || k == OTFWI_MISFIT_W2 || k == OTFWI_MISFIT_ENV;
+}
+
+// The envelope misfit's Hilbert kernel for this model's T, on the device.
+int ensure_hilb(otfwi_model *m)
+{
+    if (m->hilb && m->hilb_n == m->T) return OTFWI_OK;
+    std::vector<double> c(m->T);
+    hilbert_table(m->T, c.data());
+    dfree(m->hilb);
+    m->hilb = nullptr;
+    m->hilb_n = 0;
+    RCK(dalloc(&m->hilb, c.size()));
+    HIPCK(hipMemcpy(m->hilb, c.data(), c.size() * sizeof(double), hipMemcpyHostToDevice));
+    m->hilb_n = m->T;
+    return OTFWI_OK;
+}
+
 // One batch of a gradient: forward with ring save, misfit + adjoint source
 // (device kind) or host adjoint source already in b.adj (kind < 0), adjoint
 // run, and accumulation of the batch's per-shot gradients into m->gacc.
@@ -1088,7 +1110,7 @@ int grad_batch_device(otfwi_model *m, int s0, int nb, const int *shot_src, int k
     int *flags = reinterpret_cast<int *>(b.scratch + (size_t)ntr * 3 * m->T);
     if (kind == OTFWI_MISFIT_W2) HIPCK(hipMemsetAsync(flags, 0, ntr * sizeof(int), m->stream));
     HIPCK((launch_misfit<R, float>(kind, (const R *)b.rec, b.obs, (R *)b.adj, (R *)b.jcum,
-                                   b.mis_tr, ntr, m->T, b.scratch, m->stream)));
+                                   b.mis_tr, ntr, m->T, b.scratch, m->stream, m->hilb)));
     HIPCK(launch_reduce_misfit(b.mis_tr, nb, m->n_rx, b.mis_shot, m->stream));
     HIPCK(hipEventRecord(m->ev[2], m->stream));
     RCK(run_adjoint<R>(m, nb));
@@ -1144,8 +1166,8 @@ template <typename R>
 int gradient_impl(otfwi_model *m, int n, const int *shot_src, int kind, double *mis_out,
                   float *grad_out, bool out_dev, float *rec_out)
 {
-    if (kind != OTFWI_MISFIT_L2 && kind != OTFWI_MISFIT_W2)
-        return fail(OTFWI_E_ARG, "unknown misfit kind %d", kind);
+    if (!misfit_kind_ok(kind)) return fail(OTFWI_E_ARG, "unknown misfit kind %d", kind);
+    if (kind == OTFWI_MISFIT_ENV) RCK(ensure_hilb(m));
     if (n <= 0) return fail(OTFWI_E_ARG, "n_shots must be positive");
     if (m->n_rx <= 0) return fail(OTFWI_E_STATE, "receivers not set");
     if (kind == OTFWI_MISFIT_W2 && m->T < 2) return fail(OTFWI_E_ARG, "W2 needs T >= 2");
@@ -1174,8 +1196,8 @@ int gradient_impl(otfwi_model *m, int n, const int *shot_src, int kind, double *
 template <typename R>
 int objective_impl(otfwi_model *m, int n, const int *shot_src, int kind, double *mis_out)
 {
-    if (kind != OTFWI_MISFIT_L2 && kind != OTFWI_MISFIT_W2)
-        return fail(OTFWI_E_ARG, "unknown misfit kind %d", kind);
+    if (!misfit_kind_ok(kind)) return fail(OTFWI_E_ARG, "unknown misfit kind %d", kind);
+    if (kind == OTFWI_MISFIT_ENV) RCK(ensure_hilb(m));
     if (n <= 0) return fail(OTFWI_E_ARG, "n_shots must be positive");
     if (m->n_rx <= 0) return fail(OTFWI_E_STATE, "receivers not set");
     if (kind == OTFWI_MISFIT_W2 && m->T < 2) return fail(OTFWI_E_ARG, "W2 needs T >= 2");
@@ -1192,7 +1214,7 @@ int objective_impl(otfwi_model *m, int n, const int *shot_src, int kind, double 
         int *flags = reinterpret_cast<int *>(b.scratch + (size_t)ntr * 3 * m->T);
         if (kind == OTFWI_MISFIT_W2) HIPCK(hipMemsetAsync(flags, 0, ntr * sizeof(int), m->stream));
         HIPCK((launch_misfit<R, float>(kind, (const R *)b.rec, b.obs, (R *)b.adj, nullptr,
-                                       b.mis_tr, ntr, m->T, b.scratch, m->stream)));
+                                       b.mis_tr, ntr, m->T, b.scratch, m->stream, m->hilb)));
         HIPCK(launch_reduce_misfit(b.mis_tr, nb, m->n_rx, b.mis_shot, m->stream));
         std::vector<double> hmis(nb);
         HIPCK(hipMemcpyAsync(hmis.data(), b.mis_shot, nb * sizeof(double), hipMemcpyDeviceToHost,
@@ -1302,7 +1324,7 @@ int exact_gradient_impl(otfwi_model *m, int n, const int *shot_src, int kind, in
                         double *mis_out, float *grad_out, double *grad64_out)
 {
     if (kind != OTFWI_MISFIT_L2 && kind != OTFWI_MISFIT_W2)
-        return fail(OTFWI_E_ARG, "unknown misfit kind %d", kind);
+        return fail(OTFWI_E_ARG, "misfit kind %d: the exact gradient supports L2 and W2", kind);
     if (n <= 0) return fail(OTFWI_E_ARG, "n_shots must be positive");
     if (m->n_rx <= 0) return fail(OTFWI_E_STATE, "receivers not set");
     if (m->T < 2) return fail(OTFWI_E_ARG, "exact gradient needs T >= 2");
@@ -1683,7 +1705,7 @@ int otfwi_destroy(otfwi_model *m)
     free_batch(m->b);
     dfree(m->cm); dfree(m->kx); dfree(m->kz); dfree(m->icum); dfree(m->cq32);
     dfree(m->evt); dfree(m->pts); dfree(m->tf_fwd); dfree(m->tf_adj); dfree(m->evrow);
-    dfree(m->pmap); dfree(m->ringtab);
+    dfree(m->pmap); dfree(m->ringtab); dfree(m->hilb);
     dfree(m->gacc); dfree(m->gout); dfree(m->ill);
     dfree(m->xsnap); dfree(m->xseg); dfree(m->rxmap);
     for (int i = 0; i < 4; i++)
@@ -2098,18 +2120,23 @@ namespace {
 int misfit_host(int kind, bool exact, int n_traces, int T, const double *sim, const double *obs,
                 double *mis_out, double *adj_out)
 {
-    if (kind != OTFWI_MISFIT_L2 && kind != OTFWI_MISFIT_W2)
-        return fail(OTFWI_E_ARG, "unknown misfit kind %d", kind);
+    if (!misfit_kind_ok(kind) || (exact && kind == OTFWI_MISFIT_ENV))
+        return fail(OTFWI_E_ARG, "unknown misfit kind %d%s", kind,
+                    exact ? " (the exact derivative supports L2 and W2)" : "");
     if (n_traces <= 0 || T < 2 || !sim || !obs)
         return fail(OTFWI_E_ARG, "bad misfit arguments (n_traces=%d, T=%d)", n_traces, T);
     if (kind == OTFWI_MISFIT_W2 && (size_t)2 * T * sizeof(double) > 160 * 1024)
         return fail(OTFWI_E_ARG, "W2 supports T <= %d", (int)(160 * 1024 / 16));
     const size_t cnt = (size_t)n_traces * T;
     double *dsim = nullptr, *dobs = nullptr, *dadj = nullptr, *dmis = nullptr, *dscr = nullptr;
+    double *dtab = nullptr;
     int rc = OTFWI_OK;
-    auto cleanup = [&] { dfree(dsim); dfree(dobs); dfree(dadj); dfree(dmis); dfree(dscr); };
+    auto cleanup = [&] {
+        dfree(dsim); dfree(dobs); dfree(dadj); dfree(dmis); dfree(dscr); dfree(dtab);
+    };
     if ((rc = dalloc(&dsim, cnt)) || (rc = dalloc(&dobs, cnt)) || (rc = dalloc(&dadj, cnt)) ||
-        (rc = dalloc(&dmis, (size_t)n_traces)) || (rc = dalloc(&dscr, cnt * 3 + n_traces))) {
+        (rc = dalloc(&dmis, (size_t)n_traces)) || (rc = dalloc(&dscr, cnt * 3 + n_traces)) ||
+        (kind == OTFWI_MISFIT_ENV && (rc = dalloc(&dtab, (size_t)T)))) {
         cleanup();
         return rc;
     }
@@ -2117,11 +2144,16 @@ int misfit_host(int kind, bool exact, int n_traces, int T, const double *sim, co
     hipError_t e = hipMemcpy(dsim, sim, cnt * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(dobs, obs, cnt * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(dscr, 0, (cnt * 3 + n_traces) * sizeof(double));
+    if (e == hipSuccess && dtab) {
+        std::vector<double> tab(T);
+        hilbert_table(T, tab.data());
+        e = hipMemcpy(dtab, tab.data(), T * sizeof(double), hipMemcpyHostToDevice);
+    }
     if (e == hipSuccess)
         e = exact ? launch_misfit_exact<double, double>(kind, dsim, dobs, dadj, dmis, n_traces, T,
                                                         dscr, flags, nullptr)
                   : launch_misfit<double, double>(kind, dsim, dobs, dadj, nullptr, dmis, n_traces,
-                                                  T, dscr, nullptr);
+                                                  T, dscr, nullptr, dtab);
     std::vector<int> fl(n_traces, 0);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e == hipSuccess && mis_out)

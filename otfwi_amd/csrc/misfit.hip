@@ -19,6 +19,8 @@
 // scratch area first (rank sort: O(T^2) but only for such traces).
 #include "otfwi_internal.h"
 
+#include <cmath>
+
 namespace otfwi {
 
 constexpr int WAVE = 64;
@@ -412,6 +414,148 @@ l2_kernel(const R *sim, const O *obs, R *adj, double *mis, int T)
     if (lane == 0) mis[tr] = w;
 }
 
+// ---------------------------------------------------------------------------
+// Squared-envelope misfit (adjoint_functions.quad_envelope,
+// adjoint_functions.py:14-22):
+//   Hs = hilbert(sim), Ho = hilbert(obs) (analytic signals, per trace);
+//   E = |Hs|^2 - |Ho|^2; misfit = sum E^2;
+//   adj = sim E - Im(hilbert(E Im(Hs))).
+// scipy.signal.hilbert is ifft(fft(x) h) with h = 1 at k = 0 (and k = N/2 for
+// even N), 2 for 0 < k < N/2, 0 above.  Its real part is x and its imaginary
+// part is the circular convolution y[n] = sum_m c[(n - m) mod N] x[m] with
+// c[j] = (2/N) sum_{k=1}^{K} sin(2 pi k j / N), K = ceil(N/2) - 1
+// (hilbert_table).  Here y is formed in FP64 as a tiled product of the
+// traces with the circulant matrix, without an FFT.
+// ---------------------------------------------------------------------------
+void hilbert_table(int N, double *c)
+{
+    // sum_{k=1}^{K} sin(k t) = sin(K t / 2) sin((K + 1) t / 2) / sin(t / 2),
+    // t = 2 pi j / N; the arguments are reduced modulo 2 pi in integers
+    const int K = (N + 1) / 2 - 1;
+    const double pi = 3.14159265358979323846;
+    auto sinpi = [&](long long num) {   // sin(pi num / N)
+        const long long r = ((num % (2LL * N)) + 2LL * N) % (2LL * N);
+        return std::sin(pi * (double)r / (double)N);
+    };
+    c[0] = 0.0;
+    for (int j = 1; j < N; j++) {
+        const double v = sinpi((long long)K * j) * sinpi((long long)(K + 1) * j) / sinpi(j);
+        c[j] = 2.0 * v / (double)N;
+    }
+}
+
+// y[tr][n] = sum_m c[(n - m) mod N] x[tr][m] for a 64-trace x 64-sample tile
+// per work-group: the K dimension (m) in chunks of 32 through LDS, the 95
+// circulant values a chunk needs staged beside it; each thread keeps a 4 x 4
+// block of outputs (traces ty + 16 i, samples tx + 16 j).
+constexpr int HB = 64, HK = 32;
+template <typename I>
+__global__ void __launch_bounds__(256)
+hilbert_kernel(const I *x, double *y, const double *c, int ntr, int N)
+{
+    __shared__ double xs[HB][HK + 1];
+    __shared__ double cs[HB + HK];
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    const int n0 = blockIdx.x * HB, t0 = blockIdx.y * HB;
+    double acc[4][4] = {};
+    for (int m0 = 0; m0 < N; m0 += HK) {
+        for (int v = tid; v < HB * HK; v += 256) {
+            const int r = v / HK, k = v - r * HK;
+            const int tr = t0 + r, m = m0 + k;
+            xs[r][k] = (tr < ntr && m < N) ? (double)x[(size_t)tr * N + m] : 0.0;
+        }
+        // cs[d] = c[(n0 - m0 - (HK - 1) + d) mod N], d = 0 .. HB + HK - 2
+        for (int d = tid; d < HB + HK - 1; d += 256) {
+            int j = (n0 - m0 - (HK - 1) + d) % N;
+            if (j < 0) j += N;
+            cs[d] = c[j];
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int k = 0; k < HK; k++) {
+            double xv[4], cv[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) xv[i] = xs[ty + 16 * i][k];
+#pragma unroll
+            for (int j = 0; j < 4; j++) cv[j] = cs[tx + 16 * j - k + (HK - 1)];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[i][j] = fma(cv[j], xv[i], acc[i][j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int tr = t0 + ty + 16 * i;
+        if (tr >= ntr) continue;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int n = n0 + tx + 16 * j;
+            if (n < N) y[(size_t)tr * N + n] = acc[i][j];
+        }
+    }
+}
+
+// E = |Hs|^2 - |Ho|^2 into e, q = E Im(Hs) into q (may alias hs / ho: each
+// element is read before it is written), misfit of the trace = sum E^2.
+template <typename R, typename O>
+__global__ void __launch_bounds__(WAVE)
+env_mid_kernel(const R *sim, const O *obs, const double *hs, const double *ho, double *e,
+               double *q, double *mis, int T)
+{
+    const int tr = blockIdx.x, lane = threadIdx.x;
+    double w = 0.0;
+    for (int k = lane; k < T; k += WAVE) {
+        const size_t i = (size_t)tr * T + k;
+        const double s = (double)sim[i], o = (double)obs[i], a = hs[i], b = ho[i];
+        const double E = (s * s + a * a) - (o * o + b * b);
+        e[i] = E;
+        q[i] = E * a;
+        w += E * E;
+    }
+    w = wave_sum(w);
+    if (lane == 0) mis[tr] = w;
+}
+
+// adj = sim E - Im(hilbert(E Im(Hs)))
+template <typename R>
+__global__ void env_adj_kernel(const R *sim, const double *e, const double *hq, R *adj, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) adj[i] = (R)((double)sim[i] * e[i] - hq[i]);
+}
+
+template <typename I>
+hipError_t launch_hilbert(const I *x, double *y, const double *c, int ntr, int N, hipStream_t s)
+{
+    const dim3 grid((N + HB - 1) / HB, (ntr + HB - 1) / HB);
+    hipLaunchKernelGGL((hilbert_kernel<I>), grid, dim3(256), 0, s, x, y, c, ntr, N);
+    return hipGetLastError();
+}
+
+// Envelope misfit of ntraces traces: scratch holds 3 ntraces T doubles.
+template <typename R, typename O>
+hipError_t launch_envelope(const R *sim, const O *obs, R *adj, double *mis, int ntraces, int T,
+                           double *scratch, const double *hilb, hipStream_t s)
+{
+    if (!hilb) return hipErrorInvalidValue;
+    const size_t n = (size_t)ntraces * T;
+    double *hs = scratch, *ho = scratch + n, *hq = scratch + 2 * n;
+    hipError_t e = launch_hilbert<R>(sim, hs, hilb, ntraces, T, s);
+    if (e == hipSuccess) e = launch_hilbert<O>(obs, ho, hilb, ntraces, T, s);
+    if (e != hipSuccess) return e;
+    // E over hs, q over ho
+    hipLaunchKernelGGL((env_mid_kernel<R, O>), dim3(ntraces), dim3(WAVE), 0, s, sim, obs, hs, ho,
+                       hs, ho, mis, T);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = launch_hilbert<double>(ho, hq, hilb, ntraces, T, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((env_adj_kernel<R>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, sim,
+                       hs, hq, adj, n);
+    return hipGetLastError();
+}
+
 // jcum[t] = sum_{tau=1..t} adj[T-1-tau] (jcum[0] = 0): the running integral of
 // the flipped adjoint source (somaFonteIntegral with flip=1,
 // regression_PML_single.cu:276-292, 803), accumulated in FP64 then rounded.
@@ -446,12 +590,15 @@ template hipError_t launch_jcum<double>(const double *, double *, int, int, hipS
 
 template <typename R, typename O>
 hipError_t launch_misfit(int kind, const R *sim, const O *obs, R *adj, R *jcum, double *mis,
-                         int ntraces, int T, double *scratch, hipStream_t s)
+                         int ntraces, int T, double *scratch, hipStream_t s, const double *hilb)
 {
     if (ntraces <= 0) return hipSuccess;
     if (kind == 0) {
         hipLaunchKernelGGL((l2_kernel<R, O>), dim3(ntraces), dim3(WAVE), 0, s, sim, obs, adj, mis,
                            T);
+    } else if (kind == 2) {
+        const hipError_t e = launch_envelope<R, O>(sim, obs, adj, mis, ntraces, T, scratch, hilb, s);
+        if (e != hipSuccess) return e;
     } else {
         const size_t lds = (size_t)2 * T * sizeof(double);
         hipError_t e = hipFuncSetAttribute((const void *)w2_kernel<R, O>,
@@ -468,13 +615,13 @@ hipError_t launch_misfit(int kind, const R *sim, const O *obs, R *adj, R *jcum, 
 }
 template hipError_t launch_misfit<float, float>(int, const float *, const float *, float *,
                                                 float *, double *, int, int, double *,
-                                                hipStream_t);
+                                                hipStream_t, const double *);
 template hipError_t launch_misfit<double, float>(int, const double *, const float *, double *,
                                                  double *, double *, int, int, double *,
-                                                 hipStream_t);
+                                                 hipStream_t, const double *);
 template hipError_t launch_misfit<double, double>(int, const double *, const double *, double *,
                                                   double *, double *, int, int, double *,
-                                                  hipStream_t);
+                                                  hipStream_t, const double *);
 
 template <typename R, typename O>
 hipError_t launch_misfit_exact(int kind, const R *sim, const O *obs, R *rho, double *mis,

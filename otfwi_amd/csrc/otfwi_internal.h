@@ -175,11 +175,17 @@ hipError_t launch_pair2(const Pair2Args<R> &a, int mode, hipStream_t s);
 int step_occupancy(bool adj, int prec, int th, bool persistent);
 
 // Misfit / reduction launchers (misfit.hip).
-// kind: 0 = L2, 1 = W2.  sim/obs/adj/jcum: [nshots][n_rx][T]; mis: [nshots*n_rx] FP64 per trace.
-// jcum may be null (no running sum wanted).  O is the observed-trace type.
+// kind: 0 = L2, 1 = W2, 2 = squared envelope.  sim/obs/adj/jcum: [nshots][n_rx][T]; mis:
+// [nshots*n_rx] FP64 per trace.  jcum may be null (no running sum wanted).  O is the
+// observed-trace type.  scratch: 3 ntraces T doubles (+ ntraces ints of W2 flags).
+// hilb: hilbert_table(T) on the device (kind 2 only).
 template <typename R, typename O>
 hipError_t launch_misfit(int kind, const R *sim, const O *obs, R *adj, R *jcum, double *mis,
-                         int ntraces, int T, double *scratch, hipStream_t s);
+                         int ntraces, int T, double *scratch, hipStream_t s,
+                         const double *hilb = nullptr);
+// c[0..N): the circular-convolution kernel whose product with a trace is the
+// imaginary part of scipy.signal.hilbert's analytic signal (host, FP64).
+void hilbert_table(int N, double *c);
 // Exact derivative of the misfit value w.r.t. the simulated traces (rho):
 // kind 0: 2 (sim - obs); kind 1: derivative of the reference W2 value
 // (needs a monotone CDF of sim + c; flags as launch_misfit,

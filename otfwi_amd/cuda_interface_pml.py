@@ -35,8 +35,8 @@ def _device_kind(adj_fun):
         return kind
     mod = getattr(adj_fun, "__module__", "") or ""
     name = getattr(adj_fun, "__name__", "")
-    if mod.split(".")[-1] == "adjoint_functions" and name in ("L2", "wasserstein"):
-        return {"L2": "l2", "wasserstein": "w2"}[name]
+    if mod.split(".")[-1] == "adjoint_functions" and name in ("L2", "wasserstein", "quad_envelope"):
+        return {"L2": "l2", "wasserstein": "w2", "quad_envelope": "env"}[name]
     return None
 
 

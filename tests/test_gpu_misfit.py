@@ -63,3 +63,27 @@ def test_w2_all_zero_observed_is_nan_like_reference(gpu):
     obs[1] = 0.0
     w, a = AF.wasserstein(sim, obs)
     assert np.isnan(w) and np.isnan(a[1]).all() and np.isfinite(a[0]).all()
+
+
+@pytest.mark.parametrize("n,T", [(1, 64), (4, 1499), (3, 4000)])
+def test_envelope_matches_oracle(gpu, n, T):
+    """Squared-envelope misfit (adjoint_functions.py:14-22) on the device
+    against the oracle in FP64 (obs given as float64: 1e-10), and against the
+    reference's own precision mix (obs float32, so its analytic signal is
+    complex64: 1e-5)."""
+    sim, obs = traces(n, T, seed=T + 7 * n)
+    m, a = AF.quad_envelope(sim, obs)
+    m0, a0 = O.quad_envelope(sim, obs)
+    m1, a1 = O.quad_envelope(sim, obs.astype(np.float32))
+    em, ea = abs(m - m0) / abs(m0), np.linalg.norm(a - a0) / np.linalg.norm(a0)
+    em1, ea1 = abs(m - m1) / abs(m1), np.linalg.norm(a - a1) / np.linalg.norm(a1)
+    print(f"n={n} T={T}: vs FP64 misfit {em:.2e} adj {ea:.2e}; vs reference mix "
+          f"misfit {em1:.2e} adj {ea1:.2e}")
+    assert em <= 1e-10 and ea <= 1e-10
+    assert em1 <= 1e-5 and ea1 <= 1e-5
+
+
+def test_envelope_identical_traces(gpu):
+    sim, _ = traces(2, 500, seed=5)
+    m, a = AF.quad_envelope(sim, sim.copy())
+    assert m == 0.0 and not np.any(a)

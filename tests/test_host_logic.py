@@ -115,3 +115,17 @@ def test_array_mask_shape():
     assert m[:20].max() == 0 and m[:, :8].max() == 0 and m[-8:].max() == 0
     assert m[30, 30] == 1.0
     assert np.all(np.diff(m[20:26, 30]) > 0)
+
+
+def test_device_misfit_kinds():
+    """The drop-in keeps L2, W2 and the squared-envelope misfit on the device
+    (by marker or by name from an adjoint_functions module) and any other
+    callable on the host callback path."""
+    from otfwi_amd import _lib
+    from otfwi_amd import adjoint_functions as AF
+    from otfwi_amd.cuda_interface_pml import _device_kind
+    from otfwi_amd.model import PMLModel  # noqa: F401  (import only)
+
+    assert [_device_kind(f) for f in (AF.L2, AF.wasserstein, AF.quad_envelope)] == ["l2", "w2", "env"]
+    assert _device_kind(lambda s, o: (0.0, s)) is None
+    assert (_lib.MISFIT_L2, _lib.MISFIT_W2, _lib.MISFIT_ENV) == (0, 1, 2)
