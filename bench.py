@@ -230,7 +230,7 @@ def main():
     ap.add_argument("--shots", type=int, default=64)
     ap.add_argument("--th-fwd", type=int, default=0)
     ap.add_argument("--th-adj", type=int, default=0)
-    ap.add_argument("--misfit", default="w2", choices=["w2", "l2"])
+    ap.add_argument("--misfit", default="w2", choices=["w2", "l2", "env"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--pair-streams", type=int, default=0, choices=[0, 1, 2, 4],
