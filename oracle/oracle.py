@@ -17,6 +17,8 @@ Contents, each a restatement of one reference function:
 * :func:`wasserstein`    -- ``adjoint_functions.wasserstein`` (adjoint_functions.py:25-67)
 * :func:`hilbert`        -- ``scipy.signal.hilbert`` as adjoint_functions.py:15-16 calls it
 * :func:`quad_envelope`  -- ``adjoint_functions.quad_envelope`` (adjoint_functions.py:14-22)
+* :func:`cpml_forward`   -- ``cuda_simulate2`` of the second-order C-PML scheme
+                            (regression_CPML.cu:427-501), via the C library
 * :func:`gradient`       -- ``Cuda_interface_PML.calc_grad`` (cuda_interface_pml.py:208-249)
 """
 from __future__ import annotations
@@ -365,3 +367,25 @@ def gradient(cquad, d_x, d_z, src_xz, src, rx_xz, ring_xz, obs, T, misfit,
         mses.append(float(mse))
         recs.append(fw["rec"])
     return np.mean(grads, axis=0), float(np.mean(mses)), np.array(recs)
+
+
+def cpml_forward(p, idx=-1):
+    """One run of the second-order C-PML scheme (regression_CPML.cu:427-501,
+    oracle_cpml.c) for the problem dict p (keys as cpml_problem in
+    tests/problems.py builds them); returns rec (n_rx, T) float32."""
+    L = lib()
+    f = L.orc_cpml_forward
+    f.restype = None
+    X, Z, T = p["X"], p["Z"], p["T"]
+    i32 = lambda a: np.ascontiguousarray(a, np.int32)
+    f32 = lambda a: np.ascontiguousarray(a, np.float32)
+    sx, sz, rx, rz = i32(p["sx"]), i32(p["sz"]), i32(p["rx"]), i32(p["rz"])
+    arrs = [f32(p[k]) for k in ("cquad", "kappa", "rho_hx", "rho_hz", "K_x", "a_x", "b_x",
+                                "K_xh", "a_xh", "b_xh", "K_z", "a_z", "b_z", "K_zh", "a_zh",
+                                "b_zh", "src", "initial")]
+    rec = np.zeros((len(rx), T), np.float32)
+    c = ctypes
+    f(c.c_int(X), c.c_int(Z), c.c_int(T), c.c_int(len(sx)), c.c_int(len(rx)), c.c_int(p["ppe"]),
+      _p(sx), _p(sz), _p(rx), _p(rz), c.c_float(p["dx"]), c.c_float(p["dz"]), c.c_float(p["dt2"]),
+      c.c_float(p["qt_pi"]), *[_p(a) for a in arrs], c.c_int(idx), _p(rec))
+    return rec
