@@ -131,3 +131,55 @@ def learn_config(model_seed=0, N=256, T=1200, n_elem=32, pitch=6, pml=20, n_max=
     return dict(X=N, Z=N, T=T, pml=pml, R=1e-3, c_true=c_true, c_bg=c_bg,
                 src_mask=mask, rx_mask=mask.copy(), src=src, dx=dx, dt=dt,
                 n_shots=n_elem)
+
+
+def cpml_profiles(n, L, dt, d0, alpha_max, kmax=1.0, half=False):
+    """C-PML coefficients along one axis of n cells with an L-cell layer at
+    each end: d = d0 q^2, K = 1 + (kmax - 1) q^2, alpha = alpha_max (1 - q)
+    inside the layer (q: depth into it, 0..1, at the cell or, with half, at
+    the half point i + 1/2), b = exp(-(d/K + alpha) dt),
+    a = d (b - 1) / (K (d + K alpha)).  Returns (K, a, b) float32."""
+    pos = np.arange(n, dtype=np.float64) + (0.5 if half else 0.0)
+    depth = np.maximum(np.maximum(L - pos, pos - (n - 1 - L)), 0.0) / L
+    d = d0 * depth ** 2
+    K = 1.0 + (kmax - 1.0) * depth ** 2
+    alpha = alpha_max * (1.0 - depth) * (depth > 0)
+    b = np.exp(-(d / K + alpha) * dt)
+    a = np.where(d > 1e-9, d * (b - 1.0) / (K * (d + K * alpha) + 1e-300), 0.0)
+    return K.astype(np.float32), a.astype(np.float32), b.astype(np.float32)
+
+
+def cpml_config(X, Z, T, n_src, n_rx, L=20, c0=0.4, f0=0.05, seed=0, ppe=1):
+    """Second-order C-PML setup (the inputs of regression_CPML.cu's
+    init_memory_sim) in normalized units (dx = dz = dt = 1): velocity c0 with
+    a slower blob, density 1 with a denser blob (half-point densities as
+    arithmetic means), sources and receivers on row L + 2, Ricker wavelets of
+    f0 cycles per sample, reflection 1e-3 over the layer."""
+    rng = np.random.default_rng(seed)
+    zz, xx = np.mgrid[0:Z, 0:X]
+    c = np.full((Z, X), c0)
+    cz, cx = rng.uniform(0.5 * Z, 0.7 * Z), rng.uniform(0.35 * X, 0.65 * X)
+    c[(xx - cx) ** 2 + (zz - cz) ** 2 <= (0.1 * min(X, Z)) ** 2] = 0.7 * c0
+    rho = np.ones((Z, X))
+    rho[(xx - 0.5 * X) ** 2 + (zz - 0.4 * Z) ** 2 <= (0.08 * min(X, Z)) ** 2] = 1.5
+    rho_hx = rho.copy()
+    rho_hx[:, :-1] = 0.5 * (rho[:, :-1] + rho[:, 1:])
+    rho_hz = rho.copy()
+    rho_hz[:-1, :] = 0.5 * (rho[:-1, :] + rho[1:, :])
+    d0 = 3.0 * c0 * np.log(1e3) / (2.0 * L)
+    prof = {}
+    for ax, n in (("x", X), ("z", Z)):
+        for half in (False, True):
+            K, a, b = cpml_profiles(n, L, 1.0, d0, np.pi * f0, 1.0, half)
+            h = "h" if half else ""
+            prof[f"K_{ax}{h}"], prof[f"a_{ax}{h}"], prof[f"b_{ax}{h}"] = K, a, b
+    sx = np.round(np.linspace(L + 6, X - L - 7, n_src)).astype(np.int32)
+    rx = np.round(np.linspace(L + 3, X - L - 4, n_rx)).astype(np.int32)
+    w = ricker(f0, 1.0, T, t0=1.2 / f0)
+    src = np.tile(w, (n_src, 1)) * (1.0 + 0.1 * np.arange(n_src))[:, None]
+    return dict(X=X, Z=Z, T=T, ppe=ppe, dx=1.0, dz=1.0, dt=1.0, dt2=1.0, qt_pi=4 * np.pi,
+                sx=sx, sz=np.full(n_src, L + 2, np.int32), rx=rx,
+                rz=np.full(n_rx, L + 2, np.int32), cquad=(c ** 2).astype(np.float32),
+                kappa=(rho * c ** 2).astype(np.float32), rho=rho.astype(np.float32),
+                rho_hx=rho_hx.astype(np.float32), rho_hz=rho_hz.astype(np.float32),
+                src=src.astype(np.float32), initial=np.zeros((2, Z, X), np.float32), **prof)

@@ -9,6 +9,12 @@ c3     16 independent models with grid sizes drawn from {512, 768, 1024,
        1 and with --workers host threads.
 exact  one checkpointed exact-adjoint gradient (L2) of the 1024^2 FMC block,
        beside the reference-mode gradient of the same shots.
+cpml   the second-order C-PML scheme (regression_CPML.cu): --shots runs of a
+       --N^2 grid, T = --T steps, each shot firing its own source, advanced in
+       one batch; 76 algorithmic bytes per cell-step (frame 1: P, 2 memory
+       variables read, 2 written, 2 derivative fields written, 2 densities
+       read; frame 2: the 2 derivative fields, 2 memory variables read and
+       written, P twice, kappa, P written).
 """
 from __future__ import annotations
 
@@ -97,14 +103,37 @@ def run_exact(N=1024, T=4000, shots=8, snap=0):
             "interior_cosine_exact_vs_reference": cos}
 
 
+CPML_B = 76
+
+
+def run_cpml(N=1024, T=1000, shots=64):
+    from . import cpml
+
+    p = configs.cpml_config(N, N, T, n_src=shots, n_rx=shots, L=20)
+    m = cpml.CPMLModel.from_problem(p)
+    idx = np.arange(shots)
+    m.forward(idx[:1])
+    t0 = time.perf_counter()
+    rec = m.forward(idx)
+    dt = time.perf_counter() - t0
+    assert np.isfinite(rec).all()
+    cells = float(N) * N * shots * (T - 2)
+    log(f"[cpml] {shots} shots {N}^2 T={T}: {dt:.2f} s, "
+        f"{CPML_B * cells / dt / 8e12:.1%} of HBM peak (algorithmic)")
+    return {"workload": f"C-PML second-order scheme, {shots} shots of {N}^2, T={T}",
+            "seconds": dt, "cell_steps_per_s": cells / dt, "bytes_per_cell_step": CPML_B,
+            "hbm_frac": CPML_B * cells / dt / 8e12}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["c3", "exact"])
+    ap.add_argument("what", choices=["c3", "exact", "cpml"])
     ap.add_argument("--t-scale", type=float, default=1.0)
     ap.add_argument("--shots", type=int, default=32)
     ap.add_argument("--workers", type=int, default=4)
     ap.add_argument("--models", type=int, default=16)
     ap.add_argument("--T", type=int, default=4000)
+    ap.add_argument("--N", type=int, default=1024, help="cpml: grid size")
     ap.add_argument("--misfit", default="w2", choices=["w2", "l2"])
     ap.add_argument("--max-batch", type=int, default=0,
                     help="c3: shots resident per model (0, the default: all of its shots)")
@@ -112,6 +141,8 @@ def main(argv=None):
     if a.what == "c3":
         out = run_c3(a.models, a.shots, a.t_scale, a.workers, max_batch=a.max_batch,
                      misfit=a.misfit)
+    elif a.what == "cpml":
+        out = run_cpml(N=a.N, T=a.T, shots=a.shots)
     else:
         out = run_exact(T=a.T, shots=min(a.shots, 64))
     print(json.dumps(out))

@@ -81,48 +81,7 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def _cpml_profiles(n, L, dt, d0, alpha_max, kmax, half):
-    """C-PML coefficient profiles along one axis (quadratic d and K, linear
-    alpha; b = exp(-(d/K + alpha) dt), a = d (b - 1) / (K (d + K alpha)))."""
-    pos = np.arange(n, dtype=np.float64) + (0.5 if half else 0.0)
-    depth = np.maximum(np.maximum(L - pos, pos - (n - 1 - L)), 0.0) / L
-    d = d0 * depth ** 2
-    K = 1.0 + (kmax - 1.0) * depth ** 2
-    alpha = alpha_max * (1.0 - depth) * (depth > 0)
-    b = np.exp(-(d / K + alpha) * dt)
-    a = np.where(d > 1e-9, d * (b - 1.0) / (K * (d + K * alpha) + 1e-300), 0.0)
-    return K.astype(np.float32), a.astype(np.float32), b.astype(np.float32)
-
-
 def cpml_problem(X=96, Z=80, T=300, L=12, n_src=3, n_rx=8, seed=0, ppe=1, f0=0.05):
-    """Second-order C-PML setup (regression_CPML.cu): normalized dx = dz = 1,
-    dt = 1, velocity 0.4 with a slow blob, density 1 with a denser blob,
-    sources and receivers on row L + 2, Ricker sources."""
-    rng = np.random.default_rng(seed)
-    zz, xx = np.mgrid[0:Z, 0:X]
-    c = np.full((Z, X), 0.4)
-    cz, cx = rng.uniform(0.5 * Z, 0.7 * Z), rng.uniform(0.35 * X, 0.65 * X)
-    c[(xx - cx) ** 2 + (zz - cz) ** 2 <= (0.1 * min(X, Z)) ** 2] = 0.28
-    rho = np.ones((Z, X))
-    rho[(xx - 0.5 * X) ** 2 + (zz - 0.4 * Z) ** 2 <= (0.08 * min(X, Z)) ** 2] = 1.5
-    rho_hx = rho.copy()
-    rho_hx[:, :-1] = 0.5 * (rho[:, :-1] + rho[:, 1:])
-    rho_hz = rho.copy()
-    rho_hz[:-1, :] = 0.5 * (rho[:-1, :] + rho[1:, :])
-    d0 = 3.0 * 0.4 * np.log(1e3) / (2.0 * L)
-    prof = {}
-    for ax, n in (("x", X), ("z", Z)):
-        for half in (False, True):
-            K, a, b = _cpml_profiles(n, L, 1.0, d0, np.pi * f0, 1.0, half)
-            h = "h" if half else ""
-            prof[f"K_{ax}{h}"], prof[f"a_{ax}{h}"], prof[f"b_{ax}{h}"] = K, a, b
-    sx = np.round(np.linspace(L + 6, X - L - 7, n_src)).astype(np.int32)
-    rx = np.round(np.linspace(L + 3, X - L - 4, n_rx)).astype(np.int32)
-    w = configs.ricker(f0, 1.0, T, t0=1.2 / f0)
-    src = np.tile(w, (n_src, 1)) * (1.0 + 0.1 * np.arange(n_src))[:, None]
-    return dict(X=X, Z=Z, T=T, ppe=ppe, dx=1.0, dz=1.0, dt=1.0, dt2=1.0, qt_pi=4 * np.pi,
-                sx=sx, sz=np.full(n_src, L + 2, np.int32), rx=rx, rz=np.full(n_rx, L + 2, np.int32),
-                cquad=(c ** 2).astype(np.float32), kappa=(rho * c ** 2).astype(np.float32),
-                rho=rho.astype(np.float32), rho_hx=rho_hx.astype(np.float32),
-                rho_hz=rho_hz.astype(np.float32), src=src.astype(np.float32),
-                initial=np.zeros((2, Z, X), np.float32), **prof)
+    """Second-order C-PML setup (regression_CPML.cu) on a small grid:
+    configs.cpml_config with a 12-cell layer, velocity 0.4."""
+    return configs.cpml_config(X, Z, T, n_src, n_rx, L=L, c0=0.4, f0=f0, seed=seed, ppe=ppe)

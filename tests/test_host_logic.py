@@ -55,8 +55,8 @@ def test_point_order_row_major():
     assert list(zip(z, x)) == [(1, 2), (1, 4), (3, 1)]
 
 
-def header_functions():
-    txt = open(os.path.join(ROOT, "include", "otfwi_pml.h")).read()
+def header_functions(header="otfwi_pml.h"):
+    txt = open(os.path.join(ROOT, "include", header)).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     txt = "\n".join(line for line in txt.split("\n") if not line.strip().startswith("typedef"))
     names = re.findall(r"^\s*(?:const\s+)?[a-z_]+\s*\*?\s*([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt, re.M)
@@ -129,3 +129,16 @@ def test_device_misfit_kinds():
     assert [_device_kind(f) for f in (AF.L2, AF.wasserstein, AF.quad_envelope)] == ["l2", "w2", "env"]
     assert _device_kind(lambda s, o: (0.0, s)) is None
     assert (_lib.MISFIT_L2, _lib.MISFIT_W2, _lib.MISFIT_ENV) == (0, 1, 2)
+
+
+def test_cpml_library_exports_every_declared_symbol():
+    """libotfwi_cpml.so loads and exports every function include/otfwi_cpml.h
+    declares (no GPU call)."""
+    from otfwi_amd import cpml
+
+    names = header_functions("otfwi_cpml.h")
+    assert "init_memory_sim" in names and "otfwi_cpml_forward" in names
+    lib = ctypes.CDLL(cpml.LIB_PATH)
+    for n in names:
+        assert hasattr(lib, n), n
+    assert sorted(cpml.EXPORTED) == names
