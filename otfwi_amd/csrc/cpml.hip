@@ -18,8 +18,7 @@
 // A launch advances every shot of a batch (grid z); each shot has its own
 // fields.  P keeps four time slots as the reference does (t % 4; the first
 // two from `initial`), the memory variables two (a step reads only t - 1).
-// Work-groups of 64 x 4 cells read their stencils straight from global
-// memory (L1 / L2 serve the neighbours' re-reads).
+// Work-groups of 64 x 16 cells stage their stencil inputs in LDS.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -51,7 +50,6 @@ int cfail(int code, const char *fmt, ...)
     } while (0)
 
 constexpr double C1 = 1.196289e+00, C2 = 7.975260e-02, C3 = 9.570313e-03, C4 = 6.975446e-04;
-constexpr int BX = 64, BZ = 4;
 
 struct Dims {
     int X, Z;
@@ -67,15 +65,6 @@ __device__ __forceinline__ float boxv(const float *f, const Dims &d, int x, int 
 {
     return inbox(d, x, z) ? f[(long long)z * d.X + x] : 0.0f;
 }
-// derivPMLdx (deriv_macros.h:85-95) with offset `off` along (sx, sz)
-template <int OFF>
-__device__ __forceinline__ float dpml(const float *f, const Dims &d, int x, int z, int sx, int sz)
-{
-    auto v = [&](int k) { return (double)boxv(f, d, x + k * sx, z + k * sz); };
-    return (float)(-C4 * v(4 - OFF) + C3 * v(3 - OFF) - C2 * v(2 - OFF) + C1 * v(1 - OFF) -
-                   C1 * v(0 - OFF) + C2 * v(-1 - OFF) - C3 * v(-2 - OFF) + C4 * v(-3 - OFF));
-}
-
 struct Fields {
     float *P;          // [S][4][plane]
     float *mx, *mz;    // [S][2][plane]
@@ -90,47 +79,113 @@ struct Medium {
     float dx, dz, dt2, qt_pi;
 };
 
-__global__ void __launch_bounds__(BX *BZ)
-cpml_frame1(Fields f, Medium m, Dims d, int t)
+// A work-group of 256 threads owns a TX x TZ tile (64 x 16): it stages the
+// stencil inputs of the tile (P(t-1) with a 4-cell halo for frame 1; the x
+// derivative field with an x halo and the z one with a z halo for frame 2)
+// in LDS once, with the box rule applied while loading; each thread then
+// computes 4 cells of one column.  skip*: tiles whose profile values are all
+// a = 0, b = 1 (the interior): there the memory variables start at 0 and
+// stay exactly 0, so they are neither read nor written.
+constexpr int TX = 64, TZ = 16, TH = 4;
+
+// derivPMLdx / derivPMLdz (deriv_macros.h:85-108) of 8 staged values, the
+// reference's term order: v[0..8) = taps -3-off .. 4-off
+__device__ __forceinline__ float dsum(const float *v)
 {
-    const int x = blockIdx.x * BX + threadIdx.x, z = blockIdx.y * BZ + threadIdx.y;
-    const int s = blockIdx.z;
-    if (!inbox(d, x, z)) return;
-    const long long i = (long long)z * d.X + x, pl = d.plane;
-    const float *P1 = f.P + ((long long)s * 4 + (t - 1) % 4) * pl;
-    const int c = t & 1, p = c ^ 1;
-    float *mx = f.mx + (long long)s * 2 * pl, *mz = f.mz + (long long)s * 2 * pl;
-    const float v0 = dpml<0>(P1, d, x, z, 1, 0) / m.dx;
-    const float mxt = m.bxh[x] * mx[p * pl + i] + m.axh[x] * v0;
-    mx[c * pl + i] = mxt;
-    f.vx1[s * pl + i] = (v0 / m.Kxh[x] + mxt) / m.rho_hx[i];
-    const float w0 = dpml<0>(P1, d, x, z, 0, 1) / m.dz;
-    const float mzt = m.bzh[z] * mz[p * pl + i] + m.azh[z] * w0;
-    mz[c * pl + i] = mzt;
-    f.vz1[s * pl + i] = (w0 / m.Kzh[z] + mzt) / m.rho_hz[i];
+    return (float)(-C4 * (double)v[7] + C3 * (double)v[6] - C2 * (double)v[5] +
+                   C1 * (double)v[4] - C1 * (double)v[3] + C2 * (double)v[2] -
+                   C3 * (double)v[1] + C4 * (double)v[0]);
 }
 
-__global__ void __launch_bounds__(BX *BZ)
-cpml_frame2(Fields f, Medium m, Dims d, int t)
+__global__ void __launch_bounds__(256)
+cpml_frame1(Fields f, Medium m, Dims d, int t, const unsigned char *skipx,
+            const unsigned char *skipz)
 {
-    const int x = blockIdx.x * BX + threadIdx.x, z = blockIdx.y * BZ + threadIdx.y;
-    const int s = blockIdx.z;
-    if (!inbox(d, x, z)) return;
-    const long long i = (long long)z * d.X + x, pl = d.plane;
+    __shared__ float sp[TZ + 8][TX + 8];
+    const int tid = threadIdx.x, s = blockIdx.z;
+    const int x0 = blockIdx.x * TX, z0 = blockIdx.y * TZ;
+    const long long pl = d.plane;
+    const float *P1 = f.P + ((long long)s * 4 + (t - 1) % 4) * pl;
+    for (int k = tid; k < (TZ + 8) * (TX + 8); k += 256) {
+        const int r = k / (TX + 8), c = k - r * (TX + 8);
+        sp[r][c] = boxv(P1, d, x0 - 4 + c, z0 - 4 + r);
+    }
+    __syncthreads();
+    const int tx = tid & (TX - 1), zg = (tid / TX) * TH;
+    const int x = x0 + tx;
+    const bool kx = !skipx[blockIdx.x], kz = !skipz[blockIdx.y];
+    const int c = t & 1, p = c ^ 1;
+    float *mx = f.mx + (long long)s * 2 * pl, *mz = f.mz + (long long)s * 2 * pl;
+#pragma unroll
+    for (int j = 0; j < TH; j++) {
+        const int zl = zg + j, z = z0 + zl;
+        if (!inbox(d, x, z)) continue;
+        const long long i = (long long)z * d.X + x;
+        float v[8], w[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {   // taps -3 .. 4 (offset 0)
+            v[k] = sp[zl + 4][tx + 4 - 3 + k];
+            w[k] = sp[zl + 4 - 3 + k][tx + 4];
+        }
+        const float v0 = dsum(v) / m.dx;
+        const float mxt = kx ? m.bxh[x] * mx[p * pl + i] + m.axh[x] * v0 : 0.0f;
+        if (kx) mx[c * pl + i] = mxt;
+        f.vx1[s * pl + i] = (v0 / m.Kxh[x] + mxt) / m.rho_hx[i];
+        const float w0 = dsum(w) / m.dz;
+        const float mzt = kz ? m.bzh[z] * mz[p * pl + i] + m.azh[z] * w0 : 0.0f;
+        if (kz) mz[c * pl + i] = mzt;
+        f.vz1[s * pl + i] = (w0 / m.Kzh[z] + mzt) / m.rho_hz[i];
+    }
+}
+
+__global__ void __launch_bounds__(256)
+cpml_frame2(Fields f, Medium m, Dims d, int t, const unsigned char *skipx,
+            const unsigned char *skipz)
+{
+    __shared__ float sx[TZ][TX + 8];
+    __shared__ float sz[TZ + 8][TX];
+    const int tid = threadIdx.x, s = blockIdx.z;
+    const int x0 = blockIdx.x * TX, z0 = blockIdx.y * TZ;
+    const long long pl = d.plane;
+    const float *VX = f.vx1 + s * pl, *VZ = f.vz1 + s * pl;
+    for (int k = tid; k < TZ * (TX + 8); k += 256) {
+        const int r = k / (TX + 8), c = k - r * (TX + 8);
+        sx[r][c] = boxv(VX, d, x0 - 4 + c, z0 + r);
+    }
+    for (int k = tid; k < (TZ + 8) * TX; k += 256) {
+        const int r = k / TX, c = k - r * TX;
+        sz[r][c] = boxv(VZ, d, x0 + c, z0 - 4 + r);
+    }
+    __syncthreads();
+    const int tx = tid & (TX - 1), zg = (tid / TX) * TH;
+    const int x = x0 + tx;
+    const bool kx = !skipx[blockIdx.x], kz = !skipz[blockIdx.y];
     const int c = t & 1, p = c ^ 1;
     float *mxx = f.mxx + (long long)s * 2 * pl, *mzz = f.mzz + (long long)s * 2 * pl;
-    const float v0 = dpml<1>(f.vx1 + s * pl, d, x, z, 1, 0) / m.dx;
-    const float mxt = m.bx[x] * mxx[p * pl + i] + m.ax[x] * v0;
-    mxx[c * pl + i] = mxt;
-    const float vxx = v0 / m.Kx[x] + mxt;
-    const float w0 = dpml<1>(f.vz1 + s * pl, d, x, z, 0, 1) / m.dz;
-    const float mzt = m.bz[z] * mzz[p * pl + i] + m.az[z] * w0;
-    mzz[c * pl + i] = mzt;
-    const float vzz = w0 / m.Kz[z] + mzt;
-    const float lap = vxx + vzz;
     float *P = f.P + (long long)s * 4 * pl;
-    const float p1 = P[((t - 1) % 4) * pl + i], p2 = P[((t - 2) % 4) * pl + i];
-    P[(t % 4) * pl + i] = 2.0f * p1 - p2 + lap * m.dt2 * m.kappa[i];
+#pragma unroll
+    for (int j = 0; j < TH; j++) {
+        const int zl = zg + j, z = z0 + zl;
+        if (!inbox(d, x, z)) continue;
+        const long long i = (long long)z * d.X + x;
+        float v[8], w[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {   // taps -4 .. 3 (offset 1)
+            v[k] = sx[zl][tx + 4 - 4 + k];
+            w[k] = sz[zl + 4 - 4 + k][tx];
+        }
+        const float v0 = dsum(v) / m.dx;
+        const float mxt = kx ? m.bx[x] * mxx[p * pl + i] + m.ax[x] * v0 : 0.0f;
+        if (kx) mxx[c * pl + i] = mxt;
+        const float vxx = v0 / m.Kx[x] + mxt;
+        const float w0 = dsum(w) / m.dz;
+        const float mzt = kz ? m.bz[z] * mzz[p * pl + i] + m.az[z] * w0 : 0.0f;
+        if (kz) mzz[c * pl + i] = mzt;
+        const float vzz = w0 / m.Kz[z] + mzt;
+        const float lap = vxx + vzz;
+        const float p1 = P[((t - 1) % 4) * pl + i], p2 = P[((t - 2) % 4) * pl + i];
+        P[(t % 4) * pl + i] = 2.0f * p1 - p2 + lap * m.dt2 * m.kappa[i];
+    }
 }
 
 // Sources of shot s (in source order, on one lane) at t >= 2, then every
@@ -184,6 +239,8 @@ struct otfwi_cpml {
     int n_src = 0, n_rx = 0;
     int *pts = nullptr;          // sx, sz, rx, rz
     float *src = nullptr;        // [n_src][T]
+    unsigned char *skip = nullptr;   // per tile column / row: frame 1 x, z; frame 2 x, z
+    int ntx = 0, ntz = 0;
     int cap = 0;                 // shots the state buffers hold
     float *state = nullptr;      // P 4 + mx, mz, mxx, mzz 2 each + vx1, vz1: 14 planes per shot
     float *rec = nullptr;        // [cap][n_rx][T]
@@ -256,6 +313,32 @@ int otfwi_cpml_create(otfwi_cpml **out, int X, int Z, int T, int ppe, const floa
             rc = cfail(-2, "profile upload failed");
     }
     if (!rc) {
+        // tiles whose memory variables stay 0: a == 0 and b == 1 on every
+        // column (row) of the tile, for the half-point (frame 1) and the
+        // full-point (frame 2) profiles
+        h->ntx = (X + TX - 1) / TX;
+        h->ntz = (Z + TZ - 1) / TZ;
+        std::vector<unsigned char> sk(2 * (h->ntx + h->ntz), 1);
+        auto still = [](const float *a, const float *b, int lo, int hi) {
+            for (int k = lo; k < hi; k++)
+                if (a[k] != 0.0f || b[k] != 1.0f) return false;
+            return true;
+        };
+        for (int b = 0; b < h->ntx; b++) {
+            const int lo = b * TX, hi = std::min(X, lo + TX);
+            sk[b] = still(prof[4], prof[5], lo, hi);                          // axh, bxh
+            sk[h->ntx + h->ntz + b] = still(prof[1], prof[2], lo, hi);        // ax, bx
+        }
+        for (int b = 0; b < h->ntz; b++) {
+            const int lo = b * TZ, hi = std::min(Z, lo + TZ);
+            sk[h->ntx + b] = still(prof[10], prof[11], lo, hi);               // azh, bzh
+            sk[2 * h->ntx + h->ntz + b] = still(prof[7], prof[8], lo, hi);    // az, bz
+        }
+        rc = dalloc(&h->skip, sk.size());
+        if (!rc && hipMemcpy(h->skip, sk.data(), sk.size(), hipMemcpyHostToDevice))
+            rc = cfail(-2, "tile flags upload failed");
+    }
+    if (!rc) {
         if (initial)
             e = hipMemcpy(h->init, initial, 2 * pl * sizeof(float), hipMemcpyHostToDevice);
         else
@@ -274,7 +357,7 @@ int otfwi_cpml_destroy(otfwi_cpml *h)
 {
     if (!h) return 0;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    dfree(h->med); dfree(h->prof); dfree(h->init); dfree(h->pts); dfree(h->src);
+    dfree(h->med); dfree(h->prof); dfree(h->init); dfree(h->pts); dfree(h->src); dfree(h->skip);
     dfree(h->state); dfree(h->rec); dfree(h->shot_idx);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -340,13 +423,15 @@ int otfwi_cpml_forward(otfwi_cpml *h, int n_shots, const int *shot_idx, float *r
                            hipMemcpyDeviceToDevice, st));
     const Medium m = medium(h);
     const Dims d{h->X, h->Z, (long long)pl};
-    const dim3 grid((h->X + BX - 1) / BX, (h->Z + BZ - 1) / BZ, n_shots), blk(BX, BZ);
+    const dim3 grid(h->ntx, h->ntz, n_shots), blk(256);
+    const unsigned char *sk1x = h->skip, *sk1z = h->skip + h->ntx;
+    const unsigned char *sk2x = h->skip + h->ntx + h->ntz, *sk2z = sk2x + h->ntx;
     const int *sx = h->pts, *sz = h->pts + h->n_src, *rx = h->pts + 2 * h->n_src;
     const int *rz = rx + h->n_rx;
     for (int t = 0; t < h->T; t++) {
         if (t > 1) {
-            hipLaunchKernelGGL(cpml_frame1, grid, blk, 0, st, f, m, d, t);
-            hipLaunchKernelGGL(cpml_frame2, grid, blk, 0, st, f, m, d, t);
+            hipLaunchKernelGGL(cpml_frame1, grid, blk, 0, st, f, m, d, t, sk1x, sk1z);
+            hipLaunchKernelGGL(cpml_frame2, grid, blk, 0, st, f, m, d, t, sk2x, sk2z);
         }
         hipLaunchKernelGGL(cpml_src_rec, dim3(n_shots), dim3(64), 0, st, f, m, d, t, h->T, h->ppe,
                            h->n_src, sx, sz, h->src, h->shot_idx, h->n_rx, rx, rz, h->rec);
