@@ -235,6 +235,10 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--pair-streams", type=int, default=0, choices=[0, 1, 2, 4],
                     help="streams of the two-step phases (0: by launch size)")
+    ap.add_argument("--direct", action="store_true",
+                    help="launch every time loop directly instead of replaying captured graphs "
+                         "(loops whose two-step phase splits over streams are launched "
+                         "directly either way)")
     ap.add_argument("--steps-per-launch", type=int, default=0, choices=[0, 1, 2],
                     help="time steps per launch of both loops (0: the autotuner's choice)")
     ap.add_argument("--cpu-steps", type=int, default=2000)
@@ -270,6 +274,8 @@ def main():
         tune = m.autotune(S, steps=20)
     if args.steps_per_launch:
         m.set_steps_per_launch(args.steps_per_launch, args.steps_per_launch)
+    if args.direct:
+        m.set_graph(False)
     if args.pair_streams:
         m.set_pair_streams(args.pair_streams)
     ktimes = m.kernel_times(S, reps=10)
@@ -352,7 +358,8 @@ def main():
         "config": {"workload": f"fmc{S}_{N}x{N}_T{T}_{args.misfit}_gradient", "grid": [N, N],
                    "T": T, "shots": S, "receivers": S, "misfit": args.misfit, "pml": 20,
                    "tiles": list(m.get_tuning()[:2]), "persistent": list(m.get_launch()),
-                   "steps_per_launch": [fs, ads], "autotune": tune},
+                   "steps_per_launch": [fs, ads], "autotune": tune,
+                   "graphs": not args.direct, "pair_streams": args.pair_streams},
         "roofline": {"bound": "hbm", "kernel": unit_name,
                      "achieved": unit_ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": unit_ach / HBM_PEAK,

@@ -187,7 +187,9 @@ int otfwi_set_pair_width(otfwi_model *m, int nw);
  * else 1).  With k > 1 the shots are split into k groups of consecutive
  * shots, each advancing on the model's stream or one of the library's extra
  * streams, so each group's launches fill the others' launch tails; results
- * are bitwise identical.  No reference counterpart. */
+ * are bitwise identical.  A loop whose two-step phase splits is launched
+ * directly, also with graphs on (otfwi_set_graph): only one-stream loops are
+ * captured.  No reference counterpart. */
 int otfwi_set_pair_streams(otfwi_model *m, int k);
 
 /* The steps per launch (1 or 2) the forward and adjoint loops use for a
@@ -198,7 +200,8 @@ int otfwi_steps_per_launch(otfwi_model *m, int n_shots, int *fwd, int *adj);
 
 /* Replay the per-step launch loops as captured hipGraphs (default 1); a
  * graph is re-captured only when shots, tiles, launch style or device buffers
- * change.  No reference counterpart (regression_PML_single.cu:661-678
+ * change.  Loops whose two-step phase splits over streams
+ * (otfwi_set_pair_streams) are launched directly either way.  No reference counterpart (regression_PML_single.cu:661-678
  * launches and synchronizes four kernels per step). */
 int otfwi_set_graph(otfwi_model *m, int use_graph);
 

@@ -532,6 +532,12 @@ bool use_pairs(const otfwi_model *m, int n, bool adj)
 // two when each half still launches 2 work-groups per CU (measured at
 // 64 x 1024^2: two concurrent 32-shot halves take 0.92x the time of one
 // 64-shot launch sequence, their launches filling each other's tails).
+// A loop whose pair phase splits is launched directly, never captured into
+// a hipGraph: a process ended with a host segmentation fault inside the first
+// forward call of a model whose loop was captured across 2 or 4 streams, with
+// no cause found in this code.  At the sizes that split, direct launches are
+// also the faster form (64 x 1024^2 W2 gradient: 3.53 s direct and split,
+// 3.80 s from one-stream graphs).
 int pair_split(const otfwi_model *m, int n)
 {
     if (n < 2 || !m->xs[0]) return 1;
@@ -606,13 +612,18 @@ int persistent_wgs(const otfwi_model *m, bool adj)
 }
 
 // Runs body() (which enqueues kernel launches only) directly, or captures it
-// once into a hipGraph for slot `slot` and replays that.
+// once into a hipGraph for slot `slot` and replays that.  A body that uses the
+// extra streams (split) always runs directly.
 template <typename F>
-int run_loop(otfwi_model *m, int slot, unsigned long long key, F body)
+int run_loop(otfwi_model *m, int slot, unsigned long long key, bool split, F body)
 {
-    if (!m->use_graph) return body();
+    if (!m->use_graph || split) return body();
     if (!m->gexec[slot] || m->gkey[slot] != key) {
-        if (m->gexec[slot]) (void)hipGraphExecDestroy(m->gexec[slot]);
+        // the previous replay of this slot may still be running
+        if (m->gexec[slot]) {
+            HIPCK(hipStreamSynchronize(m->stream));
+            (void)hipGraphExecDestroy(m->gexec[slot]);
+        }
         m->gexec[slot] = nullptr;
         hipGraph_t graph = nullptr;
         HIPCK(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
@@ -847,7 +858,7 @@ int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum)
             3ULL + (unsigned long long)pair_split(m, n);
     return run_loop(m, save_ring ? 1 : 0,
                     (loop_key(m, n, m->th_fwd, pw, t0, t1) * 31ULL + (unsigned long long)ist) *
-                            1000003ULL + kb, body);
+                            1000003ULL + kb, two && pair_split(m, n) > 1, body);
 }
 
 template <typename R>
@@ -985,7 +996,8 @@ int adj_loop(otfwi_model *m, int n, int t0, int t1)
     const unsigned long long kb =
         (((unsigned long long)K * 4099ULL + (unsigned long long)G) * 3ULL + (two ? 1ULL : 0ULL)) *
             3ULL + (unsigned long long)pair_split(m, n);
-    return run_loop(m, 2, loop_key(m, n, m->th_adj, pw, t0, t1) * 1000003ULL + kb, body);
+    return run_loop(m, 2, loop_key(m, n, m->th_adj, pw, t0, t1) * 1000003ULL + kb,
+                    two && pair_split(m, n) > 1, body);
 }
 
 int upload_shot_src(otfwi_model *m, int n, const int *shot_src)
@@ -1701,25 +1713,27 @@ int otfwi_create(otfwi_model **out, int X, int Z, int T, int precision, const fl
 int otfwi_destroy(otfwi_model *m)
 {
     if (!m) return OTFWI_OK;
+    // Order: no work in flight on any stream; then the instantiated graphs
+    // (they were captured with the fork / join events); then the events and
+    // streams; then the memory.
     (void)hipStreamSynchronize(m->stream);
+    for (int i = 0; i < 3; i++)
+        if (m->xs[i]) (void)hipStreamSynchronize(m->xs[i]);
+    for (int i = 0; i < 3; i++)
+        if (m->gexec[i]) (void)hipGraphExecDestroy(m->gexec[i]);
+    for (int i = 0; i < 4; i++)
+        if (m->ev[i]) (void)hipEventDestroy(m->ev[i]);
+    for (int i = 0; i < 4; i++)
+        if (m->evfj[i]) (void)hipEventDestroy(m->evfj[i]);
+    for (int i = 0; i < 3; i++)
+        if (m->xs[i]) (void)hipStreamDestroy(m->xs[i]);
+    (void)hipStreamDestroy(m->stream);
     free_batch(m->b);
     dfree(m->cm); dfree(m->kx); dfree(m->kz); dfree(m->icum); dfree(m->cq32);
     dfree(m->evt); dfree(m->pts); dfree(m->tf_fwd); dfree(m->tf_adj); dfree(m->evrow);
     dfree(m->pmap); dfree(m->ringtab); dfree(m->hilb);
     dfree(m->gacc); dfree(m->gout); dfree(m->ill);
     dfree(m->xsnap); dfree(m->xseg); dfree(m->rxmap);
-    for (int i = 0; i < 4; i++)
-        if (m->ev[i]) (void)hipEventDestroy(m->ev[i]);
-    for (int i = 0; i < 4; i++)
-        if (m->evfj[i]) (void)hipEventDestroy(m->evfj[i]);
-    for (int i = 0; i < 3; i++)
-        if (m->gexec[i]) (void)hipGraphExecDestroy(m->gexec[i]);
-    for (int i = 0; i < 3; i++)
-        if (m->xs[i]) {
-            (void)hipStreamSynchronize(m->xs[i]);
-            (void)hipStreamDestroy(m->xs[i]);
-        }
-    (void)hipStreamDestroy(m->stream);
     delete m;
     return OTFWI_OK;
 }

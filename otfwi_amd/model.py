@@ -151,7 +151,8 @@ class PMLModel:
         return int(f.value), int(a.value)
 
     def set_pair_streams(self, k=0):
-        """Streams of the two-step loops: 1, 2, 4 or 0 (by size)."""
+        """Streams of the two-step loops: 1, 2, 4 or 0 (by size); a loop that
+        splits is launched directly, also with graphs on (set_graph)."""
         check(self._lib.otfwi_set_pair_streams(self._h, int(k)))
 
     def set_pair_width(self, nw=0):
