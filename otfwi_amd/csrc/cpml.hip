@@ -18,7 +18,8 @@
 // A launch advances every shot of a batch (grid z); each shot has its own
 // fields.  P keeps four time slots as the reference does (t % 4; the first
 // two from `initial`), the memory variables two (a step reads only t - 1).
-// Work-groups of 64 x 16 cells stage their stencil inputs in LDS.
+// One launch per step (both frames) on 64 x 32-cell work-group tiles whose
+// stencil inputs and derivative fields live in LDS.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -69,7 +70,6 @@ struct Fields {
     float *P;          // [S][4][plane]
     float *mx, *mz;    // [S][2][plane]
     float *mxx, *mzz;  // [S][2][plane]
-    float *vx1, *vz1;  // [S][plane]
 };
 
 struct Medium {
@@ -78,15 +78,6 @@ struct Medium {
     const float *Kz, *az, *bz, *Kzh, *azh, *bzh;         // [Z]
     float dx, dz, dt2, qt_pi;
 };
-
-// A work-group of 256 threads owns a TX x TZ tile (64 x 16): it stages the
-// stencil inputs of the tile (P(t-1) with a 4-cell halo for frame 1; the x
-// derivative field with an x halo and the z one with a z halo for frame 2)
-// in LDS once, with the box rule applied while loading; each thread then
-// computes 4 cells of one column.  skip*: tiles whose profile values are all
-// a = 0, b = 1 (the interior): there the memory variables start at 0 and
-// stay exactly 0, so they are neither read nor written.
-constexpr int TX = 64, TZ = 16, TH = 4;
 
 // derivPMLdx / derivPMLdz (deriv_macros.h:85-108) of 8 staged values, the
 // reference's term order: v[0..8) = taps -3-off .. 4-off
@@ -97,94 +88,148 @@ __device__ __forceinline__ float dsum(const float *v)
                    C3 * (double)v[1] + C4 * (double)v[0]);
 }
 
-__global__ void __launch_bounds__(256)
-cpml_frame1(Fields f, Medium m, Dims d, int t, const unsigned char *skipx,
-            const unsigned char *skipz)
+// A memory variable whose profile has a = 0 and b = 1 starts at 0 and stays
+// exactly 0 (b m + a v = m): it is neither read nor written there.
+__device__ __forceinline__ bool live(float a, float b) { return a != 0.0f || b != 1.0f; }
+
+// One time step t >= 2 of every shot (grid z): frame 1 and frame 2 fused.
+// A work-group of 256 threads owns a FX x FZ tile.  It stages P(t-1) with a
+// 7-cell halo in LDS (box rule applied while loading), computes the x
+// derivative field vx1 on the tile's rows over columns x0-4 .. x0+FX+2 and the
+// z field vz1 on its columns over rows z0-4 .. z0+FZ+2 into registers (the
+// halo cells recomputed from their neighbours' P, memory variables read but
+// only the owner's written), then stores them in LDS over the staged P and
+// runs frame 2 on the owned cells.  vx1 and vz1 never reach memory; sharing
+// the LDS keeps 19 KB per work-group (8 per CU).  Neighbouring tiles read
+// slot t-1 of P and of the memory variables while owners write slot t, so one
+// launch per step suffices.
+#ifndef CPML_FX
+#define CPML_FX 64
+#endif
+#ifndef CPML_FZ
+#define CPML_FZ 32
+#endif
+constexpr int FX = CPML_FX, FZ = CPML_FZ, FH = FZ * FX / 256;   // FH rows per thread in frame 2
+static_assert(256 % FX == 0 && (FZ * FX) % 256 == 0, "tile must split over 256 threads");
+constexpr int PW = FX + 14, PH = FZ + 14;   // staged P(t-1): x0-7 .., z0-7 ..
+constexpr int VXW = FX + 7, VZH = FZ + 7;   // vx1 columns, vz1 rows in use
+constexpr int SXW = FX + 8;                 // row pitch of the vx1 block
+constexpr int NVX = (FZ * VXW + 255) / 256, NVZ = (VZH * FX + 255) / 256;
+constexpr int LDS_P = PH * PW, LDS_V = FZ * SXW + (FZ + 8) * FX;
+constexpr int LDS_N = LDS_P > LDS_V ? LDS_P : LDS_V;
+
+__global__ void __launch_bounds__(256, 8)   // 8 waves per SIMD: at most 64 VGPRs
+cpml_step(Fields f, Medium m, Dims d, int t)
 {
-    __shared__ float sp[TZ + 8][TX + 8];
+    __shared__ float lds[LDS_N];
+    float (*sp)[PW] = reinterpret_cast<float (*)[PW]>(lds);
+    float (*sx)[SXW] = reinterpret_cast<float (*)[SXW]>(lds);              // vx1 at (x0 - 4 + c, z0 + r)
+    float (*sz)[FX] = reinterpret_cast<float (*)[FX]>(lds + FZ * SXW);     // vz1 at (x0 + c, z0 - 4 + r)
     const int tid = threadIdx.x, s = blockIdx.z;
-    const int x0 = blockIdx.x * TX, z0 = blockIdx.y * TZ;
+    const int x0 = blockIdx.x * FX, z0 = blockIdx.y * FZ;
     const long long pl = d.plane;
     const float *P1 = f.P + ((long long)s * 4 + (t - 1) % 4) * pl;
-    for (int k = tid; k < (TZ + 8) * (TX + 8); k += 256) {
-        const int r = k / (TX + 8), c = k - r * (TX + 8);
-        sp[r][c] = boxv(P1, d, x0 - 4 + c, z0 - 4 + r);
+    for (int k = tid; k < PH * PW; k += 256) {
+        const int r = k / PW, c = k - r * PW;
+        const bool used = (r >= 7 && r < 7 + FZ) || (c >= 7 && c < 7 + FX);   // not a corner
+        sp[r][c] = used ? boxv(P1, d, x0 - 7 + c, z0 - 7 + r) : 0.0f;
     }
     __syncthreads();
-    const int tx = tid & (TX - 1), zg = (tid / TX) * TH;
-    const int x = x0 + tx;
-    const bool kx = !skipx[blockIdx.x], kz = !skipz[blockIdx.y];
-    const int c = t & 1, p = c ^ 1;
-    float *mx = f.mx + (long long)s * 2 * pl, *mz = f.mz + (long long)s * 2 * pl;
+    const int c1 = t & 1, p1 = c1 ^ 1;
+    const float *mxp = f.mx + ((long long)s * 2 + p1) * pl, *mzp = f.mz + ((long long)s * 2 + p1) * pl;
+    float *mxc = f.mx + ((long long)s * 2 + c1) * pl, *mzc = f.mz + ((long long)s * 2 + c1) * pl;
+    // frame 1, x: vx1 = (D+x P / dx / Kxh + mx(t)) / rho_hx, taps x-3 .. x+4
+    float vx[NVX], vz[NVZ];
 #pragma unroll
-    for (int j = 0; j < TH; j++) {
-        const int zl = zg + j, z = z0 + zl;
-        if (!inbox(d, x, z)) continue;
-        const long long i = (long long)z * d.X + x;
-        float v[8], w[8];
+    for (int n = 0; n < NVX; n++) {
+        const int k = tid + n * 256;
+        const int r = k / VXW, c = k - r * VXW;
+        const int x = x0 - 4 + c, z = z0 + r;
+        float out = 0.0f;
+        if (k < FZ * VXW && inbox(d, x, z)) {
+            const long long i = (long long)z * d.X + x;
+            float v[8];
 #pragma unroll
-        for (int k = 0; k < 8; k++) {   // taps -3 .. 4 (offset 0)
-            v[k] = sp[zl + 4][tx + 4 - 3 + k];
-            w[k] = sp[zl + 4 - 3 + k][tx + 4];
+            for (int q = 0; q < 8; q++) v[q] = sp[r + 7][c + q];
+            const float v0 = dsum(v) / m.dx;
+            float mt = 0.0f;
+            if (live(m.axh[x], m.bxh[x])) {
+                mt = m.bxh[x] * mxp[i] + m.axh[x] * v0;
+                if (c >= 4 && c < 4 + FX) mxc[i] = mt;
+            }
+            out = (v0 / m.Kxh[x] + mt) / m.rho_hx[i];
         }
-        const float v0 = dsum(v) / m.dx;
-        const float mxt = kx ? m.bxh[x] * mx[p * pl + i] + m.axh[x] * v0 : 0.0f;
-        if (kx) mx[c * pl + i] = mxt;
-        f.vx1[s * pl + i] = (v0 / m.Kxh[x] + mxt) / m.rho_hx[i];
-        const float w0 = dsum(w) / m.dz;
-        const float mzt = kz ? m.bzh[z] * mz[p * pl + i] + m.azh[z] * w0 : 0.0f;
-        if (kz) mz[c * pl + i] = mzt;
-        f.vz1[s * pl + i] = (w0 / m.Kzh[z] + mzt) / m.rho_hz[i];
+        vx[n] = out;
     }
-}
-
-__global__ void __launch_bounds__(256)
-cpml_frame2(Fields f, Medium m, Dims d, int t, const unsigned char *skipx,
-            const unsigned char *skipz)
-{
-    __shared__ float sx[TZ][TX + 8];
-    __shared__ float sz[TZ + 8][TX];
-    const int tid = threadIdx.x, s = blockIdx.z;
-    const int x0 = blockIdx.x * TX, z0 = blockIdx.y * TZ;
-    const long long pl = d.plane;
-    const float *VX = f.vx1 + s * pl, *VZ = f.vz1 + s * pl;
-    for (int k = tid; k < TZ * (TX + 8); k += 256) {
-        const int r = k / (TX + 8), c = k - r * (TX + 8);
-        sx[r][c] = boxv(VX, d, x0 - 4 + c, z0 + r);
+    // frame 1, z: vz1 = (D+z P / dz / Kzh + mz(t)) / rho_hz, taps z-3 .. z+4
+#pragma unroll
+    for (int n = 0; n < NVZ; n++) {
+        const int k = tid + n * 256;
+        const int r = k / FX, c = k - r * FX;
+        const int x = x0 + c, z = z0 - 4 + r;
+        float out = 0.0f;
+        if (k < VZH * FX && inbox(d, x, z)) {
+            const long long i = (long long)z * d.X + x;
+            float w[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) w[q] = sp[r + q][c + 7];
+            const float w0 = dsum(w) / m.dz;
+            float mt = 0.0f;
+            if (live(m.azh[z], m.bzh[z])) {
+                mt = m.bzh[z] * mzp[i] + m.azh[z] * w0;
+                if (r >= 4 && r < 4 + FZ) mzc[i] = mt;
+            }
+            out = (w0 / m.Kzh[z] + mt) / m.rho_hz[i];
+        }
+        vz[n] = out;
     }
-    for (int k = tid; k < (TZ + 8) * TX; k += 256) {
-        const int r = k / TX, c = k - r * TX;
-        sz[r][c] = boxv(VZ, d, x0 + c, z0 - 4 + r);
+    const int tx = tid & (FX - 1), zg = (tid / FX) * FH;
+    float pm1[FH];
+#pragma unroll
+    for (int j = 0; j < FH; j++) pm1[j] = sp[zg + j + 7][tx + 7];
+    __syncthreads();   // every read of the staged P is done
+#pragma unroll
+    for (int n = 0; n < NVX; n++) {
+        const int k = tid + n * 256;
+        const int r = k / VXW, c = k - r * VXW;
+        if (k < FZ * VXW) sx[r][c] = vx[n];
+    }
+#pragma unroll
+    for (int n = 0; n < NVZ; n++) {
+        const int k = tid + n * 256;
+        const int r = k / FX, c = k - r * FX;
+        if (k < VZH * FX) sz[r][c] = vz[n];
     }
     __syncthreads();
-    const int tx = tid & (TX - 1), zg = (tid / TX) * TH;
+    // frame 2 on the owned cells: taps x-4 .. x+3 of vx1, z-4 .. z+3 of vz1
     const int x = x0 + tx;
-    const bool kx = !skipx[blockIdx.x], kz = !skipz[blockIdx.y];
-    const int c = t & 1, p = c ^ 1;
-    float *mxx = f.mxx + (long long)s * 2 * pl, *mzz = f.mzz + (long long)s * 2 * pl;
+    float *mxxp = f.mxx + ((long long)s * 2 + p1) * pl, *mzzp = f.mzz + ((long long)s * 2 + p1) * pl;
+    float *mxxc = f.mxx + ((long long)s * 2 + c1) * pl, *mzzc = f.mzz + ((long long)s * 2 + c1) * pl;
     float *P = f.P + (long long)s * 4 * pl;
+    const bool kx = x < d.X && live(m.ax[x], m.bx[x]);
 #pragma unroll
-    for (int j = 0; j < TH; j++) {
+    for (int j = 0; j < FH; j++) {
         const int zl = zg + j, z = z0 + zl;
         if (!inbox(d, x, z)) continue;
         const long long i = (long long)z * d.X + x;
         float v[8], w[8];
 #pragma unroll
-        for (int k = 0; k < 8; k++) {   // taps -4 .. 3 (offset 1)
-            v[k] = sx[zl][tx + 4 - 4 + k];
-            w[k] = sz[zl + 4 - 4 + k][tx];
+        for (int q = 0; q < 8; q++) {
+            v[q] = sx[zl][tx + q];
+            w[q] = sz[zl + q][tx];
         }
         const float v0 = dsum(v) / m.dx;
-        const float mxt = kx ? m.bx[x] * mxx[p * pl + i] + m.ax[x] * v0 : 0.0f;
-        if (kx) mxx[c * pl + i] = mxt;
+        const float mxt = kx ? m.bx[x] * mxxp[i] + m.ax[x] * v0 : 0.0f;
+        if (kx) mxxc[i] = mxt;
         const float vxx = v0 / m.Kx[x] + mxt;
         const float w0 = dsum(w) / m.dz;
-        const float mzt = kz ? m.bz[z] * mzz[p * pl + i] + m.az[z] * w0 : 0.0f;
-        if (kz) mzz[c * pl + i] = mzt;
+        const bool kz = live(m.az[z], m.bz[z]);
+        const float mzt = kz ? m.bz[z] * mzzp[i] + m.az[z] * w0 : 0.0f;
+        if (kz) mzzc[i] = mzt;
         const float vzz = w0 / m.Kz[z] + mzt;
         const float lap = vxx + vzz;
-        const float p1 = P[((t - 1) % 4) * pl + i], p2 = P[((t - 2) % 4) * pl + i];
-        P[(t % 4) * pl + i] = 2.0f * p1 - p2 + lap * m.dt2 * m.kappa[i];
+        const float pm2 = P[((t - 2) % 4) * pl + i];
+        P[(t % 4) * pl + i] = 2.0f * pm1[j] - pm2 + lap * m.dt2 * m.kappa[i];
     }
 }
 
@@ -239,10 +284,9 @@ struct otfwi_cpml {
     int n_src = 0, n_rx = 0;
     int *pts = nullptr;          // sx, sz, rx, rz
     float *src = nullptr;        // [n_src][T]
-    unsigned char *skip = nullptr;   // per tile column / row: frame 1 x, z; frame 2 x, z
-    int ntx = 0, ntz = 0;
+    int ntx = 0, ntz = 0;            // tiles along x and z
     int cap = 0;                 // shots the state buffers hold
-    float *state = nullptr;      // P 4 + mx, mz, mxx, mzz 2 each + vx1, vz1: 14 planes per shot
+    float *state = nullptr;      // P 4 + mx, mz, mxx, mzz 2 each: 12 planes per shot
     float *rec = nullptr;        // [cap][n_rx][T]
     int *shot_idx = nullptr;     // [cap]
     hipStream_t stream = nullptr;
@@ -271,7 +315,7 @@ int ensure_state(otfwi_cpml *h, int n)
     if (n <= h->cap && h->state) return 0;
     dfree(h->state); dfree(h->rec); dfree(h->shot_idx);
     h->cap = 0;
-    if (int rc = dalloc(&h->state, (size_t)n * 14 * plane_of(h))) return rc;
+    if (int rc = dalloc(&h->state, (size_t)n * 12 * plane_of(h))) return rc;
     if (int rc = dalloc(&h->rec, (size_t)n * std::max(h->n_rx, 1) * h->T)) return rc;
     if (int rc = dalloc(&h->shot_idx, (size_t)n)) return rc;
     h->cap = n;
@@ -312,32 +356,8 @@ int otfwi_cpml_create(otfwi_cpml **out, int X, int Z, int T, int ppe, const floa
         if (hipMemcpy(dst, prof[k], n * sizeof(float), hipMemcpyHostToDevice))
             rc = cfail(-2, "profile upload failed");
     }
-    if (!rc) {
-        // tiles whose memory variables stay 0: a == 0 and b == 1 on every
-        // column (row) of the tile, for the half-point (frame 1) and the
-        // full-point (frame 2) profiles
-        h->ntx = (X + TX - 1) / TX;
-        h->ntz = (Z + TZ - 1) / TZ;
-        std::vector<unsigned char> sk(2 * (h->ntx + h->ntz), 1);
-        auto still = [](const float *a, const float *b, int lo, int hi) {
-            for (int k = lo; k < hi; k++)
-                if (a[k] != 0.0f || b[k] != 1.0f) return false;
-            return true;
-        };
-        for (int b = 0; b < h->ntx; b++) {
-            const int lo = b * TX, hi = std::min(X, lo + TX);
-            sk[b] = still(prof[4], prof[5], lo, hi);                          // axh, bxh
-            sk[h->ntx + h->ntz + b] = still(prof[1], prof[2], lo, hi);        // ax, bx
-        }
-        for (int b = 0; b < h->ntz; b++) {
-            const int lo = b * TZ, hi = std::min(Z, lo + TZ);
-            sk[h->ntx + b] = still(prof[10], prof[11], lo, hi);               // azh, bzh
-            sk[2 * h->ntx + h->ntz + b] = still(prof[7], prof[8], lo, hi);    // az, bz
-        }
-        rc = dalloc(&h->skip, sk.size());
-        if (!rc && hipMemcpy(h->skip, sk.data(), sk.size(), hipMemcpyHostToDevice))
-            rc = cfail(-2, "tile flags upload failed");
-    }
+    h->ntx = (X + FX - 1) / FX;
+    h->ntz = (Z + FZ - 1) / FZ;
     if (!rc) {
         if (initial)
             e = hipMemcpy(h->init, initial, 2 * pl * sizeof(float), hipMemcpyHostToDevice);
@@ -357,7 +377,7 @@ int otfwi_cpml_destroy(otfwi_cpml *h)
 {
     if (!h) return 0;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    dfree(h->med); dfree(h->prof); dfree(h->init); dfree(h->pts); dfree(h->src); dfree(h->skip);
+    dfree(h->med); dfree(h->prof); dfree(h->init); dfree(h->pts); dfree(h->src);
     dfree(h->state); dfree(h->rec); dfree(h->shot_idx);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -410,29 +430,23 @@ int otfwi_cpml_forward(otfwi_cpml *h, int n_shots, const int *shot_idx, float *r
     const size_t pl = plane_of(h);
     hipStream_t st = h->stream;
     CHK(hipMemcpyAsync(h->shot_idx, shot_idx, n_shots * sizeof(int), hipMemcpyHostToDevice, st));
-    // P slots 0, 1 = initial, 2, 3 = 0; memory variables and derivatives 0
-    CHK(hipMemsetAsync(h->state, 0, (size_t)n_shots * 14 * pl * sizeof(float), st));
+    // P slots 0, 1 = initial, 2, 3 = 0; memory variables 0
+    CHK(hipMemsetAsync(h->state, 0, (size_t)n_shots * 12 * pl * sizeof(float), st));
     Fields f;
     f.P = h->state;
     float *rest = h->state + (size_t)n_shots * 4 * pl;
     f.mx = rest; f.mz = rest + (size_t)n_shots * 2 * pl;
     f.mxx = rest + (size_t)n_shots * 4 * pl; f.mzz = rest + (size_t)n_shots * 6 * pl;
-    f.vx1 = rest + (size_t)n_shots * 8 * pl; f.vz1 = rest + (size_t)n_shots * 9 * pl;
     for (int s = 0; s < n_shots; s++)
         CHK(hipMemcpyAsync(f.P + (size_t)s * 4 * pl, h->init, 2 * pl * sizeof(float),
                            hipMemcpyDeviceToDevice, st));
     const Medium m = medium(h);
     const Dims d{h->X, h->Z, (long long)pl};
     const dim3 grid(h->ntx, h->ntz, n_shots), blk(256);
-    const unsigned char *sk1x = h->skip, *sk1z = h->skip + h->ntx;
-    const unsigned char *sk2x = h->skip + h->ntx + h->ntz, *sk2z = sk2x + h->ntx;
     const int *sx = h->pts, *sz = h->pts + h->n_src, *rx = h->pts + 2 * h->n_src;
     const int *rz = rx + h->n_rx;
     for (int t = 0; t < h->T; t++) {
-        if (t > 1) {
-            hipLaunchKernelGGL(cpml_frame1, grid, blk, 0, st, f, m, d, t, sk1x, sk1z);
-            hipLaunchKernelGGL(cpml_frame2, grid, blk, 0, st, f, m, d, t, sk2x, sk2z);
-        }
+        if (t > 1) hipLaunchKernelGGL(cpml_step, grid, blk, 0, st, f, m, d, t);
         hipLaunchKernelGGL(cpml_src_rec, dim3(n_shots), dim3(64), 0, st, f, m, d, t, h->T, h->ppe,
                            h->n_src, sx, sz, h->src, h->shot_idx, h->n_rx, rx, rz, h->rec);
     }

@@ -103,7 +103,10 @@ def run_exact(N=1024, T=4000, shots=8, snap=0):
             "interior_cosine_exact_vs_reference": cos}
 
 
-CPML_B = 76
+# Bytes per cell-step of the second-order C-PML step with both frames in one
+# pass: P(t-1), P(t-2) read, P(t) written, kappa, rho_hx, rho_hz read, the four
+# memory variables read and written (the x / z derivative fields stay on chip).
+CPML_B = 56
 
 
 def run_cpml(N=1024, T=1000, shots=64):
