@@ -139,6 +139,29 @@ def test_pair_streams_bitwise(gpu, graph):
         m.set_pair_streams(3)
 
 
+def test_split_loops_launch_directly_with_graphs_on(gpu):
+    """Graphs on (the default) with the two-step phase split over two
+    streams: the split loops run as direct launches, the one-step loops
+    replay graphs, and results stay bitwise those of one step per launch
+    over repeated calls (the bench's configuration at 64 x 1024^2)."""
+    p = small_problem(X=420, Z=300, T=140, pml=10, seed=12, n_src=5, n_rx=30)
+    shots = [1, 4, 0, 3, 2]
+    obs = _obs(p, shots)
+    out = {}
+    for steps, k in ((1, 1), (2, 2)):
+        m = make_model(p, cq="cq_bg")
+        m.set_steps_per_launch(steps, steps)
+        m.set_pair_streams(k)
+        m.set_observed(obs)
+        res = [m.forward(shots)]
+        for _ in range(3):
+            g, mis = m.gradient(shots, "l2")
+            res += [g, mis, m.forward(shots)]
+        out[(steps, k)] = res
+    for u, v in zip(out[(1, 1)], out[(2, 2)]):
+        assert np.array_equal(u, v)
+
+
 def test_pairs_batch_bound(gpu):
     """Two-step launches address both slots of a field from one shot's base
     with 32-bit buffer offsets below 0x7FFFFFF0 bytes; a larger batch runs one
