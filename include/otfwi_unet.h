@@ -36,6 +36,15 @@ int otfwi_unet_conv3x3(const otfwi_bf16 *A, int CA, const otfwi_bf16 *B, int CB,
                        const otfwi_bf16 *Wt, const float *bias, const otfwi_bf16 *mask,
                        otfwi_bf16 *Y, int N, int H, int W, int Cout, int relu, void *stream);
 
+/* Pixel tiles per work-group of otfwi_unet_conv3x3 where the input channels
+ * fit one 32- or 16-channel chunk and the output block is 32 channels: 0 (the
+ * default) = 4 when the launch keeps >= 1024 work-groups, else 1; 1; or 4.
+ * With 4, a work-group stages the weights once and fetches the next tile
+ * while computing the current one.  Results are bitwise the same for every
+ * value.  Process-wide.  No reference counterpart (the reference's network
+ * runs in PyTorch). */
+int otfwi_unet_set_conv_tiles(int nt);
+
 /* Weight gradient of that convolution: dW[co][tap][ci] += sum over pixels of
  * G[p][co] * input[p + tap][ci], and (dB may be NULL) the bias gradient
  * dB[co] += sum over pixels of G[p][co] (FP32, atomically added: zero both

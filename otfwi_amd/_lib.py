@@ -144,6 +144,7 @@ _f = ctypes.c_float
 UNET_SIGS = [
     ("otfwi_unet_last_error", ctypes.c_char_p, []),
     ("otfwi_unet_conv3x3", _i, [_vp, _i, _vp, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
+    ("otfwi_unet_set_conv_tiles", _i, [_i]),
     ("otfwi_unet_wgrad", _i, [_vp, _i, _vp, _i, _i, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
     ("otfwi_unet_flip_weights", _i, [_vp, _i, _i, _vp, _vp]),
     ("otfwi_unet_pool2", _i, [_vp, _i, _i, _i, _i, _vp, _vp]),

@@ -246,6 +246,173 @@ int launch_conv(const ConvArgs &a, int N, hipStream_t st)
     return hipck(hipGetLastError());
 }
 
+// Convolutions whose input channels fit one chunk (Cin == CK: the level-0
+// layers of the network and their data gradients).  conv3x3_kernel can only
+// load, wait and compute there; here a work-group stages the weights once and
+// runs NT consecutive pixel tiles, fetching the next tile's patch into
+// registers while the current tile's MFMAs and epilogue run.  The output tile
+// has its own LDS region, so two barriers per tile suffice.  Every output is
+// the same MFMA sequence as in conv3x3_kernel (bitwise equal results).
+template <int TX, int TN, int CK>
+struct ConvTiles {
+    using S = ConvShape<TX, TN, CK>;
+    static constexpr size_t patch_elems = (size_t)S::PH * S::PW * S::PS;
+    static constexpr size_t w_elems = (size_t)9 * TN * S::WS;
+    static constexpr size_t lds_bytes = (patch_elems + w_elems) * 2 + S::out_bytes;
+};
+
+#ifndef CONV_TAP_UNROLL
+#define CONV_TAP_UNROLL 9
+#endif
+template <int TX, int TN, int CK, int NT>
+__global__ __launch_bounds__(256) void conv3x3_tiles_kernel(ConvArgs a)
+{
+    using S = ConvShape<TX, TN, CK>;
+    using L = ConvTiles<TX, TN, CK>;
+    constexpr int TY = S::TY, PW = S::PW, PH = S::PH, PS = S::PS, WS = S::WS;
+    constexpr int NJ = TN / 32, PV = CK / 8;
+    extern __shared__ __attribute__((aligned(16))) bf16 lds[];
+    bf16 *patch = lds;
+    bf16 *wl = lds + L::patch_elems;
+    bf16 *ot = wl + L::w_elems;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int H = a.in.H, W = a.in.W;
+    const int tiles_x = W / TX;
+    const int co0 = blockIdx.y * TN, n = blockIdx.z;
+    const int r = lane & 31, kh8 = (lane >> 5) * 8;
+    int pbase[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const int q = wave * 64 + i * 32 + r;
+        pbase[i] = (q / TX) * PW + (q % TX);
+    }
+    constexpr int NPV = (PH * PW * PV + 255) / 256, NWV = (9 * TN * PV + 255) / 256;
+#pragma unroll
+    for (int k = 0; k < NWV; k++) {   // the weights, once (Cin == CK)
+        const int v = tid + 256 * k;
+        if (v < 9 * TN * PV) {
+            const int row = v / PV, part = v - row * PV;   // row = tap*TN + co
+            const int tap = row / TN, co = row - tap * TN;
+            *(u32x4 *)(wl + row * WS + part * 8) =
+                *(const u32x4 *)(a.Wt + ((size_t)(co0 + co) * 9 + tap) * CK + part * 8);
+        }
+    }
+    u32x4 rp[NPV];
+    auto fetch = [&](int tile) {
+        const int tx0 = (tile % tiles_x) * TX, ty0 = (tile / tiles_x) * TY;
+#pragma unroll
+        for (int k = 0; k < NPV; k++) {
+            const int v = tid + 256 * k;
+            if (v < PH * PW * PV) {
+                const int pp = v / PV, part = v - pp * PV;
+                const int py = pp / PW, px = pp - py * PW;
+                rp[k] = load8(a.in, n, ty0 + py - 1, tx0 + px - 1, part * 8);
+            }
+        }
+    };
+    const int tile0 = blockIdx.x * NT;
+    fetch(tile0);
+    for (int t = 0; t < NT; t++) {
+        const int tile = tile0 + t;
+        const int tx0 = (tile % tiles_x) * TX, ty0 = (tile / tiles_x) * TY;
+#pragma unroll
+        for (int k = 0; k < NPV; k++) {
+            const int v = tid + 256 * k;
+            if (v < PH * PW * PV) {
+                const int pp = v / PV, part = v - pp * PV;
+                *(u32x4 *)(patch + pp * PS + part * 8) = rp[k];
+            }
+        }
+        __syncthreads();   // patch (and, the first time, weights) staged; last tile's output read
+        if (t + 1 < NT) fetch(tile + 1);
+        f32x16 acc[2][NJ];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < NJ; j++)
+#pragma unroll
+                for (int e = 0; e < 16; e++) acc[i][j][e] = 0.0f;
+#pragma unroll CONV_TAP_UNROLL
+        for (int tap = 0; tap < 9; tap++) {
+            const int dy = tap / 3, dx = tap - 3 * (tap / 3);
+#pragma unroll
+            for (int kk = 0; kk < CK; kk += 16) {
+                bf16x8 af[2], bfm[NJ];
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+                    af[i] = *(const bf16x8 *)(patch + (pbase[i] + dy * PW + dx) * PS + kk + kh8);
+#pragma unroll
+                for (int j = 0; j < NJ; j++)
+                    bfm[j] = *(const bf16x8 *)(wl + (tap * TN + j * 32 + r) * WS + kk + kh8);
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+#pragma unroll
+                    for (int j = 0; j < NJ; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i], bfm[j],
+                                                                            acc[i][j], 0, 0, 0);
+            }
+        }
+        // epilogue as conv3x3_kernel's, through the output region
+#pragma unroll
+        for (int j = 0; j < NJ; j++) {
+            const int cl = j * 32 + r;
+            const float b = a.bias ? a.bias[co0 + cl] : 0.0f;
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int e = 0; e < 16; e++) {
+                    const int row = 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+                    const int q = wave * 64 + i * 32 + row;
+                    float v = acc[i][j][e] + b;
+                    if (a.relu) v = fmaxf(v, 0.0f);
+                    ot[q * S::OS + cl] = (bf16)v;
+                }
+        }
+        __syncthreads();   // output tile complete; every read of this patch done
+        constexpr int PV8 = TN / 8;
+        for (int v = tid; v < 256 * PV8; v += 256) {
+            const int q = v / PV8, part = v - q * PV8;
+            const int y = ty0 + q / TX, x = tx0 + q % TX;
+            const size_t o = (((size_t)n * H + y) * W + x) * a.Cout + co0 + part * 8;
+            u32x4 val = *(const u32x4 *)(ot + q * S::OS + part * 8);
+            if (a.mask) {
+                const bf16x8 mk = __builtin_bit_cast(bf16x8, *(const u32x4 *)(a.mask + o));
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const unsigned lo = (float)mk[2 * k] > 0.0f ? 0x0000FFFFu : 0u;
+                    const unsigned hi = (float)mk[2 * k + 1] > 0.0f ? 0xFFFF0000u : 0u;
+                    val[k] &= lo | hi;
+                }
+            }
+            *(u32x4 *)(a.Y + o) = val;
+        }
+    }
+}
+
+constexpr int CONV_NT = 4;   // pixel tiles per work-group of conv3x3_tiles_kernel
+
+template <int TX, int TN, int CK>
+int launch_conv_tiles(const ConvArgs &a, int N, hipStream_t st)
+{
+    using L = ConvTiles<TX, TN, CK>;
+    const void *fn = (const void *)conv3x3_tiles_kernel<TX, TN, CK, CONV_NT>;
+    if (int rc = hipck(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)L::lds_bytes)))
+        return rc;
+    const int ntiles = (a.in.W / TX) * (a.in.H / (256 / TX));
+    dim3 grid(ntiles / CONV_NT, a.Cout / TN, N);
+    hipLaunchKernelGGL((conv3x3_tiles_kernel<TX, TN, CK, CONV_NT>), grid, dim3(256), L::lds_bytes,
+                       st, a);
+    return hipck(hipGetLastError());
+}
+
+// 0: by shape (the tile loop where Cin fits one chunk, the output block is 32
+// channels and the launch still has >= 1024 work-groups), 1: one tile per
+// work-group, CONV_NT: the tile loop wherever Cin fits one chunk, the block is
+// 32 channels and the tiles divide evenly.
+int g_conv_tiles = 0;
+
 // ---------------------------------------------------------------------------
 // weight gradient: dW[co][tap][ci] += sum_p G[p][co] * X[p + tap][ci]
 // M = 32*MJ output channels, N = 32 input channels, K = pixels.  Wave w
@@ -684,6 +851,19 @@ int otfwi_unet_conv3x3(const otfwi_bf16 *A, int CA, const otfwi_bf16 *B, int CB,
     if (!TX || H % (256 / TX)) return ufail("W must be a multiple of 32 and H of the tile height");
     const int TN = Cout % 64 == 0 ? 64 : 32;
     hipStream_t st = ST(stream);
+    const long long ntiles = (long long)(W / TX) * (H / (256 / TX));
+    // the tile loop for 32-channel output blocks (75 KB of LDS, two work-groups
+    // per CU as conv3x3_kernel; 64-channel blocks would need 112 KB)
+    const bool tiles = Cin == CK && TN == 32 && ntiles % CONV_NT == 0 && g_conv_tiles != 1 &&
+                       (g_conv_tiles == CONV_NT ||
+                        ntiles / CONV_NT * (Cout / TN) * (long long)N >= 1024);
+    if (tiles) {
+        if (CK == 16)
+            return TX == 64 ? launch_conv_tiles<64, 32, 16>(a, N, st)
+                            : launch_conv_tiles<32, 32, 16>(a, N, st);
+        return TX == 64 ? launch_conv_tiles<64, 32, 32>(a, N, st)
+                        : launch_conv_tiles<32, 32, 32>(a, N, st);
+    }
     if (CK == 16) {
         if (TX == 64)
             return TN == 64 ? launch_conv<64, 64, 16>(a, N, st) : launch_conv<64, 32, 16>(a, N, st);
@@ -692,6 +872,13 @@ int otfwi_unet_conv3x3(const otfwi_bf16 *A, int CA, const otfwi_bf16 *B, int CB,
     if (TX == 64)
         return TN == 64 ? launch_conv<64, 64, 32>(a, N, st) : launch_conv<64, 32, 32>(a, N, st);
     return TN == 64 ? launch_conv<32, 64, 32>(a, N, st) : launch_conv<32, 32, 32>(a, N, st);
+}
+
+int otfwi_unet_set_conv_tiles(int nt)
+{
+    if (nt != 0 && nt != 1 && nt != CONV_NT) return ufail("conv tiles per work-group: 0, 1 or 4");
+    g_conv_tiles = nt;
+    return 0;
 }
 
 int otfwi_unet_wgrad(const otfwi_bf16 *A, int CA, const otfwi_bf16 *B, int CB, int b_half,

@@ -142,3 +142,16 @@ def test_cpml_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(lib, n), n
     assert sorted(cpml.EXPORTED) == names
+
+
+def test_unet_library_exports_every_declared_symbol():
+    """libotfwi_unet.so loads and exports every function include/otfwi_unet.h
+    declares, and the ctypes table binds exactly those (no GPU call)."""
+    from otfwi_amd import _lib
+
+    names = header_functions("otfwi_unet.h")
+    assert "otfwi_unet_conv3x3" in names and "otfwi_unet_set_conv_tiles" in names
+    lib = ctypes.CDLL(_lib.UNET_LIB_PATH)
+    for n in names:
+        assert hasattr(lib, n), n
+    assert sorted(_lib.UNET_EXPORTED) == names
