@@ -38,9 +38,9 @@ int otfwi_unet_conv3x3(const otfwi_bf16 *A, int CA, const otfwi_bf16 *B, int CB,
 
 /* Pixel tiles per work-group of otfwi_unet_conv3x3 where the input channels
  * fit one 32- or 16-channel chunk and the output block is 32 channels: 0 (the
- * default) = 4 when the launch keeps >= 1024 work-groups, else 1; 1; or 4.
- * With 4, a work-group stages the weights once and fetches the next tile
- * while computing the current one.  Results are bitwise the same for every
+ * default) = 8 when the launch keeps >= 1024 work-groups, else 1; 1; or 8.
+ * With 8, a work-group stages the weights once and fetches the next two
+ * tiles while computing the current one.  Results are bitwise the same for every
  * value.  Process-wide.  No reference counterpart (the reference's network
  * runs in PyTorch). */
 int otfwi_unet_set_conv_tiles(int nt);

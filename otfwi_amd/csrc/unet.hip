@@ -249,8 +249,11 @@ int launch_conv(const ConvArgs &a, int N, hipStream_t st)
 // Convolutions whose input channels fit one chunk (Cin == CK: the level-0
 // layers of the network and their data gradients).  conv3x3_kernel can only
 // load, wait and compute there; here a work-group stages the weights once and
-// runs NT consecutive pixel tiles, fetching the next tile's patch into
-// registers while the current tile's MFMAs and epilogue run.  The output tile
+// runs NT consecutive pixel tiles, with the patches of the next CONV_PF tiles
+// in flight into registers while the current tile's MFMAs and epilogue run
+// (measured at 8 x 1024^2, 32 -> 32 channels: one tile per work-group
+// 420-503 us; 4 tiles with 1 patch ahead 341-357 us; 4 with 2 ahead 340-348;
+// 8 with 2 ahead 324-332).  The output tile
 // has its own LDS region, so two barriers per tile suffice.  Every output is
 // the same MFMA sequence as in conv3x3_kernel (bitwise equal results).
 template <int TX, int TN, int CK>
@@ -263,6 +266,9 @@ struct ConvTiles {
 
 #ifndef CONV_TAP_UNROLL
 #define CONV_TAP_UNROLL 9
+#endif
+#ifndef CONV_PF
+#define CONV_PF 2
 #endif
 template <int TX, int TN, int CK, int NT>
 __global__ __launch_bounds__(256) void conv3x3_tiles_kernel(ConvArgs a)
@@ -298,8 +304,9 @@ __global__ __launch_bounds__(256) void conv3x3_tiles_kernel(ConvArgs a)
                 *(const u32x4 *)(a.Wt + ((size_t)(co0 + co) * 9 + tap) * CK + part * 8);
         }
     }
-    u32x4 rp[NPV];
-    auto fetch = [&](int tile) {
+    // CONV_PF patches in registers: tile t + CONV_PF is fetched while tile t runs
+    u32x4 rp[CONV_PF][NPV];
+    auto fetch = [&](int tile, u32x4 (&dst)[NPV]) {
         const int tx0 = (tile % tiles_x) * TX, ty0 = (tile / tiles_x) * TY;
 #pragma unroll
         for (int k = 0; k < NPV; k++) {
@@ -307,12 +314,15 @@ __global__ __launch_bounds__(256) void conv3x3_tiles_kernel(ConvArgs a)
             if (v < PH * PW * PV) {
                 const int pp = v / PV, part = v - pp * PV;
                 const int py = pp / PW, px = pp - py * PW;
-                rp[k] = load8(a.in, n, ty0 + py - 1, tx0 + px - 1, part * 8);
+                dst[k] = load8(a.in, n, ty0 + py - 1, tx0 + px - 1, part * 8);
             }
         }
     };
     const int tile0 = blockIdx.x * NT;
-    fetch(tile0);
+#pragma unroll
+    for (int u = 0; u < CONV_PF; u++)
+        if (u < NT) fetch(tile0 + u, rp[u]);
+#pragma unroll
     for (int t = 0; t < NT; t++) {
         const int tile = tile0 + t;
         const int tx0 = (tile % tiles_x) * TX, ty0 = (tile / tiles_x) * TY;
@@ -321,11 +331,11 @@ __global__ __launch_bounds__(256) void conv3x3_tiles_kernel(ConvArgs a)
             const int v = tid + 256 * k;
             if (v < PH * PW * PV) {
                 const int pp = v / PV, part = v - pp * PV;
-                *(u32x4 *)(patch + pp * PS + part * 8) = rp[k];
+                *(u32x4 *)(patch + pp * PS + part * 8) = rp[t % CONV_PF][k];
             }
         }
         __syncthreads();   // patch (and, the first time, weights) staged; last tile's output read
-        if (t + 1 < NT) fetch(tile + 1);
+        if (t + CONV_PF < NT) fetch(tile + CONV_PF, rp[t % CONV_PF]);
         f32x16 acc[2][NJ];
 #pragma unroll
         for (int i = 0; i < 2; i++)
@@ -390,7 +400,10 @@ __global__ __launch_bounds__(256) void conv3x3_tiles_kernel(ConvArgs a)
     }
 }
 
-constexpr int CONV_NT = 4;   // pixel tiles per work-group of conv3x3_tiles_kernel
+#ifndef CONV_TILES
+#define CONV_TILES 8
+#endif
+constexpr int CONV_NT = CONV_TILES;   // pixel tiles per work-group of conv3x3_tiles_kernel
 
 template <int TX, int TN, int CK>
 int launch_conv_tiles(const ConvArgs &a, int N, hipStream_t st)
@@ -876,7 +889,7 @@ int otfwi_unet_conv3x3(const otfwi_bf16 *A, int CA, const otfwi_bf16 *B, int CB,
 
 int otfwi_unet_set_conv_tiles(int nt)
 {
-    if (nt != 0 && nt != 1 && nt != CONV_NT) return ufail("conv tiles per work-group: 0, 1 or 4");
+    if (nt != 0 && nt != 1 && nt != CONV_NT) return ufail("conv tiles per work-group: 0, 1 or 8");
     g_conv_tiles = nt;
     return 0;
 }

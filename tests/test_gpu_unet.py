@@ -71,9 +71,9 @@ def test_conv3x3_vs_oracle(gpu, N, H, W, CA, CB, Cout):
     (2, 64, 96, 32, 32, 0, True),     # 32-wide tiles (W % 64 != 0)
 ])
 def test_conv_tile_loop_bitwise(gpu, N, H, W, CA, Cout, relu, masked):
-    """Four pixel tiles per work-group (weights staged once, next patch
-    fetched during the current tile) give bitwise the one-tile results, and
-    match the FP32 oracle."""
+    """Eight pixel tiles per work-group (weights staged once, the next two
+    patches fetched during the current tile) give bitwise the one-tile
+    results, and match the FP32 oracle."""
     rng = np.random.default_rng(N + H + W + CA)
     lib = _lib.load_unet()
     A = bf(rng.standard_normal((N, H, W, CA)))
@@ -84,7 +84,7 @@ def test_conv_tile_loop_bitwise(gpu, N, H, W, CA, Cout, relu, masked):
     dm = None if mask is None else mask.cuda()
     out = {}
     try:
-        for nt in (1, 4, 0):
+        for nt in (1, 8, 0):
             _lib.ucheck(lib.otfwi_unet_set_conv_tiles(nt))
             Y = torch.empty((N, H, W, Cout), dtype=torch.bfloat16, device="cuda")
             _lib.ucheck(lib.otfwi_unet_conv3x3(vp(dA), CA, None, 0, 0, vp(dw), vp(db), vp(dm),
@@ -93,14 +93,14 @@ def test_conv_tile_loop_bitwise(gpu, N, H, W, CA, Cout, relu, masked):
             out[nt] = Y.view(torch.int16).cpu().numpy()
     finally:
         _lib.ucheck(lib.otfwi_unet_set_conv_tiles(0))
-    assert np.array_equal(out[1], out[4]) and np.array_equal(out[1], out[0])
+    assert np.array_equal(out[1], out[8]) and np.array_equal(out[1], out[0])
     with pytest.raises(Exception):
         _lib.ucheck(lib.otfwi_unet_set_conv_tiles(2))
     ref = R.conv3x3(A.float().numpy(), None, w.float().numpy(), bias, relu=bool(relu))
     if masked:
         ref = np.where(mask.float().numpy() > 0, ref, 0.0)
-    y4 = torch.from_numpy(out[4]).view(torch.bfloat16).float().numpy()
-    e = rel(y4, ref)
+    y8 = torch.from_numpy(out[8]).view(torch.bfloat16).float().numpy()
+    e = rel(y8, ref)
     print(f"conv tiles N{N} {H}x{W} {CA}->{Cout}: rel {e:.2e}")
     assert e < 5e-3
 
