@@ -11,10 +11,9 @@ exact  one checkpointed exact-adjoint gradient (L2) of the 1024^2 FMC block,
        beside the reference-mode gradient of the same shots.
 cpml   the second-order C-PML scheme (regression_CPML.cu): --shots runs of a
        --N^2 grid, T = --T steps, each shot firing its own source, advanced in
-       one batch; 76 algorithmic bytes per cell-step (frame 1: P, 2 memory
-       variables read, 2 written, 2 derivative fields written, 2 densities
-       read; frame 2: the 2 derivative fields, 2 memory variables read and
-       written, P twice, kappa, P written).
+       one batch; 56 algorithmic bytes per cell-step (both frames in one
+       pass: P(t-1) and P(t-2) read, P(t) written, kappa and the 2 densities
+       read, the 4 memory variables read and written).
 """
 from __future__ import annotations
 
