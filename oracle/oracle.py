@@ -32,6 +32,25 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 
+def load(path):
+    """ctypes handle of a library built from oracle_pml.c, with the argument
+    types of its entry points set."""
+    L = ctypes.CDLL(path)
+    fp = ctypes.c_void_p
+    i = ctypes.c_int
+    for sfx in ("_ref", "_f64"):
+        f = getattr(L, "orc_forward" + sfx)
+        f.argtypes = [i, i, i, fp, fp, fp, fp, i, fp, fp, fp, i, i, fp, fp, fp,
+                      i, fp, fp, fp, fp]
+        f.restype = i
+        a = getattr(L, "orc_adjoint" + sfx)
+        a.argtypes = [i, i, i, fp, fp, fp, fp, i, fp, fp, fp, i, fp, fp, fp, fp, fp, fp, fp]
+        a.restype = i
+    L.orc_num_threads.restype = i
+    L.orc_set_threads.argtypes = [i]
+    return L
+
+
 def lib():
     """Load (building on demand when a compiler is present) liboracle_pml.so."""
     global _LIB
@@ -40,20 +59,7 @@ def lib():
         if not os.path.exists(path):
             import subprocess
             subprocess.check_call(["make", "-s", "-C", _HERE])
-        L = ctypes.CDLL(path)
-        fp = ctypes.c_void_p
-        i = ctypes.c_int
-        for sfx in ("_ref", "_f64"):
-            f = getattr(L, "orc_forward" + sfx)
-            f.argtypes = [i, i, i, fp, fp, fp, fp, i, fp, fp, fp, i, i, fp, fp, fp,
-                          i, fp, fp, fp, fp]
-            f.restype = i
-            a = getattr(L, "orc_adjoint" + sfx)
-            a.argtypes = [i, i, i, fp, fp, fp, fp, i, fp, fp, fp, i, fp, fp, fp, fp, fp]
-            a.restype = i
-        L.orc_num_threads.restype = i
-        L.orc_set_threads.argtypes = [i]
-        _LIB = L
+        _LIB = load(path)
     return _LIB
 
 
@@ -114,9 +120,10 @@ def _prep(dtype, *arrs):
 
 
 def forward(cquad, d_x, d_z, src_xz, src, rx_xz, T, idx=-1, ring_xz=None,
-            constvec=(1.0, 1.0, 1.0), precision="ref", want_seed=False):
+            constvec=(1.0, 1.0, 1.0), precision="ref", want_seed=False, _lib=None):
     """Forward run.  precision "ref" = FP32 state with FP64 derivatives (the
-    reference's arithmetic), "f64" = FP64 throughout.
+    reference's arithmetic), "f64" = FP64 throughout.  _lib: a handle from
+    load() to run instead of liboracle_pml.so.
 
     Returns dict(rec=(n_rx,T), ring=(n_rev,T) or None, seed=(4,Z,X) or None).
     """
@@ -136,7 +143,8 @@ def forward(cquad, d_x, d_z, src_xz, src, rx_xz, T, idx=-1, ring_xz=None,
         vx = vz = ring = None
         nv = 0
     seed = np.zeros((4, Z, X), dt) if want_seed else None
-    fn = lib().orc_forward_ref if precision == "ref" else lib().orc_forward_f64
+    L = _lib or lib()
+    fn = L.orc_forward_ref if precision == "ref" else L.orc_forward_f64
     rc = fn(X, Z, T, _p(cq), _p(cv), _p(dxm), _p(dzm), ns, _p(sx), _p(sz), _p(s), idx,
             nr, _p(rx), _p(rz), _p(rec), nv, _p(vx), _p(vz), _p(ring), _p(seed))
     if rc != 0:
@@ -145,8 +153,11 @@ def forward(cquad, d_x, d_z, src_xz, src, rx_xz, T, idx=-1, ring_xz=None,
 
 
 def adjoint(cquad, d_x, d_z, rx_xz, adj, ring_xz, ring, seed,
-            constvec=(1.0, 1.0, 1.0), precision="ref"):
-    """Phase 2: reverse reconstruction + adjoint run + imaging -> grad (Z,X)."""
+            constvec=(1.0, 1.0, 1.0), precision="ref", want_hist=False, _lib=None):
+    """Phase 2: reverse reconstruction + adjoint run + imaging -> grad (Z,X).
+    With want_hist, returns (grad, U (T,Z,X), Lambda (T,Z,X)): the
+    reconstructed forward field Px_b+Pz_b and the adjoint field Px_f+Pz_f of
+    every reverse step t, as the imaging condition reads them."""
     dt = np.float32 if precision == "ref" else np.float64
     Z, X = cquad.shape
     T = adj.shape[1]
@@ -155,12 +166,15 @@ def adjoint(cquad, d_x, d_z, rx_xz, adj, ring_xz, ring, seed,
     rx, rz = _prep(np.int32, *rx_xz)
     vx, vz = _prep(np.int32, *ring_xz)
     grad = np.zeros((Z, X), dt)
-    fn = lib().orc_adjoint_ref if precision == "ref" else lib().orc_adjoint_f64
+    uh = np.zeros((T, Z, X), dt) if want_hist else None
+    lh = np.zeros((T, Z, X), dt) if want_hist else None
+    L = _lib or lib()
+    fn = L.orc_adjoint_ref if precision == "ref" else L.orc_adjoint_f64
     rc = fn(X, Z, T, _p(cq), _p(cv), _p(dxm), _p(dzm), len(rx), _p(rx), _p(rz), _p(a),
-            len(vx), _p(vx), _p(vz), _p(rg), _p(sd), _p(grad))
+            len(vx), _p(vx), _p(vz), _p(rg), _p(sd), _p(grad), _p(uh), _p(lh))
     if rc != 0:
         raise RuntimeError(f"oracle adjoint failed ({rc})")
-    return grad
+    return (grad, uh, lh) if want_hist else grad
 
 
 # --------------------------------------------------------------------------

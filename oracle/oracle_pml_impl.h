@@ -195,12 +195,15 @@ static inline AR CAT(lap, SFX)(const ST *u, int X, int Z, int x, int z)
 /* Phase 2 of cuda_grad_ext (:783-836): time-reversed reconstruction of the
  * forward field from `seed` with ring restore, adjoint propagation with the
  * running-sum of the flipped adjoint source at the receivers, and the
- * imaging condition grad += (Px_f+Pz_f) * lap8(Px_b+Pz_b) at every t. */
+ * imaging condition grad += (Px_f+Pz_f) * lap8(Px_b+Pz_b) at every t.
+ * uhist (T*Z*X, may be NULL) receives the reconstructed U = Px_b+Pz_b and
+ * lhist (T*Z*X, may be NULL) the adjoint field Px_f+Pz_f of every t, as the
+ * imaging condition reads them. */
 int CAT(orc_adjoint, SFX)(int X, int Z, int T, const ST *cq, const float *cv,
                           const ST *dxm, const ST *dzm,
                           int nr, const int *rx, const int *rz, const ST *adj,
                           int nv, const int *vx, const int *vz, const ST *ring,
-                          const ST *seed, ST *grad)
+                          const ST *seed, ST *grad, ST *uhist, ST *lhist)
 {
     if (X < 2 * BOX + 1 || Z < 2 * BOX + 1 || T < 2)
         return -1;
@@ -241,6 +244,10 @@ int CAT(orc_adjoint, SFX)(int X, int Z, int T, const ST *cq, const float *cv,
             U[i] = (ST)((AR)Pxb[cur * XZ + i] + (AR)Pzb[cur * XZ + i]);
             L[i] = (ST)((AR)Pxf[cur * XZ + i] + (AR)Pzf[cur * XZ + i]);
         }
+        if (uhist)
+            memcpy(uhist + (size_t)t * XZ, U, XZ * sizeof(ST));
+        if (lhist)
+            memcpy(lhist + (size_t)t * XZ, L, XZ * sizeof(ST));
 #pragma omp parallel for schedule(static)
         for (int z = 0; z < Z; z++)
             for (int x = 0; x < X; x++) {
