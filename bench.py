@@ -33,30 +33,20 @@ ADJ_BYTES = 60              # lambda 3R+3W, u_b 3R+3W, c^2 R, grad R+W
 
 
 def build_problem(N, T, n_elem, seed):
-    from oracle import oracle as O  # noqa: F401  (pml profile comes from the wrapper formula)
-    from otfwi_amd import configs
+    """The configuration-2 setup through the drop-in wrapper, as a reference
+    driver builds it (regression.py:34-44): Cuda_interface_PML computes the
+    PML profile (_calcPML) and, in init_regression with prec_deriv=4, the
+    boundary ring; the observed traces come from a forward run of the true
+    model.  Returns (wrapper, config)."""
+    from otfwi_amd import adjoint_functions, configs
+    from otfwi_amd.cuda_interface_pml import Cuda_interface_PML
 
     cfg = configs.config2(model_seed=seed, N=N, T=T, n_elem=n_elem)
-    L = cfg["pml"]
-    d0 = 3 * 0.5 * np.log(1 / cfg["R"]) / (2 * L ** 3)
-    prof = d0 * np.linspace(0, L, L) ** 2
-    d_x = np.zeros((N, N))
-    d_z = np.zeros((N, N))
-    d_z[:L, :] = prof[::-1, None]
-    d_z[N - L:, :] = prof[:, None]
-    d_x[:, N - L:] = prof[None, :]
-    d_x[:, :L] = prof[None, ::-1]
     m = cfg["src_mask"]
-    ring = np.zeros((N, N), bool)
-    ring[L:L + 5, L:N - L] = True
-    ring[N - L - 5:N - L, L:N - L] = True
-    ring[L:N - L, L:L + 5] = True
-    ring[L:N - L, N - L - 5:N - L] = True
-    ring |= m
-    zz, xx = np.nonzero(m)
-    rz, rx = np.nonzero(ring)
-    return dict(cfg=cfg, d_x=d_x.astype(np.float32), d_z=d_z.astype(np.float32),
-                src_x=xx, src_z=zz, ring_x=rx, ring_z=rz)
+    n = int(m.sum())
+    iface = Cuda_interface_PML(N, N, T, cfg["pml"], cfg["c_true"], n, m, n, m, cfg["src"],
+                               multishot=True, R=cfg["R"])
+    return iface, cfg, adjoint_functions
 
 
 def cpu_baseline(N, T, n_shots, n_rx, sample_steps):
@@ -258,15 +248,9 @@ def main():
     from otfwi_amd import PMLModel, _lib
 
     N, T, S = args.N, args.T, args.shots
-    prob = build_problem(N, T, S, seed=0)
-    cfg = prob["cfg"]
-    m = PMLModel(N, N, T)
-    m.set_damping(prob["d_x"], prob["d_z"])
-    m.set_sources(prob["src_x"], prob["src_z"], cfg["src"])
-    m.set_receivers(prob["src_x"], prob["src_z"])
-    m.set_ring(prob["ring_x"], prob["ring_z"])
+    iface, cfg, AF = build_problem(N, T, S, seed=0)
+    m = iface.model
     shots = np.arange(S)
-    m.set_cquad((cfg["c_true"] ** 2).astype(np.float32))
     tune = None
     if args.th_fwd or args.th_adj:
         m.set_tuning(args.th_fwd, args.th_adj, 0)
@@ -279,27 +263,29 @@ def main():
     if args.pair_streams:
         m.set_pair_streams(args.pair_streams)
     ktimes = m.kernel_times(S, reps=10)
-    obs = m.forward(shots)
-    m.set_observed(obs)
-    m.set_cquad((cfg["c_bg"] ** 2).astype(np.float32))
+    iface.simulate()                                  # observed traces of the true model
+    misfit_fn = {"w2": AF.wasserstein, "l2": AF.L2, "env": AF.quad_envelope}[args.misfit]
+    iface.init_regression(iface.recording.copy(), misfit_fn, prec_deriv=4)
+    iface._set_c_quad(cfg["c_bg"] ** 2)               # gradient at the defect-free background
     fs, ads = m.steps_per_launch(S)
+    graphs = m.graph_modes(S)
     print(f"[bench] device {_lib.device_info()}; {S} shots {N}x{N} T={T}; tiles {m.get_tuning()}"
           f" persistent {m.get_launch()} autotune {tune}; steps per launch fwd {fs} adj {ads};"
-          f" kernel ms {ktimes}", file=sys.stderr, flush=True)
+          f" graphs {graphs}; kernel ms {ktimes}", file=sys.stderr, flush=True)
 
     for i in range(args.warmup):
-        m.gradient(shots, misfit=args.misfit)
+        iface.calc_grad()
         print(f"[bench] warmup {i} done", file=sys.stderr, flush=True)
     t_ev = dict(fwd_ms=0.0, misfit_ms=0.0, adj_ms=0.0, n_fwd=0, n_adj=0)
     t0 = time.perf_counter()
     for i in range(args.steps):
-        grad, mis = m.gradient(shots, misfit=args.misfit)
+        iface.calc_grad()                             # whole gradient, device misfit
         tm = m.timing()
         for k in t_ev:
             t_ev[k] += tm[k]
-        print(f"[bench] step {i}: misfit {mis.mean():.6e}", file=sys.stderr, flush=True)
+        print(f"[bench] step {i}: misfit {iface.mse[0]:.6e}", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
-    assert np.isfinite(grad).all() and np.isfinite(mis).all()
+    assert np.isfinite(iface.grad).all() and np.isfinite(iface.mse).all()
     ms_step = 1e3 * wall / args.steps
     value = args.steps / wall
     cells = S * N * N
@@ -359,7 +345,7 @@ def main():
                    "T": T, "shots": S, "receivers": S, "misfit": args.misfit, "pml": 20,
                    "tiles": list(m.get_tuning()[:2]), "persistent": list(m.get_launch()),
                    "steps_per_launch": [fs, ads], "autotune": tune,
-                   "graphs": not args.direct, "pair_streams": args.pair_streams},
+                   "graphs": graphs, "pair_streams": args.pair_streams},
         "roofline": {"bound": "hbm", "kernel": unit_name,
                      "achieved": unit_ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": unit_ach / HBM_PEAK,
@@ -384,7 +370,7 @@ def main():
     if not args.no_cpu:
         cb = cpu_baseline(N, T, S, S, args.cpu_steps)
         out["cpu_baseline"] = cb
-    del m
+    del m, iface
     cp = runtime_copy()
     out["runtime_copy_GBps"] = cp
     out["roofline"]["frac_of_runtime_copy"] = out["roofline"]["achieved"] / cp

@@ -198,6 +198,13 @@ int otfwi_set_pair_streams(otfwi_model *m, int k);
  * counterpart. */
 int otfwi_steps_per_launch(otfwi_model *m, int n_shots, int *fwd, int *adj);
 
+/* How the forward and adjoint time loops of a batch of n_shots shots run
+ * under the current settings: out[0] / out[2] = streams of the forward /
+ * adjoint two-step phase (1 when the loop runs one step per launch or does
+ * not split), out[1] / out[3] = 1 when that loop is replayed from a captured
+ * hipGraph, 0 when it is launched directly.  No reference counterpart. */
+int otfwi_loop_modes(otfwi_model *m, int n_shots, int *out);
+
 /* Replay the per-step launch loops as captured hipGraphs (default 1); a
  * graph is re-captured only when shots, tiles, launch style or device buffers
  * change.  Loops whose two-step phase splits over streams

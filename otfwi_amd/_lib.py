@@ -53,6 +53,7 @@ _SIGS = [
     ("otfwi_set_pair_width", _i, [_vp, _i]),
     ("otfwi_set_pair_streams", _i, [_vp, _i]),
     ("otfwi_steps_per_launch", _i, [_vp, _i, _vp, _vp]),
+    ("otfwi_loop_modes", _i, [_vp, _i, _vp]),
     ("otfwi_autotune", _i, [_vp, _i, _i, _vp, _vp]),
     ("otfwi_get_timing", _i, [_vp, _vp]),
     ("otfwi_forward", _i, [_vp, _i, _vp, _vp]),

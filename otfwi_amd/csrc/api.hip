@@ -1909,6 +1909,21 @@ int otfwi_steps_per_launch(otfwi_model *m, int n_shots, int *fwd, int *adj)
     return OTFWI_OK;
 }
 
+int otfwi_loop_modes(otfwi_model *m, int n_shots, int *out)
+{
+    RCK(check_model(m));
+    if (n_shots <= 0 || !out) return fail(OTFWI_E_ARG, "loop_modes: bad arguments");
+    int f = 1, a = 1;
+    RCK(otfwi_steps_per_launch(m, n_shots, &f, &a));
+    const int ns = pair_split(m, n_shots);
+    const int fs = f == 2 ? ns : 1, as = a == 2 ? ns : 1;
+    out[0] = fs;
+    out[1] = m->use_graph && fs == 1 ? 1 : 0;
+    out[2] = as;
+    out[3] = m->use_graph && as == 1 ? 1 : 0;
+    return OTFWI_OK;
+}
+
 int otfwi_set_pair_streams(otfwi_model *m, int k)
 {
     RCK(check_model(m));

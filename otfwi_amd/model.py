@@ -150,6 +150,14 @@ class PMLModel:
                                                ctypes.byref(a)))
         return int(f.value), int(a.value)
 
+    def graph_modes(self, n_shots):
+        """How the forward and adjoint loops of n_shots shots run:
+        {"fwd": "graph" | "direct", "fwd_streams": k, "adj": ..., "adj_streams": k}."""
+        out = (ctypes.c_int * 4)()
+        check(self._lib.otfwi_loop_modes(self._h, int(n_shots), out))
+        return {"fwd": "graph" if out[1] else "direct", "fwd_streams": int(out[0]),
+                "adj": "graph" if out[3] else "direct", "adj_streams": int(out[2])}
+
     def set_pair_streams(self, k=0):
         """Streams of the two-step loops: 1, 2, 4 or 0 (by size); a loop that
         splits is launched directly, also with graphs on (set_graph)."""

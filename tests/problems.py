@@ -61,6 +61,30 @@ def config1_problem(seed=0, prec_deriv=4):
     )
 
 
+def fmc_problem(N=1024, T=600, n_elem=64, pitch=12, blob=(120, 500, 30), prec_deriv=4):
+    """The full-matrix-capture geometry of the 64-shot 1024^2 workload (pitch
+    12 cells on row 20, 20-cell PML, R = 1e-3, 5 MHz Ricker at CFL 0.5) with
+    a truncated T; the true model carries one slow blob (z, x, radius in
+    cells) shallow enough to answer within T steps."""
+    cfg = configs.config2(model_seed=0, N=N, T=T, n_elem=n_elem, pitch=pitch)
+    pml = cfg["pml"]
+    zz, xx = np.mgrid[0:N, 0:N]
+    c_true = cfg["c_bg"].copy()
+    bz, bx, br = blob
+    c_true[(zz - bz) ** 2 + (xx - bx) ** 2 <= br * br] = configs.courant(3200.0)
+    d_x, d_z = O.calc_pml(N, N, pml, R=cfg["R"])
+    ring = O.revert_mask(N, N, pml, prec_deriv, cfg["src_mask"])
+    return dict(
+        X=N, Z=N, T=T, pml=pml,
+        cq_true=_round(c_true ** 2), cq_bg=_round(cfg["c_bg"] ** 2),
+        d_x=_round(d_x), d_z=_round(d_z),
+        src_mask=cfg["src_mask"], rx_mask=cfg["rx_mask"],
+        src_xz=O.point_order(cfg["src_mask"]), rx_xz=O.point_order(cfg["rx_mask"]),
+        ring_xz=O.point_order(ring),
+        src=_round(cfg["src"]),
+    )
+
+
 def make_model(p, precision=4, cq="cq_true", th_fwd=0, th_adj=0, max_batch=0):
     from otfwi_amd import PMLModel
 

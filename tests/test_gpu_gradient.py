@@ -1,12 +1,18 @@
 """Adjoint gradient on the GPU against the CPU oracle (reference phase 2,
 regression_PML_single.cu:704-846, with adjoint_functions.L2 / wasserstein).
 
-Bars: L2 misfit within 1e-6 and L2 gradient within 1e-4 of the
-reference-precision oracle; the FP64 build within 1e-9 of the FP64 oracle.
-W2 amplifies FP32 trace rounding (about 300x in the adjoint source), so the
-FP32 W2 gradient is held to the reference-vs-FP64 spread measured in the same
-test instead (see the printed numbers), and the W2 stage itself is checked on
-identical traces in test_gpu_misfit.py.
+Bars (fixed):
+* L2: misfit within 1e-6 and gradient within 1e-4 of the reference-precision
+  oracle; the FP64 build within 1e-9 / 1e-6 of the FP64 oracle;
+* W2 adjoint propagation given the oracle's own FP32 adjoint source (the
+  library's host-adjoint entry point): gradient within 1e-4;
+* W2 end to end: gradient within 2e-3 of both the reference-precision and
+  the FP64 oracle, misfit within 1e-6 of the reference-precision oracle.
+  The W2 adjoint source amplifies the FP32 rounding of the traces about
+  300x (relative noise 1e-7 on the traces moves it by 6e-4), so an FP32
+  propagation cannot sit within 1e-4 of FP64 end to end; the reference's own
+  arithmetic is 1.2e-3 from FP64 at the 256^2 configuration.  The W2 stage
+  itself is checked on identical traces in test_gpu_misfit.py.
 """
 import numpy as np
 import pytest
@@ -27,6 +33,26 @@ def oracle_grad(p, obs, shots, misfit, precision="ref"):
                       p["ring_xz"], obs, p["T"], misfit, shots=shots, precision=precision)
 
 
+def oracle_adjoint_sources(p, obs, shots, misfit):
+    """The reference-precision oracle's own FP32 adjoint source per shot."""
+    out = []
+    for k, idx in enumerate(shots):
+        rec = O.forward(p["cq_bg"], p["d_x"], p["d_z"], p["src_xz"], p["src"], p["rx_xz"],
+                        p["T"], idx=idx)["rec"]
+        out.append(misfit(rec.astype(np.float64), obs[k])[1].astype(np.float32))
+    return np.stack(out)
+
+
+def check_w2_propagation(m, p, obs, shots, g_ref, label):
+    """Adjoint propagation alone: the library's forward with ring save, then
+    the oracle's FP32 W2 adjoint sources through the host-adjoint entry."""
+    adj = oracle_adjoint_sources(p, obs, shots, O.wasserstein)
+    m.forward_for_grad(shots)
+    e = rel(m.adjoint_from_host(adj), g_ref)
+    print(f"{label} W2 adjoint propagation vs oracle: {e:.3e}")
+    assert e <= 1e-4
+
+
 @pytest.mark.parametrize("kind", ["l2", "w2"])
 def test_small_gradient(gpu, kind):
     p = small_problem(seed=11)
@@ -44,8 +70,9 @@ def test_small_gradient(gpu, kind):
     if kind == "l2":
         assert eg < 1e-4 and em < 1e-6
     else:
-        assert eg < max(1e-4, 5 * rel(g_ref, g64))
-        assert em < max(1e-6, 5 * abs(mse_ref - mse64) / abs(mse64))
+        print(f"w2: grad vs f64 {rel(grad, g64):.3e}")
+        assert eg <= 2e-3 and rel(grad, g64) <= 2e-3 and em <= 1e-6
+        check_w2_propagation(m, p, obs, shots, g_ref, "small")
 
 
 @pytest.mark.parametrize("kind", ["l2", "w2"])
@@ -79,7 +106,8 @@ def test_config1_l2_and_w2(gpu):
         if kind == "l2":
             assert eg < 1e-4 and em < 1e-6
         else:
-            assert eg < max(1e-4, 5 * rel(g_ref, g64))
+            assert eg <= 2e-3 and rel(grad, g64) <= 2e-3 and em <= 1e-6
+            check_w2_propagation(m, p, obs, [-1], g_ref, "config1")
 
 
 def test_host_adjoint_matches_device(gpu):
@@ -141,7 +169,7 @@ def test_tile_invariance_with_interior_tiles(gpu):
     e, spread = rel(base[0], g_ref), rel(g_ref, g64)
     print(f"330x160 W2: gradient vs reference-precision oracle {e:.3e} (oracle ref vs f64 "
           f"{spread:.3e}); misfit {base[1].mean():.6e} vs {mse_ref:.6e} (f64 {mse64:.6e})")
-    assert e < max(1e-4, 5 * spread)
+    assert e <= 2e-3 and rel(base[0], g64) <= 2e-3
 
 
 def test_all_zero_observed_trace_is_reported(gpu):
@@ -177,8 +205,9 @@ def test_persistent_launch_bitwise(gpu, thf, tha):
 def test_envelope_gradient(gpu):
     """Device gradient with the squared-envelope misfit against the oracle's
     gradient with the reference's quad_envelope steps (host adjoint source
-    rounded to FP32, as adj_wrapper does); bars as for W2: the
-    reference-vs-FP64 spread of the same test."""
+    rounded to FP32, as adj_wrapper does).  Fixed bars: gradient within
+    5e-3, misfit within 1e-5 (the envelope difference amplifies FP32 trace
+    rounding like W2; the oracle's own FP32 vs FP64 spread is printed)."""
     p = small_problem(seed=14)
     shots = [0, 1]
     obs = observed(p, shots)
@@ -191,8 +220,8 @@ def test_envelope_gradient(gpu):
     eg, em = rel(grad, g_ref), abs(mis.mean() - mse_ref) / abs(mse_ref)
     print(f"env: grad {eg:.3e} (oracle ref vs f64 {rel(g_ref, g64):.3e}), "
           f"misfit {em:.3e} (ref vs f64 {abs(mse_ref - mse64) / abs(mse64):.3e})")
-    assert eg < max(1e-4, 5 * rel(g_ref, g64))
-    assert em < max(1e-6, 5 * abs(mse_ref - mse64) / abs(mse64))
+    assert eg <= 5e-3 and rel(grad, g64) <= 5e-3
+    assert em <= 1e-5
 
 
 def test_envelope_drop_in_device_path(gpu):
