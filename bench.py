@@ -251,6 +251,10 @@ def main():
     iface, cfg, AF = build_problem(N, T, S, seed=0)
     m = iface.model
     shots = np.arange(S)
+    iface.simulate()                                  # observed traces of the true model
+    misfit_fn = {"w2": AF.wasserstein, "l2": AF.L2, "env": AF.quad_envelope}[args.misfit]
+    iface.init_regression(iface.recording.copy(), misfit_fn, prec_deriv=4)
+    iface._set_c_quad(cfg["c_bg"] ** 2)               # gradient at the defect-free background
     tune = None
     if args.th_fwd or args.th_adj:
         m.set_tuning(args.th_fwd, args.th_adj, 0)
@@ -263,10 +267,6 @@ def main():
     if args.pair_streams:
         m.set_pair_streams(args.pair_streams)
     ktimes = m.kernel_times(S, reps=10)
-    iface.simulate()                                  # observed traces of the true model
-    misfit_fn = {"w2": AF.wasserstein, "l2": AF.L2, "env": AF.quad_envelope}[args.misfit]
-    iface.init_regression(iface.recording.copy(), misfit_fn, prec_deriv=4)
-    iface._set_c_quad(cfg["c_bg"] ** 2)               # gradient at the defect-free background
     fs, ads = m.steps_per_launch(S)
     graphs = m.graph_modes(S)
     print(f"[bench] device {_lib.device_info()}; {S} shots {N}x{N} T={T}; tiles {m.get_tuning()}"

@@ -1719,6 +1719,16 @@ int otfwi_destroy(otfwi_model *m)
     (void)hipStreamSynchronize(m->stream);
     for (int i = 0; i < 3; i++)
         if (m->xs[i]) (void)hipStreamSynchronize(m->xs[i]);
+    // every stream must be idle before anything it may still read is
+    // released; if one is not, keep the model's memory (a leak, never a
+    // use-after-free) and report it
+    bool idle = hipStreamQuery(m->stream) == hipSuccess;
+    for (int i = 0; i < 3; i++)
+        if (m->xs[i]) idle = idle && hipStreamQuery(m->xs[i]) == hipSuccess;
+    if (!idle) {
+        fprintf(stderr, "otfwi_destroy: a stream of the model is not idle; its memory is kept\n");
+        return fail(OTFWI_E_STATE, "destroy: a stream of the model is not idle");
+    }
     for (int i = 0; i < 3; i++)
         if (m->gexec[i]) (void)hipGraphExecDestroy(m->gexec[i]);
     for (int i = 0; i < 4; i++)

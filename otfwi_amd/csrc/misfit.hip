@@ -558,23 +558,22 @@ hipError_t launch_envelope(const R *sim, const O *obs, R *adj, double *mis, int 
 
 // jcum[t] = sum_{tau=1..t} adj[T-1-tau] (jcum[0] = 0): the running integral of
 // the flipped adjoint source (somaFonteIntegral with flip=1,
-// regression_PML_single.cu:276-292, 803), accumulated in FP64 then rounded.
+// regression_PML_single.cu:276-292, 803), summed sequentially in the state
+// precision as the reference's integral[n] += source[...] does: one thread
+// per trace, so every J[k] rounds exactly as the reference's running sum.
 template <typename R>
 __global__ void __launch_bounds__(WAVE)
-jcum_kernel(const R *adj, R *jcum, int T)
+jcum_kernel(const R *adj, R *jcum, int ntraces, int T)
 {
-    const int tr = blockIdx.x, lane = threadIdx.x;
+    const int tr = blockIdx.x * WAVE + threadIdx.x;
+    if (tr >= ntraces) return;
     const R *a = adj + (size_t)tr * T;
     R *J = jcum + (size_t)tr * T;
-    const int C = (T + WAVE - 1) / WAVE;
-    const int k0 = lane * C, k1 = min(k0 + C, T);
-    double local = 0.0;
-    for (int k = max(k0, 1); k < k1; k++)
-        local += (double)a[T - 1 - k];
-    double run = wave_excl_scan(local);
-    for (int k = k0; k < k1; k++) {
-        if (k > 0) run += (double)a[T - 1 - k];
-        J[k] = (R)run;
+    R run = R(0);
+    J[0] = run;
+    for (int k = 1; k < T; k++) {
+        run = run + a[T - 1 - k];
+        J[k] = run;
     }
 }
 
@@ -582,7 +581,8 @@ template <typename R>
 hipError_t launch_jcum(const R *adj, R *jcum, int ntraces, int T, hipStream_t s)
 {
     if (ntraces <= 0) return hipSuccess;
-    hipLaunchKernelGGL((jcum_kernel<R>), dim3(ntraces), dim3(WAVE), 0, s, adj, jcum, T);
+    hipLaunchKernelGGL((jcum_kernel<R>), dim3((ntraces + WAVE - 1) / WAVE), dim3(WAVE), 0, s,
+                       adj, jcum, ntraces, T);
     return hipGetLastError();
 }
 template hipError_t launch_jcum<float>(const float *, float *, int, int, hipStream_t);
@@ -610,8 +610,7 @@ hipError_t launch_misfit(int kind, const R *sim, const O *obs, R *adj, R *jcum, 
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || !jcum) return e;
-    hipLaunchKernelGGL((jcum_kernel<R>), dim3(ntraces), dim3(WAVE), 0, s, adj, jcum, T);
-    return hipGetLastError();
+    return launch_jcum<R>(adj, jcum, ntraces, T, s);
 }
 template hipError_t launch_misfit<float, float>(int, const float *, const float *, float *,
                                                 float *, double *, int, int, double *,

@@ -168,22 +168,50 @@ class FWI:
         return c, hist
 
 
-def lbfgs(cp: Cuda_interface_PML, c0, vmin, vmax, gmask, grad_scale, maxiter=200, callback=None,
-          options=None):
-    """regression.py:45-87 on this library: minimize 1e3*mse over the
-    velocity with L-BFGS-B, jac = grad_scale*(gmask*grad), bounds [vmin, vmax].
-    Returns scipy's OptimizeResult."""
-    from scipy.optimize import Bounds, minimize
-
+def regression_objective(cp: Cuda_interface_PML, gmask, grad_scale):
+    """The objective regression.py:45-49 hands to L-BFGS-B: calc_grad at the
+    velocity ``params`` (flattened (Z, X)), value ``1e3*mse``, gradient
+    ``grad_scale*(gmask*grad)`` flattened."""
     Z, X = cp.Z, cp.X
 
     def objfun(params):
-        cp.calc_grad(c=params.reshape(Z, X))
+        cp.calc_grad(c=np.asarray(params).reshape(Z, X))
         grad = grad_scale * (gmask * cp.grad).flatten()
         return 1e3 * float(cp.mse[0]), grad.astype(np.float64)
+
+    return objfun
+
+
+def model_misfit(params, c_true):
+    """Model misfit the reference's callback logs per iteration
+    (regression.py:76-77): ||params - c_true||^2 / (Nz*Nx)."""
+    c_true = np.asarray(c_true, np.float64)
+    return float(np.linalg.norm(np.asarray(params).reshape(c_true.shape) - c_true) ** 2
+                 / c_true.size)
+
+
+def lbfgs(cp: Cuda_interface_PML, c0, vmin, vmax, gmask, grad_scale, maxiter=200, callback=None,
+          options=None, c_true=None):
+    """regression.py:45-87 on this library: minimize 1e3*mse over the
+    velocity with L-BFGS-B, jac = grad_scale*(gmask*grad), bounds [vmin, vmax]
+    (options gtol = ftol = 1e-8, maxcor 20, maxls 10).  With c_true, the
+    result carries ``model_misfit``: the model misfit after each iteration,
+    as the reference's callback records it.  Returns scipy's OptimizeResult."""
+    from scipy.optimize import Bounds, minimize
+
+    objfun = regression_objective(cp, gmask, grad_scale)
+    log = []
+
+    def cb(x):
+        if c_true is not None:
+            log.append(model_misfit(x, c_true))
+        if callback is not None:
+            callback(x)
 
     opt = {"gtol": 1e-8, "ftol": 1e-8, "maxiter": maxiter, "maxcor": 20, "maxls": 10}
     if options:
         opt.update(options)
-    return minimize(objfun, np.asarray(c0, np.float64).flatten(), jac=True, method="L-BFGS-B",
-                    bounds=Bounds(vmin, vmax), options=opt, callback=callback)
+    r = minimize(objfun, np.asarray(c0, np.float64).flatten(), jac=True, method="L-BFGS-B",
+                 bounds=Bounds(vmin, vmax), options=opt, callback=cb)
+    r.model_misfit = np.array(log)
+    return r

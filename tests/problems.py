@@ -61,7 +61,7 @@ def config1_problem(seed=0, prec_deriv=4):
     )
 
 
-def fmc_problem(N=1024, T=600, n_elem=64, pitch=12, blob=(120, 500, 30), prec_deriv=4):
+def fmc_problem(N=1024, T=600, n_elem=64, pitch=12, blob=(40, 524, 14), prec_deriv=4):
     """The full-matrix-capture geometry of the 64-shot 1024^2 workload (pitch
     12 cells on row 20, 20-cell PML, R = 1e-3, 5 MHz Ricker at CFL 0.5) with
     a truncated T; the true model carries one slow blob (z, x, radius in

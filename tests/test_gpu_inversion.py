@@ -121,3 +121,41 @@ def test_lbfgs_regression_flow(gpu):
     print("L-BFGS-B:", j0, "->", r.fun, r.message, len(seen))
     assert r.fun < j0
     assert np.all(r.x >= 0.25 - 1e-12) and np.all(r.x <= 0.5 + 1e-12)
+
+
+@pytest.mark.parametrize("kind", ["l2", "w2"])
+def test_regression_objective_matches_oracle(gpu, kind):
+    """The value and gradient regression.py:45-49 hands to L-BFGS-B
+    (1e3*mse, grad_scale*gmask*grad over a multishot drop-in) against the
+    oracle's mean misfit and mean gradient over the shots, with the same
+    mask and scale.  Bars: L2 value 1e-6, gradient 1e-4; W2 value 1e-6,
+    gradient 2e-3 (the W2 bar of test_gpu_gradient.py)."""
+    from otfwi_amd import Cuda_interface_PML, adjoint_functions
+
+    p = small_problem(X=128, Z=112, T=360, seed=26, n_src=2, n_rx=10)
+    c_true = np.sqrt(p["cq_true"].astype(np.float64))
+    c0 = np.sqrt(p["cq_bg"].astype(np.float64))
+    cp = Cuda_interface_PML(p["X"], p["Z"], p["T"], p["pml"], c_true, 2, p["src_mask"],
+                            int(p["rx_mask"].sum()), p["rx_mask"], p["src"], multishot=True,
+                            R=1e-3)
+    cp.simulate()
+    obs = cp.recording.copy()
+    fn = adjoint_functions.L2 if kind == "l2" else adjoint_functions.wasserstein
+    cp.init_regression(obs, fn, prec_deriv=4)
+    gmask = inversion.array_mask(p["Z"], p["X"], p["pml"], p["pml"] + 6, taper=4)
+    scale = 4e3 * float(c0.mean())
+    val, grad = inversion.regression_objective(cp, gmask, scale)(c0.flatten())
+    # the oracle at the same c^2 (FP32, as _set_c_quad stores it) and observed traces
+    cq = (c0 ** 2).astype(np.float32)
+    ofn = O.l2 if kind == "l2" else O.wasserstein
+    g_ref, mse_ref, _ = O.gradient(cq, p["d_x"], p["d_z"], p["src_xz"], p["src"], p["rx_xz"],
+                                   p["ring_xz"], obs, p["T"], ofn, shots=(0, 1))
+    want_val = 1e3 * mse_ref
+    want_grad = (scale * (gmask * g_ref)).flatten()
+    ev = abs(val - want_val) / abs(want_val)
+    eg = rel(grad, want_grad)
+    print(f"regression objective {kind}: value {ev:.3e}, gradient {eg:.3e}")
+    assert ev <= 1e-6 and eg <= (1e-4 if kind == "l2" else 2e-3)
+    assert inversion.model_misfit(c_true.flatten(), c_true) == 0.0
+    want_mm = np.sum((c0 - c_true) ** 2) / c0.size
+    assert abs(inversion.model_misfit(c0.flatten(), c_true) - want_mm) <= 1e-12 * want_mm
