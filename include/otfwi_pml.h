@@ -333,6 +333,21 @@ int otfwi_misfit(int misfit_kind, int n_traces, int T, const double *sim, const 
 int otfwi_misfit_exact(int misfit_kind, int n_traces, int T, const double *sim,
                        const double *obs, double *misfit_per_trace, double *rho_out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): otfwi_misfit on device arrays that allocates nothing.
+ * sim, obs, adj_out: n_traces x T FP64 device arrays; misfit_out: n_traces
+ * FP64 device array; stream: the hipStream_t the work is enqueued on (NULL =
+ * the null stream); the call returns after that stream has finished it.
+ * Kinds L2 and W2 (the envelope misfit is refused: use otfwi_misfit).  The
+ * W2 kernels' work arrays live in a fixed 32 MiB device array of the library
+ * (one per GPU, held until the process ends): W2 needs T <= 10240 and runs
+ * the traces in chunks of (32 MiB - 256) / (24 T + 4) at a time (349 traces
+ * at T = 4000).  Every call on one GPU shares that array, so calls that
+ * overlap on one GPU (several host threads or streams) race; serialize them. */
+int otfwi_misfit_noalloc(int misfit_kind, int n_traces, int T, const double *sim,
+                         const double *obs, double *misfit_out, double *adj_out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -69,6 +69,7 @@ _SIGS = [
     ("otfwi_get_illum", _i, [_vp, _vp, _i]),
     ("otfwi_misfit", _i, [_i, _i, _i, _vp, _vp, _vp, _vp]),
     ("otfwi_misfit_exact", _i, [_i, _i, _i, _vp, _vp, _vp, _vp]),
+    ("otfwi_misfit_noalloc", _i, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
 ]
 
 EXPORTED = [s[0] for s in _SIGS]

@@ -11,7 +11,9 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <climits>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -2223,6 +2225,115 @@ int otfwi_misfit_exact(int kind, int n_traces, int T, const double *sim, const d
                        double *mis_out, double *rho_out)
 {
     return misfit_host(kind, true, n_traces, T, sim, obs, mis_out, rho_out);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free misfit on device arrays: the W2 kernels' per-trace work
+// arrays live in a fixed file-scope device array (one copy per GPU, held
+// until the process ends); its address on each device is taken once with
+// hipGetSymbolAddress and only that address is handed to kernels and copies.
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)32 << 20;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+
+__device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// Address of this device's copy of `scratch` (looked up once per device,
+// with that device current).
+int scratch_on_current_device(void **out)
+{
+    int dev = -1;
+    HIPCK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+        return fail(OTFWI_E_ARG, "device %d beyond the %d the library tracks", dev,
+                    NOALLOC_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(OTFWI_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                        hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return OTFWI_OK;
+}
+
+// Fill n ints with v (the clears a captured graph may replay; no memset nodes).
+__global__ void fill_int_kernel(int *p, int n, int v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// status = min(status, base + i) over the traces i whose W2 flag is 1 (an
+// observed trace with a non-finite CDF).
+__global__ void w2_status_kernel(const int *flags, int n, int base, int *status)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && flags[i] == 1) atomicMin(status, base + i);
+}
+
+}  // namespace
+
+extern "C" {
+
+int otfwi_misfit_noalloc(int kind, int n_traces, int T, const double *sim, const double *obs,
+                         double *mis_out, double *adj_out, void *stream)
+{
+    if (kind != OTFWI_MISFIT_L2 && kind != OTFWI_MISFIT_W2)
+        return fail(OTFWI_E_ARG,
+                    "misfit_noalloc: kind %d not supported (L2 = 0 and W2 = 1; the envelope "
+                    "misfit needs a Hilbert table: use otfwi_misfit)", kind);
+    if (n_traces <= 0 || T < 2 || !sim || !obs || !mis_out || !adj_out)
+        return fail(OTFWI_E_ARG, "misfit_noalloc: bad arguments (n_traces=%d, T=%d)", n_traces, T);
+    const size_t per_trace = (size_t)3 * T * sizeof(double) + sizeof(int);
+    const size_t avail = NOALLOC_SCRATCH_BYTES - 256;
+    if (kind == OTFWI_MISFIT_W2 &&
+        ((size_t)2 * T * sizeof(double) > 160 * 1024 || per_trace > avail))
+        return fail(OTFWI_E_ARG, "misfit_noalloc: W2 supports T <= %d (T = %d)",
+                    (int)(160 * 1024 / 16), T);
+    hipStream_t st = (hipStream_t)stream;
+    if (kind == OTFWI_MISFIT_L2) {
+        HIPCK((launch_misfit<double, double>(kind, sim, obs, adj_out, nullptr, mis_out, n_traces,
+                                             T, nullptr, st)));
+        HIPCK(hipStreamSynchronize(st));
+        return OTFWI_OK;
+    }
+    void *base = nullptr;
+    RCK(scratch_on_current_device(&base));
+    // scratch: [status int, padded to 256 B][3 nc T doubles][nc flag ints] per chunk
+    int *status = (int *)base;
+    double *work = (double *)((unsigned char *)base + 256);
+    const int chunk = (int)std::min<size_t>(avail / per_trace, (size_t)n_traces);
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, st, status, 1, INT_MAX);
+    HIPCK(hipGetLastError());
+    for (int c0 = 0; c0 < n_traces; c0 += chunk) {
+        const int nc = std::min(chunk, n_traces - c0);
+        int *flags = (int *)(work + (size_t)3 * nc * T);
+        hipLaunchKernelGGL(fill_int_kernel, dim3((nc + 255) / 256), dim3(256), 0, st, flags, nc, 0);
+        HIPCK(hipGetLastError());
+        const size_t off = (size_t)c0 * T;
+        HIPCK((launch_misfit<double, double>(kind, sim + off, obs + off, adj_out + off, nullptr,
+                                             mis_out + c0, nc, T, work, st)));
+        hipLaunchKernelGGL(w2_status_kernel, dim3((nc + 255) / 256), dim3(256), 0, st, flags, nc,
+                           c0, status);
+        HIPCK(hipGetLastError());
+    }
+    int first_bad = INT_MAX;
+    HIPCK(hipMemcpyAsync(&first_bad, status, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCK(hipStreamSynchronize(st));
+    if (first_bad != INT_MAX)
+        return fail(OTFWI_E_NUMERIC, "W2: observed trace %d has a non-finite CDF", first_bad);
+    return OTFWI_OK;
 }
 
 // ===========================================================================
