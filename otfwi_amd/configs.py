@@ -110,6 +110,51 @@ def config2(model_seed=0, N=1024, T=4000, n_elem=64, pitch=12, pml=20):
                 n_shots=n_elem)
 
 
+C3_SIZES = (512, 768, 1024, 1536, 2048)
+
+
+def config3_T(N, t_scale=1.0):
+    """Steps of an N x N model of the uneven batch: 2500 at 512^2 to 6000 at
+    2048^2, linear in the grid depth (times t_scale)."""
+    return int(round((2500 + 3500 * (N - 512) / 1536) * t_scale))
+
+
+def config3_pitch(N, n_elem=32, pml=20):
+    """Array pitch (cells) of the uneven batch's 32-element arrays: half the
+    spacing that would span the undamped width less 40 cells."""
+    return max(2, (N - 2 * pml - 40) // (n_elem - 1) // 2)
+
+
+def config3(batch_seed=0, n_models=16, t_scale=1.0, n_elem=32):
+    """Uneven batch of independent inversions: n_models grid sizes drawn from
+    {512, 768, 1024, 1536, 2048}^2 (seeded), each with dx = 0.05 mm, a
+    32-element array on the top row (32 shots), config-2 defects (model seed
+    1000 + index) and config3_T(N) steps.  Returns a list of per-model dicts
+    in the form of config2."""
+    rng = np.random.default_rng(batch_seed)
+    sizes = rng.choice(C3_SIZES, size=n_models)
+    return [config2(model_seed=1000 + i, N=int(N), T=config3_T(int(N), t_scale), n_elem=n_elem,
+                    pitch=config3_pitch(int(N), n_elem))
+            for i, N in enumerate(sizes)]
+
+
+def config4(model_seed=0, N=2048, T=8000, n_elem=16, pitch=24, pml=20):
+    """Differentiation with checkpointing: N x N at dx = 0.025 mm (CFL 0.5),
+    a 16-element array at pitch 24 cells (0.6 mm) on the top row, config-2
+    defects at this dx (semi-axes 0.3-3 mm = 12-120 cells), T = 8000."""
+    dx = 2.5e-5
+    dt = CFL * dx / V_MAX
+    rng = np.random.default_rng(model_seed)
+    c_bg = np.full((N, N), float(courant(V_BG)))
+    c_true = draw_defects(c_bg.copy(), rng, pml, semi_min=12.0, semi_max=120.0)
+    mask, xs = linear_array(N, n_elem, pml, pitch=pitch)
+    w = ricker(5e6, dt, T)
+    src = np.tile(w, (n_elem, 1))
+    return dict(X=N, Z=N, T=T, pml=pml, R=1e-3, c_true=c_true, c_bg=c_bg,
+                src_mask=mask, rx_mask=mask.copy(), src=src, dx=dx, dt=dt,
+                n_shots=n_elem)
+
+
 def learn_config(model_seed=0, N=256, T=1200, n_elem=32, pitch=6, pml=20, n_max=3,
                  semi_min=6.0, semi_max=30.0):
     """Full-matrix capture block for the inversion comparisons and the learned

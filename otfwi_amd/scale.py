@@ -49,11 +49,11 @@ def _model(N, T, n_elem, pitch, seed, max_batch):
 
 def run_c3(n_models=16, shots=32, t_scale=1.0, workers=4, max_batch=0, seed=0, misfit="w2"):
     rng = np.random.default_rng(seed)
-    sizes = rng.choice([512, 768, 1024, 1536, 2048], size=n_models)
+    sizes = rng.choice(configs.C3_SIZES, size=n_models)
     models, cps, alg = [], [], 0.0
     for i, N in enumerate(sizes):
-        T = int(round((2500 + 3500 * (N - 512) / 1536) * t_scale))
-        pitch = max(2, (N - 2 * 20 - 40) // (shots - 1) // 2)
+        T = configs.config3_T(int(N), t_scale)
+        pitch = configs.config3_pitch(int(N), shots)
         cp = _model(int(N), T, shots, pitch, seed=1000 + i, max_batch=max_batch)
         cps.append(cp)
         models.append(cp.model)

@@ -85,6 +85,40 @@ def fmc_problem(N=1024, T=600, n_elem=64, pitch=12, blob=(40, 524, 14), prec_der
     )
 
 
+def _from_config(cfg, c_true=None, prec_deriv=4):
+    N, pml = cfg["X"], cfg["pml"]
+    c_true = cfg["c_true"] if c_true is None else c_true
+    d_x, d_z = O.calc_pml(N, N, pml, R=cfg["R"])
+    ring = O.revert_mask(N, N, pml, prec_deriv, cfg["src_mask"])
+    return dict(
+        X=N, Z=N, T=cfg["T"], pml=pml,
+        cq_true=_round(c_true ** 2), cq_bg=_round(cfg["c_bg"] ** 2),
+        d_x=_round(d_x), d_z=_round(d_z),
+        src_mask=cfg["src_mask"], rx_mask=cfg["rx_mask"],
+        src_xz=O.point_order(cfg["src_mask"]), rx_xz=O.point_order(cfg["rx_mask"]),
+        ring_xz=O.point_order(ring),
+        src=_round(cfg["src"]),
+    )
+
+
+def c3_problem(N, T, model_seed=1000, blob_depth=40):
+    """One model of the uneven batch (N x N, its 32-element array and
+    config-2 defects) with a truncated T and, added, one shallow blob under
+    the array's centre so that residuals arrive within T."""
+    cfg = configs.config2(model_seed=model_seed, N=N, T=T, n_elem=32,
+                          pitch=configs.config3_pitch(N))
+    zz, xx = np.mgrid[0:N, 0:N]
+    c_true = cfg["c_true"].copy()
+    c_true[(zz - blob_depth) ** 2 + (xx - N // 2) ** 2 <= 14 * 14] = configs.courant(3200.0)
+    return _from_config(cfg, c_true)
+
+
+def c4_problem(T, model_seed=0, n_elem=16):
+    """The 2048^2 differentiation configuration (dx = 0.025 mm, 16-element
+    array at pitch 24) with a truncated T."""
+    return _from_config(configs.config4(model_seed=model_seed, T=T, n_elem=n_elem))
+
+
 def make_model(p, precision=4, cq="cq_true", th_fwd=0, th_adj=0, max_batch=0):
     from otfwi_amd import PMLModel
 
