@@ -89,6 +89,20 @@ enum {
  * hilbert as scipy.signal.hilbert forms it; FP64 on the device). */
 enum { OTFWI_MISFIT_L2 = 0, OTFWI_MISFIT_W2 = 1, OTFWI_MISFIT_ENV = 2 };
 
+/* The four further Wasserstein-type misfits of adjoint_functions.py, same
+ * values and adjoint sources (FP64 on the device, one wavefront per trace):
+ * wasserstein_quad (:70-104, densities sim^2 and obs^2), wasserstein_experimental
+ * (:107-141), wasserstein_pm (:144-223, positive parts; the reference builds its
+ * negative-part CDFs from the positive parts) and wasserstein_yang (:226-266).
+ * Accepted wherever a misfit kind is (gradient, objective, otfwi_misfit); not
+ * by the exact-derivative calls. */
+enum {
+    OTFWI_MISFIT_W2_QUAD = 3,
+    OTFWI_MISFIT_W2_EXPERIMENTAL = 4,
+    OTFWI_MISFIT_W2_PM = 5,
+    OTFWI_MISFIT_W2_YANG = 6
+};
+
 typedef struct otfwi_model otfwi_model;
 
 /* Message of the last failed call on this thread ("" if none). */

@@ -15,6 +15,8 @@ Contents, each a restatement of one reference function:
 * :func:`adjoint`        -- phase 2 of ``cuda_grad_ext`` (regression_PML_single.cu:783-836)
 * :func:`l2`             -- ``adjoint_functions.L2`` (adjoint_functions.py:8-11)
 * :func:`wasserstein`    -- ``adjoint_functions.wasserstein`` (adjoint_functions.py:25-67)
+* :func:`wasserstein_quad`, :func:`wasserstein_experimental`, :func:`wasserstein_pm`,
+  :func:`wasserstein_yang` -- the further misfits of adjoint_functions.py:70-266
 * :func:`hilbert`        -- ``scipy.signal.hilbert`` as adjoint_functions.py:15-16 calls it
 * :func:`quad_envelope`  -- ``adjoint_functions.quad_envelope`` (adjoint_functions.py:14-22)
 * :func:`cpml_forward`   -- ``cuda_simulate2`` of the second-order C-PML scheme
@@ -192,7 +194,11 @@ def _interp_linear(xs, ys, q):
     stable (mergesort) sort of xs, searchsorted-left, index clipped to
     [1, n-1], then slope*(q-x_lo)+y_lo with slope=(y_hi-y_lo)/(x_hi-x_lo)."""
     xs = np.asarray(xs, dtype=np.float64)
-    ys = np.asarray(ys, dtype=np.float64)
+    ys = np.asarray(ys)
+    if ys.dtype.kind != "f":
+        ys = ys.astype(np.float64)
+    # y_hi - y_lo is formed in ys's own dtype (float32 data keeps it), the
+    # rest in float64, as interp1d does
     order = np.argsort(xs, kind="mergesort")
     xs = xs[order]
     ys = ys[order]
@@ -304,6 +310,137 @@ def wasserstein(sim, obs):
     for s in range(n):
         w, out[s] = wasserstein_trace(sim[s], obs[s])
         W2 += w
+    return W2, out
+
+
+def _trace_sum(v):
+    """Python's builtin sum over an array (sequential), as the reference's
+    misfit accumulations use it."""
+    w = 0
+    for x in v:
+        w += x
+    return w
+
+
+def wasserstein_quad(sim, obs):
+    """adjoint_functions.wasserstein_quad (:70-104), per trace: densities
+    sim^2 and obs^2 (in the traces' own dtypes) shifted by 0.05 max(obs^2),
+    normalized by their rectangle sums, CDFs cumsum(x dt); adjoint
+    dist(t) - dist(1) with dist = interp1d(t, 2 cumsum(t - G^-1(F(t))) dt);
+    misfit sum (F^-1(t) - G^-1(t))^2."""
+    sim, obs = np.asarray(sim), np.asarray(obs)
+    n, T = sim.shape
+    t = np.linspace(0, 1, T)
+    dt = t[1] - t[0]
+    W2 = 0
+    out = np.zeros((n, T))
+    for s in range(n):
+        g = obs[s, :] ** 2
+        f = sim[s, :] ** 2
+        c = np.max(g) * 0.05
+        g += c
+        f += c
+        g /= np.sum(g * dt)
+        f /= np.sum(f * dt)
+        G = np.cumsum(g * dt)
+        F = np.cumsum(f * dt)
+        q = _interp_linear(G, t, _interp_linear(t, F, t))
+        D = 2 * np.cumsum(t - q) * dt
+        out[s] = _interp_linear(t, D, t) - _interp_linear(t, D, np.array([1.0]))[0]
+        W2 += _trace_sum(np.abs(_interp_linear(F, t, t) - _interp_linear(G, t, t)) ** 2)
+    return W2, out
+
+
+def wasserstein_experimental(sim, obs):
+    """adjoint_functions.wasserstein_experimental (:107-141), per trace: the
+    shift of wasserstein (from obs), rectangle-rule CDFs; adjoint
+    Y(t) - Y(1) with Y = t ((t - q)^2 - 2 t q), q = G^-1(F(t)); misfit
+    sum (F^-1(t) - G^-1(t))^2 F^-1(t)."""
+    sim, obs = np.asarray(sim), np.asarray(obs)
+    n, T = sim.shape
+    t = np.linspace(0, 1, T)
+    dt = t[1] - t[0]
+    W2 = 0
+    out = np.zeros((n, T))
+    for s in range(n):
+        f0 = np.concatenate([np.zeros(0), sim[s, :]])
+        g0 = np.concatenate([np.zeros(0), obs[s, :]])
+        c = np.maximum(-1.1 * np.min(g0), 0.1 * np.max(g0))
+        g = g0 + c
+        f = f0 + c
+        g /= np.sum(g * dt)
+        f /= np.sum(f * dt)
+        G = np.cumsum(g * dt)
+        F = np.cumsum(f * dt)
+        q = _interp_linear(G, t, _interp_linear(t, F, t))
+        Y = t * ((t - q) ** 2 - 2 * t * q)
+        out[s] = _interp_linear(t, Y, t) - _interp_linear(t, Y, np.array([1.0]))[0]
+        F1 = _interp_linear(F, t, t)
+        W2 += _trace_sum(np.abs(F1 - _interp_linear(G, t, t)) ** 2 * F1)
+    return W2, out
+
+
+def wasserstein_pm(sim, obs):
+    """adjoint_functions.wasserstein_pm (:144-223), per trace: positive parts
+    max(0, x) (traces' own dtypes) shifted by 0.01 max(max(0, obs)) and
+    normalized; the reference's negative-part CDFs are built from the
+    positive parts (G_n = cumsum(g_p dt), F_n = cumsum(f_p dt), :180-183), so
+    both halves equal the positive one: adjoint dist(t) - dist(1) with
+    dist = cumsum(t - q) dt + cumsum(t - q) dt, misfit twice
+    sum (F_p^-1(t) - G_p^-1(t))^2.  (The intermediate TT, DD and I of the
+    reference do not reach its outputs.)"""
+    sim, obs = np.asarray(sim), np.asarray(obs)
+    n, T = sim.shape
+    t = np.linspace(0, 1, T)
+    dt = t[1] - t[0]
+    W2 = 0
+    out = np.zeros((n, T))
+    for s in range(n):
+        g_p = np.maximum(0, obs[s, :])
+        f_p = np.maximum(0, sim[s, :])
+        c_p = np.max(g_p) * 0.01
+        g_p += c_p
+        f_p += c_p
+        g_p /= np.sum(g_p * dt)
+        f_p /= np.sum(f_p * dt)
+        G_p = np.cumsum(g_p * dt)
+        F_p = np.cumsum(f_p * dt)
+        q = _interp_linear(G_p, t, _interp_linear(t, F_p, t))
+        D = np.cumsum(t - q) * dt + np.cumsum(t - q) * dt
+        out[s] = _interp_linear(t, D, t) - _interp_linear(t, D, np.array([t[-1]]))[0]
+        e = _trace_sum(np.abs(_interp_linear(F_p, t, t) - _interp_linear(G_p, t, t)) ** 2)
+        W2 += e + e
+    return W2, out
+
+
+def wasserstein_yang(sim, obs):
+    """adjoint_functions.wasserstein_yang (:226-266), per trace: shift
+    c = -1.1 min(obs) (no floor), normalized densities g, f, rectangle-rule
+    CDFs, q = G^-1(F(t)) (interp1d sorts a decreasing G stably), r = t - q,
+    w = -2 f(t) / g(q); adjoint (U diag(w) + diag(r)) r with U the upper
+    triangle of ones, i.e. s_k = sum_{j>k} w_j r_j + (w_k + r_k) r_k; misfit
+    sum (t - G^-1(F))^2 f."""
+    sim, obs = np.asarray(sim), np.asarray(obs)
+    n, T = sim.shape
+    t = np.linspace(0, 1, T)
+    dt = t[1] - t[0]
+    W2 = 0
+    out = np.zeros((n, T))
+    U = np.triu(np.ones((T, T)))
+    for s in range(n):
+        c = -1.1 * np.min(obs[s, :])
+        g = obs[s, :] + c
+        f = sim[s, :] + c
+        g /= np.sum(g * dt)
+        f /= np.sum(f * dt)
+        G = np.cumsum(g * dt)
+        F = np.cumsum(f * dt)
+        q = _interp_linear(G, t, _interp_linear(t, F, t))
+        r = t - q
+        w = -2 * _interp_linear(t, f, t) / _interp_linear(t, g, q)
+        # U @ diag(w) has the entries U_jk w_k exactly (one nonzero product each)
+        out[s] = (U * w[None, :] + np.diag(r)) @ r
+        W2 += _trace_sum(np.abs(t - _interp_linear(G, t, F)) ** 2 * f)
     return W2, out
 
 

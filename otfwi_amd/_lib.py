@@ -15,6 +15,11 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libotfwi_pm
 
 OK, E_ARG, E_HIP, E_MEM, E_STATE, E_NUMERIC = 0, -1, -2, -3, -4, -5
 MISFIT_L2, MISFIT_W2, MISFIT_ENV = 0, 1, 2
+MISFIT_W2_QUAD, MISFIT_W2_EXPERIMENTAL, MISFIT_W2_PM, MISFIT_W2_YANG = 3, 4, 5, 6
+# misfit names of the Python layers -> kind codes of the C ABI
+MISFIT_KINDS = {"l2": MISFIT_L2, "w2": MISFIT_W2, "env": MISFIT_ENV,
+                "w2_quad": MISFIT_W2_QUAD, "w2_experimental": MISFIT_W2_EXPERIMENTAL,
+                "w2_pm": MISFIT_W2_PM, "w2_yang": MISFIT_W2_YANG}
 
 # (name, restype, argtypes) of every exported symbol declared in include/otfwi_pml.h
 _vp = ctypes.c_void_p

@@ -234,7 +234,7 @@ int ensure_batch(otfwi_model *m, int n, bool need_adj, bool need_ring = true)
         RCK(fa(&b.jcum, ntr * m->T));
         RCK(dalloc(&b.mis_tr, ntr));
         RCK(dalloc(&b.mis_shot, (size_t)cap));
-        RCK(dalloc(&b.scratch, ntr * 3 * m->T + ntr));
+        RCK(dalloc(&b.scratch, ntr * 6 * m->T + ntr));   // room for every misfit kind
         RCK(dalloc(&b.obs, ntr * m->T));
     }
     RCK(dalloc(&b.shot_src, (size_t)cap));
@@ -1088,7 +1088,13 @@ int check_w2_flags(const otfwi_model *m, const std::vector<int> &fl, int s0)
 
 bool misfit_kind_ok(int k)
 {
-    return k == OTFWI_MISFIT_L2 || k == OTFWI_MISFIT_W2 || k == OTFWI_MISFIT_ENV;
+    return k >= OTFWI_MISFIT_L2 && k <= OTFWI_MISFIT_W2_YANG;
+}
+
+// W2 and the four further Wasserstein-type misfits report per-trace flags.
+bool w2_family(int k)
+{
+    return k == OTFWI_MISFIT_W2 || (k >= OTFWI_MISFIT_W2_QUAD && k <= OTFWI_MISFIT_W2_YANG);
 }
 
 // The envelope misfit's Hilbert kernel for this model's T, on the device.
@@ -1121,8 +1127,8 @@ int grad_batch_device(otfwi_model *m, int s0, int nb, const int *shot_src, int k
     HIPCK(hipEventRecord(m->ev[1], m->stream));
     if (rec_out) RCK(copy_rec_out<R>(m, nb, rec_out + (size_t)s0 * m->n_rx * m->T));
     const int ntr = nb * m->n_rx;
-    int *flags = reinterpret_cast<int *>(b.scratch + (size_t)ntr * 3 * m->T);
-    if (kind == OTFWI_MISFIT_W2) HIPCK(hipMemsetAsync(flags, 0, ntr * sizeof(int), m->stream));
+    int *flags = misfit_flags(kind, b.scratch, ntr, m->T);
+    if (w2_family(kind)) HIPCK(hipMemsetAsync(flags, 0, ntr * sizeof(int), m->stream));
     HIPCK((launch_misfit<R, float>(kind, (const R *)b.rec, b.obs, (R *)b.adj, (R *)b.jcum,
                                    b.mis_tr, ntr, m->T, b.scratch, m->stream, m->hilb)));
     HIPCK(launch_reduce_misfit(b.mis_tr, nb, m->n_rx, b.mis_shot, m->stream));
@@ -1135,8 +1141,8 @@ int grad_batch_device(otfwi_model *m, int s0, int nb, const int *shot_src, int k
     std::vector<double> hmis(nb);
     HIPCK(hipMemcpyAsync(hmis.data(), b.mis_shot, nb * sizeof(double), hipMemcpyDeviceToHost,
                          m->stream));
-    std::vector<int> fl(kind == OTFWI_MISFIT_W2 ? ntr : 0);
-    if (kind == OTFWI_MISFIT_W2)
+    std::vector<int> fl(w2_family(kind) ? ntr : 0);
+    if (w2_family(kind))
         HIPCK(hipMemcpyAsync(fl.data(), flags, ntr * sizeof(int), hipMemcpyDeviceToHost, m->stream));
     HIPCK(hipStreamSynchronize(m->stream));
     float ms[3];
@@ -1184,7 +1190,7 @@ int gradient_impl(otfwi_model *m, int n, const int *shot_src, int kind, double *
     if (kind == OTFWI_MISFIT_ENV) RCK(ensure_hilb(m));
     if (n <= 0) return fail(OTFWI_E_ARG, "n_shots must be positive");
     if (m->n_rx <= 0) return fail(OTFWI_E_STATE, "receivers not set");
-    if (kind == OTFWI_MISFIT_W2 && m->T < 2) return fail(OTFWI_E_ARG, "W2 needs T >= 2");
+    if (w2_family(kind) && m->T < 2) return fail(OTFWI_E_ARG, "W2 needs T >= 2");
     RCK(ready(m, true));
     const int B = batch_size(m, n);
     RCK(ensure_batch(m, B, true));
@@ -1214,7 +1220,7 @@ int objective_impl(otfwi_model *m, int n, const int *shot_src, int kind, double 
     if (kind == OTFWI_MISFIT_ENV) RCK(ensure_hilb(m));
     if (n <= 0) return fail(OTFWI_E_ARG, "n_shots must be positive");
     if (m->n_rx <= 0) return fail(OTFWI_E_STATE, "receivers not set");
-    if (kind == OTFWI_MISFIT_W2 && m->T < 2) return fail(OTFWI_E_ARG, "W2 needs T >= 2");
+    if (w2_family(kind) && m->T < 2) return fail(OTFWI_E_ARG, "W2 needs T >= 2");
     RCK(ready(m, false));
     const int B = batch_size(m, n);
     RCK(ensure_batch(m, B, true, false));
@@ -1225,16 +1231,16 @@ int objective_impl(otfwi_model *m, int n, const int *shot_src, int kind, double 
         RCK(upload_obs_batch<R>(m, s0, nb));
         RCK(run_forward<R>(m, nb, false));
         const int ntr = nb * m->n_rx;
-        int *flags = reinterpret_cast<int *>(b.scratch + (size_t)ntr * 3 * m->T);
-        if (kind == OTFWI_MISFIT_W2) HIPCK(hipMemsetAsync(flags, 0, ntr * sizeof(int), m->stream));
+        int *flags = misfit_flags(kind, b.scratch, ntr, m->T);
+        if (w2_family(kind)) HIPCK(hipMemsetAsync(flags, 0, ntr * sizeof(int), m->stream));
         HIPCK((launch_misfit<R, float>(kind, (const R *)b.rec, b.obs, (R *)b.adj, nullptr,
                                        b.mis_tr, ntr, m->T, b.scratch, m->stream, m->hilb)));
         HIPCK(launch_reduce_misfit(b.mis_tr, nb, m->n_rx, b.mis_shot, m->stream));
         std::vector<double> hmis(nb);
         HIPCK(hipMemcpyAsync(hmis.data(), b.mis_shot, nb * sizeof(double), hipMemcpyDeviceToHost,
                              m->stream));
-        std::vector<int> fl(kind == OTFWI_MISFIT_W2 ? ntr : 0);
-        if (kind == OTFWI_MISFIT_W2)
+        std::vector<int> fl(w2_family(kind) ? ntr : 0);
+        if (w2_family(kind))
             HIPCK(hipMemcpyAsync(fl.data(), flags, ntr * sizeof(int), hipMemcpyDeviceToHost,
                                  m->stream));
         HIPCK(hipStreamSynchronize(m->stream));
@@ -2161,12 +2167,12 @@ namespace {
 int misfit_host(int kind, bool exact, int n_traces, int T, const double *sim, const double *obs,
                 double *mis_out, double *adj_out)
 {
-    if (!misfit_kind_ok(kind) || (exact && kind == OTFWI_MISFIT_ENV))
+    if (!misfit_kind_ok(kind) || (exact && kind != OTFWI_MISFIT_L2 && kind != OTFWI_MISFIT_W2))
         return fail(OTFWI_E_ARG, "unknown misfit kind %d%s", kind,
                     exact ? " (the exact derivative supports L2 and W2)" : "");
     if (n_traces <= 0 || T < 2 || !sim || !obs)
         return fail(OTFWI_E_ARG, "bad misfit arguments (n_traces=%d, T=%d)", n_traces, T);
-    if (kind == OTFWI_MISFIT_W2 && (size_t)2 * T * sizeof(double) > 160 * 1024)
+    if (w2_family(kind) && (size_t)2 * T * sizeof(double) > 160 * 1024)
         return fail(OTFWI_E_ARG, "W2 supports T <= %d", (int)(160 * 1024 / 16));
     const size_t cnt = (size_t)n_traces * T;
     double *dsim = nullptr, *dobs = nullptr, *dadj = nullptr, *dmis = nullptr, *dscr = nullptr;
@@ -2176,15 +2182,16 @@ int misfit_host(int kind, bool exact, int n_traces, int T, const double *sim, co
         dfree(dsim); dfree(dobs); dfree(dadj); dfree(dmis); dfree(dscr); dfree(dtab);
     };
     if ((rc = dalloc(&dsim, cnt)) || (rc = dalloc(&dobs, cnt)) || (rc = dalloc(&dadj, cnt)) ||
-        (rc = dalloc(&dmis, (size_t)n_traces)) || (rc = dalloc(&dscr, cnt * 3 + n_traces)) ||
+        (rc = dalloc(&dmis, (size_t)n_traces)) || (rc = dalloc(&dscr, cnt * misfit_scratch_per_sample(kind) + n_traces)) ||
         (kind == OTFWI_MISFIT_ENV && (rc = dalloc(&dtab, (size_t)T)))) {
         cleanup();
         return rc;
     }
-    int *flags = reinterpret_cast<int *>(dscr + cnt * 3);
+    int *flags = misfit_flags(kind, dscr, n_traces, T);
     hipError_t e = hipMemcpy(dsim, sim, cnt * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(dobs, obs, cnt * sizeof(double), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemset(dscr, 0, (cnt * 3 + n_traces) * sizeof(double));
+    if (e == hipSuccess)
+        e = hipMemset(dscr, 0, (cnt * misfit_scratch_per_sample(kind) + n_traces) * sizeof(double));
     if (e == hipSuccess && dtab) {
         std::vector<double> tab(T);
         hilbert_table(T, tab.data());
@@ -2201,7 +2208,7 @@ int misfit_host(int kind, bool exact, int n_traces, int T, const double *sim, co
         e = hipMemcpy(mis_out, dmis, n_traces * sizeof(double), hipMemcpyDeviceToHost);
     if (e == hipSuccess && adj_out)
         e = hipMemcpy(adj_out, dadj, cnt * sizeof(double), hipMemcpyDeviceToHost);
-    if (e == hipSuccess && kind == OTFWI_MISFIT_W2)
+    if (e == hipSuccess && w2_family(kind))
         e = hipMemcpy(fl.data(), flags, n_traces * sizeof(int), hipMemcpyDeviceToHost);
     cleanup();
     if (e != hipSuccess) return fail(OTFWI_E_HIP, "misfit: %s", hipGetErrorString(e));

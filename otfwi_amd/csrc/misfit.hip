@@ -267,6 +267,217 @@ __device__ double excl_prefix_inplace(double *y, int T)
     return total;
 }
 
+// Inclusive running sum in place: y[k] <- sum_{i<=k} y[i] (lane-owned
+// contiguous chunks).
+__device__ void incl_prefix_inplace(double *y, int T)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int C = (T + WAVE - 1) / WAVE;
+    const int k0 = lane * C, k1 = min(k0 + C, T);
+    double local = 0.0;
+    for (int k = k0; k < k1; k++) local += y[k];
+    double run = wave_excl_scan(local);
+    __syncthreads();
+    for (int k = k0; k < k1; k++) {
+        run += y[k];
+        y[k] = run;
+    }
+    __syncthreads();
+}
+
+// Exclusive suffix sum in place: y[k] <- sum_{j>k} y[j].  Lane l owns the
+// l-th chunk counted from the end, so the scan over lanes runs from the end.
+__device__ void suffix_excl_inplace(double *y, int T)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int C = (T + WAVE - 1) / WAVE;
+    const int r0 = lane * C, r1 = min(r0 + C, T);     // reversed indices r = T-1-k
+    double local = 0.0;
+    for (int r = r0; r < r1; r++) local += y[T - 1 - r];
+    double run = wave_excl_scan(local);
+    __syncthreads();
+    for (int r = r0; r < r1; r++) {
+        const double v = y[T - 1 - r];
+        y[T - 1 - r] = run;
+        run += v;
+    }
+    __syncthreads();
+}
+
+// The four further Wasserstein-type misfits of adjoint_functions.py, one
+// wavefront per trace, all in FP64, t_k = k/(T-1), dt = t_1 - t_0:
+//  W2V_QUAD (wasserstein_quad, :70-104): densities sim^2, obs^2 shifted by
+//    0.05 max(obs^2); CDFs cumsum(x dt) (rectangle rule, first entry x_0 dt);
+//    adjoint D(t) - D(1), D = (2 cumsum(t - G^-1(F(t)))) dt;
+//    misfit sum (F^-1(t) - G^-1(t))^2;
+//  W2V_EXP (wasserstein_experimental, :107-141): the shift of wasserstein,
+//    rectangle-rule CDFs, adjoint Y(t) - Y(1) with
+//    Y = t ((t - q)^2 - 2 t q), q = G^-1(F(t)); misfit
+//    sum (F^-1(t) - G^-1(t))^2 F^-1(t);
+//  W2V_PM (wasserstein_pm, :144-223): densities max(0, x) shifted by
+//    0.01 max(max(0, obs)); the reference builds its negative-part CDFs from
+//    the positive parts, so both halves are the positive part: adjoint
+//    dist(t) - dist(1), dist = cumsum(t - q) dt + cumsum(t - q) dt; misfit
+//    twice sum (F^-1(t) - G^-1(t))^2;
+//  W2V_YANG (wasserstein_yang, :226-266): shift c = -1.1 min(obs) (no
+//    floor), rectangle-rule CDFs, q = G^-1(F(t)) (G stably sorted when it
+//    decreases), r = t - q, w = -2 f(t) / g(q) (f, g the normalized
+//    densities, g interpolated on the t grid with extrapolation); adjoint
+//    s_k = sum_{j>k} w_j r_j + (w_k + r_k) r_k (the reference's
+//    (triu(1) diag(w) + diag(r)) r); misfit sum (t - G^-1(F))^2 f.
+// Inverse CDFs are interp1d's: a stable sort of the abscissae, searchsorted
+// left, clipped brackets, linear extrapolation.  scratch: 6 T doubles per
+// trace; flags as w2_kernel (2: the observed density sums to 0, 1: the
+// observed CDF of QUAD, EXP or PM decreases).
+constexpr int W2V_QUAD = 3, W2V_EXP = 4, W2V_PM = 5, W2V_YANG = 6;
+
+template <int V>
+__device__ __forceinline__ double w2v_density(double x)
+{
+    if constexpr (V == W2V_QUAD) return x * x;
+    else if constexpr (V == W2V_PM) return x > 0.0 ? x : 0.0;   // np.maximum(0, x)
+    else return x;
+}
+
+template <typename R, typename O, int V>
+__global__ void __launch_bounds__(WAVE)
+w2v_kernel(const R *sim, const O *obs, R *adj, double *mis, int T, double *scratch, int *flags)
+{
+    extern __shared__ double lds[];
+    double *Gs = lds;          // G
+    double *Fs = lds + T;      // F
+    const int tr = blockIdx.x;
+    const int lane = threadIdx.x;
+    const R *sv = sim + (size_t)tr * T;
+    const O *ov = obs + (size_t)tr * T;
+    double *gr = scratch + (size_t)tr * 6 * T;   // normalized obs density g
+    double *fr = gr + T;                          // normalized sim density f
+    double *sx = gr + 2 * T, *st = gr + 3 * T;    // stably sorted CDF and its times
+    double *h = gr + 4 * T;                       // transport term, then the adjoint
+    double *dg = gr + 5 * T;                      // YANG: (w_k + r_k) r_k
+    const double step = 1.0 / (double)(T - 1);
+    const double dt = tval(1, T, step) - tval(0, T, step);
+
+    double mn = INFINITY, mx = -INFINITY;
+    for (int k = lane; k < T; k += WAVE) {
+        const double o = w2v_density<V>((double)ov[k]);
+        mn = fmin(mn, o);
+        mx = fmax(mx, o);
+    }
+    mn = wave_min(mn);
+    mx = wave_max(mx);
+    double c;
+    if constexpr (V == W2V_QUAD) c = mx * 0.05;
+    else if constexpr (V == W2V_PM) c = mx * 0.01;
+    else if constexpr (V == W2V_EXP) c = fmax(-1.1 * mn, 0.1 * mx);
+    else c = -1.1 * mn;
+    double sg = 0.0, sf = 0.0;
+    for (int k = lane; k < T; k += WAVE) {
+        const double g = w2v_density<V>((double)ov[k]) + c;
+        const double f = w2v_density<V>((double)sv[k]) + c;
+        gr[k] = g;
+        fr[k] = f;
+        sg += g * dt;
+        sf += f * dt;
+    }
+    sg = wave_sum(sg);
+    sf = wave_sum(sf);
+    if (!(sg != 0.0) && lane == 0) flags[tr] = 2;
+    __syncthreads();
+    for (int k = lane; k < T; k += WAVE) {
+        const double g = gr[k] / sg, f = fr[k] / sf;
+        gr[k] = g;
+        fr[k] = f;
+        Gs[k] = g * dt;
+        Fs[k] = f * dt;
+    }
+    __syncthreads();
+    incl_prefix_inplace(Gs, T);
+    incl_prefix_inplace(Fs, T);
+
+    const bool gmono = is_monotone(Gs, T), fmono = is_monotone(Fs, T);
+    const double *gx = Gs, *gy = nullptr, *fx = Fs, *fy = nullptr;
+    if constexpr (V == W2V_YANG) {
+        if (!gmono) {
+            rank_sort(Gs, T, step, sx, st);
+            gx = sx;
+            gy = st;
+        }
+    } else {
+        if (!gmono && lane == 0 && flags[tr] == 0) flags[tr] = 1;
+        if (V == W2V_EXP && !fmono) {
+            rank_sort(Fs, T, step, sx, st);
+            fx = sx;
+            fy = st;
+        }
+    }
+    auto tgrid = [=](int i) { return tval(i, T, step); };
+    auto Gx = [=](int i) { return gx[i]; };
+    auto Gy = [=](int i) { return gy ? gy[i] : tval(i, T, step); };
+    auto Fx = [=](int i) { return fx[i]; };
+    auto Fy = [=](int i) { return fy ? fy[i] : tval(i, T, step); };
+    auto gv = [=](int i) { return gr[i]; };
+
+    double w = 0.0;
+    for (int k = lane; k < T; k += WAVE) {
+        const double tk = tgrid(k);
+        const double q = interp_lin(Gx, Gy, T, interp_at_grid(Fs, k, T, step));
+        const double r = tk - q;
+        if constexpr (V == W2V_QUAD || V == W2V_PM) {
+            h[k] = r;
+            const double e = interp_lin(Fx, Fy, T, tk) - interp_lin(Gx, Gy, T, tk);
+            w += e * e;
+        } else if constexpr (V == W2V_EXP) {
+            h[k] = tk * (r * r - 2.0 * tk * q);
+            const double F1 = interp_lin(Fx, Fy, T, tk);
+            const double e = F1 - interp_lin(Gx, Gy, T, tk);
+            w += e * e * F1;
+        } else {
+            const double fi = interp_at_grid(fr, k, T, step);
+            const double wk = -2.0 * fi / interp_lin(tgrid, gv, T, q);
+            h[k] = wk * r;
+            dg[k] = (wk + r) * r;
+            const double e = tk - interp_lin(Gx, Gy, T, Fs[k]);
+            w += e * e * fr[k];
+        }
+    }
+    w = wave_sum(w);
+    __syncthreads();
+    R *a = adj + (size_t)tr * T;
+    if constexpr (V == W2V_YANG) {
+        suffix_excl_inplace(h, T);
+        for (int k = lane; k < T; k += WAVE) a[k] = (R)(h[k] + dg[k]);
+    } else {
+        if constexpr (V == W2V_QUAD || V == W2V_PM) {
+            incl_prefix_inplace(h, T);
+            for (int k = lane; k < T; k += WAVE) {
+                if constexpr (V == W2V_QUAD) h[k] = 2.0 * h[k] * dt;
+                else h[k] = h[k] * dt + h[k] * dt;
+            }
+            __syncthreads();
+        }
+        const double xl = tval(T - 2, T, step), xh = tval(T - 1, T, step);
+        const double D1 = (h[T - 1] - h[T - 2]) / (xh - xl) * (1.0 - xl) + h[T - 2];
+        for (int k = lane; k < T; k += WAVE) a[k] = (R)(interp_at_grid(h, k, T, step) - D1);
+    }
+    if (lane == 0) mis[tr] = (V == W2V_PM) ? w + w : w;
+    __syncthreads();
+}
+
+template <typename R, typename O, int V>
+hipError_t launch_w2v(const R *sim, const O *obs, R *adj, double *mis, int ntraces, int T,
+                      double *scratch, hipStream_t s)
+{
+    const size_t lds = (size_t)2 * T * sizeof(double);
+    hipError_t e = hipFuncSetAttribute((const void *)w2v_kernel<R, O, V>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    int *flags = misfit_flags(V, scratch, ntraces, T);
+    hipLaunchKernelGGL((w2v_kernel<R, O, V>), dim3(ntraces), dim3(WAVE), lds, s, sim, obs, adj,
+                       mis, T, scratch, flags);
+    return hipGetLastError();
+}
+
 // Exact derivative of the reference W2 value W = sum_k (F^-1(t_k) - G^-1(t_k))^2
 // with respect to the simulated trace, for a monotone F (sim + c >= 0).
 // F^-1(t_k) = t_lo + (t_hi - t_lo) (t_k - F_lo) / (F_hi - F_lo) on the
@@ -599,12 +810,21 @@ hipError_t launch_misfit(int kind, const R *sim, const O *obs, R *adj, R *jcum, 
     } else if (kind == 2) {
         const hipError_t e = launch_envelope<R, O>(sim, obs, adj, mis, ntraces, T, scratch, hilb, s);
         if (e != hipSuccess) return e;
+    } else if (kind >= W2V_QUAD && kind <= W2V_YANG) {
+        hipError_t e;
+        switch (kind) {
+        case W2V_QUAD: e = launch_w2v<R, O, W2V_QUAD>(sim, obs, adj, mis, ntraces, T, scratch, s); break;
+        case W2V_EXP: e = launch_w2v<R, O, W2V_EXP>(sim, obs, adj, mis, ntraces, T, scratch, s); break;
+        case W2V_PM: e = launch_w2v<R, O, W2V_PM>(sim, obs, adj, mis, ntraces, T, scratch, s); break;
+        default: e = launch_w2v<R, O, W2V_YANG>(sim, obs, adj, mis, ntraces, T, scratch, s); break;
+        }
+        if (e != hipSuccess) return e;
     } else {
         const size_t lds = (size_t)2 * T * sizeof(double);
         hipError_t e = hipFuncSetAttribute((const void *)w2_kernel<R, O>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
-        int *flags = reinterpret_cast<int *>(scratch + (size_t)ntraces * 3 * T);
+        int *flags = misfit_flags(kind, scratch, ntraces, T);
         hipLaunchKernelGGL((w2_kernel<R, O>), dim3(ntraces), dim3(WAVE), lds, s, sim, obs, adj,
                            mis, T, scratch, flags);
     }

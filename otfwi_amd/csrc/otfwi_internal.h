@@ -175,10 +175,20 @@ hipError_t launch_pair2(const Pair2Args<R> &a, int mode, hipStream_t s);
 int step_occupancy(bool adj, int prec, int th, bool persistent);
 
 // Misfit / reduction launchers (misfit.hip).
-// kind: 0 = L2, 1 = W2, 2 = squared envelope.  sim/obs/adj/jcum: [nshots][n_rx][T]; mis:
+// kind: 0 = L2, 1 = W2, 2 = squared envelope, 3-6 = wasserstein_quad,
+// _experimental, _pm, _yang.  sim/obs/adj/jcum: [nshots][n_rx][T]; mis:
 // [nshots*n_rx] FP64 per trace.  jcum may be null (no running sum wanted).  O is the
-// observed-trace type.  scratch: 3 ntraces T doubles (+ ntraces ints of W2 flags).
+// observed-trace type.  scratch: misfit_scratch_per_sample(kind) ntraces T
+// doubles, then ntraces ints of flags (misfit_flags).
 // hilb: hilbert_table(T) on the device (kind 2 only).
+// Per-trace flags of the W2-type kernels inside a misfit scratch area: after
+// 3 T doubles per trace (W2, envelope) or 6 T (the four further
+// Wasserstein-type misfits, kinds 3-6).
+inline int misfit_scratch_per_sample(int kind) { return kind >= 3 ? 6 : 3; }
+inline int *misfit_flags(int kind, double *scratch, int ntraces, int T)
+{
+    return reinterpret_cast<int *>(scratch + (size_t)ntraces * misfit_scratch_per_sample(kind) * T);
+}
 template <typename R, typename O>
 hipError_t launch_misfit(int kind, const R *sim, const O *obs, R *adj, R *jcum, double *mis,
                          int ntraces, int T, double *scratch, hipStream_t s,

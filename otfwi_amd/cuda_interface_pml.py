@@ -6,8 +6,8 @@ batched handle API instead of the per-shot ctypes loop:
 
 * ``simulate`` runs every shot of a multishot setup in one batched forward
   (cuda_interface_pml.py:185-206 loops over ``cuda_simulate``);
-* ``calc_grad`` with ``adjoint_functions.L2`` or ``.wasserstein`` (either the
-  reference's module or :mod:`otfwi_amd.adjoint_functions`, recognized by
+* ``calc_grad`` with any of the seven ``adjoint_functions`` misfits (either
+  the reference's module or :mod:`otfwi_amd.adjoint_functions`, recognized by
   name) computes the misfit and adjoint source on the device and reduces the
   per-shot gradients in FP64 (cuda_interface_pml.py:208-236 reduces in FP32);
   any other ``adj_fun`` is called on the host once per shot with
@@ -35,8 +35,11 @@ def _device_kind(adj_fun):
         return kind
     mod = getattr(adj_fun, "__module__", "") or ""
     name = getattr(adj_fun, "__name__", "")
-    if mod.split(".")[-1] == "adjoint_functions" and name in ("L2", "wasserstein", "quad_envelope"):
-        return {"L2": "l2", "wasserstein": "w2", "quad_envelope": "env"}[name]
+    known = {"L2": "l2", "wasserstein": "w2", "quad_envelope": "env",
+             "wasserstein_quad": "w2_quad", "wasserstein_experimental": "w2_experimental",
+             "wasserstein_pm": "w2_pm", "wasserstein_yang": "w2_yang"}
+    if mod.split(".")[-1] == "adjoint_functions" and name in known:
+        return known[name]
     return None
 
 

@@ -208,7 +208,7 @@ class PMLModel:
     def gradient(self, shot_src=(-1,), misfit="w2", want_rec=False):
         """Device misfit + adjoint gradient.  Returns (grad (Z,X) FP32 mean
         over shots, per-shot misfit FP64 [, traces])."""
-        kind = {"l2": _lib.MISFIT_L2, "w2": _lib.MISFIT_W2, "env": _lib.MISFIT_ENV}[misfit]
+        kind = _lib.MISFIT_KINDS[misfit]
         s, n = self._shots(shot_src)
         mis = np.zeros(n, np.float64)
         grad = np.zeros((self.Z, self.X), np.float32)
@@ -218,7 +218,7 @@ class PMLModel:
 
     def gradient_device(self, shot_src, misfit, grad_dev_ptr):
         """Same as gradient(), writing the (Z,X) FP32 mean gradient to device memory."""
-        kind = {"l2": _lib.MISFIT_L2, "w2": _lib.MISFIT_W2, "env": _lib.MISFIT_ENV}[misfit]
+        kind = _lib.MISFIT_KINDS[misfit]
         s, n = self._shots(shot_src)
         mis = np.zeros(n, np.float64)
         check(self._lib.otfwi_gradient_device(self._h, n, ptr(s), kind, ptr(mis),
@@ -236,7 +236,7 @@ class PMLModel:
         """Exact discrete-adjoint gradient dJ/d(c^2) of J = mean over shots of
         the misfit value, with snapshot checkpointing every snap_every steps
         (0: sqrt(T-1)).  Returns (grad (Z,X) FP32 or FP64, per-shot misfit)."""
-        kind = {"l2": _lib.MISFIT_L2, "w2": _lib.MISFIT_W2, "env": _lib.MISFIT_ENV}[misfit]
+        kind = _lib.MISFIT_KINDS[misfit]
         s, n = self._shots(shot_src)
         mis = np.zeros(n, np.float64)
         if fp64:
@@ -252,7 +252,7 @@ class PMLModel:
     def objective(self, shot_src=(-1,), misfit="w2"):
         """Per-shot misfit (FP64) of the current model: forward runs and the
         device misfit, no adjoint."""
-        kind = {"l2": _lib.MISFIT_L2, "w2": _lib.MISFIT_W2, "env": _lib.MISFIT_ENV}[misfit]
+        kind = _lib.MISFIT_KINDS[misfit]
         s, n = self._shots(shot_src)
         mis = np.zeros(n, np.float64)
         check(self._lib.otfwi_objective(self._h, n, ptr(s), kind, ptr(mis)))
@@ -296,7 +296,7 @@ def misfit_exact(kind, sim, obs):
 
 def misfit(kind, sim, obs, _exact=False):
     """Per-trace misfit and adjoint source of (n, T) traces on the GPU (FP64)."""
-    k = {"l2": _lib.MISFIT_L2, "w2": _lib.MISFIT_W2, "env": _lib.MISFIT_ENV}[kind]
+    k = _lib.MISFIT_KINDS[kind]
     s = np.ascontiguousarray(sim, dtype=np.float64)
     o = np.ascontiguousarray(obs, dtype=np.float64)
     if s.ndim == 1:
@@ -318,7 +318,7 @@ def gradient_multi(models, shots, misfit="w2", workers=4):
     list.  Returns [(grad (Z,X) FP32, per-shot misfit FP64)] in model order."""
     n = len(models)
     kinds = [misfit] * n if isinstance(misfit, str) else list(misfit)
-    code = {"l2": _lib.MISFIT_L2, "w2": _lib.MISFIT_W2, "env": _lib.MISFIT_ENV}
+    code = _lib.MISFIT_KINDS
     ss = [i32(np.atleast_1d(s)) for s in shots]
     grads = [np.zeros((m.Z, m.X), np.float32) for m in models]
     mis = [np.zeros(len(s), np.float64) for s in ss]
