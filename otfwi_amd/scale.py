@@ -9,6 +9,9 @@ c3     16 independent models with grid sizes drawn from {512, 768, 1024,
        1 and with --workers host threads.
 exact  one checkpointed exact-adjoint gradient (L2) of the 1024^2 FMC block,
        beside the reference-mode gradient of the same shots.
+c4     the 2048^2 differentiation configuration (16 shots, T = --T): FP32 and
+       FP64 checkpointed exact gradients, FP64 central differences along 8
+       Gaussian bumps, the reference-mode gradient (run_c4).
 cpml   the second-order C-PML scheme (regression_CPML.cu): --shots runs of a
        --N^2 grid, T = --T steps, each shot firing its own source, advanced in
        one batch; 56 algorithmic bytes per cell-step (both frames in one
@@ -102,6 +105,101 @@ def run_exact(N=1024, T=4000, shots=8, snap=0):
             "interior_cosine_exact_vs_reference": cos}
 
 
+def run_c4(T=8000, shots=16, n_bumps=8, misfit="l2", fp64_batch=8, seed=0):
+    """The differentiation configuration: 2048^2 at dx = 0.025 mm, 16 shots,
+    T = 8000.  Observed traces from the FP64 build on the true model; at the
+    background + half the model error: the FP32 exact gradient (snapshots
+    every sqrt(T-1) steps, all shots resident), the FP64 exact gradient
+    (fp64_batch shots resident at a time, to fit HBM) as its oracle, the
+    reference-mode FP32 gradient, and central differences of the FP64
+    objective (step 1e-3 m/s of velocity) along n_bumps Gaussian bumps of
+    sigma 1 mm (40 cells) placed at seeded positions below the array."""
+    from .model import PMLModel
+
+    cfg = configs.config4(model_seed=seed, T=T, n_elem=shots)
+    N, pml = cfg["X"], cfg["pml"]
+    cp = Cuda_interface_PML(N, N, T, pml, cfg["c_true"], shots, cfg["src_mask"], shots,
+                            cfg["rx_mask"], cfg["src"], multishot=True, R=cfg["R"])
+    from . import adjoint_functions as adj
+    cp.init_regression(np.zeros((shots, shots, T), np.float32), adj.L2, prec_deriv=4)
+    d_x, d_z = cp.d_x, cp.d_z
+    sx, sz = cp.pos_source_x, cp.pos_source_z
+    rx, rz = cp.pos_sensor_x, cp.pos_sensor_z
+    vx, vz = cp._pos_revert_x, cp._pos_revert_z
+    cp.model.close()
+
+    def model(prec, max_batch=0):
+        m = PMLModel(N, N, T, precision=prec)
+        m.set_damping(d_x, d_z)
+        m.set_sources(sx, sz, cfg["src"].astype(np.float32))
+        m.set_receivers(rx, rz)
+        m.set_ring(vx, vz)
+        if max_batch:
+            m.set_tuning(0, 0, max_batch)
+        return m
+
+    s = np.arange(shots)
+    cq_true = cfg["c_true"] ** 2
+    cq = 0.5 * (cfg["c_bg"] ** 2 + cq_true)
+    out = {"workload": f"C4: {shots}-shot {N}^2 T={T} {misfit} gradient, dx 0.025 mm"}
+    m64 = model(8, fp64_batch)
+    m64.set_cquad64(cq_true)
+    t0 = time.perf_counter()
+    obs = m64.forward(s)
+    log(f"[c4] FP64 observed traces in {time.perf_counter() - t0:.1f} s")
+    m64.set_observed(obs)
+    m64.set_cquad64(cq)
+    k = int(round(np.sqrt(T - 1)))
+    t0 = time.perf_counter()
+    g64, mis64 = m64.gradient_exact(s, misfit, snap_every=k, fp64=True)
+    out["fp64_exact_seconds"] = time.perf_counter() - t0
+    log(f"[c4] FP64 exact gradient in {out['fp64_exact_seconds']:.1f} s")
+    rng = np.random.default_rng(seed)
+    h = 2 * 0.5 * float(configs.courant(1e-3))
+    zz, xx = np.mgrid[0:N, 0:N]
+    fd = []
+    t0 = time.perf_counter()
+    for b in range(n_bumps):
+        cz, cx = rng.uniform(0.2 * N, N - pml - 60), rng.uniform(pml + 60, N - pml - 60)
+        d = np.exp(-((zz - cz) ** 2 + (xx - cx) ** 2) / (2 * 40.0 ** 2))
+        m64.set_cquad64(cq + h * d)
+        jp = m64.objective(s, misfit).mean()
+        m64.set_cquad64(cq - h * d)
+        jm = m64.objective(s, misfit).mean()
+        f = (jp - jm) / (2 * h)
+        a = float((g64 * d).sum())
+        fd.append({"bump": [round(cz, 1), round(cx, 1)], "fd": f, "adjoint": a,
+                   "rel": abs(f - a) / abs(f)})
+        log(f"[c4] bump {b}: fd {f:.9e} adjoint {a:.9e} rel {abs(f - a) / abs(f):.2e}")
+    out["fd_seconds"] = time.perf_counter() - t0
+    out["fd"] = fd
+    out["fd_worst_rel"] = max(r["rel"] for r in fd)
+    m64.close()
+    del m64
+    m32 = model(4)
+    m32.set_observed(obs.astype(np.float32))
+    m32.set_cquad((cq).astype(np.float32))
+    t0 = time.perf_counter()
+    g32, mis32 = m32.gradient_exact(s, misfit, snap_every=k)
+    out["fp32_exact_seconds"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g_ref, _ = m32.gradient(s, misfit)
+    out["fp32_reference_mode_seconds"] = time.perf_counter() - t0
+    m32.close()
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    inner = np.zeros((N, N), bool)     # away from the ring and the PML edges, as run_exact
+    inner[pml + 16:N - pml - 16, pml + 16:N - pml - 16] = True
+    a, b = g32[inner].astype(np.float64), g_ref[inner].astype(np.float64)
+    out.update(snap_every=k, fp32_exact_vs_fp64=rel(g32.astype(np.float64), g64),
+               fp32_misfit_vs_fp64=abs(mis32.mean() - mis64.mean()) / abs(mis64.mean()),
+               cosine_exact_vs_reference_mode=float(a @ b / np.linalg.norm(a) / np.linalg.norm(b)),
+               checkpoint_bytes_fp32=(int(np.ceil((T - 1) / k)) * 4 + k * 2)
+               * (N + 128) * (N + 16) * 4 * shots,
+               full_history_bytes_fp32=(T - 1) * 2 * (N + 128) * (N + 16) * 4 * shots)
+    log(f"[c4] FP32 exact {out['fp32_exact_seconds']:.1f} s vs FP64 {out['fp32_exact_vs_fp64']:.2e}")
+    return out
+
+
 # Bytes per cell-step of the second-order C-PML step with both frames in one
 # pass: P(t-1), P(t-2) read, P(t) written, kappa, rho_hx, rho_hz read, the four
 # memory variables read and written (the x / z derivative fields stay on chip).
@@ -129,7 +227,7 @@ def run_cpml(N=1024, T=1000, shots=64):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["c3", "exact", "cpml"])
+    ap.add_argument("what", choices=["c3", "exact", "cpml", "c4"])
     ap.add_argument("--t-scale", type=float, default=1.0)
     ap.add_argument("--shots", type=int, default=32)
     ap.add_argument("--workers", type=int, default=4)
@@ -145,6 +243,8 @@ def main(argv=None):
                      misfit=a.misfit)
     elif a.what == "cpml":
         out = run_cpml(N=a.N, T=a.T, shots=a.shots)
+    elif a.what == "c4":
+        out = run_c4(T=a.T, shots=a.shots, misfit=a.misfit)
     else:
         out = run_exact(T=a.T, shots=min(a.shots, 64))
     print(json.dumps(out))
