@@ -179,6 +179,7 @@ def unet_timing(N=8, H=1024, reps=5):
     train_s = e0.elapsed_time(e1) / 1e3 / reps
     fwd_s = e1.elapsed_time(e2) / 1e3 / reps
     ftrain, ffwd = U.train_flops(N, H, H), sum(U.conv_flops(N, H, H).values())
+    roof = U.train_roofline(N, H, H, hbm=HBM_PEAK, mfma=MFMA_BF16_PEAK)
     print(f"[bench] U-Net {N}x{H}^2: train step {train_s * 1e3:.2f} ms, forward {fwd_s * 1e3:.2f} ms",
           file=sys.stderr, flush=True)
     return {"bound": "mfma", "kernel": "conv3x3 + wgrad (BF16, FP32 accumulate)",
@@ -186,7 +187,11 @@ def unet_timing(N=8, H=1024, reps=5):
             "achieved": ftrain / train_s / 1e12, "peak": MFMA_BF16_PEAK / 1e12,
             "unit": "TFLOP/s", "frac": ftrain / train_s / MFMA_BF16_PEAK,
             "train_step_ms": train_s * 1e3, "train_flops": ftrain,
-            "forward_ms": fwd_s * 1e3, "forward_tflops": ffwd / fwd_s / 1e12}
+            "forward_ms": fwd_s * 1e3, "forward_tflops": ffwd / fwd_s / 1e12,
+            # per-kernel roofline: each convolution bound by max(HBM bytes, FLOPs), the
+            # elementwise kernels by their bytes (unet.train_roofline)
+            "kernel_roofline_ms": roof["ms"], "frac_of_kernel_roofline": roof["ms"] / 1e3 / train_s,
+            "hbm_bound_convs": roof["hbm_bound_convs"], "mfma_bound_convs": roof["mfma_bound_convs"]}
 
 
 def inversion_compare(n_models, n_iter):

@@ -95,6 +95,64 @@ def train_flops(N, H, W):
     return sum(f.values()) * 3 - f["enc0a"]
 
 
+def train_roofline(N, H, W, hbm=8.0e12, mfma=2.5e15):
+    """Per-kernel roofline of one training step (forward, backward, Adam) of
+    N images of H x W: each 3x3 convolution kernel (forward, data gradient,
+    weight gradient) is bound by max(HBM bytes / hbm, FLOPs / mfma) with its
+    operands read and its result written once in BF16; the elementwise
+    kernels (input packing, pooling, upsample and skip gradients, head,
+    Adam) by their bytes.  Returns {"ms", "conv_ms", "other_ms", "bytes",
+    "flops", "hbm_bound_convs", "mfma_bound_convs"}."""
+    lvl = {"enc0": 0, "enc1": 1, "enc2": 2, "bot": 3, "dec2": 2, "dec1": 1, "dec0": 0}
+    t_conv = t_other = 0.0
+    tot_b = tot_f = 0.0
+    nh = nm = 0
+
+    def kern(b, f):
+        nonlocal t_conv, tot_b, tot_f, nh, nm
+        t_conv += max(b / hbm, f / mfma)
+        tot_b += b
+        tot_f += f
+        if b / hbm >= f / mfma:
+            nh += 1
+        else:
+            nm += 1
+
+    def elem(b):
+        nonlocal t_other, tot_b
+        t_other += b / hbm
+        tot_b += b
+
+    for name, ca, cb, co in CONVS:
+        l = lvl[name[:-1]]
+        px = N * (H >> l) * (W >> l)
+        cin = ca + cb
+        x_bytes = 2.0 * px * ca + 2.0 * (px / 4) * cb    # [skip | upsampled coarse]
+        f = 2.0 * px * co * 9 * cin
+        kern(x_bytes + 2.0 * px * co, f)                 # forward
+        kern(2.0 * px * co + x_bytes, f)                 # weight gradient
+        if name != "enc0a":                              # data gradient (+ ReLU mask)
+            masked = name.endswith("b") or name == "botb"
+            kern(2.0 * px * co + 2.0 * px * cin * (2 if masked else 1), f)
+        if name.endswith("a") and l > 0 and not name.startswith("dec"):
+            elem(2.0 * (4 * px) * ca + 2.0 * px * ca)     # 2x2 pooling of the finer level
+        if cb:
+            # upsample gradient: fine gradient and coarse mask read, coarse written
+            elem(2.0 * px * cb + 2.0 * 2 * (px / 4) * cb)
+            # skip combine of the encoder output: skip gradient, mask and result at
+            # this level, the pooled path's gradient at the next coarser one
+            elem(2.0 * px * ca * 3 + 2.0 * (px / 4) * ca)
+    px0 = N * H * W
+    elem(4.0 * 2 * px0 + 2.0 * PAD_CH * px0)             # input packing
+    elem(2.0 * 32 * px0 + 4.0 * px0)                     # head forward
+    elem(2.0 * 32 * px0 + 8.0 * px0 + 2.0 * 32 * px0)    # head backward
+    n_par = param_layout()[1]
+    elem(n_par * (4.0 * 7 + 2.0))                        # Adam: p, g, m, v read/written, BF16 copy
+    return {"ms": 1e3 * (t_conv + t_other), "conv_ms": 1e3 * t_conv,
+            "other_ms": 1e3 * t_other, "bytes": tot_b, "flops": tot_f,
+            "hbm_bound_convs": nh, "mfma_bound_convs": nm}
+
+
 def _vp(t):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
 

@@ -64,3 +64,21 @@ def test_oracle_backward_finite_difference():
     fd = (Lp - Lm) / (2 * h)
     an = float(np.dot(g.astype(np.float64), d))
     assert abs(fd - an) < 2e-3 * max(abs(an), 1e-12) + 1e-7, (fd, an)
+
+
+def test_train_roofline_counts():
+    """The per-kernel roofline of a training step counts the same FLOPs as
+    train_flops, and classifies the 32-channel full-resolution convolutions
+    as HBM-bound and the 256-channel bottleneck as MFMA-bound."""
+    from otfwi_amd import unet as U
+
+    r = U.train_roofline(8, 1024, 1024)
+    assert r["flops"] == U.train_flops(8, 1024, 1024)
+    assert r["hbm_bound_convs"] + r["mfma_bound_convs"] == 3 * len(U.CONVS) - 1
+    px = 8 * 1024 * 1024
+    b = 2.0 * px * 32 * 2                          # enc0b forward: 32 channels in and out
+    f = 2.0 * px * 32 * 9 * 32
+    assert b / 8e12 > f / 2.5e15                   # HBM-bound
+    px3 = px / 64
+    assert 2.0 * px3 * 512 / 8e12 < 2.0 * px3 * 256 * 9 * 256 / 2.5e15   # botb: MFMA-bound
+    assert 0 < r["conv_ms"] < r["ms"]
