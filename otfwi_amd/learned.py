@@ -29,6 +29,14 @@ from .cuda_interface_pml import Cuda_interface_PML
 from . import adjoint_functions as adj
 
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "unet_fmc256.npz")
+# the network fine-tuned at the 64-shot 1024^2 full-matrix-capture geometry
+WEIGHTS_1024 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "unet_fmc1024.npz")
+# Problem keywords of the two geometries: the 256^2 block (32 elements, T = 1200, 1-3 defects
+# of semi-axes 6-30 cells) and the 64-shot 1024^2 configuration itself (T = 4000, 1-5 defects
+# of semi-axes 6-60 cells)
+GEOMETRIES = {"fmc256": {},
+              "fmc1024": dict(N=1024, T=4000, n_elem=64, pitch=12, n_max=5, semi_min=6.0,
+                              semi_max=60.0)}
 C_SCALE = 0.25        # target = model error / C_SCALE
 ILL_DECADES = 6.0
 G_PCT = 99.0          # gradient scaled by this percentile of |g| on the mask ...
@@ -336,7 +344,8 @@ def summarize(res, tol=0.02):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description="train / evaluate the learned preconditioner")
-    ap.add_argument("cmd", choices=["train", "refine", "compare"])
+    ap.add_argument("cmd", choices=["train", "refine", "finetune", "compare"])
+    ap.add_argument("--geometry", default="fmc256", choices=sorted(GEOMETRIES))
     ap.add_argument("--pairs", type=int, default=2000)
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--stage2", type=int, default=1000)
@@ -352,7 +361,7 @@ def main(argv=None):
     ap.add_argument("--save", default=None, help="refine: write the refined weights here")
     a = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)  # noqa: E731
-    prob = Problem()
+    prob = Problem(**GEOMETRIES[a.geometry])
     summary = {}
     if a.cmd == "train":
         net, info = train_recipe(prob, a.pairs, a.epochs, a.stage2, a.epochs2, a.lr, log=log)
@@ -360,6 +369,21 @@ def main(argv=None):
         summary["train"] = info
         with open(os.path.splitext(a.weights)[0] + ".json", "w") as f:
             json.dump({k: v for k, v in info.items() if not k.startswith("loss")}, f, indent=1)
+    elif a.cmd == "finetune":
+        # pairs at this geometry (seeds 0..pairs-1), the network in --weights as the start,
+        # written to --save (default: the 1024^2 weights)
+        from .unet import UNet
+        X, Y, t_pairs = make_pairs(prob, range(a.pairs), log=log, log_every=10)
+        net = UNet(params=load_weights(a.weights))
+        h, t_train = train(net, X, Y, a.epochs, lr=a.lr, log=log)
+        out_w = a.save or WEIGHTS_1024
+        save_weights(net, out_w)
+        info = dict(pairs=a.pairs, epochs=a.epochs, lr=a.lr, t_pairs=t_pairs, t_train=t_train,
+                    init=os.path.basename(a.weights), loss_first_epoch=h[0], loss_last_epoch=h[-1])
+        summary["finetune"] = info
+        with open(os.path.splitext(out_w)[0] + ".json", "w") as f:
+            json.dump(info, f, indent=1)
+        a.weights = out_w
     elif a.cmd == "refine":
         from .unet import UNet
         net = UNet(params=load_weights(a.weights))
@@ -370,7 +394,7 @@ def main(argv=None):
         summary["refine"] = info
         with open(os.path.splitext(out_w)[0] + ".refine.json", "w") as f:
             json.dump({k: v for k, v in info.items() if not k.startswith("loss")}, f, indent=1)
-    else:
+    if a.cmd in ("compare", "finetune"):
         from .unet import UNet
         net = UNet(params=load_weights(a.weights))
     res = compare(prob, net, [100000 + k for k in range(a.models)], n_iter=a.iters, log=log)
