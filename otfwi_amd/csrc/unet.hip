@@ -26,6 +26,7 @@ typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 namespace {
 
@@ -441,9 +442,15 @@ struct WgradArgs {
 };
 
 // The transposed tiles' channel rows are padded by 8 (gradient) or 16
-// (patch) elements per group of 8 rows: the parts of one pixel that
-// neighbouring lanes write (channels 8 apart) then fall in different LDS
-// banks, and every 8-element fragment read stays 16-byte aligned.
+// (patch) elements per group of 8 rows, and every 8-element fragment read
+// stays 16-byte aligned.  Staging the tiles in units of 4 pixels x 8
+// channels transposed in registers (ds_write_b64 per channel row) instead of
+// per 16-byte piece (eight ds_write_b16) also frees registers: the 64-channel
+// kernel fits 2 waves per SIMD (158 VGPRs instead of 198) and ran 15-27 %
+// faster on MI355X at batch 8 (Cout 64-256); with one 32-channel block the
+// patch keeps the per-piece staging (evenly spread over the 192 threads,
+// where 4-pixel units leave a third round to a quarter of them: 12 % slower),
+// the gradient tile takes the units (4-10 % faster than before).
 template <int TX, int MJ>
 struct WgShape {
     static constexpr int TY = 256 / TX;
@@ -456,6 +463,9 @@ struct WgShape {
     static __device__ int xofs(int c) { return c * PH * PW + (c >> 3) * 16; }
 };
 
+#ifndef WG_XUNITS_MJ
+#define WG_XUNITS_MJ 2
+#endif
 template <int TX, int MJ>
 __global__ __launch_bounds__(192) void wgrad_kernel(WgradArgs a)
 {
@@ -486,56 +496,100 @@ __global__ __launch_bounds__(192) void wgrad_kernel(WgradArgs a)
     float bsum = 0.0f;
     const int t_begin = blockIdx.x * a.tiles_per_wg;
     const int t_end = min(t_begin + a.tiles_per_wg, tiles_all);
-    // the next tile's gradient and input patch are loaded into registers
-    // while the current tile's MFMAs run
-    constexpr int NG = 256 * (CO / 8), NX = PH * (TX + 2) * 4;
-    constexpr int KG = (NG + 191) / 192, KX = (NX + 191) / 192;
-    u32x4 rg[KG], rx[KX];
+    // The next tile's gradient and input patch are loaded into registers
+    // while the current tile's MFMAs run.  Staging unit: 4 consecutive pixels
+    // x 8 channels, loaded as four 16-byte pixel pieces and transposed in
+    // registers (v_perm) into eight 8-byte channel rows, one ds_write_b64 each
+    // (instead of eight 2-byte writes per piece).
+    constexpr int NBX = PW / 4;                          // 4-pixel blocks of a patch row
+    constexpr int NUG = 64 * (CO / 8), NUX = PH * NBX * 4;
+    constexpr int UG = (NUG + 191) / 192, UX = (NUX + 191) / 192;
+    // with one 32-channel output block the patch is staged per 16-byte piece
+    // (TX + 2 pixels a row, spread evenly over the 192 threads)
+    constexpr bool XU = MJ >= WG_XUNITS_MJ;
+    constexpr int NX = PH * (TX + 2) * 4, KX = (NX + 191) / 192;
+    u32x4 rg[UG][4], rx[XU ? UX : 1][4], rxp[XU ? 1 : KX];
     auto fetch = [&](int t) {
         const int n = t / tiles_img, tt = t - n * tiles_img;
         const int tx0 = (tt % tiles_x) * TX, ty0 = (tt / tiles_x) * TY;
 #pragma unroll
-        for (int k = 0; k < KG; k++) {
-            const int v = tid + 192 * k;
-            if (v < NG) {
-                const int q = v / (CO / 8), part = v - q * (CO / 8);
-                const int y = ty0 + q / TX, x = tx0 + q % TX;
-                rg[k] = *(const u32x4 *)(a.G + (((size_t)n * H + y) * W + x) * a.Cout + co0 +
-                                         part * 8);
-            }
-        }
+        for (int k = 0; k < UG; k++) {
+            const int u = tid + 192 * k;
+            if (u < NUG) {
+                const int part = u % (CO / 8), q0 = 4 * (u / (CO / 8));
+                const int y = ty0 + q0 / TX, x = tx0 + q0 % TX;
+                const bf16 *g = a.G + (((size_t)n * H + y) * W + x) * a.Cout + co0 + part * 8;
 #pragma unroll
-        for (int k = 0; k < KX; k++) {
-            const int v = tid + 192 * k;
-            if (v < NX) {
-                const int pp = v >> 2, part = v & 3;
-                const int py = pp / (TX + 2), px = pp - py * (TX + 2);
-                rx[k] = load8(a.in, n, ty0 + py - 1, tx0 + px - 1, ci0 + part * 8);
+                for (int p = 0; p < 4; p++) rg[k][p] = *(const u32x4 *)(g + (size_t)p * a.Cout);
             }
         }
+        if constexpr (XU) {
+#pragma unroll
+            for (int k = 0; k < UX; k++) {
+                const int u = tid + 192 * k;
+                if (u < NUX) {
+                    const int part = u & 3, pb = u >> 2;
+                    const int py = pb / NBX, px0 = (pb - py * NBX) * 4;
+#pragma unroll
+                    for (int p = 0; p < 4; p++)
+                        rx[k][p] = load8(a.in, n, ty0 + py - 1, tx0 + px0 + p - 1, ci0 + part * 8);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < KX; k++) {
+                const int v = tid + 192 * k;
+                if (v < NX) {
+                    const int pp = v >> 2, part = v & 3;
+                    const int py = pp / (TX + 2), px = pp - py * (TX + 2);
+                    rxp[k] = load8(a.in, n, ty0 + py - 1, tx0 + px - 1, ci0 + part * 8);
+                }
+            }
+        }
+    };
+    // channel c of four pixel pieces v[0..3] (8 channels each) as 4 bf16
+    auto column = [](const u32x4 (&v)[4], int c) {
+        const unsigned sel = (c & 1) ? 0x07060302u : 0x05040100u;
+        const int d = c >> 1;
+        return u32x2{__builtin_amdgcn_perm(v[1][d], v[0][d], sel),
+                     __builtin_amdgcn_perm(v[3][d], v[2][d], sel)};
     };
     if (t_begin < t_end) fetch(t_begin);
     for (int t = t_begin; t < t_end; t++) {
         // gradient tile -> gT[co][q], input patch -> xT[ci][py][px]
 #pragma unroll
-        for (int k = 0; k < KG; k++) {
-            const int v = tid + 192 * k;
-            if (v < NG) {
-                const int q = v / (CO / 8), part = v - q * (CO / 8);
-                const bf16 *gv = (const bf16 *)&rg[k];
+        for (int k = 0; k < UG; k++) {
+            const int u = tid + 192 * k;
+            if (u < NUG) {
+                const int part = u % (CO / 8), q0 = 4 * (u / (CO / 8));
 #pragma unroll
-                for (int e = 0; e < 8; e++) gT[S::gofs(part * 8 + e) + q] = gv[e];
+                for (int c = 0; c < 8; c++)
+                    *(u32x2 *)(gT + S::gofs(part * 8 + c) + q0) = column(rg[k], c);
             }
         }
+        if constexpr (XU) {
 #pragma unroll
-        for (int k = 0; k < KX; k++) {
-            const int v = tid + 192 * k;
-            if (v < NX) {
-                const int pp = v >> 2, part = v & 3;
-                const int py = pp / (TX + 2), px = pp - py * (TX + 2);
-                const bf16 *xs = (const bf16 *)&rx[k];
+            for (int k = 0; k < UX; k++) {
+                const int u = tid + 192 * k;
+                if (u < NUX) {
+                    const int part = u & 3, pb = u >> 2;
+                    const int py = pb / NBX, px0 = (pb - py * NBX) * 4;
 #pragma unroll
-                for (int e = 0; e < 8; e++) xT[S::xofs(part * 8 + e) + py * PW + px] = xs[e];
+                    for (int c = 0; c < 8; c++)
+                        *(u32x2 *)(xT + S::xofs(part * 8 + c) + py * PW + px0) = column(rx[k], c);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < KX; k++) {
+                const int v = tid + 192 * k;
+                if (v < NX) {
+                    const int pp = v >> 2, part = v & 3;
+                    const int py = pp / (TX + 2), px = pp - py * (TX + 2);
+                    const bf16 *xs = (const bf16 *)&rxp[k];
+#pragma unroll
+                    for (int e = 0; e < 8; e++) xT[S::xofs(part * 8 + e) + py * PW + px] = xs[e];
+                }
             }
         }
         __syncthreads();
