@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <string>
@@ -424,8 +425,9 @@ int launch_conv_tiles(const ConvArgs &a, int N, hipStream_t st)
 // 0: by shape (the tile loop where Cin fits one chunk, the output block is 32
 // channels and the launch still has >= 1024 work-groups), 1: one tile per
 // work-group, CONV_NT: the tile loop wherever Cin fits one chunk, the block is
-// 32 channels and the tiles divide evenly.
-int g_conv_tiles = 0;
+// 32 channels and the tiles divide evenly.  Read once per launch; setting it
+// while another thread launches convolutions picks either value.
+std::atomic<int> g_conv_tiles{0};
 
 // ---------------------------------------------------------------------------
 // weight gradient: dW[co][tap][ci] += sum_p G[p][co] * X[p + tap][ci]
@@ -921,9 +923,9 @@ int otfwi_unet_conv3x3(const otfwi_bf16 *A, int CA, const otfwi_bf16 *B, int CB,
     const long long ntiles = (long long)(W / TX) * (H / (256 / TX));
     // the tile loop for 32-channel output blocks (75 KB of LDS, two work-groups
     // per CU as conv3x3_kernel; 64-channel blocks would need 112 KB)
-    const bool tiles = Cin == CK && TN == 32 && ntiles % CONV_NT == 0 && g_conv_tiles != 1 &&
-                       (g_conv_tiles == CONV_NT ||
-                        ntiles / CONV_NT * (Cout / TN) * (long long)N >= 1024);
+    const int nt = g_conv_tiles.load(std::memory_order_relaxed);
+    const bool tiles = Cin == CK && TN == 32 && ntiles % CONV_NT == 0 && nt != 1 &&
+                       (nt == CONV_NT || ntiles / CONV_NT * (Cout / TN) * (long long)N >= 1024);
     if (tiles) {
         if (CK == 16)
             return TX == 64 ? launch_conv_tiles<64, 32, 16>(a, N, st)
@@ -944,7 +946,7 @@ int otfwi_unet_conv3x3(const otfwi_bf16 *A, int CA, const otfwi_bf16 *B, int CB,
 int otfwi_unet_set_conv_tiles(int nt)
 {
     if (nt != 0 && nt != 1 && nt != CONV_NT) return ufail("conv tiles per work-group: 0, 1 or 8");
-    g_conv_tiles = nt;
+    g_conv_tiles.store(nt, std::memory_order_relaxed);
     return 0;
 }
 

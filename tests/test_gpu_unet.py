@@ -39,6 +39,44 @@ def st():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+def test_conv_tile_loop_chosen_by_size(gpu):
+    """At 2 x 1024^2, 32 -> 32 channels the automatic setting runs the tile
+    loop (4096 tiles / 8 per work-group x 2 images = 1024 work-groups, the
+    threshold): its output is bitwise that of one tile per work-group.  (An
+    input joined from two tensors never reaches the tile loop: the joined
+    form needs CA a multiple of 32 and CB > 0, so Cin exceeds one chunk.)"""
+    N, H, W, C = 2, 1024, 1024, 32
+    rng = np.random.default_rng(5)
+    lib = _lib.load_unet()
+    A = bf(rng.standard_normal((N, H, W, C))).cuda()
+    w = bf(rng.standard_normal((C, 9, C)) / np.sqrt(9 * C)).cuda()
+    b = torch.tensor(rng.standard_normal(C).astype(np.float32) * 0.1).cuda()
+    out = {}
+    try:
+        for nt in (0, 1):
+            _lib.ucheck(lib.otfwi_unet_set_conv_tiles(nt))
+            Y = torch.empty((N, H, W, C), dtype=torch.bfloat16, device="cuda")
+            _lib.ucheck(lib.otfwi_unet_conv3x3(vp(A), C, None, 0, 0, vp(w), vp(b), None, vp(Y),
+                                               N, H, W, C, 1, st()))
+            torch.cuda.synchronize()
+            out[nt] = Y.view(torch.int16)
+    finally:
+        _lib.ucheck(lib.otfwi_unet_set_conv_tiles(0))
+    assert torch.equal(out[0], out[1])
+    # and the result is the convolution: one interior row checked on the CPU
+    y = 517
+    Ar = A[0, y - 1:y + 2].float().cpu().numpy()
+    wr = w.float().cpu().numpy().reshape(C, 3, 3, C)
+    ref = np.zeros((W, C))
+    Ap = np.pad(Ar, ((0, 0), (1, 1), (0, 0)))
+    for dy in range(3):
+        for dx in range(3):
+            ref += Ap[dy, dx:dx + W] @ wr[:, dy, dx, :].T
+    ref = np.maximum(ref + b.cpu().numpy()[None], 0)
+    got = out[0][0, y].view(torch.bfloat16).float().cpu().numpy()
+    assert rel(got, ref) < 5e-3
+
+
 @pytest.mark.parametrize("N,H,W,CA,CB,Cout", [
     (2, 64, 64, 32, 64, 32),      # decoder concat, 64-wide tiles, TN 32
     (1, 32, 64, 64, 0, 64),       # plain, 64-wide tiles, TN 64
@@ -103,6 +141,44 @@ def test_conv_tile_loop_bitwise(gpu, N, H, W, CA, Cout, relu, masked):
     e = rel(y8, ref)
     print(f"conv tiles N{N} {H}x{W} {CA}->{Cout}: rel {e:.2e}")
     assert e < 5e-3
+
+
+def test_conv_tile_loop_chosen_by_size(gpu):
+    """At 2 x 1024^2, 32 -> 32 channels the automatic setting runs the tile
+    loop (4096 tiles / 8 per work-group x 2 images = 1024 work-groups, the
+    threshold): its output is bitwise that of one tile per work-group.  (An
+    input joined from two tensors never reaches the tile loop: the joined
+    form needs CA a multiple of 32 and CB > 0, so Cin exceeds one chunk.)"""
+    N, H, W, C = 2, 1024, 1024, 32
+    rng = np.random.default_rng(5)
+    lib = _lib.load_unet()
+    A = bf(rng.standard_normal((N, H, W, C))).cuda()
+    w = bf(rng.standard_normal((C, 9, C)) / np.sqrt(9 * C)).cuda()
+    b = torch.tensor(rng.standard_normal(C).astype(np.float32) * 0.1).cuda()
+    out = {}
+    try:
+        for nt in (0, 1):
+            _lib.ucheck(lib.otfwi_unet_set_conv_tiles(nt))
+            Y = torch.empty((N, H, W, C), dtype=torch.bfloat16, device="cuda")
+            _lib.ucheck(lib.otfwi_unet_conv3x3(vp(A), C, None, 0, 0, vp(w), vp(b), None, vp(Y),
+                                               N, H, W, C, 1, st()))
+            torch.cuda.synchronize()
+            out[nt] = Y.view(torch.int16)
+    finally:
+        _lib.ucheck(lib.otfwi_unet_set_conv_tiles(0))
+    assert torch.equal(out[0], out[1])
+    # and the result is the convolution: one interior row checked on the CPU
+    y = 517
+    Ar = A[0, y - 1:y + 2].float().cpu().numpy()
+    wr = w.float().cpu().numpy().reshape(C, 3, 3, C)
+    ref = np.zeros((W, C))
+    Ap = np.pad(Ar, ((0, 0), (1, 1), (0, 0)))
+    for dy in range(3):
+        for dx in range(3):
+            ref += Ap[dy, dx:dx + W] @ wr[:, dy, dx, :].T
+    ref = np.maximum(ref + b.cpu().numpy()[None], 0)
+    got = out[0][0, y].view(torch.bfloat16).float().cpu().numpy()
+    assert rel(got, ref) < 5e-3
 
 
 @pytest.mark.parametrize("N,H,W,CA,CB,Cout", [
