@@ -54,6 +54,19 @@ int otfwi_unet_wgrad(const otfwi_bf16 *A, int CA, const otfwi_bf16 *B, int CB, i
                      const otfwi_bf16 *G, float *dW, float *dB, int N, int H, int W, int Cout,
                      int tiles_per_wg, void *stream);
 
+/* The same weight gradient, bitwise reproducible: each work-group stores its
+ * partial sums to its row of `work` and a fixed-order reduction (sums of 16
+ * rows, repeated) adds them to dW and dB.  work_elems must be at least
+ * otfwi_unet_wgrad_work_elems(N, H, W, CA + CB, Cout, tiles_per_wg) floats;
+ * work = NULL falls back to the atomics of otfwi_unet_wgrad.  Calls sharing a
+ * workspace must be ordered (one stream).  No reference counterpart (the
+ * reference's network runs in PyTorch). */
+int otfwi_unet_wgrad_work(const otfwi_bf16 *A, int CA, const otfwi_bf16 *B, int CB, int b_half,
+                          const otfwi_bf16 *G, float *dW, float *dB, int N, int H, int W, int Cout,
+                          int tiles_per_wg, float *work, size_t work_elems, void *stream);
+/* Workspace floats otfwi_unet_wgrad_work needs for this shape (-1: bad shape). */
+long long otfwi_unet_wgrad_work_elems(int N, int H, int W, int Cin, int Cout, int tiles_per_wg);
+
 /* Wf[ci][tap][co] = W[co][8 - tap][ci]. */
 int otfwi_unet_flip_weights(const otfwi_bf16 *W, int Cout, int Cin, otfwi_bf16 *Wf, void *stream);
 
@@ -84,6 +97,16 @@ int otfwi_unet_head_fwd(const otfwi_bf16 *X, const float *w, const float *b, lon
 int otfwi_unet_head_bwd(const otfwi_bf16 *X, const float *w, const float *out,
                         const float *target, long long np, float scale, otfwi_bf16 *gX, float *dw,
                         float *db, double *loss, void *stream);
+
+/* The same with the block sums stored to `work` (at least
+ * OTFWI_UNET_HEAD_WORK floats) and added in block order: bitwise
+ * reproducible.  work = NULL: the atomics of otfwi_unet_head_bwd.  No
+ * reference counterpart. */
+#define OTFWI_UNET_HEAD_WORK (1024 * 34)
+int otfwi_unet_head_bwd_work(const otfwi_bf16 *X, const float *w, const float *out,
+                             const float *target, long long np, float scale, otfwi_bf16 *gX,
+                             float *dw, float *db, double *loss, float *work, size_t work_elems,
+                             void *stream);
 
 /* dB[c] += sum over np pixels of G[p][c] (C divides 256). */
 int otfwi_unet_bias_grad(const otfwi_bf16 *G, long long np, int C, float *dB, void *stream);

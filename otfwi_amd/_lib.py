@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libotfwi_pm
 
 OK, E_ARG, E_HIP, E_MEM, E_STATE, E_NUMERIC = 0, -1, -2, -3, -4, -5
 MISFIT_L2, MISFIT_W2, MISFIT_ENV = 0, 1, 2
+UNET_HEAD_WORK = 1024 * 34   # floats of the head-backward workspace (include/otfwi_unet.h)
 MISFIT_W2_QUAD, MISFIT_W2_EXPERIMENTAL, MISFIT_W2_PM, MISFIT_W2_YANG = 3, 4, 5, 6
 # misfit names of the Python layers -> kind codes of the C ABI
 MISFIT_KINDS = {"l2": MISFIT_L2, "w2": MISFIT_W2, "env": MISFIT_ENV,
@@ -153,6 +154,9 @@ UNET_SIGS = [
     ("otfwi_unet_conv3x3", _i, [_vp, _i, _vp, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
     ("otfwi_unet_set_conv_tiles", _i, [_i]),
     ("otfwi_unet_wgrad", _i, [_vp, _i, _vp, _i, _i, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
+    ("otfwi_unet_wgrad_work", _i, [_vp, _i, _vp, _i, _i, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp,
+                                   ctypes.c_size_t, _vp]),
+    ("otfwi_unet_wgrad_work_elems", ctypes.c_longlong, [_i, _i, _i, _i, _i, _i]),
     ("otfwi_unet_flip_weights", _i, [_vp, _i, _i, _vp, _vp]),
     ("otfwi_unet_pool2", _i, [_vp, _i, _i, _i, _i, _vp, _vp]),
     ("otfwi_unet_upsum_mask", _i, [_vp, _i, _i, _vp, _i, _i, _i, _i, _vp, _vp]),
@@ -160,6 +164,8 @@ UNET_SIGS = [
     ("otfwi_unet_pack_input", _i, [_vp, _i, _i, _i, _vp, _vp]),
     ("otfwi_unet_head_fwd", _i, [_vp, _vp, _vp, _ll, _vp, _vp]),
     ("otfwi_unet_head_bwd", _i, [_vp, _vp, _vp, _vp, _ll, _f, _vp, _vp, _vp, _vp, _vp]),
+    ("otfwi_unet_head_bwd_work", _i, [_vp, _vp, _vp, _vp, _ll, _f, _vp, _vp, _vp, _vp, _vp,
+                                      ctypes.c_size_t, _vp]),
     ("otfwi_unet_bias_grad", _i, [_vp, _ll, _i, _vp, _vp]),
     ("otfwi_unet_adam", _i, [_vp, _vp, _vp, _vp, _vp, _ll, _f, _f, _f, _f, _i, _vp]),
     ("otfwi_unet_to_bf16", _i, [_vp, _ll, _vp, _vp]),

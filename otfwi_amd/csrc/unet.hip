@@ -46,6 +46,8 @@ int hipck(hipError_t e)
     return -2;
 }
 
+inline unsigned blocks(size_t n, unsigned b = 256) { return (unsigned)((n + b - 1) / b); }
+
 // ---------------------------------------------------------------------------
 // input of a convolution: [A | B] along channels; B at half resolution when
 // b_half (nearest upsample), zero outside the image and beyond Cin
@@ -433,13 +435,18 @@ std::atomic<int> g_conv_tiles{0};
 // weight gradient: dW[co][tap][ci] += sum_p G[p][co] * X[p + tap][ci]
 // M = 32*MJ output channels, N = 32 input channels, K = pixels.  Wave w
 // (of 3) takes the taps of row dy = w.  Each work-group runs tiles_per_wg
-// pixel tiles and then adds its partial sums with FP32 atomics.
+// pixel tiles and then either adds its partial sums with FP32 atomics or
+// stores them to its row of a workspace, which wgrad_reduce_kernel sums in a
+// fixed order (bitwise reproducible).
 // ---------------------------------------------------------------------------
 struct WgradArgs {
     Src in;
     const bf16 *G;   // [N][H][W][Cout]
     float *dW;       // [Cout][9][Cin]
     float *dB;       // [Cout] bias gradient (added by the ci-block 0 work-groups) or null
+    // null: atomics; else row blockIdx.x of part (row width Cout*9*Cin + Cout:
+    // the weight sums, then the bias sums) takes this work-group's sums
+    float *part;
     int N, Cout, tiles_per_wg;
 };
 
@@ -629,7 +636,12 @@ __global__ __launch_bounds__(192) void wgrad_kernel(WgradArgs a)
         }
         __syncthreads();
     }
-    if (do_bias) atomicAdd(a.dB + co0 + tid, bsum);
+    const size_t nw = (size_t)a.Cout * 9 * Cin;
+    float *row = a.part ? a.part + (size_t)blockIdx.x * (nw + a.Cout) : nullptr;
+    if (do_bias) {
+        if (row) row[nw + co0 + tid] = bsum;
+        else atomicAdd(a.dB + co0 + tid, bsum);
+    }
     const int ci = ci0 + r;
     if (ci >= Cin) return;
 #pragma unroll
@@ -639,8 +651,49 @@ __global__ __launch_bounds__(192) void wgrad_kernel(WgradArgs a)
 #pragma unroll
             for (int e = 0; e < 16; e++) {
                 const int co = co0 + m * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
-                atomicAdd(a.dW + ((size_t)co * 9 + 3 * wave + dx) * Cin + ci, acc[m][dx][e]);
+                const size_t o = ((size_t)co * 9 + 3 * wave + dx) * Cin + ci;
+                if (row) row[o] = acc[m][dx][e];
+                else atomicAdd(a.dW + o, acc[m][dx][e]);
             }
+}
+
+// One pass of the fixed-order reduction of the weight-gradient workspace:
+// out[j][i] = sum over g in [16 j, min(16 j + 16, rows)) of in[g][i], summed
+// in order; the last pass (rows <= 16) adds the sum to dW (i < nw) and dB
+// (nw <= i < nw + Cout, when dB is given) instead.
+constexpr int WG_RED = 16;
+__global__ void wgrad_reduce_kernel(const float *in, int rows, size_t width, size_t nw, float *out,
+                                    float *dW, float *dB)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= width) return;
+    const int g0 = j * WG_RED, g1 = min(g0 + WG_RED, rows);
+    float s = 0.0f;
+    for (int g = g0; g < g1; g++) s += in[(size_t)g * width + i];
+    if (rows > WG_RED) {
+        out[(size_t)j * width + i] = s;
+    } else if (i < nw) {
+        dW[i] += s;
+    } else if (dB) {
+        dB[i - nw] += s;
+    }
+}
+
+// Pixel groups (work-group rows along x) of a weight-gradient launch.
+inline int wgrad_groups(int N, int H, int W, int tiles_per_wg)
+{
+    const long long tiles = (long long)N * H * W / 256;
+    return (int)((tiles + tiles_per_wg - 1) / tiles_per_wg);
+}
+
+// Workspace floats of the fixed-order weight gradient: the rows of every
+// pixel group, then the rows of the first reduction pass.
+inline size_t wgrad_work_elems(int N, int H, int W, int Cin, int Cout, int tiles_per_wg)
+{
+    const size_t g = (size_t)wgrad_groups(N, H, W, tiles_per_wg);
+    const size_t width = (size_t)Cout * 9 * Cin + Cout;
+    return (g + (g + WG_RED - 1) / WG_RED) * width;
 }
 
 template <int TX, int MJ>
@@ -655,7 +708,22 @@ int launch_wgrad(const WgradArgs &a, hipStream_t st)
     const int Cin = a.in.CA + a.in.CB;
     dim3 grid((tiles + a.tiles_per_wg - 1) / a.tiles_per_wg, a.Cout / (32 * MJ), (Cin + 31) / 32);
     hipLaunchKernelGGL((wgrad_kernel<TX, MJ>), grid, dim3(192), S::lds_bytes, st, a);
-    return hipck(hipGetLastError());
+    if (int rc = hipck(hipGetLastError())) return rc;
+    if (!a.part) return 0;
+    // fixed-order reduction over the pixel groups, 16 rows per sum, ping-pong
+    // between the group rows and the first pass's rows
+    const size_t nw = (size_t)a.Cout * 9 * Cin, width = nw + a.Cout;
+    float *bufs[2] = {a.part, a.part + (size_t)grid.x * width};
+    int rows = (int)grid.x, k = 0;
+    for (;;) {
+        const int out_rows = rows > WG_RED ? (rows + WG_RED - 1) / WG_RED : 1;
+        hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(blocks(width), out_rows), dim3(256), 0, st,
+                           bufs[k], rows, width, nw, bufs[k ^ 1], a.dW, a.dB);
+        if (int rc = hipck(hipGetLastError())) return rc;
+        if (rows <= WG_RED) return 0;
+        rows = out_rows;
+        k ^= 1;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -793,10 +861,14 @@ __global__ void head_fwd_kernel(const bf16 *X, const float *w, const float *b, s
 // MSE head backward: g = scale*(out - target); loss += sum g^2-like term
 // (sum (out-target)^2); gX[p][c] = X[p][c] > 0 ? g*w[c] : 0 (BF16);
 // dw[c] += sum_p g*X[p][c]; db += sum_p g.  256 threads per block.
+// bsums (may be null): block b stores its 34 sums (dw[0..31], db, loss) to
+// bsums[34 b ..] instead of adding them atomically; head_bwd_sum_kernel adds
+// them in block order.
 __global__ __launch_bounds__(256) void head_bwd_kernel(const bf16 *X, const float *w,
                                                        const float *out, const float *target,
                                                        size_t np, float scale, bf16 *gX,
-                                                       float *dw, float *db, double *loss)
+                                                       float *dw, float *db, double *loss,
+                                                       float *bsums)
 {
     __shared__ float red[4][34];
     float part[34];
@@ -832,10 +904,28 @@ __global__ __launch_bounds__(256) void head_bwd_kernel(const bf16 *X, const floa
     if (threadIdx.x < 34) {
         const int k = threadIdx.x;
         const float s = red[0][k] + red[1][k] + red[2][k] + red[3][k];
-        if (k < 32) atomicAdd(dw + k, s);
+        if (bsums) bsums[34 * blockIdx.x + k] = s;
+        else if (k < 32) atomicAdd(dw + k, s);
         else if (k == 32) atomicAdd(db, s);
         else atomicAdd(loss, (double)s);
     }
+}
+
+// The head's block sums added in block order: dw, db in FP32, the loss in FP64.
+__global__ void head_bwd_sum_kernel(const float *part, int nb, float *dw, float *db, double *loss)
+{
+    const int k = threadIdx.x;
+    if (k >= 34) return;
+    if (k == 33) {
+        double l = 0.0;
+        for (int b = 0; b < nb; b++) l += (double)part[34 * b + k];
+        *loss += l;
+        return;
+    }
+    float s = 0.0f;
+    for (int b = 0; b < nb; b++) s += part[34 * b + k];
+    if (k < 32) dw[k] += s;
+    else *db += s;
 }
 
 // dB[c] += sum over pixels of G[p][c]; each block sums 1024 pixels
@@ -878,7 +968,6 @@ __global__ void to_bf16_kernel(const float *x, size_t n, bf16 *y)
     if (i < n) y[i] = (bf16)x[i];
 }
 
-inline unsigned blocks(size_t n, unsigned b = 256) { return (unsigned)((n + b - 1) / b); }
 
 bool al16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
@@ -954,13 +1043,30 @@ int otfwi_unet_wgrad(const otfwi_bf16 *A, int CA, const otfwi_bf16 *B, int CB, i
                      const otfwi_bf16 *G, float *dW, float *dB, int N, int H, int W, int Cout,
                      int tiles_per_wg, void *stream)
 {
+    return otfwi_unet_wgrad_work(A, CA, B, CB, b_half, G, dW, dB, N, H, W, Cout, tiles_per_wg,
+                                 nullptr, 0, stream);
+}
+
+long long otfwi_unet_wgrad_work_elems(int N, int H, int W, int Cin, int Cout, int tiles_per_wg)
+{
+    if (N <= 0 || H <= 0 || W <= 0 || Cin <= 0 || Cout <= 0 || tiles_per_wg <= 0) return -1;
+    return (long long)wgrad_work_elems(N, H, W, Cin, Cout, tiles_per_wg);
+}
+
+int otfwi_unet_wgrad_work(const otfwi_bf16 *A, int CA, const otfwi_bf16 *B, int CB, int b_half,
+                          const otfwi_bf16 *G, float *dW, float *dB, int N, int H, int W, int Cout,
+                          int tiles_per_wg, float *work, size_t work_elems, void *stream)
+{
     WgradArgs a{};
     a.in = Src{BF(A), CA, BF(B), CB, b_half, H, W};
     a.G = BF(G); a.dW = dW; a.dB = dB; a.N = N; a.Cout = Cout; a.tiles_per_wg = tiles_per_wg;
+    a.part = work;
     if (int rc = check_src(a.in, N)) return rc;
     if (!G || !dW || !al16(G) || Cout <= 0 || Cout % 32)
         return ufail("bad gradient/output (Cout must be a multiple of 32)");
     if (tiles_per_wg <= 0) return ufail("tiles_per_wg must be positive");
+    if (work && work_elems < wgrad_work_elems(N, H, W, CA + CB, Cout, tiles_per_wg))
+        return ufail("weight-gradient workspace too small (otfwi_unet_wgrad_work_elems)");
     if (CB && CA % 32) return ufail("CA must be a multiple of 32 with a second input");
     const int TX = W % 64 == 0 ? 64 : (W % 32 == 0 ? 32 : 0);
     if (!TX || H % (256 / TX)) return ufail("W must be a multiple of 32 and H of the tile height");
@@ -1036,12 +1142,27 @@ int otfwi_unet_head_bwd(const otfwi_bf16 *X, const float *w, const float *out,
                         const float *target, long long np, float scale, otfwi_bf16 *gX, float *dw,
                         float *db, double *loss, void *stream)
 {
+    return otfwi_unet_head_bwd_work(X, w, out, target, np, scale, gX, dw, db, loss, nullptr, 0,
+                                    stream);
+}
+
+int otfwi_unet_head_bwd_work(const otfwi_bf16 *X, const float *w, const float *out,
+                             const float *target, long long np, float scale, otfwi_bf16 *gX,
+                             float *dw, float *db, double *loss, float *work, size_t work_elems,
+                             void *stream)
+{
     if (!X || !w || !out || !target || !gX || !dw || !db || !loss || np <= 0 || !al16(X) ||
         !al16(gX))
         return ufail("bad head-backward args");
+    if (work && work_elems < (size_t)OTFWI_UNET_HEAD_WORK)
+        return ufail("head-backward workspace too small (OTFWI_UNET_HEAD_WORK floats)");
     const unsigned nb = std::min(blocks((size_t)np), 1024u);
     hipLaunchKernelGGL(head_bwd_kernel, dim3(nb), dim3(256), 0, ST(stream), BF(X),
-                       w, out, target, (size_t)np, scale, BFW(gX), dw, db, loss);
+                       w, out, target, (size_t)np, scale, BFW(gX), dw, db, loss, work);
+    if (int rc = hipck(hipGetLastError())) return rc;
+    if (!work) return 0;
+    hipLaunchKernelGGL(head_bwd_sum_kernel, dim3(1), dim3(64), 0, ST(stream), work, (int)nb, dw, db,
+                       loss);
     return hipck(hipGetLastError());
 }
 

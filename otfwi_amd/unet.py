@@ -159,9 +159,12 @@ def _vp(t):
 
 class UNet:
     """Device-resident network.  forward() keeps the activations that
-    backward() needs; train_step() = forward + backward + Adam."""
+    backward() needs; train_step() = forward + backward + Adam.  With
+    deterministic=True (default) the weight gradients are summed in a fixed
+    order from per-work-group partial sums (otfwi_unet_wgrad_work), so a
+    training run is bitwise reproducible; False adds them with FP32 atomics."""
 
-    def __init__(self, params=None, seed=0, device="cuda"):
+    def __init__(self, params=None, seed=0, device="cuda", deterministic=True):
         import torch
 
         self.torch = torch
@@ -183,6 +186,8 @@ class UNet:
                 self.flip[name] = torch.empty(co * 9 * (ca + cb), dtype=torch.bfloat16,
                                               device=self.dev)
         self._bufs = {}
+        self.deterministic = deterministic
+        self._ws = None        # weight-gradient workspace (FP32), grown on demand
         self.loss_acc = torch.zeros(1, dtype=torch.float64, device=self.dev)
         self._sync_bf16()
 
@@ -244,9 +249,20 @@ class UNet:
         tiles = N * H * W // 256
         other = max(1, co // 64) * ((ca + cb + 31) // 32)
         tpw = max(1, -(-tiles * other // 2048))
-        _lib.ucheck(self.lib.otfwi_unet_wgrad(_vp(A), ca, _vp(B), cb, 1 if cb else 0, _vp(G),
-                                              self._gw(name), self._gb(name), N, H, W, co, tpw,
-                                              self._st()))
+        if not self.deterministic:
+            _lib.ucheck(self.lib.otfwi_unet_wgrad(_vp(A), ca, _vp(B), cb, 1 if cb else 0, _vp(G),
+                                                  self._gw(name), self._gb(name), N, H, W, co,
+                                                  tpw, self._st()))
+            return
+        need = int(self.lib.otfwi_unet_wgrad_work_elems(N, H, W, ca + cb, co, tpw))
+        if need <= 0:
+            raise ValueError(f"bad weight-gradient shape for {name}")
+        if self._ws is None or self._ws.numel() < need:
+            self._ws = self.torch.empty(need, dtype=self.torch.float32, device=self.dev)
+        _lib.ucheck(self.lib.otfwi_unet_wgrad_work(_vp(A), ca, _vp(B), cb, 1 if cb else 0, _vp(G),
+                                                   self._gw(name), self._gb(name), N, H, W, co,
+                                                   tpw, _vp(self._ws), self._ws.numel(),
+                                                   self._st()))
 
     def _pool(self, X, Y, N, H, W, C):
         _lib.ucheck(self.lib.otfwi_unet_pool2(_vp(X), N, H, W, C, _vp(Y), self._st()))
@@ -313,10 +329,12 @@ class UNet:
         npx = N * H * W
         hw, hb = self.lay["head"][0], self.lay["head"][2]
         g = b("g_dec0b", (N, H, W, 32))
-        _lib.ucheck(self.lib.otfwi_unet_head_bwd(
+        hwork = b("head_work", (_lib.UNET_HEAD_WORK,), t.float32) if self.deterministic else None
+        _lib.ucheck(self.lib.otfwi_unet_head_bwd_work(
             _vp(a["dec0b"]), ctypes.c_void_p(self.p.data_ptr() + 4 * hw), _vp(out), _vp(target),
             npx, 2.0 / npx, _vp(g), ctypes.c_void_p(self.g.data_ptr() + 4 * hw),
-            ctypes.c_void_p(self.g.data_ptr() + 4 * hb), _vp(self.loss_acc), self._st()))
+            ctypes.c_void_p(self.g.data_ptr() + 4 * hb), _vp(self.loss_acc), _vp(hwork),
+            0 if hwork is None else hwork.numel(), self._st()))
         # decoder, finest level first
         dxc = {}
         h, w = H, W

@@ -230,6 +230,50 @@ def test_wgrad_and_data_grad_vs_autograd(gpu, N, H, W, CA, CB, Cout):
         assert ex < 5e-3
 
 
+@pytest.mark.parametrize("N,H,W,CA,CB,Cout,tpw", [
+    (2, 64, 64, 32, 64, 32, 1),       # 32 pixel groups: two reduction passes
+    (2, 256, 256, 32, 0, 64, 1),      # 512 groups: three passes
+    (1, 32, 32, 128, 256, 128, 3),    # 2 groups: one pass
+    (1, 64, 64, 16, 0, 32, 2),        # Cin 16 (half a channel block)
+])
+def test_wgrad_workspace_fixed_order(gpu, N, H, W, CA, CB, Cout, tpw):
+    """The workspace form of the weight gradient (per-work-group partial
+    sums, fixed-order reduction): bitwise equal over repeated calls and over
+    a dW that already holds values (it adds), equal to the atomic form up to
+    summation order, and a too-small workspace is refused."""
+    rng = np.random.default_rng(11 + H + CA + Cout)
+    lib = _lib.load_unet()
+    A = bf(rng.standard_normal((N, H, W, CA))).cuda()
+    B = bf(rng.standard_normal((N, H // 2, W // 2, CB))).cuda() if CB else None
+    G = bf(rng.standard_normal((N, H, W, Cout))).cuda()
+    cin = CA + CB
+    need = int(lib.otfwi_unet_wgrad_work_elems(N, H, W, cin, Cout, tpw))
+    assert need > 0
+    ws = torch.full((need,), float("nan"), dtype=torch.float32, device="cuda")
+    base = torch.tensor(rng.standard_normal((Cout, 9, cin)).astype(np.float32)).cuda()
+    bbase = torch.tensor(rng.standard_normal(Cout).astype(np.float32)).cuda()
+    outs = []
+    for _ in range(2):
+        dW, dB = base.clone(), bbase.clone()
+        _lib.ucheck(lib.otfwi_unet_wgrad_work(vp(A), CA, vp(B), CB, 1 if CB else 0, vp(G), vp(dW),
+                                              vp(dB), N, H, W, Cout, tpw, vp(ws), need, st()))
+        torch.cuda.synchronize()
+        outs.append((dW.cpu(), dB.cpu()))
+    assert torch.equal(outs[0][0], outs[1][0]) and torch.equal(outs[0][1], outs[1][1])
+    dWa, dBa = base.clone(), bbase.clone()
+    _lib.ucheck(lib.otfwi_unet_wgrad(vp(A), CA, vp(B), CB, 1 if CB else 0, vp(G), vp(dWa),
+                                     vp(dBa), N, H, W, Cout, tpw, st()))
+    torch.cuda.synchronize()
+    ew = rel(outs[0][0].numpy() - base.cpu().numpy(), dWa.cpu().numpy() - base.cpu().numpy())
+    eb = rel(outs[0][1].numpy() - bbase.cpu().numpy(), dBa.cpu().numpy() - bbase.cpu().numpy())
+    print(f"wgrad workspace {H}x{W} {CA}+{CB}->{Cout} tpw {tpw}: vs atomics {ew:.2e}, bias {eb:.2e}")
+    assert ew < 1e-5 and eb < 1e-5
+    with pytest.raises(Exception):
+        _lib.ucheck(lib.otfwi_unet_wgrad_work(vp(A), CA, vp(B), CB, 1 if CB else 0, vp(G),
+                                              vp(dWa), vp(dBa), N, H, W, Cout, tpw, vp(ws),
+                                              need - 1, st()))
+
+
 def test_network_forward_backward_vs_oracle(gpu):
     N, H, W = 2, 256, 256
     rng = np.random.default_rng(3)
@@ -319,3 +363,9 @@ def test_training_reduces_loss(gpu):
     print("loss", losses[0], "->", losses[-1])
     assert np.all(np.isfinite(losses))
     assert losses[-1] < 0.5 * losses[0]
+    # fixed-order weight gradients: a second run from the same start is bitwise equal
+    net2 = U.UNet(seed=2)
+    for _ in range(40):
+        net2.train_step(x, t, lr=1e-3)
+    torch.cuda.synchronize()
+    assert torch.equal(net.p, net2.p)
