@@ -59,7 +59,8 @@ def header_functions(header="otfwi_pml.h"):
     txt = open(os.path.join(ROOT, "include", header)).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     txt = "\n".join(line for line in txt.split("\n") if not line.strip().startswith("typedef"))
-    names = re.findall(r"^\s*(?:const\s+)?[a-z_]+\s*\*?\s*([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt, re.M)
+    names = re.findall(r"^\s*(?:const\s+)?(?:long\s+long|unsigned|[a-z_]+)\s*\*?\s*"
+                       r"([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt, re.M)
     return sorted(set(n for n in names if n not in ("typedef",)))
 
 
