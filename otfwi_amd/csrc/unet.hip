@@ -944,21 +944,27 @@ __global__ __launch_bounds__(256) void head_bwd_kernel(const bf16 *X, const floa
     }
 }
 
-// The head's block sums added in block order: dw, db in FP32, the loss in FP64.
-__global__ void head_bwd_sum_kernel(const float *part, int nb, float *dw, float *db, double *loss)
+// The head's block sums added in a fixed order: work-group k (one wave)
+// takes sum k; lane l adds blocks l, l + 64, ... in order, then the lanes'
+// sums meet in a fixed butterfly.  dw, db in FP32, the loss in FP64.
+__global__ __launch_bounds__(64) void head_bwd_sum_kernel(const float *part, int nb, float *dw,
+                                                          float *db, double *loss)
 {
-    const int k = threadIdx.x;
-    if (k >= 34) return;
-    if (k == 33) {
-        double l = 0.0;
-        for (int b = 0; b < nb; b++) l += (double)part[34 * b + k];
-        *loss += l;
-        return;
+    const int k = blockIdx.x, l = threadIdx.x;
+    double s = 0.0;
+    float f = 0.0f;
+    for (int b = l; b < nb; b += 64) {
+        if (k == 33) s += (double)part[34 * b + k];
+        else f += part[34 * b + k];
     }
-    float s = 0.0f;
-    for (int b = 0; b < nb; b++) s += part[34 * b + k];
-    if (k < 32) dw[k] += s;
-    else *db += s;
+    for (int o = 32; o > 0; o >>= 1) {
+        s += __shfl_xor(s, o, 64);
+        f += __shfl_xor(f, o, 64);
+    }
+    if (l != 0) return;
+    if (k == 33) *loss += s;
+    else if (k == 32) *db += f;
+    else dw[k] += f;
 }
 
 // dB[c] += sum over pixels of G[p][c]; each block sums 1024 pixels
@@ -1194,7 +1200,7 @@ int otfwi_unet_head_bwd_work(const otfwi_bf16 *X, const float *w, const float *o
                        w, out, target, (size_t)np, scale, BFW(gX), dw, db, loss, work);
     if (int rc = hipck(hipGetLastError())) return rc;
     if (!work) return 0;
-    hipLaunchKernelGGL(head_bwd_sum_kernel, dim3(1), dim3(64), 0, ST(stream), work, (int)nb, dw, db,
+    hipLaunchKernelGGL(head_bwd_sum_kernel, dim3(34), dim3(64), 0, ST(stream), work, (int)nb, dw, db,
                        loss);
     return hipck(hipGetLastError());
 }
