@@ -358,7 +358,11 @@ def main(argv=None):
     ap.add_argument("--chain", type=int, default=1000, help="refine: seeds of step-chain pairs")
     ap.add_argument("--chain-steps", type=int, default=3)
     ap.add_argument("--first", type=int, default=1000, help="refine: first-step pairs")
-    ap.add_argument("--save", default=None, help="refine: write the refined weights here")
+    ap.add_argument("--save", default=None, help="refine / finetune: write the weights here")
+    ap.add_argument("--seed0", type=int, default=0,
+                    help="finetune: first model seed of the pairs (seeds seed0 .. seed0+pairs-1)")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="finetune: skip the held-out comparison")
     a = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)  # noqa: E731
     prob = Problem(**GEOMETRIES[a.geometry])
@@ -370,16 +374,18 @@ def main(argv=None):
         with open(os.path.splitext(a.weights)[0] + ".json", "w") as f:
             json.dump({k: v for k, v in info.items() if not k.startswith("loss")}, f, indent=1)
     elif a.cmd == "finetune":
-        # pairs at this geometry (seeds 0..pairs-1), the network in --weights as the start,
-        # written to --save (default: the 1024^2 weights)
+        # pairs at this geometry (seeds seed0 .. seed0+pairs-1), the network in --weights as
+        # the start, written to --save (default: the 1024^2 weights)
         from .unet import UNet
-        X, Y, t_pairs = make_pairs(prob, range(a.pairs), log=log, log_every=10)
+        X, Y, t_pairs = make_pairs(prob, range(a.seed0, a.seed0 + a.pairs), rng_seed=a.seed0,
+                                   log=log, log_every=10)
         net = UNet(params=load_weights(a.weights))
-        h, t_train = train(net, X, Y, a.epochs, lr=a.lr, log=log)
+        h, t_train = train(net, X, Y, a.epochs, lr=a.lr, seed=a.seed0, log=log)
         out_w = a.save or WEIGHTS_1024
         save_weights(net, out_w)
-        info = dict(pairs=a.pairs, epochs=a.epochs, lr=a.lr, t_pairs=t_pairs, t_train=t_train,
-                    init=os.path.basename(a.weights), loss_first_epoch=h[0], loss_last_epoch=h[-1])
+        info = dict(pairs=a.pairs, seeds=[a.seed0, a.seed0 + a.pairs - 1], epochs=a.epochs,
+                    lr=a.lr, t_pairs=t_pairs, t_train=t_train, init=os.path.basename(a.weights),
+                    loss_first_epoch=h[0], loss_last_epoch=h[-1])
         summary["finetune"] = info
         with open(os.path.splitext(out_w)[0] + ".json", "w") as f:
             json.dump(info, f, indent=1)
@@ -394,6 +400,9 @@ def main(argv=None):
         summary["refine"] = info
         with open(os.path.splitext(out_w)[0] + ".refine.json", "w") as f:
             json.dump({k: v for k, v in info.items() if not k.startswith("loss")}, f, indent=1)
+    if a.cmd == "finetune" and a.no_compare:
+        log(json.dumps(summary["finetune"]))
+        return
     if a.cmd in ("compare", "finetune"):
         from .unet import UNet
         net = UNet(params=load_weights(a.weights))
