@@ -201,9 +201,10 @@ int otfwi_set_pair_width(otfwi_model *m, int nw);
  * else 1).  With k > 1 the shots are split into k groups of consecutive
  * shots, each advancing on the model's stream or one of the library's extra
  * streams, so each group's launches fill the others' launch tails; results
- * are bitwise identical.  A loop whose two-step phase splits is launched
- * directly, also with graphs on (otfwi_set_graph): only one-stream loops are
- * captured.  No reference counterpart. */
+ * are bitwise identical.  With graphs on (otfwi_set_graph) each group's
+ * two-step launches are captured on that group's own stream and replayed
+ * there between a fork and a join recorded outside the graphs.  No reference
+ * counterpart. */
 int otfwi_set_pair_streams(otfwi_model *m, int k);
 
 /* The steps per launch (1 or 2) the forward and adjoint loops use for a
@@ -215,14 +216,16 @@ int otfwi_steps_per_launch(otfwi_model *m, int n_shots, int *fwd, int *adj);
 /* How the forward and adjoint time loops of a batch of n_shots shots run
  * under the current settings: out[0] / out[2] = streams of the forward /
  * adjoint two-step phase (1 when the loop runs one step per launch or does
- * not split), out[1] / out[3] = 1 when that loop is replayed from a captured
- * hipGraph, 0 when it is launched directly.  No reference counterpart. */
+ * not split), out[1] / out[3] = 1 when that loop is replayed from captured
+ * hipGraphs (one per stream group for a split two-step phase, whose one to
+ * three single steps before and after it launch directly), 0 when it is
+ * launched directly.  No reference counterpart. */
 int otfwi_loop_modes(otfwi_model *m, int n_shots, int *out);
 
 /* Replay the per-step launch loops as captured hipGraphs (default 1); a
  * graph is re-captured only when shots, tiles, launch style or device buffers
- * change.  Loops whose two-step phase splits over streams
- * (otfwi_set_pair_streams) are launched directly either way.  No reference counterpart (regression_PML_single.cu:661-678
+ * change.  A loop whose two-step phase splits over streams
+ * (otfwi_set_pair_streams) is captured per stream group.  No reference counterpart (regression_PML_single.cu:661-678
  * launches and synchronizes four kernels per step). */
 int otfwi_set_graph(otfwi_model *m, int use_graph);
 

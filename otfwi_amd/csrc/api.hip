@@ -138,6 +138,11 @@ struct otfwi_model {
     unsigned long long epoch = 1;
     hipGraphExec_t gexec[3] = {};
     unsigned long long gkey[3] = {};
+    // loops whose two-step phase splits over stream groups: per slot, the
+    // pairs of each group captured on that group's own stream
+    hipGraphExec_t gsplit[3][4] = {};
+    unsigned long long gskey[3] = {};
+    bool gsvalid[3] = {};
     uint8_t *tf_fwd = nullptr, *tf_adj = nullptr;
     int tf_fwd_th = 0, tf_adj_th = 0;
     std::vector<float> obs_h;
@@ -534,12 +539,12 @@ bool use_pairs(const otfwi_model *m, int n, bool adj)
 // two when each half still launches 2 work-groups per CU (measured at
 // 64 x 1024^2: two concurrent 32-shot halves take 0.92x the time of one
 // 64-shot launch sequence, their launches filling each other's tails).
-// A loop whose pair phase splits is launched directly, never captured into
-// a hipGraph: a process ended with a host segmentation fault inside the first
-// forward call of a model whose loop was captured across 2 or 4 streams, with
-// no cause found in this code.  At the sizes that split, direct launches are
-// also the faster form (64 x 1024^2 W2 gradient: 3.53 s direct and split,
-// 3.80 s from one-stream graphs).
+// A split loop is never captured as one graph across its streams (a process
+// ended with a host segmentation fault inside the first forward call of a
+// model whose loop had been captured across 2 or 4 streams, with no cause
+// found in this code): with graphs on, each shot group's pairs are captured
+// on that group's own stream by itself (run_split), and the fork and join
+// stay outside the graphs.
 int pair_split(const otfwi_model *m, int n)
 {
     if (n < 2 || !m->xs[0]) return 1;
@@ -643,6 +648,65 @@ int run_loop(otfwi_model *m, int slot, unsigned long long key, bool split, F bod
     }
     HIPCK(hipGraphLaunch(m->gexec[slot], m->stream));
     return OTFWI_OK;
+}
+
+// Shot range [g0, g1) of group k of ns (consecutive shots, as launch_pair_split
+// splits them).
+inline void group_range(int n, int k, int ns, int &g0, int &g1)
+{
+    g0 = (int)((long long)n * k / ns);
+    g1 = (int)((long long)n * (k + 1) / ns);
+}
+
+// A loop whose two-step phase splits over ns stream groups, with graphs on:
+// lead() and trail() enqueue the single steps before and after the pairs on
+// the model's stream, directly; group(k, s) enqueues the pairs of shot group
+// k on stream s.  Each group's pairs are captured once, by themselves, on the
+// stream they run on (a one-stream capture, as every other captured loop) and
+// replayed there between a fork from and a join into the model's stream; the
+// fork and join events are recorded outside the graphs.
+template <typename FL, typename FG, typename FT>
+int run_split(otfwi_model *m, int slot, unsigned long long key, int n, int ns, FL lead, FG group,
+              FT trail)
+{
+    RCK(lead());
+    if (!m->gsvalid[slot] || m->gskey[slot] != key) {
+        // earlier replays of this slot may still run on any of the streams
+        HIPCK(hipStreamSynchronize(m->stream));
+        for (int k = 0; k < 3; k++)
+            if (m->xs[k]) HIPCK(hipStreamSynchronize(m->xs[k]));
+        for (int k = 0; k < 4; k++) {
+            if (m->gsplit[slot][k]) (void)hipGraphExecDestroy(m->gsplit[slot][k]);
+            m->gsplit[slot][k] = nullptr;
+        }
+        m->gsvalid[slot] = false;
+        for (int k = 0; k < ns; k++) {
+            int g0, g1;
+            group_range(n, k, ns, g0, g1);
+            if (g1 <= g0) continue;
+            hipStream_t s = k ? m->xs[k - 1] : m->stream;
+            hipGraph_t graph = nullptr;
+            HIPCK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            const int rc = group(k, s);
+            const hipError_t e = hipStreamEndCapture(s, &graph);
+            if (rc != OTFWI_OK) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+            HIPCK(e);
+            const hipError_t ei = hipGraphInstantiate(&m->gsplit[slot][k], graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            HIPCK(ei);
+        }
+        m->gskey[slot] = key;
+        m->gsvalid[slot] = true;
+    }
+    RCK(fork_streams(m, ns));
+    for (int k = 0; k < ns; k++)
+        if (m->gsplit[slot][k])
+            HIPCK(hipGraphLaunch(m->gsplit[slot][k], k ? m->xs[k - 1] : m->stream));
+    RCK(join_streams(m, ns));
+    return trail();
 }
 
 unsigned long long loop_key(const otfwi_model *m, int n, int th, int pw, int t0, int t1)
@@ -791,40 +855,64 @@ int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum)
     // adjoint's reconstruction
     const bool two = use_pairs(m, n, false) && ist == 0 && K == 1 && G == n && m->evrow &&
                      pair2_supported((long long)plane, (long long)b.cap * (long long)plane, (int)sizeof(R));
+    const int S2 = t1 - t0;
+    int npairs = S2 > 1 ? (S2 - 1) / 2 : 0;
+    npairs -= npairs & 1;
+    const int nlead = (S2 - 1) - 2 * npairs;
+    auto single = [&](int tt) -> int {
+        const int p = (tt - 1) & 1, c = tt & 1;
+        a.t = tt;
+        a.P0 = (const R *)b.P[p]; a.P1 = (R *)b.P[c];
+        a.Ax0 = (const R *)b.Ax[p]; a.Ax1 = (R *)b.Ax[c];
+        a.Az0 = (const R *)b.Az[p]; a.Az1 = (R *)b.Az[c];
+        a.Px0 = (const R *)b.Px[p]; a.Px1 = (R *)b.Px[c];
+        HIPCK(launch_fwd_step<R>(a, m->th_fwd, pw, st));
+        return OTFWI_OK;
+    };
+    auto lead_fn = [&]() -> int {
+        for (int t = t0; t < t0 + nlead; t++) RCK(single(t));
+        return OTFWI_OK;
+    };
+    auto trail_fn = [&]() -> int {
+        for (int t = t0 + std::max(nlead, 0) + 2 * npairs; t < t1; t++) RCK(single(t));
+        return OTFWI_OK;
+    };
+    // slot pointers of pair k (starting at step t): each pair writes the slot
+    // it does not read, so after k pairs the state sits in the other slot
+    // than one step per launch leaves it
+    auto pair_slots = [&](Pair2Args<R> &q, int k, int t) {
+        const int in = ((t - 1) & 1) ^ (k & 1), out = in ^ 1;
+        q.t = t;
+        q.P0 = (const R *)b.P[in]; q.P1 = (R *)b.P[out];
+        q.Ax0 = (const R *)b.Ax[in]; q.Ax1 = (R *)b.Ax[out];
+        q.Az0 = (const R *)b.Az[in]; q.Az1 = (R *)b.Az[out];
+        q.Px0 = (const R *)b.Px[in]; q.Px1 = (R *)b.Px[out];
+    };
+    // all pairs of shot group g of ns on stream s (captured by run_split)
+    auto group_fn = [&](int g, int ns, hipStream_t s) -> int {
+        int g0, g1;
+        group_range(n, g, ns, g0, g1);
+        Pair2Args<R> q = pair_fwd_args<R>(m, n, a);
+        int t = t0 + nlead;
+        for (int k = 0; k < npairs; k++, t += 2) {
+            pair_slots(q, k, t);
+            HIPCK(launch_pair2<R>(pair_group<R>(m, q, g0, g1 - g0), 0, s));
+        }
+        return OTFWI_OK;
+    };
+    // direct launches: the groups' launches of each pair enqueued together
     auto body2 = [&]() -> int {
-        const int S = t1 - t0;
-        int pairs = S > 1 ? (S - 1) / 2 : 0;
-        pairs -= pairs & 1;
-        const int lead = (S - 1) - 2 * pairs;
-        int t = t0;
-        auto single = [&](int tt) -> int {
-            const int p = (tt - 1) & 1, c = tt & 1;
-            a.t = tt;
-            a.P0 = (const R *)b.P[p]; a.P1 = (R *)b.P[c];
-            a.Ax0 = (const R *)b.Ax[p]; a.Ax1 = (R *)b.Ax[c];
-            a.Az0 = (const R *)b.Az[p]; a.Az1 = (R *)b.Az[c];
-            a.Px0 = (const R *)b.Px[p]; a.Px1 = (R *)b.Px[c];
-            HIPCK(launch_fwd_step<R>(a, m->th_fwd, pw, st));
-            return OTFWI_OK;
-        };
-        for (int k = 0; k < lead; k++, t++) RCK(single(t));
+        RCK(lead_fn());
         Pair2Args<R> q = pair_fwd_args<R>(m, n, a);
         const int ns = pair_split(m, n);
-        if (ns > 1 && pairs > 0) RCK(fork_streams(m, ns));
-        for (int k = 0; k < pairs; k++, t += 2) {
-            // each pair writes the slot it does not read: after k pairs the
-            // state sits in the other slot than one step per launch leaves it
-            const int in = ((t - 1) & 1) ^ (k & 1), out = in ^ 1;
-            q.t = t;
-            q.P0 = (const R *)b.P[in]; q.P1 = (R *)b.P[out];
-            q.Ax0 = (const R *)b.Ax[in]; q.Ax1 = (R *)b.Ax[out];
-            q.Az0 = (const R *)b.Az[in]; q.Az1 = (R *)b.Az[out];
-            q.Px0 = (const R *)b.Px[in]; q.Px1 = (R *)b.Px[out];
+        if (ns > 1 && npairs > 0) RCK(fork_streams(m, ns));
+        int t = t0 + nlead;
+        for (int k = 0; k < npairs; k++, t += 2) {
+            pair_slots(q, k, t);
             HIPCK(launch_pair_split<R>(m, q, n, ns, 0));
         }
-        if (ns > 1 && pairs > 0) RCK(join_streams(m, ns));
-        for (; t < t1; t++) RCK(single(t));
-        return OTFWI_OK;
+        if (ns > 1 && npairs > 0) RCK(join_streams(m, ns));
+        return trail_fn();
     };
     auto body = [&]() -> int {
         if (two) return body2();
@@ -858,9 +946,13 @@ int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum)
     const unsigned long long kb =
         (((unsigned long long)K * 4099ULL + (unsigned long long)G) * 3ULL + (two ? 1ULL : 0ULL)) *
             3ULL + (unsigned long long)pair_split(m, n);
-    return run_loop(m, save_ring ? 1 : 0,
-                    (loop_key(m, n, m->th_fwd, pw, t0, t1) * 31ULL + (unsigned long long)ist) *
-                            1000003ULL + kb, two && pair_split(m, n) > 1, body);
+    const unsigned long long key =
+        (loop_key(m, n, m->th_fwd, pw, t0, t1) * 31ULL + (unsigned long long)ist) * 1000003ULL + kb;
+    const int ns = two && npairs > 0 ? pair_split(m, n) : 1;
+    if (ns > 1 && m->use_graph)
+        return run_split(m, save_ring ? 1 : 0, key, n, ns, lead_fn,
+                         [&](int g, hipStream_t s) { return group_fn(g, ns, s); }, trail_fn);
+    return run_loop(m, save_ring ? 1 : 0, key, ns > 1, body);
 }
 
 template <typename R>
@@ -936,30 +1028,53 @@ int adj_loop(otfwi_model *m, int n, int t0, int t1)
         HIPCK(launch_adj_step<R>(a, m->th_adj, pw, st));
         return OTFWI_OK;
     };
+    const int S2 = t1 - t0;
+    const int npairs = (S2 / 2) & ~1;
+    const int nlead = S2 - 2 * npairs;
+    auto lead_fn = [&]() -> int {
+        for (int t = t0; t < t0 + nlead; t++) RCK(single(t));
+        return OTFWI_OK;
+    };
+    // slot pointers of pair k (starting at step t) of the reconstruction (u)
+    // and lambda (l) launches: each pair writes the slots it does not read
+    auto pair_slots = [&](Pair2Args<R> &u, Pair2Args<R> &l, int k, int t) {
+        const int f = k & 1;
+        const int li = ((t - 1) & 1) ^ f, ui = ((T - 2 + t) & 1) ^ f, ai = ((T - 1 + t) & 1) ^ f;
+        u.t = l.t = t;
+        u.P0 = (const R *)b.P[ui]; u.P1 = (R *)b.P[ui ^ 1];
+        u.Ax0 = (const R *)b.Ax[ai]; u.Ax1 = (R *)b.Ax[ai ^ 1];
+        u.Az0 = (const R *)b.Az[ai]; u.Az1 = (R *)b.Az[ai ^ 1];
+        l.P0 = (const R *)b.LP[li]; l.P1 = (R *)b.LP[li ^ 1];
+        l.Ax0 = (const R *)b.LAx[li]; l.Ax1 = (R *)b.LAx[li ^ 1];
+        l.Az0 = (const R *)b.LAz[li]; l.Az1 = (R *)b.LAz[li ^ 1];
+        l.Px0 = (const R *)b.LPx[li]; l.Px1 = (R *)b.LPx[li ^ 1];
+    };
+    // all pairs of shot group g of ns on stream s (captured by run_split)
+    auto group_fn = [&](int g, int ns, hipStream_t s) -> int {
+        int g0, g1;
+        group_range(n, g, ns, g0, g1);
+        Pair2Args<R> u = pair_rec_args<R>(m, n), l = pair_lam_args<R>(m, n);
+        int t = t0 + nlead;
+        for (int k = 0; k < npairs; k++, t += 2) {
+            pair_slots(u, l, k, t);
+            HIPCK(launch_pair2<R>(pair_group<R>(m, u, g0, g1 - g0), 2, s));
+            HIPCK(launch_pair2<R>(pair_group<R>(m, l, g0, g1 - g0), 1, s));
+        }
+        return OTFWI_OK;
+    };
+    // direct launches: the groups' launches of each pair enqueued together
     auto body2 = [&]() -> int {
-        const int S = t1 - t0;
-        const int pairs = (S / 2) & ~1;
-        const int lead = S - 2 * pairs;
-        int t = t0;
-        for (int k = 0; k < lead; k++, t++) RCK(single(t));
+        RCK(lead_fn());
         Pair2Args<R> u = pair_rec_args<R>(m, n), l = pair_lam_args<R>(m, n);
         const int ns = pair_split(m, n);
-        if (ns > 1 && pairs > 0) RCK(fork_streams(m, ns));
-        for (int k = 0; k < pairs; k++, t += 2) {
-            const int f = k & 1;
-            const int li = ((t - 1) & 1) ^ f, ui = ((T - 2 + t) & 1) ^ f, ai = ((T - 1 + t) & 1) ^ f;
-            u.t = l.t = t;
-            u.P0 = (const R *)b.P[ui]; u.P1 = (R *)b.P[ui ^ 1];
-            u.Ax0 = (const R *)b.Ax[ai]; u.Ax1 = (R *)b.Ax[ai ^ 1];
-            u.Az0 = (const R *)b.Az[ai]; u.Az1 = (R *)b.Az[ai ^ 1];
+        if (ns > 1 && npairs > 0) RCK(fork_streams(m, ns));
+        int t = t0 + nlead;
+        for (int k = 0; k < npairs; k++, t += 2) {
+            pair_slots(u, l, k, t);
             HIPCK(launch_pair_split<R>(m, u, n, ns, 2));
-            l.P0 = (const R *)b.LP[li]; l.P1 = (R *)b.LP[li ^ 1];
-            l.Ax0 = (const R *)b.LAx[li]; l.Ax1 = (R *)b.LAx[li ^ 1];
-            l.Az0 = (const R *)b.LAz[li]; l.Az1 = (R *)b.LAz[li ^ 1];
-            l.Px0 = (const R *)b.LPx[li]; l.Px1 = (R *)b.LPx[li ^ 1];
             HIPCK(launch_pair_split<R>(m, l, n, ns, 1));
         }
-        if (ns > 1 && pairs > 0) RCK(join_streams(m, ns));
+        if (ns > 1 && npairs > 0) RCK(join_streams(m, ns));
         return OTFWI_OK;
     };
     auto body = [&]() -> int {
@@ -998,8 +1113,13 @@ int adj_loop(otfwi_model *m, int n, int t0, int t1)
     const unsigned long long kb =
         (((unsigned long long)K * 4099ULL + (unsigned long long)G) * 3ULL + (two ? 1ULL : 0ULL)) *
             3ULL + (unsigned long long)pair_split(m, n);
-    return run_loop(m, 2, loop_key(m, n, m->th_adj, pw, t0, t1) * 1000003ULL + kb,
-                    two && pair_split(m, n) > 1, body);
+    const unsigned long long key = loop_key(m, n, m->th_adj, pw, t0, t1) * 1000003ULL + kb;
+    const int ns = two && npairs > 0 ? pair_split(m, n) : 1;
+    if (ns > 1 && m->use_graph)
+        return run_split(m, 2, key, n, ns, lead_fn,
+                         [&](int g, hipStream_t s) { return group_fn(g, ns, s); },
+                         []() { return (int)OTFWI_OK; });
+    return run_loop(m, 2, key, ns > 1, body);
 }
 
 int upload_shot_src(otfwi_model *m, int n, const int *shot_src)
@@ -1737,8 +1857,11 @@ int otfwi_destroy(otfwi_model *m)
         fprintf(stderr, "otfwi_destroy: a stream of the model is not idle; its memory is kept\n");
         return fail(OTFWI_E_STATE, "destroy: a stream of the model is not idle");
     }
-    for (int i = 0; i < 3; i++)
+    for (int i = 0; i < 3; i++) {
         if (m->gexec[i]) (void)hipGraphExecDestroy(m->gexec[i]);
+        for (int k = 0; k < 4; k++)
+            if (m->gsplit[i][k]) (void)hipGraphExecDestroy(m->gsplit[i][k]);
+    }
     for (int i = 0; i < 4; i++)
         if (m->ev[i]) (void)hipEventDestroy(m->ev[i]);
     for (int i = 0; i < 4; i++)
@@ -1936,9 +2059,9 @@ int otfwi_loop_modes(otfwi_model *m, int n_shots, int *out)
     const int ns = pair_split(m, n_shots);
     const int fs = f == 2 ? ns : 1, as = a == 2 ? ns : 1;
     out[0] = fs;
-    out[1] = m->use_graph && fs == 1 ? 1 : 0;
+    out[1] = m->use_graph ? 1 : 0;
     out[2] = as;
-    out[3] = m->use_graph && as == 1 ? 1 : 0;
+    out[3] = m->use_graph ? 1 : 0;
     return OTFWI_OK;
 }
 

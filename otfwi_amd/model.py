@@ -159,8 +159,9 @@ class PMLModel:
                 "adj": "graph" if out[3] else "direct", "adj_streams": int(out[2])}
 
     def set_pair_streams(self, k=0):
-        """Streams of the two-step loops: 1, 2, 4 or 0 (by size); a loop that
-        splits is launched directly, also with graphs on (set_graph)."""
+        """Streams of the two-step loops: 1, 2, 4 or 0 (by size); with graphs
+        on (set_graph) each stream group's two-step launches are captured and
+        replayed on that group's stream."""
         check(self._lib.otfwi_set_pair_streams(self._h, int(k)))
 
     def set_pair_width(self, nw=0):

@@ -1,8 +1,9 @@
 """The 64-shot 1024^2 full-matrix-capture launch shape against the oracle.
 
 All 64 shots are resident, so the library runs the two-step kernels split
-over two streams (32 shots each) with direct launches: the configuration
-the 64-shot W2 gradient benchmark measures.  T is truncated so the oracle
+over two streams (32 shots each), each half's two-step launches replayed
+from its own captured hipGraph: the configuration the 64-shot W2 gradient
+benchmark measures.  T is truncated so the oracle
 finishes in seconds; the true model has one shallow blob so that the
 residuals are nonzero within T.
 
@@ -40,7 +41,7 @@ def _split_shape(m, n):
           flush=True)
     assert m.steps_per_launch(n) == (2, 2)
     assert modes["fwd_streams"] == 2 and modes["adj_streams"] == 2
-    assert modes["fwd"] == "direct" and modes["adj"] == "direct"
+    assert modes["fwd"] == "graph" and modes["adj"] == "graph"
 
 
 ARRAY_ROW = 20
