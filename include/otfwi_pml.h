@@ -287,6 +287,22 @@ int otfwi_gradient_multi(int n_models, otfwi_model *const *models, const int *n_
                          const int *const *shot_src, const int *misfit_kind,
                          double *const *misfit_out, float *const *grad_out, int n_workers);
 
+/* Gradients of several independent FP32 models (uneven grids, T and shot
+ * counts) with every two-step launch covering all of them: per pair index
+ * one launch whose work-groups look up their model in a device table, over
+ * the models whose loops still have that pair; the forward and the adjoint
+ * two-step phases are each captured into one hipGraph on the first model's
+ * stream, and each model's single steps, misfit and reductions run on its
+ * own stream, joined with events.  Arguments as otfwi_gradient_multi.  Every
+ * model must run two steps per launch of 256 threads for its batch (the
+ * default at a few work-groups per CU and more), hold its shots in one
+ * batch and collect no illumination; otherwise OTFWI_E_ARG names the model.
+ * Results equal those of separate otfwi_gradient calls.  No reference
+ * counterpart (one model per process, regression_PML_single.cu:21-31). */
+int otfwi_gradient_fused(int n_models, otfwi_model *const *models, const int *n_shots,
+                         const int *const *shot_src, const int *misfit_kind,
+                         double *const *misfit_out, float *const *grad_out);
+
 /* Two-stage gradient for a host-side adjoint source (an arbitrary adj_fun,
  * cuda_interface_pml.py:238-249): otfwi_forward_for_grad runs the forward
  * with ring save and returns the traces; otfwi_adjoint_from_host takes the

@@ -720,14 +720,20 @@ unsigned long long loop_key(const otfwi_model *m, int n, int th, int pw, int t0,
     return k;
 }
 
+// Part of a time loop to enqueue: the whole loop, or (two-step loops only,
+// for the fused multi-model gradient) the single steps before the two-step
+// phase / after it; the two-step phase itself is then launched by the caller.
+enum LoopPart { LOOP_ALL = 0, LOOP_LEAD = 1, LOOP_TRAIL = 2 };
 template <typename R>
-int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum = false);
+int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum = false,
+             LoopPart part = LOOP_ALL);
 template <typename R>
-int adj_loop(otfwi_model *m, int n, int t0, int t1);
+int adj_loop(otfwi_model *m, int n, int t0, int t1, LoopPart part = LOOP_ALL);
 
 template <typename R>
-int run_forward(otfwi_model *m, int n, bool save_ring, bool illum = false)
+int run_forward(otfwi_model *m, int n, bool save_ring, bool illum = false, LoopPart part = LOOP_ALL)
 {
+    if (part == LOOP_TRAIL) return fwd_loop<R>(m, n, save_ring, 1, m->T, illum, part);
     Batch &b = m->b;
     const size_t es = sizeof(R);
     const size_t plane = (size_t)m->zp * m->ld;
@@ -745,7 +751,7 @@ int run_forward(otfwi_model *m, int n, bool save_ring, bool illum = false)
             HIPCK(hipMemsetAsync((char *)b.ring + (size_t)s * m->T * m->n_ring * es, 0,
                                  (size_t)m->n_ring * es, st));
     }
-    return fwd_loop<R>(m, n, save_ring, 1, m->T, illum);
+    return fwd_loop<R>(m, n, save_ring, 1, m->T, illum, part);
 }
 
 // Launches of forward steps t0 .. t1-1 on the current batch (no memsets).
@@ -836,8 +842,68 @@ int ensure_lap(otfwi_model *m)
     return OTFWI_OK;
 }
 
+// Two-step schedule of forward steps t0 .. t1-1: nlead single steps, then
+// npairs (even) two-step launches (each writes the slot it does not read, so
+// an even count leaves state t in slot t&1 again), then single steps -- the
+// last step is always single, so A(t1-2) stays in slot (t1-2)&1 for the
+// adjoint's reconstruction.
+inline void fwd_schedule(int t0, int t1, int &nlead, int &npairs)
+{
+    const int S = t1 - t0;
+    npairs = S > 1 ? (S - 1) / 2 : 0;
+    npairs -= npairs & 1;
+    nlead = (S - 1) - 2 * npairs;
+}
+
+// Two-step schedule of adjoint steps t0 .. t1-1: nlead single steps, then
+// npairs (even) pairs of launches, so the slots end where one step per launch
+// leaves them.
+inline void adj_schedule(int t0, int t1, int &nlead, int &npairs)
+{
+    const int S = t1 - t0;
+    npairs = (S / 2) & ~1;
+    nlead = S - 2 * npairs;
+}
+
+// Slot pointers of forward pair k starting at step t: each pair writes the
+// slot it does not read, so after k pairs the state sits in the other slot
+// than one step per launch leaves it.
 template <typename R>
-int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum)
+void fwd_pair_slots(const otfwi_model *m, Pair2Args<R> &q, int k, int t)
+{
+    const Batch &b = m->b;
+    const int in = ((t - 1) & 1) ^ (k & 1), out = in ^ 1;
+    q.t = t;
+    q.P0 = (const R *)b.P[in]; q.P1 = (R *)b.P[out];
+    q.Ax0 = (const R *)b.Ax[in]; q.Ax1 = (R *)b.Ax[out];
+    q.Az0 = (const R *)b.Az[in]; q.Az1 = (R *)b.Az[out];
+    q.Px0 = (const R *)b.Px[in]; q.Px1 = (R *)b.Px[out];
+}
+
+// Slot pointers of adjoint pair k starting at step t, reconstruction (u) and
+// lambda (l) launches: each pair writes the slots it does not read.  The
+// reconstructed field U(t) lives in P slot (T-1+t)%2 and A_b(t) in slot
+// (T-2+t)%2, starting from the forward's P(T-1) and A(T-2)
+// (regression_PML_single.cu:787-790).
+template <typename R>
+void adj_pair_slots(const otfwi_model *m, Pair2Args<R> &u, Pair2Args<R> &l, int k, int t)
+{
+    const Batch &b = m->b;
+    const int T = m->T;
+    const int f = k & 1;
+    const int li = ((t - 1) & 1) ^ f, ui = ((T - 2 + t) & 1) ^ f, ai = ((T - 1 + t) & 1) ^ f;
+    u.t = l.t = t;
+    u.P0 = (const R *)b.P[ui]; u.P1 = (R *)b.P[ui ^ 1];
+    u.Ax0 = (const R *)b.Ax[ai]; u.Ax1 = (R *)b.Ax[ai ^ 1];
+    u.Az0 = (const R *)b.Az[ai]; u.Az1 = (R *)b.Az[ai ^ 1];
+    l.P0 = (const R *)b.LP[li]; l.P1 = (R *)b.LP[li ^ 1];
+    l.Ax0 = (const R *)b.LAx[li]; l.Ax1 = (R *)b.LAx[li ^ 1];
+    l.Az0 = (const R *)b.LAz[li]; l.Az1 = (R *)b.LAz[li ^ 1];
+    l.Px0 = (const R *)b.LPx[li]; l.Px1 = (R *)b.LPx[li ^ 1];
+}
+
+template <typename R>
+int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum, LoopPart part)
 {
     Batch &b = m->b;
     const size_t plane = (size_t)m->zp * m->ld;
@@ -848,17 +914,11 @@ int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum)
     const int ist = (illum && m->ill) ? m->ill_stride : 0;
     const int K = std::max(1, m->tb_steps[0]);
     const int G = m->tb_shots[0] > 0 ? std::min(m->tb_shots[0], n) : n;
-    // two steps per launch: leading single steps, then an even number of
-    // two-step launches (each writes the slot it does not read, so an even
-    // count leaves state t in slot t&1 again), then single steps -- the last
-    // step is always single, so A(t1-2) stays in slot (t1-2)&1 for the
-    // adjoint's reconstruction
+    // two steps per launch, scheduled by fwd_schedule
     const bool two = use_pairs(m, n, false) && ist == 0 && K == 1 && G == n && m->evrow &&
                      pair2_supported((long long)plane, (long long)b.cap * (long long)plane, (int)sizeof(R));
-    const int S2 = t1 - t0;
-    int npairs = S2 > 1 ? (S2 - 1) / 2 : 0;
-    npairs -= npairs & 1;
-    const int nlead = (S2 - 1) - 2 * npairs;
+    int nlead, npairs;
+    fwd_schedule(t0, t1, nlead, npairs);
     auto single = [&](int tt) -> int {
         const int p = (tt - 1) & 1, c = tt & 1;
         a.t = tt;
@@ -877,17 +937,7 @@ int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum)
         for (int t = t0 + std::max(nlead, 0) + 2 * npairs; t < t1; t++) RCK(single(t));
         return OTFWI_OK;
     };
-    // slot pointers of pair k (starting at step t): each pair writes the slot
-    // it does not read, so after k pairs the state sits in the other slot
-    // than one step per launch leaves it
-    auto pair_slots = [&](Pair2Args<R> &q, int k, int t) {
-        const int in = ((t - 1) & 1) ^ (k & 1), out = in ^ 1;
-        q.t = t;
-        q.P0 = (const R *)b.P[in]; q.P1 = (R *)b.P[out];
-        q.Ax0 = (const R *)b.Ax[in]; q.Ax1 = (R *)b.Ax[out];
-        q.Az0 = (const R *)b.Az[in]; q.Az1 = (R *)b.Az[out];
-        q.Px0 = (const R *)b.Px[in]; q.Px1 = (R *)b.Px[out];
-    };
+    auto pair_slots = [&](Pair2Args<R> &q, int k, int t) { fwd_pair_slots<R>(m, q, k, t); };
     // all pairs of shot group g of ns on stream s (captured by run_split)
     auto group_fn = [&](int g, int ns, hipStream_t s) -> int {
         int g0, g1;
@@ -948,6 +998,10 @@ int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum)
             3ULL + (unsigned long long)pair_split(m, n);
     const unsigned long long key =
         (loop_key(m, n, m->th_fwd, pw, t0, t1) * 31ULL + (unsigned long long)ist) * 1000003ULL + kb;
+    if (part != LOOP_ALL) {
+        if (!two) return fail(OTFWI_E_STATE, "forward loop runs one step per launch");
+        return part == LOOP_LEAD ? lead_fn() : trail_fn();
+    }
     const int ns = two && npairs > 0 ? pair_split(m, n) : 1;
     if (ns > 1 && m->use_graph)
         return run_split(m, save_ring ? 1 : 0, key, n, ns, lead_fn,
@@ -956,7 +1010,7 @@ int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum)
 }
 
 template <typename R>
-int run_adjoint(otfwi_model *m, int n)
+int run_adjoint(otfwi_model *m, int n, LoopPart part = LOOP_ALL)
 {
     Batch &b = m->b;
     const size_t es = sizeof(R);
@@ -969,7 +1023,7 @@ int run_adjoint(otfwi_model *m, int n)
         HIPCK(hipMemsetAsync(b.LPx[i], 0, plane * n * es, st));
     }
     HIPCK(hipMemsetAsync(b.grad, 0, plane * n * es, st));
-    return adj_loop<R>(m, n, 1, m->T);
+    return adj_loop<R>(m, n, 1, m->T, part);
 }
 
 // Launches of adjoint steps t0 .. t1-1 on the current batch (no memsets).
@@ -995,7 +1049,7 @@ AdjArgs<R> adj_args(otfwi_model *m, int n)
 }
 
 template <typename R>
-int adj_loop(otfwi_model *m, int n, int t0, int t1)
+int adj_loop(otfwi_model *m, int n, int t0, int t1, LoopPart part)
 {
     Batch &b = m->b;
     const size_t plane = (size_t)m->zp * m->ld;
@@ -1028,26 +1082,14 @@ int adj_loop(otfwi_model *m, int n, int t0, int t1)
         HIPCK(launch_adj_step<R>(a, m->th_adj, pw, st));
         return OTFWI_OK;
     };
-    const int S2 = t1 - t0;
-    const int npairs = (S2 / 2) & ~1;
-    const int nlead = S2 - 2 * npairs;
+    int nlead, npairs;
+    adj_schedule(t0, t1, nlead, npairs);
     auto lead_fn = [&]() -> int {
         for (int t = t0; t < t0 + nlead; t++) RCK(single(t));
         return OTFWI_OK;
     };
-    // slot pointers of pair k (starting at step t) of the reconstruction (u)
-    // and lambda (l) launches: each pair writes the slots it does not read
     auto pair_slots = [&](Pair2Args<R> &u, Pair2Args<R> &l, int k, int t) {
-        const int f = k & 1;
-        const int li = ((t - 1) & 1) ^ f, ui = ((T - 2 + t) & 1) ^ f, ai = ((T - 1 + t) & 1) ^ f;
-        u.t = l.t = t;
-        u.P0 = (const R *)b.P[ui]; u.P1 = (R *)b.P[ui ^ 1];
-        u.Ax0 = (const R *)b.Ax[ai]; u.Ax1 = (R *)b.Ax[ai ^ 1];
-        u.Az0 = (const R *)b.Az[ai]; u.Az1 = (R *)b.Az[ai ^ 1];
-        l.P0 = (const R *)b.LP[li]; l.P1 = (R *)b.LP[li ^ 1];
-        l.Ax0 = (const R *)b.LAx[li]; l.Ax1 = (R *)b.LAx[li ^ 1];
-        l.Az0 = (const R *)b.LAz[li]; l.Az1 = (R *)b.LAz[li ^ 1];
-        l.Px0 = (const R *)b.LPx[li]; l.Px1 = (R *)b.LPx[li ^ 1];
+        adj_pair_slots<R>(m, u, l, k, t);
     };
     // all pairs of shot group g of ns on stream s (captured by run_split)
     auto group_fn = [&](int g, int ns, hipStream_t s) -> int {
@@ -1114,6 +1156,10 @@ int adj_loop(otfwi_model *m, int n, int t0, int t1)
         (((unsigned long long)K * 4099ULL + (unsigned long long)G) * 3ULL + (two ? 1ULL : 0ULL)) *
             3ULL + (unsigned long long)pair_split(m, n);
     const unsigned long long key = loop_key(m, n, m->th_adj, pw, t0, t1) * 1000003ULL + kb;
+    if (part != LOOP_ALL) {
+        if (!two) return fail(OTFWI_E_STATE, "adjoint loop runs one step per launch");
+        return part == LOOP_LEAD ? lead_fn() : OTFWI_OK;
+    }
     const int ns = two && npairs > 0 ? pair_split(m, n) : 1;
     if (ns > 1 && m->use_graph)
         return run_split(m, 2, key, n, ns, lead_fn,
@@ -1235,26 +1281,49 @@ int ensure_hilb(otfwi_model *m)
 // One batch of a gradient: forward with ring save, misfit + adjoint source
 // (device kind) or host adjoint source already in b.adj (kind < 0), adjoint
 // run, and accumulation of the batch's per-shot gradients into m->gacc.
+// Misfit of the batch's traces against its observed traces on the device:
+// per-trace values, adjoint sources and their running sums, per-shot sums.
 template <typename R>
-int grad_batch_device(otfwi_model *m, int s0, int nb, const int *shot_src, int kind,
-                      double *mis_out, float *rec_out)
+int batch_misfit(otfwi_model *m, int nb, int kind)
 {
     Batch &b = m->b;
-    RCK(upload_shot_src(m, nb, shot_src ? shot_src + s0 : nullptr));
-    RCK(upload_obs_batch<R>(m, s0, nb));
-    HIPCK(hipEventRecord(m->ev[0], m->stream));
-    RCK(run_forward<R>(m, nb, true, true));
-    HIPCK(hipEventRecord(m->ev[1], m->stream));
-    if (rec_out) RCK(copy_rec_out<R>(m, nb, rec_out + (size_t)s0 * m->n_rx * m->T));
     const int ntr = nb * m->n_rx;
     int *flags = misfit_flags(kind, b.scratch, ntr, m->T);
     if (w2_family(kind)) HIPCK(hipMemsetAsync(flags, 0, ntr * sizeof(int), m->stream));
     HIPCK((launch_misfit<R, float>(kind, (const R *)b.rec, b.obs, (R *)b.adj, (R *)b.jcum,
                                    b.mis_tr, ntr, m->T, b.scratch, m->stream, m->hilb)));
     HIPCK(launch_reduce_misfit(b.mis_tr, nb, m->n_rx, b.mis_shot, m->stream));
+    return OTFWI_OK;
+}
+
+template <typename R>
+int grad_batch_finish(otfwi_model *m, int s0, int nb, int kind, double *mis_out);
+
+template <typename R>
+int grad_batch_device(otfwi_model *m, int s0, int nb, const int *shot_src, int kind,
+                      double *mis_out, float *rec_out)
+{
+    RCK(upload_shot_src(m, nb, shot_src ? shot_src + s0 : nullptr));
+    RCK(upload_obs_batch<R>(m, s0, nb));
+    HIPCK(hipEventRecord(m->ev[0], m->stream));
+    RCK(run_forward<R>(m, nb, true, true));
+    HIPCK(hipEventRecord(m->ev[1], m->stream));
+    if (rec_out) RCK(copy_rec_out<R>(m, nb, rec_out + (size_t)s0 * m->n_rx * m->T));
+    RCK(batch_misfit<R>(m, nb, kind));
     HIPCK(hipEventRecord(m->ev[2], m->stream));
     RCK(run_adjoint<R>(m, nb));
     HIPCK(hipEventRecord(m->ev[3], m->stream));
+    return grad_batch_finish<R>(m, s0, nb, kind, mis_out);
+}
+
+// After a batch's adjoint: its gradients added to the FP64 sum, per-shot
+// misfits and W2 flags read back, phase times taken from ev[0..3].
+template <typename R>
+int grad_batch_finish(otfwi_model *m, int s0, int nb, int kind, double *mis_out)
+{
+    Batch &b = m->b;
+    const int ntr = nb * m->n_rx;
+    int *flags = misfit_flags(kind, b.scratch, ntr, m->T);
     const size_t plane = (size_t)m->zp * m->ld;
     HIPCK(launch_accum_grad<R>((const R *)b.grad, nb, m->X, m->Z, m->ld, (long long)plane, m->gacc,
                                m->stream));
@@ -1302,9 +1371,8 @@ int ensure_gacc(otfwi_model *m)
     return OTFWI_OK;
 }
 
-template <typename R>
-int gradient_impl(otfwi_model *m, int n, const int *shot_src, int kind, double *mis_out,
-                  float *grad_out, bool out_dev, float *rec_out)
+// Checks and device state of a gradient call over n shots (batches of *B).
+int grad_prepare(otfwi_model *m, int n, int kind, int *B)
 {
     if (!misfit_kind_ok(kind)) return fail(OTFWI_E_ARG, "unknown misfit kind %d", kind);
     if (kind == OTFWI_MISFIT_ENV) RCK(ensure_hilb(m));
@@ -1312,14 +1380,18 @@ int gradient_impl(otfwi_model *m, int n, const int *shot_src, int kind, double *
     if (m->n_rx <= 0) return fail(OTFWI_E_STATE, "receivers not set");
     if (w2_family(kind) && m->T < 2) return fail(OTFWI_E_ARG, "W2 needs T >= 2");
     RCK(ready(m, true));
-    const int B = batch_size(m, n);
-    RCK(ensure_batch(m, B, true));
+    *B = batch_size(m, n);
+    RCK(ensure_batch(m, *B, true));
     RCK(ensure_gacc(m));
     RCK(ensure_ill(m));
     m->t_fwd_ms = m->t_mis_ms = m->t_adj_ms = 0;
     m->n_fwd_launch = m->n_adj_launch = 0;
-    for (int s0 = 0; s0 < n; s0 += B)
-        RCK(grad_batch_device<R>(m, s0, std::min(B, n - s0), shot_src, kind, mis_out, rec_out));
+    return OTFWI_OK;
+}
+
+// The mean gradient over n shots from the FP64 sum, to grad_out (host or device).
+int grad_complete(otfwi_model *m, int n, float *grad_out, bool out_dev)
+{
     const int cells = m->X * m->Z;
     HIPCK(launch_finalize_grad(m->gacc, cells, 1.0 / n, m->gout, m->stream));
     if (grad_out) {
@@ -1329,6 +1401,219 @@ int gradient_impl(otfwi_model *m, int n, const int *shot_src, int kind, double *
     HIPCK(hipStreamSynchronize(m->stream));
     m->ill_valid = m->ill_stride > 0;
     return OTFWI_OK;
+}
+
+template <typename R>
+int gradient_impl(otfwi_model *m, int n, const int *shot_src, int kind, double *mis_out,
+                  float *grad_out, bool out_dev, float *rec_out)
+{
+    int B = 0;
+    RCK(grad_prepare(m, n, kind, &B));
+    for (int s0 = 0; s0 < n; s0 += B)
+        RCK(grad_batch_device<R>(m, s0, std::min(B, n - s0), shot_src, kind, mis_out, rec_out));
+    return grad_complete(m, n, grad_out, out_dev);
+}
+
+// Device copy of the two-step arguments of a fused loop: for launch k,
+// tab[k * nm ..] the problems it covers (in model order) and wg[k * (nm + 1) ..]
+// the prefix sums of their work-group counts.
+struct FusedTables {
+    int nm = 0, K = 0;
+    std::vector<Pair2Args<float>> tab[2];   // [0] forward or reconstruction, [1] lambda
+    std::vector<int> wg, cnt;
+    std::vector<long long> nwg;
+};
+
+// One captured graph on `st` holding the fused two-step launches of a
+// phase (modes[0] then, if given, modes[1] per pair index), replayed once.
+int run_fused_phase(hipStream_t st, const FusedTables &ft, char *dev, int nmodes, const int *modes)
+{
+    const size_t tb = (size_t)ft.K * ft.nm * sizeof(Pair2Args<float>);
+    const Pair2Args<float> *dt[2] = {(const Pair2Args<float> *)dev,
+                                     (const Pair2Args<float> *)(dev + tb)};
+    const int *dwg = (const int *)(dev + 2 * tb);
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIPCK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < ft.K && e == hipSuccess; k++)
+        for (int j = 0; j < nmodes && e == hipSuccess; j++)
+            e = launch_pair2_multi(dt[j] + (size_t)k * ft.nm, dwg + (size_t)k * (ft.nm + 1),
+                                   ft.cnt[k], ft.nwg[k], modes[j], st);
+    const hipError_t ec = hipStreamEndCapture(st, &graph);
+    HIPCK(e);
+    HIPCK(ec);
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCK(e);
+    e = hipGraphLaunch(exec, st);
+    const hipError_t es = hipStreamSynchronize(st);
+    (void)hipGraphExecDestroy(exec);
+    HIPCK(e);
+    HIPCK(es);
+    return OTFWI_OK;
+}
+
+// Gradients of several FP32 models with every two-step launch covering all of
+// them: per pair index one pair2_multi launch over the models whose loops
+// still have that pair (uneven grids, shot counts and step counts), the
+// launches of a phase captured into one graph on the first model's stream.
+// Each model's single steps, misfit and reductions run on its own stream,
+// joined to the fused phases with events.  Results are bitwise those of
+// separate gradient calls (the same kernels, cells and steps).
+int fused_gradient_impl(int nm, otfwi_model *const *ms, const int *ns, const int *const *shot_src,
+                        const int *kinds, double *const *mis_out, float *const *grad_out)
+{
+    using R = float;
+    for (int i = 0; i < nm; i++) {
+        otfwi_model *m = ms[i];
+        const int n = ns[i];
+        if (m->prec != 4) return fail(OTFWI_E_ARG, "fused gradient: model %d is not FP32", i);
+        int B = 0;
+        RCK(grad_prepare(m, n, kinds[i], &B));
+        if (B != n)
+            return fail(OTFWI_E_ARG, "fused gradient: model %d's %d shots exceed its batch limit %d",
+                        i, n, B);
+        if (m->ill_stride > 0)
+            return fail(OTFWI_E_ARG, "fused gradient: model %d collects illumination (one-step "
+                        "forward loops); switch it off", i);
+        // every model runs two steps per launch whatever its own launch size
+        // (the fused launches are the sum of the models'), unless one step per
+        // launch was asked for
+        const size_t plane = (size_t)m->zp * m->ld;
+        const bool pairs_ok = m->evrow && m->ringtab && m->fwd_steps != 1 && m->adj_steps != 1 &&
+                              m->tb_steps[0] <= 1 && m->tb_steps[1] <= 1 &&
+                              m->tb_shots[0] <= 0 && m->tb_shots[1] <= 0 &&
+                              pair2_supported((long long)plane, (long long)m->b.cap * (long long)plane,
+                                              (int)sizeof(R));
+        if (!pairs_ok || m->pair_nw_built != F2_MULTI_NW)
+            return fail(OTFWI_E_ARG, "fused gradient: model %d cannot run two steps per launch "
+                        "of %d threads (grid, batch or settings); otfwi_gradient_multi runs it",
+                        i, F2_MULTI_NW);
+        RCK(ensure_lap<R>(m));
+        RCK(upload_shot_src(m, n, shot_src ? shot_src[i] : nullptr));
+        RCK(upload_obs_batch<R>(m, 0, n));
+    }
+    // argument tables of both phases
+    FusedTables tf, ta;
+    tf.nm = ta.nm = nm;
+    std::vector<int> nlf(nm), npf(nm), nla(nm), npa(nm);
+    for (int i = 0; i < nm; i++) {
+        fwd_schedule(1, ms[i]->T, nlf[i], npf[i]);
+        adj_schedule(1, ms[i]->T, nla[i], npa[i]);
+        tf.K = std::max(tf.K, npf[i]);
+        ta.K = std::max(ta.K, npa[i]);
+    }
+    for (FusedTables *t : {&tf, &ta}) {
+        const bool fwd = t == &tf;
+        t->tab[0].assign((size_t)t->K * nm, Pair2Args<float>{});
+        t->tab[1].assign((size_t)t->K * nm, Pair2Args<float>{});
+        t->wg.assign((size_t)t->K * (nm + 1), 0);
+        t->cnt.assign(t->K, 0);
+        t->nwg.assign(t->K, 0);
+        for (int k = 0; k < t->K; k++) {
+            int c = 0;
+            long long w = 0;
+            for (int i = 0; i < nm; i++) {
+                otfwi_model *m = ms[i];
+                const int n = ns[i];
+                if (k >= (fwd ? npf[i] : npa[i])) continue;
+                Pair2Args<R> q, u, l;
+                if (fwd) {
+                    q = pair_fwd_args<R>(m, n, fwd_args<R>(m, n));
+                    fwd_pair_slots<R>(m, q, k, 1 + nlf[i] + 2 * k);
+                    if (!pair2_args_ok(q, 0))
+                        return fail(OTFWI_E_STATE, "fused gradient: model %d forward arguments", i);
+                    t->tab[0][(size_t)k * nm + c] = q;
+                } else {
+                    u = pair_rec_args<R>(m, n);
+                    l = pair_lam_args<R>(m, n);
+                    adj_pair_slots<R>(m, u, l, k, 1 + nla[i] + 2 * k);
+                    if (!pair2_args_ok(u, 2) || !pair2_args_ok(l, 1))
+                        return fail(OTFWI_E_STATE, "fused gradient: model %d adjoint arguments", i);
+                    t->tab[0][(size_t)k * nm + c] = u;
+                    t->tab[1][(size_t)k * nm + c] = l;
+                }
+                t->wg[(size_t)k * (nm + 1) + c] = (int)w;
+                w += pair2_groups(m->X, m->Z, F2_MULTI_NW, n);
+                if (w > 0x7FFFFFFFLL)
+                    return fail(OTFWI_E_ARG, "fused gradient: too many work-groups per launch");
+                c++;
+            }
+            t->wg[(size_t)k * (nm + 1) + c] = (int)w;
+            t->cnt[k] = c;
+            t->nwg[k] = w;
+        }
+    }
+    // one device allocation per phase: both argument tables, then the prefix table
+    char *dev[2] = {nullptr, nullptr};
+    int rc = OTFWI_OK;
+    hipStream_t main = ms[0]->stream;
+    auto upload = [&](const FusedTables &t, char **d) -> int {
+        const size_t tb = (size_t)t.K * nm * sizeof(Pair2Args<float>);
+        const size_t bytes = 2 * tb + t.wg.size() * sizeof(int);
+        if (bytes == 0) return OTFWI_OK;
+        RCK(dalloc(d, bytes));
+        HIPCK(hipMemcpyAsync(*d, t.tab[0].data(), tb, hipMemcpyHostToDevice, main));
+        HIPCK(hipMemcpyAsync(*d + tb, t.tab[1].data(), tb, hipMemcpyHostToDevice, main));
+        HIPCK(hipMemcpyAsync(*d + 2 * tb, t.wg.data(), t.wg.size() * sizeof(int),
+                             hipMemcpyHostToDevice, main));
+        HIPCK(hipStreamSynchronize(main));
+        return OTFWI_OK;
+    };
+    auto body = [&]() -> int {
+        RCK(upload(tf, &dev[0]));
+        RCK(upload(ta, &dev[1]));
+        // forward: memsets and leading single steps on each model's stream
+        for (int i = 0; i < nm; i++) {
+            otfwi_model *m = ms[i];
+            HIPCK(hipEventRecord(m->ev[0], m->stream));
+            RCK(run_forward<R>(m, ns[i], true, false, LOOP_LEAD));
+            HIPCK(hipEventRecord(m->evfj[1], m->stream));
+            HIPCK(hipStreamWaitEvent(main, m->evfj[1], 0));
+        }
+        const int mf[1] = {0}, ma[2] = {2, 1};
+        if (tf.K > 0) RCK(run_fused_phase(main, tf, dev[0], 1, mf));
+        HIPCK(hipEventRecord(ms[0]->evfj[2], main));
+        // last forward step, misfit, adjoint memsets and leading steps per model
+        for (int i = 0; i < nm; i++) {
+            otfwi_model *m = ms[i];
+            HIPCK(hipStreamWaitEvent(m->stream, ms[0]->evfj[2], 0));
+            RCK(run_forward<R>(m, ns[i], true, false, LOOP_TRAIL));
+            HIPCK(hipEventRecord(m->ev[1], m->stream));
+            RCK(batch_misfit<R>(m, ns[i], kinds[i]));
+            HIPCK(hipEventRecord(m->ev[2], m->stream));
+            RCK(run_adjoint<R>(m, ns[i], LOOP_LEAD));
+            HIPCK(hipEventRecord(m->evfj[1], m->stream));
+            HIPCK(hipStreamWaitEvent(main, m->evfj[1], 0));
+        }
+        if (ta.K > 0) RCK(run_fused_phase(main, ta, dev[1], 2, ma));
+        HIPCK(hipEventRecord(ms[0]->evfj[2], main));
+        for (int i = 0; i < nm; i++) {
+            otfwi_model *m = ms[i];
+            HIPCK(hipStreamWaitEvent(m->stream, ms[0]->evfj[2], 0));
+            HIPCK(hipEventRecord(m->ev[3], m->stream));
+            RCK(grad_batch_finish<R>(m, 0, ns[i], kinds[i], mis_out ? mis_out[i] : nullptr));
+            RCK(grad_complete(m, ns[i], grad_out ? grad_out[i] : nullptr, false));
+        }
+        return OTFWI_OK;
+    };
+    std::vector<int> steps_set(2 * nm);
+    for (int i = 0; i < nm; i++) {
+        steps_set[2 * i] = ms[i]->fwd_steps;
+        steps_set[2 * i + 1] = ms[i]->adj_steps;
+        ms[i]->fwd_steps = ms[i]->adj_steps = 2;
+    }
+    rc = body();
+    for (int i = 0; i < nm; i++) {
+        ms[i]->fwd_steps = steps_set[2 * i];
+        ms[i]->adj_steps = steps_set[2 * i + 1];
+    }
+    // every stream idle before the tables are released
+    for (int i = 0; i < nm; i++) (void)hipStreamSynchronize(ms[i]->stream);
+    dfree(dev[0]);
+    dfree(dev[1]);
+    return rc;
 }
 
 // Misfit of the current model without the adjoint: forward runs of the shots
@@ -2227,6 +2512,22 @@ int otfwi_gradient_multi(int n_models, otfwi_model *const *models, const int *n_
     for (int i = 0; i < n_models; i++)
         if (rc[i] != OTFWI_OK) return fail(rc[i], "model %d: %s", i, msg[i].c_str());
     return OTFWI_OK;
+}
+
+int otfwi_gradient_fused(int n_models, otfwi_model *const *models, const int *n_shots,
+                         const int *const *shot_src, const int *misfit_kind,
+                         double *const *misfit_out, float *const *grad_out)
+{
+    if (n_models <= 0 || !models || !n_shots || !misfit_kind)
+        return fail(OTFWI_E_ARG, "gradient_fused: bad arguments");
+    for (int i = 0; i < n_models; i++) {
+        RCK(check_model(models[i]));
+        for (int j = 0; j < i; j++)
+            if (models[j] == models[i])
+                return fail(OTFWI_E_ARG, "gradient_fused: model %d is listed twice", i);
+    }
+    return fused_gradient_impl(n_models, models, n_shots, shot_src, misfit_kind, misfit_out,
+                               grad_out);
 }
 
 int otfwi_forward_for_grad(otfwi_model *m, int n_shots, const int *shot_src, float *rec_out)

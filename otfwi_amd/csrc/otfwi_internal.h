@@ -171,6 +171,16 @@ bool pair2_supported(long long sstride, long long slot_elems, int prec);
 long long pair2_groups(int X, int Z, int nw, int nshots);
 template <typename R>
 hipError_t launch_pair2(const Pair2Args<R> &a, int mode, hipStream_t s);
+// Host-side checks launch_pair2 applies to one problem's arguments.
+template <typename R>
+bool pair2_args_ok(const Pair2Args<R> &a, int mode);
+// One FP32 two-step launch of F2_MULTI_NW-thread work-groups over nprob
+// problems: tab[nprob] their arguments (each passing pair2_args_ok, with
+// nw == F2_MULTI_NW), wg0[nprob + 1] the prefix sums of their work-group
+// counts (pair2_groups), nwg = wg0[nprob]; both tables in device memory.
+constexpr int F2_MULTI_NW = 256;
+hipError_t launch_pair2_multi(const Pair2Args<float> *tab, const int *wg0, int nprob, long long nwg,
+                              int mode, hipStream_t s);
 // Resident work-groups per CU of a step kernel (0 if the variant does not exist).
 int step_occupancy(bool adj, int prec, int th, bool persistent);
 

@@ -1003,6 +1003,8 @@ hipError_t launch_adj_step<double>(const AdjArgs<double> &a, int th, int pw, hip
 // In the forward kernel the iterations whose rows all lie inside the chunk
 // and away from the plane's edges run a copy of the body without row checks.
 constexpr int F2_FWD = 0, F2_LAM = 1, F2_REC = 2;
+// constant address space: uniform loads through it are scalar and invariant
+#define F2_CONST __attribute__((address_space(4)))
 // Rows of global loads in flight ahead of the row being computed, per mode
 // (measured at 64 x 1024^2: the reconstruction kernel gains 8 % at 4, the
 // forward kernel loses 90 % there and the lambda kernel nothing).
@@ -1451,8 +1453,23 @@ __device__ __forceinline__ void f2_iter(const Pair2Args<R> &a, const F2Ctx &c,
 template <typename R, int MODE, int NW>
 constexpr int f2_waves() { return sizeof(R) == 4 ? 3 : 1; }
 
-template <typename R, int MODE, int NW>
-__global__ void __launch_bounds__(NW, (f2_waves<R, MODE, NW>())) pair2_kernel(const Pair2Args<R> a)
+// Work-group coordinates of a two-step launch (column block x, row chunk y,
+// shot z): the launch grid's own, or given.
+struct F2GridIds {
+    __device__ unsigned x() const { return blockIdx.x; }
+    __device__ unsigned y() const { return blockIdx.y; }
+    __device__ unsigned z() const { return blockIdx.z; }
+};
+struct F2GivenIds {
+    unsigned bx, by, bz;
+    __device__ unsigned x() const { return bx; }
+    __device__ unsigned y() const { return by; }
+    __device__ unsigned z() const { return bz; }
+};
+
+// One work-group of a two-step launch of the problem described by a.
+template <typename R, int MODE, int NW, typename Ids>
+__device__ __forceinline__ void pair2_body(const Pair2Args<R> a, const Ids id)
 {
     using G = F2Geo<NW>;
     constexpr int LP = G::LP;
@@ -1465,11 +1482,11 @@ __global__ void __launch_bounds__(NW, (f2_waves<R, MODE, NW>())) pair2_kernel(co
     F2Ctx c;
     c.tid = threadIdx.x;
     c.X = a.X; c.Z = a.Z; c.ld = a.ld; c.t = a.t; c.T = a.T; c.nring = a.n_ring;
-    const int xs = blockIdx.x * G::OWN;
+    const int xs = id.x() * G::OWN;
     c.x = xs - F2_HALO + c.tid;
-    c.zc0 = blockIdx.y * F2_CH;
+    c.zc0 = id.y() * F2_CH;
     c.zc1 = min(c.zc0 + F2_CH, a.Z);
-    c.s = blockIdx.z;
+    c.s = id.z();
     c.ss = MODE == F2_FWD ? a.shot_src[c.s] : -1;
     c.lc = c.tid + 8;
     // columns beyond X + 12 never reach an owned cell: not loaded
@@ -1522,12 +1539,12 @@ __global__ void __launch_bounds__(NW, (f2_waves<R, MODE, NW>())) pair2_kernel(co
     if constexpr (MODE != F2_LAM)
         for (int k = c.tid; k < F2_NR * MW; k += NW) {
             const int z = c.zc0 - F2_RB + k / MW;
-            smask[k] = (z >= 0 && z < a.Z) ? a.mask[((size_t)blockIdx.x * a.Z + z) * MW + k % MW] : 0u;
+            smask[k] = (z >= 0 && z < a.Z) ? a.mask[((size_t)id.x() * a.Z + z) * MW + k % MW] : 0u;
         }
     if constexpr (MODE != F2_REC)
         for (int r = c.tid; r < F2_NR; r += NW) {
             const int z = c.zc0 - F2_RB + r;
-            sev[r] = (z >= 0 && z < a.Z) ? a.evrow[(size_t)blockIdx.x * a.Z + z] : (uint8_t)0;
+            sev[r] = (z >= 0 && z < a.Z) ? a.evrow[(size_t)id.x() * a.Z + z] : (uint8_t)0;
         }
     for (int r = 0; r < 8; r++)
         for (int k = c.tid; k < LP; k += NW) {
@@ -1582,6 +1599,41 @@ __global__ void __launch_bounds__(NW, (f2_waves<R, MODE, NW>())) pair2_kernel(co
     }
 }
 
+template <typename R, int MODE, int NW>
+__global__ void __launch_bounds__(NW, (f2_waves<R, MODE, NW>())) pair2_kernel(const Pair2Args<R> a)
+{
+    pair2_body<R, MODE, NW>(a, F2GridIds{});
+}
+
+// One two-step launch over several independent problems (models of uneven
+// grid size and shot count, each with its own time step): work-group b runs
+// problem i, wg0[i] <= b < wg0[i + 1], at local index l = b - wg0[i] taken as
+// (column block, row chunk, shot) with the column block fastest, as the
+// grid of a one-problem launch orders them.  The tables are read through the
+// constant address space: uniform, so the problem's arguments land in scalar
+// registers as a one-problem launch's kernel arguments do.
+template <typename R, int MODE, int NW>
+__global__ void __launch_bounds__(NW, (f2_waves<R, MODE, NW>()))
+pair2_multi_kernel(const F2_CONST Pair2Args<R> *tab, const F2_CONST int *wg0, const int nprob)
+{
+    const int b = blockIdx.x;
+    int i = 0;
+    while (i + 1 < nprob && b >= wg0[i + 1]) i++;
+    Pair2Args<R> a;
+    {
+        static_assert(sizeof(Pair2Args<R>) % 4 == 0, "argument block of whole words");
+        const F2_CONST unsigned *src = (const F2_CONST unsigned *)(tab + i);
+        unsigned *dst = reinterpret_cast<unsigned *>(&a);
+#pragma unroll
+        for (int k = 0; k < (int)(sizeof(Pair2Args<R>) / 4); k++) dst[k] = src[k];
+    }
+    const int nbx = (a.X + F2Geo<NW>::OWN - 1) / F2Geo<NW>::OWN;
+    const int nby = (a.Z + F2_CH - 1) / F2_CH;
+    const unsigned l = (unsigned)(b - wg0[i]);
+    pair2_body<R, MODE, NW>(a, F2GivenIds{l % (unsigned)nbx, (l / (unsigned)nbx) % (unsigned)nby,
+                                          l / (unsigned)(nbx * nby)});
+}
+
 long long pair2_groups(int X, int Z, int nw, int nshots)
 {
     const int own = f2_own(nw);
@@ -1597,21 +1649,31 @@ bool pair2_supported(long long sstride, long long slot_elems, int prec)
 }
 
 template <typename R>
-hipError_t launch_pair2(const Pair2Args<R> &a, int mode, hipStream_t st)
+bool pair2_args_ok(const Pair2Args<R> &a, int mode)
 {
+    if (mode != F2_FWD && mode != F2_LAM && mode != F2_REC) return false;
     if ((mode != F2_LAM && !a.mask) || (mode != F2_REC && (!a.pmap || !a.evrow)) ||
         !pair2_supported(a.sstride, a.slot_elems, (int)sizeof(R)))
-        return hipErrorInvalidValue;
+        return false;
+    if (a.X <= 0 || a.Z <= 0 || a.nshots <= 0) return false;
     // both slots of each field one allocation: the written slot is slot_elems away
     auto gap = [&](const R *u, const R *v) { return u > v ? u - v : v - u; };
     if (gap(a.P0, a.P1) != a.slot_elems || gap(a.Ax0, a.Ax1) != a.slot_elems ||
         gap(a.Az0, a.Az1) != a.slot_elems || (a.Ax0 < a.Ax1) != (a.Az0 < a.Az1) ||
         (mode != F2_FWD && a.lap1 - a.lap0 != a.slot_elems) ||
         (mode != F2_REC && (gap(a.Px0, a.Px1) != a.slot_elems || (a.P0 < a.P1) != (a.Px0 < a.Px1))))
-        return hipErrorInvalidValue;
-    if (mode != F2_LAM && (size_t)a.T * a.n_ring * sizeof(R) > F2_SENT) return hipErrorInvalidValue;
+        return false;
+    if (mode != F2_LAM && (size_t)a.T * a.n_ring * sizeof(R) > F2_SENT) return false;
+    return f2_own(a.nw) > 0;
+}
+template bool pair2_args_ok<float>(const Pair2Args<float> &, int);
+template bool pair2_args_ok<double>(const Pair2Args<double> &, int);
+
+template <typename R>
+hipError_t launch_pair2(const Pair2Args<R> &a, int mode, hipStream_t st)
+{
+    if (!pair2_args_ok(a, mode)) return hipErrorInvalidValue;
     const int own = f2_own(a.nw);
-    if (own <= 0) return hipErrorInvalidValue;
     dim3 grid((a.X + own - 1) / own, (a.Z + F2_CH - 1) / F2_CH, a.nshots);
     const dim3 blk(a.nw);
 #define F2_LAUNCH(M, W) hipLaunchKernelGGL((pair2_kernel<R, M, W>), grid, blk, 0, st, a)
@@ -1635,6 +1697,28 @@ hipError_t launch_pair2(const Pair2Args<R> &a, int mode, hipStream_t st)
 }
 template hipError_t launch_pair2<float>(const Pair2Args<float> &, int, hipStream_t);
 template hipError_t launch_pair2<double>(const Pair2Args<double> &, int, hipStream_t);
+
+hipError_t launch_pair2_multi(const Pair2Args<float> *tab, const int *wg0, int nprob, long long nwg,
+                              int mode, hipStream_t st)
+{
+    if (!tab || !wg0 || nprob <= 0 || nwg <= 0 || nwg > 0x7FFFFFFFLL) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)nwg), blk(F2_MULTI_NW);
+    const F2_CONST Pair2Args<float> *ct = (const F2_CONST Pair2Args<float> *)tab;
+    const F2_CONST int *cw = (const F2_CONST int *)wg0;
+    switch (mode) {
+    case F2_FWD:
+        hipLaunchKernelGGL((pair2_multi_kernel<float, F2_FWD, F2_MULTI_NW>), grid, blk, 0, st, ct, cw, nprob);
+        break;
+    case F2_LAM:
+        hipLaunchKernelGGL((pair2_multi_kernel<float, F2_LAM, F2_MULTI_NW>), grid, blk, 0, st, ct, cw, nprob);
+        break;
+    case F2_REC:
+        hipLaunchKernelGGL((pair2_multi_kernel<float, F2_REC, F2_MULTI_NW>), grid, blk, 0, st, ct, cw, nprob);
+        break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
 
 int step_occupancy(bool adj, int prec, int th, bool persistent)
 {

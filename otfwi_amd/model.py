@@ -312,11 +312,15 @@ def misfit(kind, sim, obs, _exact=False):
     return mis, adj
 
 
-def gradient_multi(models, shots, misfit="w2", workers=4):
-    """Gradients of several independent PMLModels (uneven sizes) at once,
-    scheduled longest-first over `workers` host threads, each model on its
-    own stream.  shots: one shot_src list per model; misfit: one kind or a
-    list.  Returns [(grad (Z,X) FP32, per-shot misfit FP64)] in model order."""
+def gradient_multi(models, shots, misfit="w2", workers=4, fused=False):
+    """Gradients of several independent PMLModels (uneven sizes) at once.
+    fused=False: scheduled longest-first over `workers` host threads, each
+    model on its own stream (otfwi_gradient_multi).  fused=True: every
+    two-step launch covers all models, the two-step phases captured into one
+    graph each (otfwi_gradient_fused; FP32 models running two steps per
+    launch, shots in one batch).  shots: one shot_src list per model;
+    misfit: one kind or a list.  Returns [(grad (Z,X) FP32, per-shot misfit
+    FP64)] in model order."""
     n = len(models)
     kinds = [misfit] * n if isinstance(misfit, str) else list(misfit)
     code = _lib.MISFIT_KINDS
@@ -330,6 +334,9 @@ def gradient_multi(models, shots, misfit="w2", workers=4):
     arr_k = (ctypes.c_int * n)(*[code[k] for k in kinds])
     arr_mis = (P * n)(*[x.ctypes.data for x in mis])
     arr_g = (P * n)(*[g.ctypes.data for g in grads])
-    check(_lib.load().otfwi_gradient_multi(n, arr_m, arr_n, arr_s, arr_k, arr_mis, arr_g,
-                                           int(workers)))
+    if fused:
+        check(_lib.load().otfwi_gradient_fused(n, arr_m, arr_n, arr_s, arr_k, arr_mis, arr_g))
+    else:
+        check(_lib.load().otfwi_gradient_multi(n, arr_m, arr_n, arr_s, arr_k, arr_mis, arr_g,
+                                               int(workers)))
     return list(zip(grads, mis))

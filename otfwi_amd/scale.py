@@ -6,7 +6,8 @@ c3     16 independent models with grid sizes drawn from {512, 768, 1024,
        1536, 2048}^2, a 32-element array and 32 shots each, T scaled with the
        grid depth as T = round(2500 + 3500 (N - 512) / 1536) (times --t-scale),
        one gradient of every model (--misfit) through otfwi_gradient_multi with
-       1 and with --workers host threads.
+       1 and with --workers host threads, and through otfwi_gradient_fused
+       (every two-step launch covering all models, each phase one graph).
 exact  one checkpointed exact-adjoint gradient (L2) of the 1024^2 FMC block,
        beside the reference-mode gradient of the same shots.
 c4     the 2048^2 differentiation configuration (16 shots, T = --T): FP32 and
@@ -66,15 +67,24 @@ def run_c3(n_models=16, shots=32, t_scale=1.0, workers=4, max_batch=0, seed=0, m
     out = {"workload": f"C3: {n_models} models, sizes {sorted(int(s) for s in sizes)}, "
                        f"{shots} shots each, T scale {t_scale}, {misfit}",
            "algorithmic_bytes": alg}
-    for w in (1, workers):
-        gradient_multi(models, shot_lists, misfit, workers=w)      # graphs, buffers
+    ref = None
+    # the fused launch needs every model's shots in one batch
+    for w in (1, workers) + (() if max_batch else ("fused",)):
+        kw = dict(fused=True) if w == "fused" else dict(workers=w)
+        gradient_multi(models, shot_lists, misfit, **kw)      # graphs, buffers
         t0 = time.perf_counter()
-        res = gradient_multi(models, shot_lists, misfit, workers=w)
+        res = gradient_multi(models, shot_lists, misfit, **kw)
         dt = time.perf_counter() - t0
         assert all(np.isfinite(g).all() for g, _ in res)
-        out[f"workers_{w}"] = {"seconds": dt, "hbm_frac": alg / dt / 8e12,
-                               "model_gradients_per_s": n_models / dt}
-        log(f"[c3] workers {w}: {dt:.2f} s ({alg / dt / 8e12:.1%} of HBM peak, algorithmic)")
+        if ref is None:
+            ref = res
+        same = all(np.array_equal(g, g0) and np.array_equal(m, m0)
+                   for (g, m), (g0, m0) in zip(res, ref))
+        key = "fused" if w == "fused" else f"workers_{w}"
+        out[key] = {"seconds": dt, "hbm_frac": alg / dt / 8e12,
+                    "model_gradients_per_s": n_models / dt, "bitwise_equal_to_first": same}
+        log(f"[c3] {key}: {dt:.2f} s ({alg / dt / 8e12:.1%} of HBM peak, algorithmic), "
+            f"bitwise equal to the first run: {same}")
     return out
 
 

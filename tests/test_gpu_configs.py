@@ -4,9 +4,10 @@ truncated T so that the oracle finishes in seconds.
 
 * uneven batch: three models of 512^2, 1024^2 and 2048^2 with 32-element
   arrays and all 32 shots resident each, run concurrently by
-  otfwi_gradient_multi; one shot per model against the oracle (traces 1e-5,
-  L2 gradient 1e-4, misfit 1e-6), every model bitwise equal to its own
-  separate gradient call;
+  otfwi_gradient_multi and with every two-step launch covering all three
+  (otfwi_gradient_fused); one shot per model against the oracle (traces
+  1e-5, L2 gradient 1e-4, misfit 1e-6), every model bitwise equal to its own
+  separate gradient call in both forms;
 * differentiation: the 2048^2 grid at dx = 0.025 mm, two shots, the exact
   adjoint with snapshots every sqrt(T) steps in the FP64 build, against
   central finite differences (step: 1e-3 m/s of velocity) along two
@@ -52,6 +53,9 @@ def test_uneven_batch_vs_oracle_and_separate_runs(gpu):
         models.append(m)
         obs_all.append(obs)
     out = gradient_multi(models, [shots] * len(sizes), "l2", workers=3)
+    fused = gradient_multi(models, [shots] * len(sizes), "l2", fused=True)
+    for (g, mis), (g0, mis0) in zip(fused, ref_runs):
+        assert np.array_equal(g, g0) and np.array_equal(mis, mis0)
     for N, p, obs, (g, mis), (g0, mis0) in zip(sizes, probs, obs_all, out, ref_runs):
         assert np.array_equal(g, g0) and np.array_equal(mis, mis0)
         others = [s for s in shots if s != chosen]

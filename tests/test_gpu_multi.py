@@ -1,6 +1,7 @@
 """Uneven multi-model batching: several models of different grid size, T and
-shot count run concurrently (longest first, one stream each) and give the
-same results, bit for bit, as separate gradient calls."""
+shot count run concurrently (longest first, one stream each), or with every
+two-step launch covering all of them (fused), and give the same results, bit
+for bit, as separate gradient calls."""
 import numpy as np
 import pytest
 
@@ -33,3 +34,34 @@ def test_multi_model_equals_separate_runs(gpu, workers):
     for (g, mis), (g0, mis0) in zip(out, ref):
         assert np.array_equal(g, g0)
         assert np.array_equal(mis, mis0)
+
+
+def test_fused_equals_separate_runs(gpu):
+    """One two-step launch per pair index over all models (uneven grids, step
+    counts, so models drop out of the later launches, shot counts and
+    misfits), twice in a row: bitwise the separate gradient calls, which run
+    these small batches one step per launch.  A model held to one step per
+    launch is refused."""
+    from otfwi_amd._lib import OtfwiError
+
+    probs = [small_problem(X=96, Z=80, T=240, seed=41, n_src=2),
+             small_problem(X=500, Z=300, T=401, seed=42, n_src=3, n_rx=10),
+             small_problem(X=128, Z=96, T=320, seed=43, n_src=2, n_rx=8),
+             small_problem(X=260, Z=530, T=90, seed=44, n_src=4, n_rx=12)]
+    shots = [[0, 1], [2, 0, 1], [1], [3, 1, 0, 2]]
+    kinds = ["w2", "l2", "w2", "l2"]
+    models, ref = [], []
+    for p, s, k in zip(probs, shots, kinds):
+        m = make_model(p, cq="cq_bg")
+        m.set_observed(observed(p, s))
+        ref.append(m.gradient(s, k))
+        models.append(m)
+    for _ in range(2):
+        out = gradient_multi(models, shots, kinds, fused=True)
+        for i, ((g, mis), (g0, mis0)) in enumerate(zip(out, ref)):
+            assert np.array_equal(g, g0), i
+            assert np.array_equal(mis, mis0), i
+    assert any(np.abs(g).max() > 0 for g, _ in out)
+    models[2].set_steps_per_launch(1, 1)
+    with pytest.raises(OtfwiError):
+        gradient_multi(models, shots, kinds, fused=True)
