@@ -293,7 +293,11 @@ int otfwi_gradient_multi(int n_models, otfwi_model *const *models, const int *n_
  * the models whose loops still have that pair; the forward and the adjoint
  * two-step phases are each captured into one hipGraph on the first model's
  * stream, and each model's single steps, misfit and reductions run on its
- * own stream, joined with events.  Arguments as otfwi_gradient_multi.  Every
+ * own stream, joined with events.  The shots of every model are split into
+ * two stream groups, each group's launches captured on its own stream, when
+ * half of the first launch still makes 2 work-groups per CU, or as the
+ * first model's otfwi_set_pair_streams says (1: one group; 2 or 4: two).
+ * Arguments as otfwi_gradient_multi.  Every
  * model must run two steps per launch of 256 threads for its batch (the
  * default at a few work-groups per CU and more), hold its shots in one
  * batch and collect no illumination; otherwise OTFWI_E_ARG names the model.
@@ -326,6 +330,13 @@ int otfwi_adjoint_from_host(otfwi_model *m, const float *adj_src, float *grad_ou
  * with the 9-point Laplacian, :390-445). */
 int otfwi_gradient_exact(otfwi_model *m, int n_shots, const int *shot_src, int misfit_kind,
                          int snap_every, double *misfit_out, float *grad_out, double *grad64_out);
+
+/* Same, writing the FP32 (Z,X) mean gradient to device memory (the FP64
+ * accumulation scaled and rounded once, as grad_out of otfwi_gradient_exact),
+ * for callers whose model and gradient stay on the GPU (the autograd
+ * wrapper).  Replaces the same reference interfaces as otfwi_gradient_exact. */
+int otfwi_gradient_exact_device(otfwi_model *m, int n_shots, const int *shot_src, int misfit_kind,
+                                int snap_every, double *misfit_out, float *grad_out_dev);
 
 /* c^2 in FP64 (for the FP64 build and finite-difference checks): the model
  * is formed from these values directly instead of their FP32 rounding.

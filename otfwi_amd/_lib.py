@@ -71,6 +71,7 @@ _SIGS = [
     ("otfwi_gradient_multi", _i, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _i]),
     ("otfwi_gradient_fused", _i, [_i, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("otfwi_gradient_exact", _i, [_vp, _i, _vp, _i, _i, _vp, _vp, _vp]),
+    ("otfwi_gradient_exact_device", _i, [_vp, _i, _vp, _i, _i, _vp, _vp]),
     ("otfwi_set_cquad64", _i, [_vp, _vp]),
     ("otfwi_set_illum", _i, [_vp, _i]),
     ("otfwi_get_illum", _i, [_vp, _vp, _i]),

@@ -15,6 +15,11 @@ mode="reference"  the reference's gradient (regression_PML_single.cu:704-846):
 
 The gradient is formed in the forward call (the adjoint needs that run's
 wavefields) and scaled by the incoming gradient in backward.
+
+A float32 CUDA tensor on an FP32 model stays on the device: the library
+reads c^2 from the tensor's memory and writes the gradient into a tensor on
+the same device (after torch's stream has finished producing c^2); other
+inputs (CPU tensors, float64, the FP64 build) go through host arrays.
 """
 from __future__ import annotations
 
@@ -27,10 +32,24 @@ from .model import PMLModel
 class PropagatorMisfit(torch.autograd.Function):
     @staticmethod
     def forward(ctx, cquad, model: PMLModel, shots, misfit, mode, snap_every):
-        cq = cquad.detach().to("cpu", torch.float64).numpy()
-        if cq.shape != (model.Z, model.X):
+        if tuple(cquad.shape) != (model.Z, model.X):
             raise ValueError(f"cquad must be ({model.Z}, {model.X})")
+        if mode not in ("exact", "reference"):
+            raise ValueError(f"unknown mode {mode!r}")
         shots = np.asarray(shots, np.int32)
+        if cquad.is_cuda and cquad.dtype == torch.float32 and model.precision == 4:
+            cq = cquad.detach().contiguous()
+            g = torch.empty_like(cq)
+            # the library's stream reads c^2: torch's work producing it must be done
+            torch.cuda.current_stream(cq.device).synchronize()
+            model.set_cquad_device(cq.data_ptr())
+            if mode == "exact":
+                mis = model.gradient_exact_device(shots, misfit, snap_every, g.data_ptr())
+            else:
+                mis = model.gradient_device(shots, misfit, g.data_ptr())
+            ctx.save_for_backward(g)
+            return torch.tensor(float(np.mean(mis)), dtype=cquad.dtype, device=cquad.device)
+        cq = cquad.detach().to("cpu", torch.float64).numpy()
         if mode == "exact":
             if model.precision == 8:
                 model.set_cquad64(cq)
@@ -40,8 +59,6 @@ class PropagatorMisfit(torch.autograd.Function):
         elif mode == "reference":
             model.set_cquad(cq.astype(np.float32))
             g, mis = model.gradient(shots, misfit)
-        else:
-            raise ValueError(f"unknown mode {mode!r}")
         ctx.save_for_backward(torch.as_tensor(np.asarray(g), dtype=cquad.dtype,
                                               device=cquad.device))
         return torch.tensor(float(np.mean(mis)), dtype=cquad.dtype, device=cquad.device)

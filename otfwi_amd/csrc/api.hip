@@ -160,6 +160,7 @@ struct otfwi_model {
     void *xsnap = nullptr, *xseg = nullptr;
     size_t xsnap_bytes = 0, xseg_bytes = 0;
     int *rxmap = nullptr;
+    int *dflag = nullptr;            // device validity flag of otfwi_set_cquad_device
     std::vector<double> cq64_h;
     float *gout = nullptr;           // FP32 mean gradient on device
     // state of a pending two-stage gradient
@@ -1414,50 +1415,73 @@ int gradient_impl(otfwi_model *m, int n, const int *shot_src, int kind, double *
     return grad_complete(m, n, grad_out, out_dev);
 }
 
-// Device copy of the two-step arguments of a fused loop: for launch k,
-// tab[k * nm ..] the problems it covers (in model order) and wg[k * (nm + 1) ..]
-// the prefix sums of their work-group counts.
-struct FusedTables {
-    int nm = 0, K = 0;
+// Two-step arguments of a fused loop phase for one stream group: for launch
+// k, tab[j][k * nm ..] the problems it covers (mode j's arguments, in model
+// order) and wg[k * (nm + 1) ..] the prefix sums of their work-group counts;
+// dev holds both tables and then the prefix table on the device.
+struct FusedGroup {
     std::vector<Pair2Args<float>> tab[2];   // [0] forward or reconstruction, [1] lambda
     std::vector<int> wg, cnt;
     std::vector<long long> nwg;
+    char *dev = nullptr;
+};
+struct FusedPhase {
+    int nm = 0, K = 0, G = 1;
+    FusedGroup grp[2];
 };
 
-// One captured graph on `st` holding the fused two-step launches of a
-// phase (modes[0] then, if given, modes[1] per pair index), replayed once.
-int run_fused_phase(hipStream_t st, const FusedTables &ft, char *dev, int nmodes, const int *modes)
+// The fused two-step launches of a phase (modes[0] then, if given, modes[1]
+// per pair index): group g's launches captured into one graph on stream g
+// (the first model's stream, then its first extra stream), the graphs
+// replayed between a fork and a join, so each group's launch tails are
+// filled by the other group's work-groups.
+int run_fused_phase(otfwi_model *m0, const FusedPhase &ph, int nmodes, const int *modes)
 {
-    const size_t tb = (size_t)ft.K * ft.nm * sizeof(Pair2Args<float>);
-    const Pair2Args<float> *dt[2] = {(const Pair2Args<float> *)dev,
-                                     (const Pair2Args<float> *)(dev + tb)};
-    const int *dwg = (const int *)(dev + 2 * tb);
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    HIPCK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    hipError_t e = hipSuccess;
-    for (int k = 0; k < ft.K && e == hipSuccess; k++)
-        for (int j = 0; j < nmodes && e == hipSuccess; j++)
-            e = launch_pair2_multi(dt[j] + (size_t)k * ft.nm, dwg + (size_t)k * (ft.nm + 1),
-                                   ft.cnt[k], ft.nwg[k], modes[j], st);
-    const hipError_t ec = hipStreamEndCapture(st, &graph);
-    HIPCK(e);
-    HIPCK(ec);
-    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    HIPCK(e);
-    e = hipGraphLaunch(exec, st);
-    const hipError_t es = hipStreamSynchronize(st);
-    (void)hipGraphExecDestroy(exec);
-    HIPCK(e);
-    HIPCK(es);
-    return OTFWI_OK;
+    hipStream_t st[2] = {m0->stream, m0->xs[0]};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    auto body = [&]() -> int {
+        const size_t tb = (size_t)ph.K * ph.nm * sizeof(Pair2Args<float>);
+        for (int g = 0; g < ph.G; g++) {
+            const FusedGroup &fg = ph.grp[g];
+            const Pair2Args<float> *dt[2] = {(const Pair2Args<float> *)fg.dev,
+                                             (const Pair2Args<float> *)(fg.dev + tb)};
+            const int *dwg = (const int *)(fg.dev + 2 * tb);
+            hipGraph_t graph = nullptr;
+            HIPCK(hipStreamBeginCapture(st[g], hipStreamCaptureModeThreadLocal));
+            hipError_t e = hipSuccess;
+            for (int k = 0; k < ph.K && e == hipSuccess; k++)
+                for (int j = 0; j < nmodes && e == hipSuccess; j++)
+                    if (fg.cnt[k] > 0)
+                        e = launch_pair2_multi(dt[j] + (size_t)k * ph.nm,
+                                               dwg + (size_t)k * (ph.nm + 1), fg.cnt[k],
+                                               fg.nwg[k], modes[j], st[g]);
+            const hipError_t ec = hipStreamEndCapture(st[g], &graph);
+            if (e != hipSuccess || ec != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                HIPCK(e);
+                HIPCK(ec);
+            }
+            e = hipGraphInstantiate(&exec[g], graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            HIPCK(e);
+        }
+        if (ph.G > 1) RCK(fork_streams(m0, ph.G));
+        for (int g = 0; g < ph.G; g++) HIPCK(hipGraphLaunch(exec[g], st[g]));
+        if (ph.G > 1) RCK(join_streams(m0, ph.G));
+        return OTFWI_OK;
+    };
+    const int rc = body();
+    for (int g = 0; g < 2; g++)
+        if (st[g]) (void)hipStreamSynchronize(st[g]);
+    for (int g = 0; g < 2; g++)
+        if (exec[g]) (void)hipGraphExecDestroy(exec[g]);
+    return rc;
 }
 
 // Gradients of several FP32 models with every two-step launch covering all of
 // them: per pair index one pair2_multi launch over the models whose loops
-// still have that pair (uneven grids, shot counts and step counts), the
-// launches of a phase captured into one graph on the first model's stream.
+// still have that pair (uneven grids, shot counts and step counts) per stream
+// group of shots, each group's launches of a phase captured into one graph.
 // Each model's single steps, misfit and reductions run on its own stream,
 // joined to the fused phases with events.  Results are bitwise those of
 // separate gradient calls (the same kernels, cells and steps).
@@ -1494,76 +1518,93 @@ int fused_gradient_impl(int nm, otfwi_model *const *ms, const int *ns, const int
         RCK(upload_shot_src(m, n, shot_src ? shot_src[i] : nullptr));
         RCK(upload_obs_batch<R>(m, 0, n));
     }
-    // argument tables of both phases
-    FusedTables tf, ta;
+    // argument tables of both phases, split into G stream groups of each
+    // model's shots (two when half of the first launch still makes 2
+    // work-groups per CU, as pair_split decides for one model)
+    FusedPhase tf, ta;
     tf.nm = ta.nm = nm;
     std::vector<int> nlf(nm), npf(nm), nla(nm), npa(nm);
+    long long half_wg = 0;
     for (int i = 0; i < nm; i++) {
         fwd_schedule(1, ms[i]->T, nlf[i], npf[i]);
         adj_schedule(1, ms[i]->T, nla[i], npa[i]);
         tf.K = std::max(tf.K, npf[i]);
         ta.K = std::max(ta.K, npa[i]);
+        half_wg += pair2_groups(ms[i]->X, ms[i]->Z, F2_MULTI_NW, ns[i] / 2);
     }
-    for (FusedTables *t : {&tf, &ta}) {
-        const bool fwd = t == &tf;
-        t->tab[0].assign((size_t)t->K * nm, Pair2Args<float>{});
-        t->tab[1].assign((size_t)t->K * nm, Pair2Args<float>{});
-        t->wg.assign((size_t)t->K * (nm + 1), 0);
-        t->cnt.assign(t->K, 0);
-        t->nwg.assign(t->K, 0);
-        for (int k = 0; k < t->K; k++) {
-            int c = 0;
-            long long w = 0;
-            for (int i = 0; i < nm; i++) {
-                otfwi_model *m = ms[i];
-                const int n = ns[i];
-                if (k >= (fwd ? npf[i] : npa[i])) continue;
-                Pair2Args<R> q, u, l;
-                if (fwd) {
-                    q = pair_fwd_args<R>(m, n, fwd_args<R>(m, n));
-                    fwd_pair_slots<R>(m, q, k, 1 + nlf[i] + 2 * k);
-                    if (!pair2_args_ok(q, 0))
-                        return fail(OTFWI_E_STATE, "fused gradient: model %d forward arguments", i);
-                    t->tab[0][(size_t)k * nm + c] = q;
-                } else {
-                    u = pair_rec_args<R>(m, n);
-                    l = pair_lam_args<R>(m, n);
-                    adj_pair_slots<R>(m, u, l, k, 1 + nla[i] + 2 * k);
-                    if (!pair2_args_ok(u, 2) || !pair2_args_ok(l, 1))
-                        return fail(OTFWI_E_STATE, "fused gradient: model %d adjoint arguments", i);
-                    t->tab[0][(size_t)k * nm + c] = u;
-                    t->tab[1][(size_t)k * nm + c] = l;
+    // the first model's pair-stream setting decides when it is not automatic
+    const int ps = ms[0]->pair_streams;
+    const int G = !ms[0]->xs[0] || ps == 1 ? 1 :
+                  ps >= 2 ? 2 : (half_wg >= 2LL * std::max(ms[0]->n_cu, 1) ? 2 : 1);
+    tf.G = ta.G = G;
+    for (FusedPhase *ph : {&tf, &ta}) {
+        const bool fwd = ph == &tf;
+        for (int g = 0; g < G; g++) {
+            FusedGroup &fg = ph->grp[g];
+            fg.tab[0].assign((size_t)ph->K * nm, Pair2Args<float>{});
+            fg.tab[1].assign((size_t)ph->K * nm, Pair2Args<float>{});
+            fg.wg.assign((size_t)ph->K * (nm + 1), 0);
+            fg.cnt.assign(ph->K, 0);
+            fg.nwg.assign(ph->K, 0);
+            for (int k = 0; k < ph->K; k++) {
+                int c = 0;
+                long long w = 0;
+                for (int i = 0; i < nm; i++) {
+                    otfwi_model *m = ms[i];
+                    const int n = ns[i];
+                    int g0, g1;
+                    group_range(n, g, G, g0, g1);
+                    if (k >= (fwd ? npf[i] : npa[i]) || g1 <= g0) continue;
+                    if (fwd) {
+                        Pair2Args<R> q = pair_fwd_args<R>(m, n, fwd_args<R>(m, n));
+                        fwd_pair_slots<R>(m, q, k, 1 + nlf[i] + 2 * k);
+                        q = pair_group<R>(m, q, g0, g1 - g0);
+                        if (!pair2_args_ok(q, 0))
+                            return fail(OTFWI_E_STATE, "fused gradient: model %d forward arguments", i);
+                        fg.tab[0][(size_t)k * nm + c] = q;
+                    } else {
+                        Pair2Args<R> u = pair_rec_args<R>(m, n), l = pair_lam_args<R>(m, n);
+                        adj_pair_slots<R>(m, u, l, k, 1 + nla[i] + 2 * k);
+                        u = pair_group<R>(m, u, g0, g1 - g0);
+                        l = pair_group<R>(m, l, g0, g1 - g0);
+                        if (!pair2_args_ok(u, 2) || !pair2_args_ok(l, 1))
+                            return fail(OTFWI_E_STATE, "fused gradient: model %d adjoint arguments", i);
+                        fg.tab[0][(size_t)k * nm + c] = u;
+                        fg.tab[1][(size_t)k * nm + c] = l;
+                    }
+                    fg.wg[(size_t)k * (nm + 1) + c] = (int)w;
+                    w += pair2_groups(m->X, m->Z, F2_MULTI_NW, g1 - g0);
+                    if (w > 0x7FFFFFFFLL)
+                        return fail(OTFWI_E_ARG, "fused gradient: too many work-groups per launch");
+                    c++;
                 }
-                t->wg[(size_t)k * (nm + 1) + c] = (int)w;
-                w += pair2_groups(m->X, m->Z, F2_MULTI_NW, n);
-                if (w > 0x7FFFFFFFLL)
-                    return fail(OTFWI_E_ARG, "fused gradient: too many work-groups per launch");
-                c++;
+                fg.wg[(size_t)k * (nm + 1) + c] = (int)w;
+                fg.cnt[k] = c;
+                fg.nwg[k] = w;
             }
-            t->wg[(size_t)k * (nm + 1) + c] = (int)w;
-            t->cnt[k] = c;
-            t->nwg[k] = w;
         }
     }
-    // one device allocation per phase: both argument tables, then the prefix table
-    char *dev[2] = {nullptr, nullptr};
+    // one device allocation per phase and group: both argument tables, then the prefix table
     int rc = OTFWI_OK;
     hipStream_t main = ms[0]->stream;
-    auto upload = [&](const FusedTables &t, char **d) -> int {
-        const size_t tb = (size_t)t.K * nm * sizeof(Pair2Args<float>);
-        const size_t bytes = 2 * tb + t.wg.size() * sizeof(int);
-        if (bytes == 0) return OTFWI_OK;
-        RCK(dalloc(d, bytes));
-        HIPCK(hipMemcpyAsync(*d, t.tab[0].data(), tb, hipMemcpyHostToDevice, main));
-        HIPCK(hipMemcpyAsync(*d + tb, t.tab[1].data(), tb, hipMemcpyHostToDevice, main));
-        HIPCK(hipMemcpyAsync(*d + 2 * tb, t.wg.data(), t.wg.size() * sizeof(int),
-                             hipMemcpyHostToDevice, main));
+    auto upload = [&](FusedPhase &ph) -> int {
+        const size_t tb = (size_t)ph.K * nm * sizeof(Pair2Args<float>);
+        for (int g = 0; g < ph.G; g++) {
+            FusedGroup &fg = ph.grp[g];
+            const size_t bytes = 2 * tb + fg.wg.size() * sizeof(int);
+            if (bytes == 0) continue;
+            RCK(dalloc(&fg.dev, bytes));
+            HIPCK(hipMemcpyAsync(fg.dev, fg.tab[0].data(), tb, hipMemcpyHostToDevice, main));
+            HIPCK(hipMemcpyAsync(fg.dev + tb, fg.tab[1].data(), tb, hipMemcpyHostToDevice, main));
+            HIPCK(hipMemcpyAsync(fg.dev + 2 * tb, fg.wg.data(), fg.wg.size() * sizeof(int),
+                                 hipMemcpyHostToDevice, main));
+        }
         HIPCK(hipStreamSynchronize(main));
         return OTFWI_OK;
     };
     auto body = [&]() -> int {
-        RCK(upload(tf, &dev[0]));
-        RCK(upload(ta, &dev[1]));
+        RCK(upload(tf));
+        RCK(upload(ta));
         // forward: memsets and leading single steps on each model's stream
         for (int i = 0; i < nm; i++) {
             otfwi_model *m = ms[i];
@@ -1573,7 +1614,7 @@ int fused_gradient_impl(int nm, otfwi_model *const *ms, const int *ns, const int
             HIPCK(hipStreamWaitEvent(main, m->evfj[1], 0));
         }
         const int mf[1] = {0}, ma[2] = {2, 1};
-        if (tf.K > 0) RCK(run_fused_phase(main, tf, dev[0], 1, mf));
+        if (tf.K > 0) RCK(run_fused_phase(ms[0], tf, 1, mf));
         HIPCK(hipEventRecord(ms[0]->evfj[2], main));
         // last forward step, misfit, adjoint memsets and leading steps per model
         for (int i = 0; i < nm; i++) {
@@ -1587,7 +1628,7 @@ int fused_gradient_impl(int nm, otfwi_model *const *ms, const int *ns, const int
             HIPCK(hipEventRecord(m->evfj[1], m->stream));
             HIPCK(hipStreamWaitEvent(main, m->evfj[1], 0));
         }
-        if (ta.K > 0) RCK(run_fused_phase(main, ta, dev[1], 2, ma));
+        if (ta.K > 0) RCK(run_fused_phase(ms[0], ta, 2, ma));
         HIPCK(hipEventRecord(ms[0]->evfj[2], main));
         for (int i = 0; i < nm; i++) {
             otfwi_model *m = ms[i];
@@ -1611,8 +1652,10 @@ int fused_gradient_impl(int nm, otfwi_model *const *ms, const int *ns, const int
     }
     // every stream idle before the tables are released
     for (int i = 0; i < nm; i++) (void)hipStreamSynchronize(ms[i]->stream);
-    dfree(dev[0]);
-    dfree(dev[1]);
+    for (int k = 0; k < 3; k++)
+        if (ms[0]->xs[k]) (void)hipStreamSynchronize(ms[0]->xs[k]);
+    for (FusedPhase *ph : {&tf, &ta})
+        for (int g = 0; g < 2; g++) dfree(ph->grp[g].dev);
     return rc;
 }
 
@@ -1746,7 +1789,7 @@ int ensure_xbuf(otfwi_model *m, size_t snap_bytes, size_t seg_bytes)
 // (mean over shots) in FP32 and/or FP64.
 template <typename R>
 int exact_gradient_impl(otfwi_model *m, int n, const int *shot_src, int kind, int k_snap,
-                        double *mis_out, float *grad_out, double *grad64_out)
+                        double *mis_out, float *grad_out, double *grad64_out, bool out_dev = false)
 {
     if (kind != OTFWI_MISFIT_L2 && kind != OTFWI_MISFIT_W2)
         return fail(OTFWI_E_ARG, "misfit kind %d: the exact gradient supports L2 and W2", kind);
@@ -1885,7 +1928,8 @@ int exact_gradient_impl(otfwi_model *m, int n, const int *shot_src, int kind, in
     const double scale = ((double)m->dt / (double)m->dx) / n;   // m = c^2 dt/dx
     if (grad_out) {
         HIPCK(launch_finalize_grad(m->gacc, cells, scale, m->gout, st));
-        HIPCK(hipMemcpyAsync(grad_out, m->gout, cells * sizeof(float), hipMemcpyDeviceToHost, st));
+        HIPCK(hipMemcpyAsync(grad_out, m->gout, cells * sizeof(float),
+                             out_dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
     }
     if (grad64_out) {
         std::vector<double> acc(cells);
@@ -2159,7 +2203,7 @@ int otfwi_destroy(otfwi_model *m)
     dfree(m->evt); dfree(m->pts); dfree(m->tf_fwd); dfree(m->tf_adj); dfree(m->evrow);
     dfree(m->pmap); dfree(m->ringtab); dfree(m->hilb);
     dfree(m->gacc); dfree(m->gout); dfree(m->ill);
-    dfree(m->xsnap); dfree(m->xseg); dfree(m->rxmap);
+    dfree(m->xsnap); dfree(m->xseg); dfree(m->rxmap); dfree(m->dflag);
     delete m;
     return OTFWI_OK;
 }
@@ -2206,6 +2250,35 @@ int otfwi_set_cquad_device(otfwi_model *m, const float *cquad_dev)
     if (!cquad_dev) return fail(OTFWI_E_ARG, "cquad is NULL");
     const size_t n = (size_t)m->X * m->Z;
     if (!m->cq32) RCK(dalloc(&m->cq32, n));
+    if (m->prec == 4) {
+        // FP32 build: the model array is formed on the device, nothing goes
+        // through host memory but the validity flag
+        const size_t np = (size_t)m->zp * m->ld;
+        if (!m->cm) {
+            RCK(dalloc((char **)&m->cm, np * sizeof(float)));
+            HIPCK(hipMemsetAsync(m->cm, 0, np * sizeof(float), m->stream));
+        }
+        if (!m->dflag) RCK(dalloc(&m->dflag, 1));
+        HIPCK(hipMemcpyAsync(m->cq32, cquad_dev, n * sizeof(float), hipMemcpyDeviceToDevice,
+                             m->stream));
+        // no bad cell: the flag keeps 0x7F7F7F7F (above any 1 + flat index)
+        HIPCK(hipMemsetAsync(m->dflag, 0x7F, sizeof(int), m->stream));
+        HIPCK(launch_model_cm<float>(m->cq32, m->X, m->Z, m->ld, m->dt, m->dx, (float *)m->cm,
+                                     m->dflag, m->stream));
+        int bad = 0;
+        HIPCK(hipMemcpyAsync(&bad, m->dflag, sizeof(int), hipMemcpyDeviceToHost, m->stream));
+        HIPCK(hipStreamSynchronize(m->stream));
+        m->cq64_h.clear();
+        if (bad != 0x7F7F7F7F) {
+            float v = 0.0f;
+            HIPCK(hipMemcpy(&v, m->cq32 + (bad - 1), sizeof(float), hipMemcpyDeviceToHost));
+            m->have_cq = false;
+            return fail(OTFWI_E_NUMERIC, "c^2 at flat index %d is %g (must be finite and >= 0)",
+                        bad - 1, (double)v);
+        }
+        m->have_cq = true;
+        return OTFWI_OK;
+    }
     HIPCK(hipMemcpy(m->cq32, cquad_dev, n * sizeof(float), hipMemcpyDeviceToDevice));
     std::vector<float> h(n);
     HIPCK(hipMemcpy(h.data(), m->cq32, n * sizeof(float), hipMemcpyDeviceToHost));
@@ -2555,7 +2628,16 @@ int otfwi_gradient_exact(otfwi_model *m, int n_shots, const int *shot_src, int m
 {
     RCK(check_model(m));
     return DISPATCH(m, exact_gradient_impl)(m, n_shots, shot_src, misfit_kind, snap_every,
-                                            misfit_out, grad_out, grad64_out);
+                                            misfit_out, grad_out, grad64_out, false);
+}
+
+int otfwi_gradient_exact_device(otfwi_model *m, int n_shots, const int *shot_src, int misfit_kind,
+                                int snap_every, double *misfit_out, float *grad_out_dev)
+{
+    RCK(check_model(m));
+    if (!grad_out_dev) return fail(OTFWI_E_ARG, "grad_out_dev is NULL");
+    return DISPATCH(m, exact_gradient_impl)(m, n_shots, shot_src, misfit_kind, snap_every,
+                                            misfit_out, grad_out_dev, nullptr, true);
 }
 
 int otfwi_set_cquad64(otfwi_model *m, const double *cquad)

@@ -911,6 +911,37 @@ hipError_t launch_finalize_grad(const double *acc, int n, double scale, float *o
     return hipGetLastError();
 }
 
+// Model array of the step kernels from c^2 on the device: cm at (z + ZOFF) *
+// ld + x + XOFF = (c^2 dt) / dx in R, the padding left as it is (zero), and
+// *bad lowered to 1 + the smallest flat index whose c^2 is not finite or is
+// negative (the caller sets it above any such value first).  One product and
+// one correctly rounded division, as the host forms it.
+template <typename R>
+__global__ void model_cm_kernel(const float *__restrict__ cq, int X, int Z, int ld, R dt, R dx,
+                                R *__restrict__ cm, int *__restrict__ bad)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int z = blockIdx.y;
+    if (x >= X || z >= Z) return;
+    const size_t q = (size_t)z * X + x;
+    const float c = cq[q];
+    if (!(c >= 0.0f && c <= 3.402823466e38f)) atomicMin(bad, (int)q + 1);
+    cm[(size_t)(z + ZOFF) * ld + x + XOFF] = ((R)c * dt) / dx;
+}
+
+template <typename R>
+hipError_t launch_model_cm(const float *cq, int X, int Z, int ld, R dt, R dx, R *cm, int *bad,
+                           hipStream_t s)
+{
+    hipLaunchKernelGGL((model_cm_kernel<R>), dim3((X + 255) / 256, Z), dim3(256), 0, s, cq, X, Z,
+                       ld, dt, dx, cm, bad);
+    return hipGetLastError();
+}
+template hipError_t launch_model_cm<float>(const float *, int, int, int, float, float, float *,
+                                           int *, hipStream_t);
+template hipError_t launch_model_cm<double>(const float *, int, int, int, double, double,
+                                            double *, int *, hipStream_t);
+
 // Source-side illumination: ill[z*X+x] += w * sum_s P_s(t)[z][x]^2 over the
 // shots of the batch (summed in shot order, FP32), one sampled step t.
 template <typename R>

@@ -221,6 +221,12 @@ template <typename R>
 hipError_t launch_accum_grad(const R *grad, int nshots, int X, int Z, int ld, long long sstride,
                              double *acc, hipStream_t s);
 hipError_t launch_finalize_grad(const double *acc, int n, double scale, float *o32, hipStream_t s);
+// cm = (c^2 dt) / dx of every grid cell into the padded plane (padding
+// untouched); *bad lowered to 1 + the smallest flat index of a c^2 that is
+// not finite or is negative, left as it is when there is none.
+template <typename R>
+hipError_t launch_model_cm(const float *cq, int X, int Z, int ld, R dt, R dx, R *cm, int *bad,
+                           hipStream_t s);
 // ill[z*X+x] += w * sum_s P[s][z][x]^2 (padded planes P, unpadded FP32 ill).
 template <typename R>
 hipError_t launch_illum(const R *P, int nshots, int X, int Z, int ld, long long sstride, float w,

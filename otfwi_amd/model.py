@@ -250,6 +250,16 @@ class PMLModel:
                                                  ptr(mis), ptr(g), None))
         return g, mis
 
+    def gradient_exact_device(self, shot_src, misfit, snap_every, grad_dev_ptr):
+        """Same as gradient_exact(), writing the (Z,X) FP32 gradient to device
+        memory; returns the per-shot misfits."""
+        kind = _lib.MISFIT_KINDS[misfit]
+        s, n = self._shots(shot_src)
+        mis = np.zeros(n, np.float64)
+        check(self._lib.otfwi_gradient_exact_device(self._h, n, ptr(s), kind, int(snap_every),
+                                                    ptr(mis), ctypes.c_void_p(int(grad_dev_ptr))))
+        return mis
+
     def objective(self, shot_src=(-1,), misfit="w2"):
         """Per-shot misfit (FP64) of the current model: forward runs and the
         device misfit, no adjoint."""

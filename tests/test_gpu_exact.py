@@ -156,6 +156,36 @@ def test_autograd_wrapper(gpu):
     assert abs(fd - an) < 1e-6 * abs(fd)
 
 
+@pytest.mark.parametrize("mode", ["exact", "reference"])
+def test_autograd_device_tensor(gpu, mode):
+    """A float32 CUDA tensor on an FP32 model: c^2 is read from the tensor's
+    memory and the gradient written into a device tensor (the model array
+    formed on the device), bitwise equal to the same values through host
+    arrays; a negative c^2 on the device is refused."""
+    import torch
+
+    from otfwi_amd._lib import OtfwiError
+    from otfwi_amd.autograd import propagator_misfit
+
+    p = small_problem(seed=35)
+    shots = [0, 1]
+    m = make_model(p)
+    m.set_observed(observed(p, shots))
+    cq_h = torch.tensor(p["cq_bg"], dtype=torch.float32, requires_grad=True)
+    J_h = propagator_misfit(cq_h, m, shots, "l2", mode)
+    J_h.backward()
+    cq_d = torch.tensor(p["cq_bg"], dtype=torch.float32, device="cuda", requires_grad=True)
+    J_d = propagator_misfit(cq_d * 1.0, m, shots, "l2", mode)
+    (2.0 * J_d).backward()
+    assert J_d.is_cuda and cq_d.grad.is_cuda
+    assert J_d.item() == J_h.item()
+    assert np.array_equal(cq_d.grad.cpu().numpy(), 2.0 * cq_h.grad.numpy())
+    bad = torch.tensor(p["cq_bg"], dtype=torch.float32, device="cuda")
+    bad[5, 7] = -1.0
+    with pytest.raises(OtfwiError):
+        m.set_cquad_device(bad.data_ptr())
+
+
 def test_exact_vs_reference_gradient(gpu):
     p = small_problem(seed=21)
     shots = [0, 1]

@@ -39,7 +39,8 @@ def test_multi_model_equals_separate_runs(gpu, workers):
 def test_fused_equals_separate_runs(gpu):
     """One two-step launch per pair index over all models (uneven grids, step
     counts, so models drop out of the later launches, shot counts and
-    misfits), twice in a row: bitwise the separate gradient calls, which run
+    misfits), in one stream group and in two (a model of one shot sits in
+    one of them), repeated: bitwise the separate gradient calls, which run
     these small batches one step per launch.  A model held to one step per
     launch is refused."""
     from otfwi_amd._lib import OtfwiError
@@ -56,11 +57,12 @@ def test_fused_equals_separate_runs(gpu):
         m.set_observed(observed(p, s))
         ref.append(m.gradient(s, k))
         models.append(m)
-    for _ in range(2):
+    for groups in (1, 2, 2, 0):
+        models[0].set_pair_streams(groups)      # stream groups of the fused launches
         out = gradient_multi(models, shots, kinds, fused=True)
         for i, ((g, mis), (g0, mis0)) in enumerate(zip(out, ref)):
-            assert np.array_equal(g, g0), i
-            assert np.array_equal(mis, mis0), i
+            assert np.array_equal(g, g0), (groups, i)
+            assert np.array_equal(mis, mis0), (groups, i)
     assert any(np.abs(g).max() > 0 for g, _ in out)
     models[2].set_steps_per_launch(1, 1)
     with pytest.raises(OtfwiError):
