@@ -114,14 +114,15 @@ class FWI:
         return np.clip(c, self.vmin, self.vmax)
 
     def run(self, c0, n_iter, learned=None, tol=None, time_limit=None, patience=1,
-            learned_scales=(1.0, 0.5)):
+            learned_scales=(1.0, 0.5), progress=None):
         """Iterate from c0.  learned(grad, illum, c) -> proposed velocity update
         (or None); the proposal, scaled by each of learned_scales in turn, is
         kept at the first scale that decreases the misfit, otherwise the plain
         step is taken; after `patience` rejected proposals no more are asked
         for.  Stops after n_iter iterations, at the velocity-error
         tolerance tol, at time_limit seconds, or when no step size decreases
-        the misfit."""
+        the misfit.  progress(record), when given, is called with every
+        iteration's record."""
         hist = History()
         t0 = time.perf_counter()
         c = np.asarray(c0, np.float64).copy()
@@ -164,6 +165,8 @@ class FWI:
                 break
             kind, c, J, g, ill, a = accepted
             hist.records.append(Record(time.perf_counter() - t0, it, J, self.err(c), kind, a))
+            if progress is not None:
+                progress(hist.records[-1])
         self.c = c
         return c, hist
 

@@ -295,9 +295,12 @@ def compare(prob: Problem, net, seeds, n_iter=30, tol=0.02, log=print, **run_kw)
         res = {"seed": int(s)}
         for kind in ("plain", "learned"):
             fwi = prob.fwi(c_true=c_true)
+            prog = (lambda r, s=s, kind=kind: log(
+                f"[compare] seed {s} {kind} it {r.it}: {r.kind} step, err {r.err:.4f}, "
+                f"{r.t:.2f} s")) if log else None
             _, h = fwi.run(prob.c_bg, n_iter=n_iter, tol=tol,
                            learned=learned_step(net, prob.mask) if kind == "learned" else None,
-                           **(run_kw if kind == "learned" else {}))
+                           progress=prog, **(run_kw if kind == "learned" else {}))
             res[kind] = dict(time_to_tol=h.time_to(tol), best_err=h.best_err,
                              err0=h.records[0].err, n_grad=h.n_grad,
                              iters=len(h.records) - 1,
@@ -315,6 +318,12 @@ def compare(prob: Problem, net, seeds, n_iter=30, tol=0.02, log=print, **run_kw)
             res[kind]["time_to_plain_final"] = res[kind].pop("_h").time_to(target)
         out.append(res)
     return out
+
+
+def start_error(prob: Problem, seed):
+    """Relative L2 velocity error of the background model against model `seed`."""
+    c_true = prob.model(int(seed))
+    return float(np.linalg.norm(prob.c_bg - c_true) / np.linalg.norm(c_true))
 
 
 def summarize(res, tol=0.02):
@@ -363,6 +372,10 @@ def main(argv=None):
                     help="finetune: first model seed of the pairs (seeds seed0 .. seed0+pairs-1)")
     ap.add_argument("--no-compare", action="store_true",
                     help="finetune: skip the held-out comparison")
+    ap.add_argument("--seed-range", type=int, nargs=2, default=None, metavar=("FIRST", "END"),
+                    help="compare: held-out seeds FIRST .. END-1 (default 100000 .. +models)")
+    ap.add_argument("--max-err0", type=float, default=None,
+                    help="compare: only the seeds whose starting velocity error is at most this")
     a = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)  # noqa: E731
     prob = Problem(**GEOMETRIES[a.geometry])
@@ -406,7 +419,11 @@ def main(argv=None):
     if a.cmd in ("compare", "finetune"):
         from .unet import UNet
         net = UNet(params=load_weights(a.weights))
-    res = compare(prob, net, [100000 + k for k in range(a.models)], n_iter=a.iters, log=log)
+    seeds = list(range(*a.seed_range)) if a.seed_range else [100000 + k for k in range(a.models)]
+    if a.max_err0 is not None:
+        seeds = [s for s in seeds if start_error(prob, s) <= a.max_err0]
+        log(f"[compare] seeds with starting error <= {a.max_err0}: {seeds}")
+    res = compare(prob, net, seeds, n_iter=a.iters, log=log)
     summary["compare"] = summarize(res)
     summary["models"] = res
     log(json.dumps(summary["compare"]))
