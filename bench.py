@@ -215,7 +215,8 @@ def inversion_compare(n_models, n_iter, geometry="fmc1024", max_err0=0.025):
         s += 1
     net = UNet(params=L.load_weights(weights))
     res = L.compare(prob, net, seeds, n_iter=n_iter,
-                    log=lambda s: print(s, file=sys.stderr, flush=True))
+                    log=lambda s: print(s, file=sys.stderr, flush=True),
+                    learned_scales=L.LEARNED_SCALES, patience=L.LEARNED_PATIENCE)
     summ = L.summarize(res)
     summ["time_to_tol_ratio"] = (summ["plain_time_to_tol"] / summ["learned_time_to_tol"]
                                  if summ["plain_time_to_tol"] and summ["learned_time_to_tol"]
@@ -237,6 +238,8 @@ def inversion_compare(n_models, n_iter, geometry="fmc1024", max_err0=0.025):
         with open(meta) as f:
             summ["training"] = json.load(f)
     summ["iterations"] = n_iter
+    summ["learned_scales"] = list(L.LEARNED_SCALES)
+    summ["learned_patience"] = L.LEARNED_PATIENCE
     summ["workload"] = (f"{geometry}: {prob.N}x{prob.N}, {prob.ns} shots, T={prob.T}, W2; "
                         f"held-out seeds {seeds} (the first from 100000 up with starting error "
                         f"<= {max_err0:.1%})")
