@@ -891,9 +891,12 @@ __global__ void head_fwd_kernel(const bf16 *X, const float *w, const float *b, s
     out[p] = s;
 }
 
-// MSE head backward: g = scale*(out - target); loss += sum g^2-like term
-// (sum (out-target)^2); gX[p][c] = X[p][c] > 0 ? g*w[c] : 0 (BF16);
-// dw[c] += sum_p g*X[p][c]; db += sum_p g.  256 threads per block.
+// MSE head backward: g = scale*(out - target); loss += sum (out-target)^2;
+// gX[p][c] = X[p][c] > 0 ? g*w[c] : 0 (BF16); dw[c] += sum_p g*X[p][c];
+// db += sum_p g.  256 threads per block; a thread takes elements e = (pixel
+// e / 4, channels 8 (e % 4) .. + 7), so the lanes of a wave read and write 16
+// consecutive pixels' 64 bytes each (coalesced), grid-stride with four
+// elements' loads issued together.
 // bsums (may be null): block b stores its 34 sums (dw[0..31], db, loss) to
 // bsums[34 b ..] instead of adding them atomically; head_bwd_sum_kernel adds
 // them in block order.
@@ -904,42 +907,70 @@ __global__ __launch_bounds__(256) void head_bwd_kernel(const bf16 *X, const floa
                                                        float *bsums)
 {
     __shared__ float red[4][34];
-    float part[34];
-    for (int k = 0; k < 34; k++) part[k] = 0.0f;
-    // grid-stride: few blocks, so that few partial sums meet in the atomics
-    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < np;
-         p += (size_t)gridDim.x * blockDim.x) {
-        const float d = out[p] - target[p];
-        const float g = scale * d;
-        part[32] += g;
-        part[33] += d * d;
-        for (int c8 = 0; c8 < 32; c8 += 8) {
-            const u32x4 v = *(const u32x4 *)(X + p * 32 + c8);
-            const bf16 *xb = (const bf16 *)&v;
+    const int k = threadIdx.x & 3;               // channel group of every element of this thread
+    float wk[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) wk[j] = w[8 * k + j];
+    float part[8] = {}, pg = 0.0f, pl = 0.0f;
+    const size_t ne = np * 4, stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t e0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e0 < ne; e0 += 4 * stride) {
+        u32x4 xv[4];
+        float ov[4], tv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const size_t e = e0 + u * stride;
+            if (e < ne) {
+                xv[u] = *(const u32x4 *)(X + e * 8);
+                ov[u] = out[e >> 2];
+                tv[u] = target[e >> 2];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const size_t e = e0 + u * stride;
+            if (e >= ne) break;
+            const float d = ov[u] - tv[u];
+            const float g = scale * d;
+            if (k == 0) {
+                pg += g;
+                pl += d * d;
+            }
+            const bf16 *xb = (const bf16 *)&xv[u];
             u32x4 o;
             bf16 *ob = (bf16 *)&o;
-            for (int k = 0; k < 8; k++) {
-                const float xv = (float)xb[k];
-                part[c8 + k] += g * xv;
-                ob[k] = (bf16)(xv > 0.0f ? g * w[c8 + k] : 0.0f);
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const float x = (float)xb[j];
+                part[j] += g * x;
+                ob[j] = (bf16)(x > 0.0f ? g * wk[j] : 0.0f);
             }
-            *(u32x4 *)(gX + p * 32 + c8) = o;
+            *(u32x4 *)(gX + e * 8) = o;
         }
     }
+    // lanes of one channel group (lane % 4) meet in a fixed butterfly
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-    for (int k = 0; k < 34; k++) {
-        float v = part[k];
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-        if (lane == 0) red[wave][k] = v;
+    for (int o = 4; o < 64; o <<= 1) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) part[j] += __shfl_xor(part[j], o, 64);
+        pg += __shfl_xor(pg, o, 64);
+        pl += __shfl_xor(pl, o, 64);
+    }
+    if (lane < 4) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) red[wave][8 * lane + j] = part[j];
+        if (lane == 0) {
+            red[wave][32] = pg;
+            red[wave][33] = pl;
+        }
     }
     __syncthreads();
     if (threadIdx.x < 34) {
-        const int k = threadIdx.x;
-        const float s = red[0][k] + red[1][k] + red[2][k] + red[3][k];
-        if (bsums) bsums[34 * blockIdx.x + k] = s;
-        else if (k < 32) atomicAdd(dw + k, s);
-        else if (k == 32) atomicAdd(db, s);
+        const int c = threadIdx.x;
+        const float s = red[0][c] + red[1][c] + red[2][c] + red[3][c];
+        if (bsums) bsums[34 * blockIdx.x + c] = s;
+        else if (c < 32) atomicAdd(dw + c, s);
+        else if (c == 32) atomicAdd(db, s);
         else atomicAdd(loss, (double)s);
     }
 }
@@ -1195,7 +1226,7 @@ int otfwi_unet_head_bwd_work(const otfwi_bf16 *X, const float *w, const float *o
         return ufail("bad head-backward args");
     if (work && work_elems < (size_t)OTFWI_UNET_HEAD_WORK)
         return ufail("head-backward workspace too small (OTFWI_UNET_HEAD_WORK floats)");
-    const unsigned nb = std::min(blocks((size_t)np), 1024u);
+    const unsigned nb = std::min(blocks((size_t)np * 4), 1024u);
     hipLaunchKernelGGL(head_bwd_kernel, dim3(nb), dim3(256), 0, ST(stream), BF(X),
                        w, out, target, (size_t)np, scale, BFW(gX), dw, db, loss, work);
     if (int rc = hipck(hipGetLastError())) return rc;
