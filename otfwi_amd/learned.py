@@ -74,6 +74,9 @@ class Problem:
         self.cp = Cuda_interface_PML(N, N, T, pml, p["c_bg"], self.ns, p["src_mask"], self.ns,
                                      p["rx_mask"], p["src"], multishot=True, R=p["R"])
         self.mask = inversion.array_mask(N, N, pml, pml + 10, taper=8)
+        # the array's mirror axis: x -> mirror - x maps its elements onto each other
+        xs = np.where(np.asarray(p["src_mask"]).any(0))[0]
+        self.mirror = int(xs.min() + xs.max())
         self.vmin = float(configs.courant(1400.0))
         self.vmax = float(configs.courant(5900.0))
         self.shots = np.arange(self.ns, dtype=np.int32)
@@ -133,9 +136,13 @@ def make_pairs(prob: Problem, seeds, augment=0.5, rng_seed=0, log_every=100, log
     return X, Y, time.perf_counter() - t0
 
 
-def train(net, X, Y, epochs, batch=8, lr=1e-4, seed=0, log=print):
+def train(net, X, Y, epochs, batch=8, lr=1e-4, seed=0, log=print, mirror=None, mask=None):
     """Adam on mean squared error over the pairs, data resident on the GPU.
-    Returns the per-epoch mean losses and the seconds spent."""
+    mirror (with mask): half of the batches, drawn at random, are mirrored
+    across the array's axis, x -> mirror - x (the geometry maps onto itself,
+    so a mirrored pair is the pair of the mirrored model), their gradient
+    channel and target multiplied by the update mask again.  Returns the
+    per-epoch mean losses and the seconds spent."""
     import torch
 
     dev = net.dev
@@ -143,6 +150,10 @@ def train(net, X, Y, epochs, batch=8, lr=1e-4, seed=0, log=print):
     Yd = torch.tensor(Y, device=dev)
     n = len(X)
     rng = np.random.default_rng(seed)
+    if mirror is not None:
+        W = X.shape[-1]
+        mcols = torch.tensor((mirror - np.arange(W)) % W, device=dev)
+        md = torch.tensor(np.asarray(mask, np.float32), device=dev)
     hist = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -151,8 +162,12 @@ def train(net, X, Y, epochs, batch=8, lr=1e-4, seed=0, log=print):
         acc = []
         for i in range(0, n - batch + 1, batch):
             idx = torch.tensor(perm[i:i + batch], device=dev)
-            acc.append(net.train_step(Xd.index_select(0, idx).contiguous(),
-                                      Yd.index_select(0, idx).contiguous(), lr=lr))
+            xb, yb = Xd.index_select(0, idx), Yd.index_select(0, idx)
+            if mirror is not None and rng.random() < 0.5:
+                xb = xb.index_select(-1, mcols)
+                yb = yb.index_select(-1, mcols) * md
+                xb[:, 0] *= md
+            acc.append(net.train_step(xb.contiguous(), yb.contiguous(), lr=lr))
         loss = float(torch.stack(acc).sum().item()) / (len(acc) * batch * X.shape[-1] ** 2)
         hist.append(loss)
         if log:
@@ -393,6 +408,8 @@ def main(argv=None):
                     help="compare: scales a learned proposal is tried at, in turn")
     ap.add_argument("--patience", type=int, default=LEARNED_PATIENCE,
                     help="compare: rejected learned proposals before plain steps only")
+    ap.add_argument("--mirror", action="store_true",
+                    help="finetune: mirror half of the batches across the array's axis")
     ap.add_argument("--learned-only", action="store_true",
                     help="compare: run the learned inversions only (no summary)")
     a = ap.parse_args(argv)
@@ -412,11 +429,13 @@ def main(argv=None):
         X, Y, t_pairs = make_pairs(prob, range(a.seed0, a.seed0 + a.pairs), rng_seed=a.seed0,
                                    log=log, log_every=10)
         net = UNet(params=load_weights(a.weights))
-        h, t_train = train(net, X, Y, a.epochs, lr=a.lr, seed=a.seed0, log=log)
+        h, t_train = train(net, X, Y, a.epochs, lr=a.lr, seed=a.seed0, log=log,
+                           mirror=prob.mirror if a.mirror else None, mask=prob.mask)
         out_w = a.save or WEIGHTS_1024
         save_weights(net, out_w)
         info = dict(pairs=a.pairs, seeds=[a.seed0, a.seed0 + a.pairs - 1], epochs=a.epochs,
-                    lr=a.lr, t_pairs=t_pairs, t_train=t_train, init=os.path.basename(a.weights),
+                    lr=a.lr, mirror=bool(a.mirror), t_pairs=t_pairs, t_train=t_train,
+                    init=os.path.basename(a.weights),
                     loss_first_epoch=h[0], loss_last_epoch=h[-1])
         summary["finetune"] = info
         with open(os.path.splitext(out_w)[0] + ".json", "w") as f:
