@@ -915,11 +915,26 @@ int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum, 
     const int ist = (illum && m->ill) ? m->ill_stride : 0;
     const int K = std::max(1, m->tb_steps[0]);
     const int G = m->tb_shots[0] > 0 ? std::min(m->tb_shots[0], n) : n;
-    // two steps per launch, scheduled by fwd_schedule
-    const bool two = use_pairs(m, n, false) && ist == 0 && K == 1 && G == n && m->evrow &&
+    // two steps per launch, scheduled by fwd_schedule.  With illumination
+    // sampled every ist-th step (ist even) the pairs start at an odd step, so
+    // every sampled step is a single step or a pair's second step, whose P
+    // plane the pair writes; such a loop runs on one stream (the illumination
+    // adds every shot into one plane).
+    const bool two = use_pairs(m, n, false) && ist % 2 == 0 && K == 1 && G == n && m->evrow &&
                      pair2_supported((long long)plane, (long long)b.cap * (long long)plane, (int)sizeof(R));
     int nlead, npairs;
     fwd_schedule(t0, t1, nlead, npairs);
+    if (ist > 0 && ((t0 + nlead) & 1) == 0) {
+        // one more leading single step and two pairs fewer: the pair count
+        // stays even and the last step single (four trailing single steps)
+        if (npairs >= 2) {
+            npairs -= 2;
+            nlead += 1;
+        } else {
+            nlead += 2 * npairs;
+            npairs = 0;
+        }
+    }
     auto single = [&](int tt) -> int {
         const int p = (tt - 1) & 1, c = tt & 1;
         a.t = tt;
@@ -928,6 +943,9 @@ int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum, 
         a.Az0 = (const R *)b.Az[p]; a.Az1 = (R *)b.Az[c];
         a.Px0 = (const R *)b.Px[p]; a.Px1 = (R *)b.Px[c];
         HIPCK(launch_fwd_step<R>(a, m->th_fwd, pw, st));
+        if (ist > 0 && tt % ist == 0)
+            HIPCK(launch_illum<R>((const R *)b.P[c], n, m->X, m->Z, m->ld, (long long)plane,
+                                  (float)ist, m->ill, st));
         return OTFWI_OK;
     };
     auto lead_fn = [&]() -> int {
@@ -955,12 +973,15 @@ int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum, 
     auto body2 = [&]() -> int {
         RCK(lead_fn());
         Pair2Args<R> q = pair_fwd_args<R>(m, n, a);
-        const int ns = pair_split(m, n);
+        const int ns = ist > 0 ? 1 : pair_split(m, n);
         if (ns > 1 && npairs > 0) RCK(fork_streams(m, ns));
         int t = t0 + nlead;
         for (int k = 0; k < npairs; k++, t += 2) {
             pair_slots(q, k, t);
             HIPCK(launch_pair_split<R>(m, q, n, ns, 0));
+            if (ist > 0 && (t + 1) % ist == 0)
+                HIPCK(launch_illum<R>((const R *)q.P1, n, m->X, m->Z, m->ld, (long long)plane,
+                                      (float)ist, m->ill, st));
         }
         if (ns > 1 && npairs > 0) RCK(join_streams(m, ns));
         return trail_fn();
@@ -1003,7 +1024,7 @@ int fwd_loop(otfwi_model *m, int n, bool save_ring, int t0, int t1, bool illum, 
         if (!two) return fail(OTFWI_E_STATE, "forward loop runs one step per launch");
         return part == LOOP_LEAD ? lead_fn() : trail_fn();
     }
-    const int ns = two && npairs > 0 ? pair_split(m, n) : 1;
+    const int ns = two && npairs > 0 && ist == 0 ? pair_split(m, n) : 1;
     if (ns > 1 && m->use_graph)
         return run_split(m, save_ring ? 1 : 0, key, n, ns, lead_fn,
                          [&](int g, hipStream_t s) { return group_fn(g, ns, s); }, trail_fn);

@@ -62,7 +62,8 @@ class Record:
 @dataclass
 class History:
     records: list = field(default_factory=list)
-    n_grad: int = 0
+    n_grad: int = 0     # gradient evaluations (forward + adjoint)
+    n_obj: int = 0      # misfit-only evaluations (forward) of trial steps
 
     def time_to(self, tol):
         """Wall time at which the velocity error first fell to tol (None if never)."""
@@ -80,8 +81,11 @@ class FWI:
     """Plain illumination-preconditioned inversion on a Cuda_interface_PML
     whose init_regression has been called (observed data and ring set).
 
-    Every objective evaluation is a full device gradient of all shots, so an
-    accepted trial also supplies the gradient of the next iteration.
+    A trial step is judged by the misfit alone (forward runs of all shots and
+    the device misfit, no adjoint); the gradient and illumination are
+    computed only at an accepted iterate, for the next iteration.  The
+    forward-only misfit equals the gradient call's misfit bit for bit, so the
+    iterates are those of judging every trial by a full gradient.
     """
 
     def __init__(self, cp: Cuda_interface_PML, mask, vmin, vmax, misfit="w2", illum_stride=4,
@@ -109,6 +113,11 @@ class FWI:
         self.m.set_cquad((np.asarray(c, np.float64) ** 2).astype(np.float32))
         g, mis = self.m.gradient(self.shots, self.misfit)
         return float(np.mean(mis)), g, self.m.illum()
+
+    def objective(self, c):
+        """Mean misfit over shots at c (forward runs only)."""
+        self.m.set_cquad((np.asarray(c, np.float64) ** 2).astype(np.float32))
+        return float(np.mean(self.m.objective(self.shots, self.misfit)))
 
     def clip(self, c):
         return np.clip(c, self.vmin, self.vmax)
@@ -140,10 +149,10 @@ class FWI:
                 dc = learned(g, ill, c)
                 for scale in (learned_scales if dc is not None else ()):
                     ct = self.clip(c + scale * dc)
-                    Jt, gt, it_ill = self.evaluate(ct)
-                    hist.n_grad += 1
+                    Jt = self.objective(ct)
+                    hist.n_obj += 1
                     if Jt < J:
-                        accepted = ("learned", ct, Jt, gt, it_ill, scale)
+                        accepted = ("learned", ct, Jt, scale)
                         break
                 if dc is not None and accepted is None:
                     rejected += 1
@@ -152,10 +161,10 @@ class FWI:
                 a = self.alpha
                 for _ in range(self.max_halvings + 1):
                     ct = self.clip(c + a * d)
-                    Jt, gt, it_ill = self.evaluate(ct)
-                    hist.n_grad += 1
+                    Jt = self.objective(ct)
+                    hist.n_obj += 1
                     if Jt < J:
-                        accepted = ("plain", ct, Jt, gt, it_ill, a)
+                        accepted = ("plain", ct, Jt, a)
                         self.alpha = min(1.5 * a, self.alpha_max)
                         break
                     a *= 0.5
@@ -163,10 +172,14 @@ class FWI:
                     self.alpha = a
             if accepted is None:
                 break
-            kind, c, J, g, ill, a = accepted
+            kind, c, J, a = accepted
             hist.records.append(Record(time.perf_counter() - t0, it, J, self.err(c), kind, a))
             if progress is not None:
                 progress(hist.records[-1])
+            # the gradient at the accepted iterate, for the next iteration
+            if it < n_iter and not (tol is not None and hist.records[-1].err <= tol):
+                J, g, ill = self.evaluate(c)
+                hist.n_grad += 1
         self.c = c
         return c, hist
 

@@ -325,7 +325,7 @@ def compare(prob: Problem, net, seeds, n_iter=30, tol=0.02, log=print, methods=(
                            learned=learned_step(net, prob.mask) if kind == "learned" else None,
                            progress=prog, **(run_kw if kind == "learned" else {}))
             res[kind] = dict(time_to_tol=h.time_to(tol), best_err=h.best_err,
-                             err0=h.records[0].err, n_grad=h.n_grad,
+                             err0=h.records[0].err, n_grad=h.n_grad, n_obj=h.n_obj,
                              iters=len(h.records) - 1,
                              wall=h.records[-1].t,
                              kinds=[r.kind for r in h.records[1:]],

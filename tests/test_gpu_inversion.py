@@ -39,6 +39,29 @@ def test_illumination_vs_oracle(gpu, stride):
         m.illum()
 
 
+@pytest.mark.parametrize("T,stride", [(400, 4), (401, 4), (402, 2), (401, 3)])
+def test_illumination_with_two_step_launches(gpu, T, stride):
+    """Illumination collected inside two-step forward loops (pairs start at an
+    odd step, sampled steps read from the pair's output, one stream) equals
+    the one-step-per-launch illumination bit for bit, and so do the gradient
+    and misfits; an odd stride runs the forward one step per launch."""
+    p = small_problem(X=420, Z=300, T=T, seed=27, n_src=5, n_rx=20)
+    shots = [0, 1, 2, 3, 4]
+    obs = observed(p, shots)
+    out = []
+    for steps in (1, 2):
+        m = make_model(p, cq="cq_bg")
+        m.set_observed(obs)
+        m.set_steps_per_launch(steps, steps)
+        m.set_pair_streams(2)
+        m.set_illum(stride)
+        g, mis = m.gradient(shots, "l2")
+        out.append((g, mis, m.illum()))
+    for u, v in zip(*out):
+        assert np.array_equal(u, v)
+    assert np.abs(out[0][2]).max() > 0
+
+
 @pytest.mark.parametrize("kind", ["l2", "w2"])
 def test_objective_equals_gradient_misfit(gpu, kind):
     p = small_problem(X=330, Z=160, T=760, seed=22)
@@ -71,6 +94,47 @@ def test_plain_direction_vs_oracle(gpu):
     e = rel(d, d_ref)
     print(f"plain direction rel {e:.3e}")
     assert e < 1e-4
+
+
+@pytest.mark.parametrize("T,stride", [(400, 4), (401, 4), (402, 2), (401, 3)])
+def test_illumination_with_two_step_launches(gpu, T, stride):
+    """Illumination collected inside two-step forward loops (pairs start at an
+    odd step, sampled steps read from the pair's output, one stream) equals
+    the one-step-per-launch illumination bit for bit, and so do the gradient
+    and misfits; an odd stride runs the forward one step per launch."""
+    p = small_problem(X=420, Z=300, T=T, seed=27, n_src=5, n_rx=20)
+    shots = [0, 1, 2, 3, 4]
+    obs = observed(p, shots)
+    out = []
+    for steps in (1, 2):
+        m = make_model(p, cq="cq_bg")
+        m.set_observed(obs)
+        m.set_steps_per_launch(steps, steps)
+        m.set_pair_streams(2)
+        m.set_illum(stride)
+        g, mis = m.gradient(shots, "l2")
+        out.append((g, mis, m.illum()))
+    for u, v in zip(*out):
+        assert np.array_equal(u, v)
+    assert np.abs(out[0][2]).max() > 0
+
+
+@pytest.mark.parametrize("kind", ["l2", "w2"])
+def test_objective_equals_gradient_misfit(gpu, kind):
+    """The forward-only misfit the inversion judges trial steps by equals the
+    gradient call's misfit bit for bit: with two steps per launch asked for,
+    the objective's forward runs two-step launches and the gradient's forward,
+    collecting illumination, one step per launch."""
+    p = small_problem(X=160, Z=128, T=400, seed=26, n_src=3, n_rx=12)
+    shots = [0, 1, 2]
+    m = make_model(p, cq="cq_bg")
+    m.set_observed(observed(p, shots))
+    m.set_steps_per_launch(2, 2)
+    m.set_illum(4)
+    _, mis_g = m.gradient(shots, kind)
+    mis_o = m.objective(shots, kind)
+    print(kind, mis_g, mis_o)
+    assert np.all(mis_g > 0) and np.array_equal(mis_g, mis_o)
 
 
 def test_plain_inversion_accepts_only_decreases(gpu):
