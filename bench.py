@@ -197,10 +197,10 @@ def unet_timing(N=8, H=1024, reps=5):
 def inversion_compare(n_models, n_iter, geometry="fmc1024", max_err0=0.025):
     """Learned vs plain inversion (30 iterations, W2 acceptance) with the
     shipped network weights, on the first n_models held-out random-defect
-    models (seeds from 100000 up) whose starting velocity error is at most
-    max_err0: the 2 % tolerance is within reach of a few steps only from
-    there (at the 64-shot 1024^2 geometry the starting errors of the held-out
-    models run from 2.2 % to 11 %, median 6.7 %)."""
+    models (seeds from 100000 up) whose starting velocity error lies in
+    (2 %, max_err0]: the 2 % tolerance is within reach of a few steps only
+    from there (at the 64-shot 1024^2 geometry the starting errors of the
+    held-out models run from about 1 % to 11 %, median 6.7 %)."""
     from otfwi_amd import learned as L
     from otfwi_amd.unet import UNet
 
@@ -210,7 +210,7 @@ def inversion_compare(n_models, n_iter, geometry="fmc1024", max_err0=0.025):
     prob = L.Problem(**L.GEOMETRIES[geometry])
     seeds, s = [], 100000
     while len(seeds) < n_models and s < 101000:
-        if L.start_error(prob, s) <= max_err0:
+        if 0.02 < L.start_error(prob, s) <= max_err0:
             seeds.append(s)
         s += 1
     net = UNet(params=L.load_weights(weights))
@@ -242,7 +242,7 @@ def inversion_compare(n_models, n_iter, geometry="fmc1024", max_err0=0.025):
     summ["learned_patience"] = L.LEARNED_PATIENCE
     summ["workload"] = (f"{geometry}: {prob.N}x{prob.N}, {prob.ns} shots, T={prob.T}, W2; "
                         f"held-out seeds {seeds} (the first from 100000 up with starting error "
-                        f"<= {max_err0:.1%})")
+                        f"in (2 %, {max_err0:.1%}])")
     return summ
 
 

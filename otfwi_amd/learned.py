@@ -402,6 +402,8 @@ def main(argv=None):
                     help="compare: held-out seeds FIRST .. END-1 (default 100000 .. +models)")
     ap.add_argument("--max-err0", type=float, default=None,
                     help="compare: only the seeds whose starting velocity error is at most this")
+    ap.add_argument("--min-err0", type=float, default=None,
+                    help="compare: only the seeds whose starting velocity error exceeds this")
     ap.add_argument("--seeds", type=int, nargs="+", default=None,
                     help="compare: these held-out seeds (overrides --seed-range)")
     ap.add_argument("--scales", type=float, nargs="+", default=list(LEARNED_SCALES),
@@ -459,9 +461,11 @@ def main(argv=None):
         net = UNet(params=load_weights(a.weights))
     seeds = (list(a.seeds) if a.seeds else list(range(*a.seed_range)) if a.seed_range
              else [100000 + k for k in range(a.models)])
-    if a.max_err0 is not None:
-        seeds = [s for s in seeds if start_error(prob, s) <= a.max_err0]
-        log(f"[compare] seeds with starting error <= {a.max_err0}: {seeds}")
+    if a.max_err0 is not None or a.min_err0 is not None:
+        hi = np.inf if a.max_err0 is None else a.max_err0
+        lo = -np.inf if a.min_err0 is None else a.min_err0
+        seeds = [s for s in seeds if lo < start_error(prob, s) <= hi]
+        log(f"[compare] seeds with starting error in ({lo}, {hi}]: {seeds}")
     res = compare(prob, net, seeds, n_iter=a.iters, log=log, patience=a.patience,
                   learned_scales=tuple(a.scales),
                   methods=("learned",) if a.learned_only else ("plain", "learned"))
