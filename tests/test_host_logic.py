@@ -156,3 +156,20 @@ def test_unet_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(lib, n), n
     assert sorted(_lib.UNET_EXPORTED) == names
+
+
+def test_fmc_array_mirror_axis():
+    """The mirror augmentation of the learned preconditioner's training maps
+    x -> xs.min() + xs.max() - x; on both full-matrix-capture geometries it
+    maps the array's source and receiver cells onto each other, so a mirrored
+    training pair is the pair of the mirrored model."""
+    from otfwi_amd import configs, learned
+
+    for kw in (learned.GEOMETRIES["fmc256"], learned.GEOMETRIES["fmc1024"]):
+        p = configs.learn_config(0, **kw)
+        src, rx = np.asarray(p["src_mask"]), np.asarray(p["rx_mask"])
+        xs = np.where(src.any(0))[0]
+        W = src.shape[1]
+        cols = (xs.min() + xs.max() - np.arange(W)) % W
+        assert np.array_equal(src[:, cols], src) and np.array_equal(rx[:, cols], rx)
+        assert src.sum() == kw.get("n_elem", 32)
