@@ -23,6 +23,7 @@ b[1]) with Adam moments beside it and a BF16 copy that the kernels read.
 """
 from __future__ import annotations
 
+
 import ctypes
 import math
 
@@ -42,6 +43,12 @@ CONVS = (
 CONV = {c[0]: c for c in CONVS}
 IN_CH = 2          # real input channels (gradient, illumination)
 PAD_CH = 16        # input channels as stored
+# work-groups per weight-gradient launch (each sums several 256-pixel tiles);
+# in the fixed-order form every work-group's sums are one workspace row that
+# wgrad_reduce_kernel reads back, so fewer groups mean fewer bytes reduced.
+# Training step at 8 x 1024^2 (MI355X): 16.28-16.30 ms with 1024, 16.38-16.42
+# with 2048, 16.96-17.01 with 512.
+WGRAD_GROUPS = 1024
 
 
 def param_layout():
@@ -248,7 +255,7 @@ class UNet:
         _, ca, cb, co = CONV[name]
         tiles = N * H * W // 256
         other = max(1, co // 64) * ((ca + cb + 31) // 32)
-        tpw = max(1, -(-tiles * other // 2048))
+        tpw = max(1, -(-tiles * other // WGRAD_GROUPS))
         if not self.deterministic:
             _lib.ucheck(self.lib.otfwi_unet_wgrad(_vp(A), ca, _vp(B), cb, 1 if cb else 0, _vp(G),
                                                   self._gw(name), self._gb(name), N, H, W, co,
