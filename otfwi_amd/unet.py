@@ -23,7 +23,6 @@ b[1]) with Adam moments beside it and a BF16 copy that the kernels read.
 """
 from __future__ import annotations
 
-
 import ctypes
 import math
 
