@@ -271,7 +271,7 @@ def main():
                     help="forward+adjoint steps of the CPU sample (0: the whole T-1)")
     ap.add_argument("--inversion-geometry", default="fmc1024", choices=["fmc256", "fmc1024"])
     ap.add_argument("--inversion-max-err0", type=float, default=0.025)
-    ap.add_argument("--inversion-models", type=int, default=1,
+    ap.add_argument("--inversion-models", type=int, default=3,
                     help="held-out models for the learned vs plain inversion (0: skip)")
     ap.add_argument("--inversion-iters", type=int, default=30)
     ap.add_argument("--no-unet", action="store_true", help="skip the U-Net MFMA timing")
