@@ -223,9 +223,14 @@ def inversion_compare(n_models, n_iter, geometry="fmc1024", max_err0=0.025):
                                  else None)
     # where the plain method never reaches the tolerance, the learned method's
     # lead is at least the plain run's whole wall time over the learned time
+    # (unbounded where the plain run stalled: it ended, no step size lowering
+    # the misfit, before its iteration budget)
     lower = [r["plain"]["wall"] / r["learned"]["time_to_tol"] for r in res
              if r["learned"]["time_to_tol"] and r["plain"]["time_to_tol"] is None]
     summ["time_to_tol_ratio_lower_bound"] = min(lower) if lower else None
+    summ["time_to_tol_ratio_lower_bound_per_model"] = lower
+    summ["plain_stalled_above_tol"] = sum(
+        1 for r in res if r["plain"]["time_to_tol"] is None and r["plain"]["iters"] < n_iter)
     summ["per_model"] = [{"seed": r["seed"], "err0": r["plain"]["err0"],
                           "plain_best_err": r["plain"]["best_err"],
                           "learned_best_err": r["learned"]["best_err"],
