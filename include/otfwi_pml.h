@@ -127,7 +127,11 @@ int otfwi_set_damping(otfwi_model *m, const float *d_x, const float *d_z);
 
 /* c^2, (Z,X) FP32 host array (setCquad, regression_PML_single.cu:253-257). */
 int otfwi_set_cquad(otfwi_model *m, const float *cquad);
-/* c^2 from device memory (FP32, Z*X contiguous). */
+/* c^2 from device memory (FP32, Z*X contiguous), read on the model's stream
+ * (the caller's stream must have finished writing it).  In the FP32 build the
+ * model array is formed on the device and only a validity flag comes back;
+ * a value that is not finite or is negative fails the call (OTFWI_E_NUMERIC)
+ * and leaves the model without c^2 until the next set. */
 int otfwi_set_cquad_device(otfwi_model *m, const float *cquad_dev);
 
 /* Sources: positions and wavelets (n_src*T FP32); the kernels inject the
