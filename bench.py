@@ -194,6 +194,33 @@ def unet_timing(N=8, H=1024, reps=5):
             "hbm_bound_convs": roof["hbm_bound_convs"], "mfma_bound_convs": roof["mfma_bound_convs"]}
 
 
+def inversion_summary(summ, res, n_iter):
+    """Time-to-tolerance figures added to a learned.summarize() dict `summ` of
+    the per-model results `res` of learned.compare() (n_iter iterations).
+    Where the plain method never reaches the tolerance, the learned method's
+    lead is at least the plain run's whole wall time over the learned time,
+    and unbounded where the plain run stalled (it ended, no step size lowering
+    the misfit, before its iteration budget)."""
+    summ = dict(summ)
+    summ["time_to_tol_ratio"] = (summ["plain_time_to_tol"] / summ["learned_time_to_tol"]
+                                 if summ["plain_time_to_tol"] and summ["learned_time_to_tol"]
+                                 else None)
+    lower = [r["plain"]["wall"] / r["learned"]["time_to_tol"] for r in res
+             if r["learned"]["time_to_tol"] and r["plain"]["time_to_tol"] is None]
+    summ["time_to_tol_ratio_lower_bound"] = min(lower) if lower else None
+    summ["time_to_tol_ratio_lower_bound_per_model"] = lower
+    summ["plain_stalled_above_tol"] = sum(
+        1 for r in res if r["plain"]["time_to_tol"] is None and r["plain"]["iters"] < n_iter)
+    summ["per_model"] = [{"seed": r["seed"], "err0": r["plain"]["err0"],
+                          "plain_best_err": r["plain"]["best_err"],
+                          "learned_best_err": r["learned"]["best_err"],
+                          "plain_time_to_tol": r["plain"]["time_to_tol"],
+                          "learned_time_to_tol": r["learned"]["time_to_tol"],
+                          "plain_wall": r["plain"]["wall"],
+                          "learned_wall": r["learned"]["wall"]} for r in res]
+    return summ
+
+
 def inversion_compare(n_models, n_iter, geometry="fmc1024", max_err0=0.025):
     """Learned vs plain inversion (30 iterations, W2 acceptance) with the
     shipped network weights, on the first n_models held-out random-defect
@@ -217,27 +244,7 @@ def inversion_compare(n_models, n_iter, geometry="fmc1024", max_err0=0.025):
     res = L.compare(prob, net, seeds, n_iter=n_iter,
                     log=lambda s: print(s, file=sys.stderr, flush=True),
                     learned_scales=L.LEARNED_SCALES, patience=L.LEARNED_PATIENCE)
-    summ = L.summarize(res)
-    summ["time_to_tol_ratio"] = (summ["plain_time_to_tol"] / summ["learned_time_to_tol"]
-                                 if summ["plain_time_to_tol"] and summ["learned_time_to_tol"]
-                                 else None)
-    # where the plain method never reaches the tolerance, the learned method's
-    # lead is at least the plain run's whole wall time over the learned time
-    # (unbounded where the plain run stalled: it ended, no step size lowering
-    # the misfit, before its iteration budget)
-    lower = [r["plain"]["wall"] / r["learned"]["time_to_tol"] for r in res
-             if r["learned"]["time_to_tol"] and r["plain"]["time_to_tol"] is None]
-    summ["time_to_tol_ratio_lower_bound"] = min(lower) if lower else None
-    summ["time_to_tol_ratio_lower_bound_per_model"] = lower
-    summ["plain_stalled_above_tol"] = sum(
-        1 for r in res if r["plain"]["time_to_tol"] is None and r["plain"]["iters"] < n_iter)
-    summ["per_model"] = [{"seed": r["seed"], "err0": r["plain"]["err0"],
-                          "plain_best_err": r["plain"]["best_err"],
-                          "learned_best_err": r["learned"]["best_err"],
-                          "plain_time_to_tol": r["plain"]["time_to_tol"],
-                          "learned_time_to_tol": r["learned"]["time_to_tol"],
-                          "plain_wall": r["plain"]["wall"],
-                          "learned_wall": r["learned"]["wall"]} for r in res]
+    summ = inversion_summary(L.summarize(res), res, n_iter)
     meta = os.path.splitext(weights)[0] + ".json"
     if os.path.exists(meta):
         with open(meta) as f:
