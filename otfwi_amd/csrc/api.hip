@@ -1464,6 +1464,8 @@ int run_fused_phase(otfwi_model *m0, const FusedPhase &ph, int nmodes, const int
         const size_t tb = (size_t)ph.K * ph.nm * sizeof(Pair2Args<float>);
         for (int g = 0; g < ph.G; g++) {
             const FusedGroup &fg = ph.grp[g];
+            // a group no model has shots in (one-shot models forced into two groups)
+            if (std::none_of(fg.cnt.begin(), fg.cnt.end(), [](int c) { return c > 0; })) continue;
             const Pair2Args<float> *dt[2] = {(const Pair2Args<float> *)fg.dev,
                                              (const Pair2Args<float> *)(fg.dev + tb)};
             const int *dwg = (const int *)(fg.dev + 2 * tb);
@@ -1487,7 +1489,8 @@ int run_fused_phase(otfwi_model *m0, const FusedPhase &ph, int nmodes, const int
             HIPCK(e);
         }
         if (ph.G > 1) RCK(fork_streams(m0, ph.G));
-        for (int g = 0; g < ph.G; g++) HIPCK(hipGraphLaunch(exec[g], st[g]));
+        for (int g = 0; g < ph.G; g++)
+            if (exec[g]) HIPCK(hipGraphLaunch(exec[g], st[g]));
         if (ph.G > 1) RCK(join_streams(m0, ph.G));
         return OTFWI_OK;
     };

@@ -67,3 +67,23 @@ def test_fused_equals_separate_runs(gpu):
     models[2].set_steps_per_launch(1, 1)
     with pytest.raises(OtfwiError):
         gradient_multi(models, shots, kinds, fused=True)
+
+
+def test_fused_one_shot_models_two_groups(gpu):
+    """Two stream groups forced on models of one shot each: every model sits
+    in the second group, the first has no launches and is skipped; results
+    bitwise those of separate calls."""
+    probs = [small_problem(X=96, Z=80, T=240, seed=45, n_src=2),
+             small_problem(X=160, Z=128, T=301, seed=46, n_src=3, n_rx=10)]
+    shots = [[1], [2]]
+    models, ref = [], []
+    for p, s in zip(probs, shots):
+        m = make_model(p, cq="cq_bg")
+        m.set_observed(observed(p, s))
+        ref.append(m.gradient(s, "l2"))
+        models.append(m)
+    models[0].set_pair_streams(2)
+    out = gradient_multi(models, shots, "l2", fused=True)
+    for (g, mis), (g0, mis0) in zip(out, ref):
+        assert np.array_equal(g, g0) and np.array_equal(mis, mis0)
+    assert all(np.abs(g).max() > 0 for g, _ in out)
